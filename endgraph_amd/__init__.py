@@ -1,0 +1,252 @@
+"""Python host mirror of the MI355X-native EndGraph library (libendgraph.so).
+
+This is plumbing over the C ABI in include/endgraph.h (ctypes); all work runs
+in the hand-written gfx950 kernels.  There is no CPU fallback: importing the
+native library fails loudly when it has not been built.
+
+The Graph class mirrors the reference's Graph<EdgeData> (core/graph.hpp:100):
+construct with the partition/socket counts, call load_directed /
+load_undirected_from_directed with a device edge list, read back the built
+arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libendgraph.so")
+_lib = None
+
+REC8 = np.dtype([("src", "<u4"), ("dst", "<u4")])
+REC12 = np.dtype([("src", "<u4"), ("dst", "<u4"), ("w", "<f4")])
+CIDX = np.dtype([("index", "<u8"), ("vertex", "<u4")], align=False)
+
+EG_OK = 0
+
+
+class EndGraphError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"endgraph status {status}: {msg}")
+        self.status = status
+
+
+def build_library(jobs: int = 8) -> str:
+    """Compile every HIP source for gfx950 into endgraph_amd/libendgraph.so."""
+    subprocess.run(["make", "-s", "-C", _HERE, f"-j{jobs}"], check=True)
+    return LIB_PATH
+
+
+class _Info(C.Structure):
+    _fields_ = [("vertices_num", C.c_uint32), ("partitions", C.c_int), ("sockets", C.c_int),
+                ("record_bytes", C.c_int), ("edges_num", C.c_uint64), ("pages_num", C.c_uint32),
+                ("symmetrized", C.c_int)]
+
+
+def lib():
+    """The loaded native library (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"native library missing at {LIB_PATH}: run `make -C endgraph_amd` "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, i32, f32, f64 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float, C.c_double
+    pp = C.POINTER(C.c_void_p)
+    sigs = {
+        "eg_last_error": ([], C.c_char_p),
+        "eg_version": ([], i32),
+        "eg_device_alloc": ([pp, u64], i32),
+        "eg_device_free": ([vp], i32),
+        "eg_memcpy_h2d": ([vp, vp, u64, vp], i32),
+        "eg_memcpy_d2h": ([vp, vp, u64, vp], i32),
+        "eg_memcpy_d2d": ([vp, vp, u64, vp], i32),
+        "eg_stream_synchronize": ([vp], i32),
+        "eg_gen_rmat": ([vp, u64, u32, f64, f64, f64, u64, i32, f32, f32, vp], i32),
+        "eg_gen_powerlaw": ([vp, u64, u32, f64, f64, f64, f64, u64, i32, vp], i32),
+        "eg_graph_create": ([u32, i32, i32, u64, u32, i32, pp], i32),
+        "eg_graph_destroy": ([vp], i32),
+        "eg_graph_build": ([vp, vp, u64, i32, vp], i32),
+        "eg_graph_get_info": ([vp, C.POINTER(_Info)], i32),
+        "eg_graph_partition_offset": ([vp, vp], i32),
+        "eg_graph_local_partition_offset": ([vp, i32, vp], i32),
+        "eg_graph_degree": ([vp, pp, pp], i32),
+        "eg_graph_page_prefix": ([vp, pp], i32),
+        "eg_graph_csc": ([vp, i32, i32, pp, C.POINTER(u64), pp, C.POINTER(u64)], i32),
+        "eg_graph_csr": ([vp, i32, i32, pp, C.POINTER(u64), pp, pp], i32),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def check(status: int):
+    if status != EG_OK:
+        raise EndGraphError(status, lib().eg_last_error().decode())
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(int(stream.cuda_stream))  # torch.cuda.Stream
+
+
+class DeviceBuffer:
+    """HBM allocation owned by Python (eg_device_alloc / eg_device_free)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().eg_device_alloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().eg_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+    def upload(self, arr: np.ndarray, stream=None):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        check(lib().eg_memcpy_h2d(C.c_void_p(self.ptr), arr.ctypes.data_as(C.c_void_p), arr.nbytes, _stream(stream)))
+
+    def download(self, dtype, count: int, offset_bytes: int = 0, stream=None) -> np.ndarray:
+        return to_host(self.ptr + offset_bytes, dtype, count, stream)
+
+    @classmethod
+    def from_array(cls, arr: np.ndarray):
+        b = cls(max(arr.nbytes, 1))
+        b.upload(arr)
+        return b
+
+
+def to_host(d_ptr: int, dtype, count: int, stream=None) -> np.ndarray:
+    """Copy `count` elements of `dtype` from device address d_ptr."""
+    dtype = np.dtype(dtype)
+    out = np.empty(count, dtype=dtype)
+    if count:
+        check(lib().eg_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(d_ptr), out.nbytes, _stream(stream)))
+    return out
+
+
+def synchronize(stream=None):
+    check(lib().eg_stream_synchronize(_stream(stream)))
+
+
+# ------------------------------------------------------------ generators --
+def gen_rmat(d_edges: int, E: int, scale: int, a=0.57, b=0.19, c=0.19, seed=1, rec_bytes=8, w_lo=0.0, w_hi=1.0,
+             stream=None):
+    check(lib().eg_gen_rmat(C.c_void_p(d_edges), E, scale, a, b, c, seed, rec_bytes, w_lo, w_hi, _stream(stream)))
+
+
+def gen_powerlaw(d_edges: int, E: int, V: int, alpha_out=1.9, alpha_in=2.1, max_out=2_997_469.0, max_in=770_155.0,
+                 seed=3, rec_bytes=8, stream=None):
+    check(lib().eg_gen_powerlaw(C.c_void_p(d_edges), E, V, alpha_out, alpha_in, max_out, max_in, seed, rec_bytes,
+                                _stream(stream)))
+
+
+# ----------------------------------------------------------------- graph --
+class Graph:
+    """Graph<EdgeData> (core/graph.hpp:100) with P partitions and S sockets on one GPU."""
+
+    def __init__(self, vertices_num: int, partitions: int = 1, sockets: int = 1, eta: int = 0, m: int = 1,
+                 record_bytes: int = 8):
+        h = C.c_void_p()
+        check(lib().eg_graph_create(vertices_num, partitions, sockets, eta, m, record_bytes, C.byref(h)))
+        self._h = h
+        self.V, self.P, self.S, self.rec = int(vertices_num), int(partitions), int(sockets), int(record_bytes)
+        self.rec_dtype = REC8 if self.rec == 8 else REC12
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().eg_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # load_directed / load_undirected_from_directed (core/graph.hpp:1358, 629)
+    def build(self, d_edges: int, E0: int, symmetrize: bool = False, stream=None):
+        check(lib().eg_graph_build(self._h, C.c_void_p(d_edges), E0, int(bool(symmetrize)), _stream(stream)))
+
+    def load_directed(self, d_edges: int, E0: int, stream=None):
+        self.build(d_edges, E0, False, stream)
+
+    def load_undirected_from_directed(self, d_edges: int, E0: int, stream=None):
+        self.build(d_edges, E0, True, stream)
+
+    def info(self) -> dict:
+        i = _Info()
+        check(lib().eg_graph_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _Info._fields_}
+
+    @property
+    def edges_num(self) -> int:
+        return int(self.info()["edges_num"])
+
+    def partition_offset(self) -> np.ndarray:
+        out = np.zeros(self.P + 1, np.uint32)
+        check(lib().eg_graph_partition_offset(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def local_partition_offset(self, p: int) -> np.ndarray:
+        out = np.zeros(self.S + 1, np.uint32)
+        check(lib().eg_graph_local_partition_offset(self._h, p, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def degree_ptrs(self):
+        o, i = C.c_void_p(), C.c_void_p()
+        check(lib().eg_graph_degree(self._h, C.byref(o), C.byref(i)))
+        return o.value, i.value
+
+    def out_degree(self) -> np.ndarray:
+        return to_host(self.degree_ptrs()[0], np.uint32, self.V)
+
+    def in_degree(self) -> np.ndarray:
+        return to_host(self.degree_ptrs()[1], np.uint32, self.V)
+
+    def page_prefix(self) -> np.ndarray:
+        p = C.c_void_p()
+        check(lib().eg_graph_page_prefix(self._h, C.byref(p)))
+        pages = (self.V + 4095) // 4096
+        return to_host(p.value, np.uint64, pages + 1)
+
+    def csc_ptrs(self, p: int, s: int = 0):
+        lst, ci = C.c_void_p(), C.c_void_p()
+        n, nci = C.c_uint64(), C.c_uint64()
+        check(lib().eg_graph_csc(self._h, p, s, C.byref(lst), C.byref(n), C.byref(ci), C.byref(nci)))
+        return lst.value, n.value, ci.value, nci.value
+
+    def csr_ptrs(self, p: int, s: int = 0):
+        lst, idx, bm = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        n = C.c_uint64()
+        check(lib().eg_graph_csr(self._h, p, s, C.byref(lst), C.byref(n), C.byref(idx), C.byref(bm)))
+        return lst.value, n.value, idx.value, bm.value
+
+    def csc(self, p: int, s: int = 0):
+        """(incoming_adj_list records, compressed_incoming_adj_index) of (p, s), copied to the host."""
+        lst, n, ci, nci = self.csc_ptrs(p, s)
+        return to_host(lst, self.rec_dtype, n), to_host(ci, CIDX, nci)
+
+    def csr(self, p: int, s: int = 0):
+        """(outgoing_adj_list records, outgoing_adj_index u64[V+1], outgoing_adj_bitmap words) of (p, s)."""
+        lst, n, idx, bm = self.csr_ptrs(p, s)
+        return (to_host(lst, self.rec_dtype, n), to_host(idx, np.uint64, self.V + 1),
+                to_host(bm, np.uint64, (self.V >> 6) + 1))

@@ -1,0 +1,681 @@
+// Graph construction: EndGraph's preprocessing (core/graph.hpp:424-1373)
+// recast for one MI355X with the P partitions side by side in HBM.
+//
+//   symmetrise   per-rank reversed copies            (:1057-1078)
+//   degree       out/in-degree histogram, id check   (:671-716, R5)
+//   page weights + CPBT boundary search             (:586-626, :718-780)
+//   CSC order    (q(src), dst, r, src, j); P=1: (q(src), dst, j)
+//   CSR order    (q(dst), src, r, dst, j); P=1: (q(dst), src, j)
+//                r = source rank, j = position in the rank's list,
+//                q = p*S + s the sub-partition (partition, socket) of the
+//                owning endpoint.  The reference reaches these orders by the
+//                shuffle's stable sort per rank (:815), concatenation by
+//                source rank (:957) and two stable sorts per partition
+//                (:1107/:1113, :1262/:1268).  Here: for P>1 one stable sort
+//                of every rank slice by the owner id, then one stable sort of
+//                everything by the composite key (q, key).
+//   indexes      compressed CSC index (:1147-1188), dense CSR index and
+//                bitmap (:1312-1353).
+#include <algorithm>
+#include <cstring>
+
+#include "graph_state.hpp"
+#include "sort.hpp"
+
+namespace eg {
+namespace {
+
+constexpr int kNT = 256;
+
+template <class T>
+__global__ void k_fill(T* __restrict__ p, uint64_t n, T v) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+template <class T>
+int fill(T* p, uint64_t n, T v, hipStream_t s) {
+  if (n == 0) return EG_OK;
+  unsigned grid = unsigned(std::min<uint64_t>(ceil_div(n, kNT), 8192));
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid), dim3(kNT), 0, s, p, n, v);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+// Per-rank symmetrisation (core/graph.hpp:1057-1078): rank r's list is its
+// slice followed by the reversed copy of that slice.
+template <class R>
+__global__ void k_symmetrize(const R* __restrict__ in, R* __restrict__ out, uint64_t E0, int P) {
+  const uint64_t base = E0 / P;
+  const uint64_t n_out = 2 * E0;
+  for (uint64_t o = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < n_out; o += uint64_t(gridDim.x) * blockDim.x) {
+    uint64_t r = base ? o / (2 * base) : uint64_t(P - 1);
+    if (r > uint64_t(P - 1)) r = P - 1;
+    const uint64_t n_r = base + (r == uint64_t(P - 1) ? E0 % P : 0);
+    const uint64_t local = o - 2 * base * r;
+    R e;
+    if (local < n_r) {
+      e = in[base * r + local];
+    } else {
+      e = in[base * r + local - n_r];
+      uint32_t t = e.src;
+      e.src = e.dst;
+      e.dst = t;
+    }
+    out[o] = e;
+  }
+}
+
+// Degree histogram (core/graph.hpp:689-707) for both directions; ids >= V are
+// skipped and flagged.  For a symmetrised build it reads the input once and
+// counts each edge at both endpoints.
+template <class R>
+__global__ __launch_bounds__(kNT) void k_degree(const R* __restrict__ e, uint64_t n, uint32_t V, int sym,
+                                                 uint32_t* __restrict__ out_deg, uint32_t* __restrict__ in_deg,
+                                                 uint32_t* __restrict__ err) {
+  bool bad = false;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    const R r = e[i];
+    if (r.src >= V || r.dst >= V) {
+      bad = true;
+      continue;
+    }
+    atomicAdd(&out_deg[r.src], 1u);
+    if (sym)
+      atomicAdd(&out_deg[r.dst], 1u);
+    else
+      atomicAdd(&in_deg[r.dst], 1u);
+  }
+  if (__ballot(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(err, 1u);
+}
+
+// page_amount (core/graph.hpp:718-741): one block per page.  The weight
+// (1+(P-1)/16)*deg + eta*(P-1 + m*deg) keeps the reference's integer types:
+// the first product and the bracket are 32-bit unsigned, the rest 64-bit.
+__global__ __launch_bounds__(kNT) void k_page_weights(const uint32_t* __restrict__ deg, uint32_t V, uint32_t f1,
+                                                       uint32_t pm1, uint32_t m, uint64_t eta,
+                                                       uint64_t* __restrict__ page_w, uint32_t pages) {
+  __shared__ uint64_t wt[kNT / kWave];
+  const uint32_t pg = blockIdx.x;
+  uint64_t acc = 0;
+  for (uint32_t k = threadIdx.x; k < kPageSize; k += kNT) {
+    const uint32_t v = pg * kPageSize + k;
+    if (v < V) {
+      const uint32_t d = deg[v];
+      const uint32_t a = f1 * d;
+      const uint32_t b = pm1 + m * d;
+      acc += uint64_t(a) + eta * uint64_t(b);
+    }
+  }
+  acc = wave_reduce(acc, OpSum{});
+  if ((threadIdx.x & (kWave - 1)) == 0) wt[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int i = 0; i < kNT / kWave; ++i) t += wt[i];
+    page_w[pg] = t;
+    if (pg == pages - 1) page_w[pages] = 0;  // slot that becomes the total after the scan
+  }
+}
+
+// index of the last prefix entry <= x among prefix[lo..hi] (upper_bound - 1);
+// prefix is non-decreasing and prefix[lo] <= x holds for every caller.
+__device__ uint32_t last_le(const uint64_t* prefix, uint32_t lo, uint32_t hi, uint64_t x) {
+  uint32_t a = lo, b = hi + 1;  // answer in [a, b)
+  while (b - a > 1) {
+    uint32_t mid = a + (b - a) / 2;
+    if (prefix[mid] <= x)
+      a = mid;
+    else
+      b = mid;
+  }
+  return a;
+}
+
+// judge (core/graph.hpp:586-594) on the pages [lo, lo+n] of the global prefix.
+__device__ bool cpbt_judge(const uint64_t* prefix, uint32_t lo, uint32_t n, uint32_t k, uint64_t s) {
+  uint64_t sum = prefix[lo];
+  for (uint32_t i = 0; i < k; ++i) sum = prefix[last_le(prefix, lo, lo + n, sum + s)];
+  return sum == prefix[lo + n];
+}
+
+// optimal_offset (core/graph.hpp:597-626) by one wave: the smallest feasible
+// bottleneck in (lb, ub] found by a 64-way search (judge is monotone in s, so
+// this is the bisection's answer), then the greedy boundaries.  Writes k+1
+// absolute page ids to out.
+__device__ void cpbt_wave(const uint64_t* prefix, uint32_t lo, uint32_t n, uint32_t k, uint32_t* out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  uint64_t lb = 0, ub = prefix[lo + n] - prefix[lo];
+  while (lb + 1 < ub) {
+    const uint64_t range = ub - lb - 1;  // candidates lb+1 .. ub-1
+    const uint64_t c = lb + 1 + (range / 64) * uint64_t(lane) + ((range % 64) * uint64_t(lane)) / 64;
+    const bool ok = cpbt_judge(prefix, lo, n, k, c);
+    const uint64_t mask = __ballot(ok);
+    if (mask) {
+      const int first = __ffsll((unsigned long long)mask) - 1;
+      const uint64_t cf = __shfl(c, first, kWave);
+      const uint64_t cp = __shfl(c, first > 0 ? first - 1 : 0, kWave);
+      ub = cf;
+      if (first > 0) lb = cp;
+    } else {
+      lb = __shfl(c, kWave - 1, kWave);
+    }
+  }
+  if (lane == 0) {
+    out[0] = lo;
+    out[k] = lo + n;
+    uint64_t sum = prefix[lo];
+    for (uint32_t i = 0; i + 1 < k; ++i) {
+      uint32_t at = last_le(prefix, lo, lo + n, sum + ub);
+      out[i + 1] = at;
+      sum = prefix[at];
+    }
+  }
+}
+
+// Machine level (k = P) then socket level (k = S) for every partition; one
+// wave.  Page boundaries become vertex boundaries clamped to V (R9).
+__global__ void k_cpbt(const uint64_t* __restrict__ prefix, uint32_t pages, uint32_t V, int P, int S,
+                       eg_graph::Meta* __restrict__ meta) {
+  cpbt_wave(prefix, 0, pages, uint32_t(P), meta->page_offset);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i <= P; ++i)
+      meta->partition_offset[i] = uint32_t(min(uint64_t(meta->page_offset[i]) * kPageSize, uint64_t(V)));
+    meta->partition_offset[0] = 0;
+    meta->partition_offset[P] = V;
+  }
+  __syncthreads();
+  for (int p = 0; p < P; ++p) {
+    const uint32_t pb = meta->page_offset[p], pe = meta->page_offset[p + 1];
+    uint32_t* lpo = meta->local_page_offset + p * (S + 1);
+    cpbt_wave(prefix, pb, pe - pb, uint32_t(S), lpo);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t* lo = meta->local_partition_offset + p * (S + 1);
+      const uint32_t vb = meta->partition_offset[p], ve = meta->partition_offset[p + 1];
+      for (int i = 0; i <= S; ++i)
+        lo[i] = uint32_t(min(uint64_t(lpo[i]) * kPageSize, uint64_t(ve)));
+      lo[0] = vb;
+      lo[S] = ve;
+    }
+    __syncthreads();
+  }
+}
+
+// qmap[page] = p*S + s of the page (vertex ids in one page share it).
+__global__ void k_qmap(const eg_graph::Meta* __restrict__ meta, uint32_t pages, int P, int S,
+                       uint16_t* __restrict__ qmap) {
+  const uint32_t pg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pg >= pages) return;
+  int p = 0;
+  while (p + 1 < P && meta->page_offset[p + 1] <= pg) ++p;
+  const uint32_t* lpo = meta->local_page_offset + p * (S + 1);
+  int s = 0;
+  while (s + 1 < S && lpo[s + 1] <= pg) ++s;
+  qmap[pg] = uint16_t(p * S + s);
+}
+
+// Edges per sub-partition: CSC owner = src (out-degree), CSR owner = dst
+// (in-degree).  A wave covers 64 consecutive ids of one page, so one q.
+__global__ __launch_bounds__(kNT) void k_q_counts(const uint32_t* __restrict__ out_deg,
+                                                   const uint32_t* __restrict__ in_deg, uint32_t V,
+                                                   const uint16_t* __restrict__ qmap, eg_graph::Meta* meta) {
+  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < V; base += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t v = base + threadIdx.x;
+    uint64_t a = 0, b = 0;
+    if (v < V) {
+      a = out_deg[v];
+      b = in_deg[v];
+    }
+    a = wave_reduce(a, OpSum{});
+    b = wave_reduce(b, OpSum{});
+    const uint64_t v0 = base + (threadIdx.x & ~uint64_t(kWave - 1));
+    if ((threadIdx.x & (kWave - 1)) == 0 && v0 < V) {
+      const int q = qmap[v0 >> kPageShift];
+      if (a) atomicAdd((unsigned long long*)&meta->q_cnt_csc[q], (unsigned long long)a);
+      if (b) atomicAdd((unsigned long long*)&meta->q_cnt_csr[q], (unsigned long long)b);
+    }
+  }
+}
+
+__global__ void k_q_offsets(eg_graph::Meta* meta, int Q) {
+  if (threadIdx.x != 0) return;
+  uint64_t a = 0, b = 0;
+  for (int q = 0; q < Q; ++q) {
+    meta->q_off_csc[q] = a;
+    meta->q_off_csr[q] = b;
+    a += meta->q_cnt_csc[q];
+    b += meta->q_cnt_csr[q];
+  }
+  meta->q_off_csc[Q] = a;
+  meta->q_off_csr[Q] = b;
+}
+
+// ----------------------------------------------------------- sort keys --
+struct KeySrc {
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.src; }
+};
+struct KeyDst {
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
+};
+// (q(owner) << vbits) | other endpoint.  kPull: owner = src, key = dst.
+template <bool kPull>
+struct KeyQ {
+  const uint16_t* qmap;
+  int vbits;
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const {
+    const uint32_t own = kPull ? r.src : r.dst;
+    const uint32_t k = kPull ? r.dst : r.src;
+    return (uint64_t(qmap[own >> kPageShift]) << vbits) | k;
+  }
+};
+
+// ------------------------------------------------------------- indexes --
+constexpr int kIdxIPT = 16;
+constexpr int kIdxTile = kNT * kIdxIPT;
+
+// Count run starts (key changes) per tile.
+template <class R, class KeyFn>
+__global__ __launch_bounds__(kNT) void k_run_count(const R* __restrict__ a, uint64_t n, KeyFn key,
+                                                   uint32_t* __restrict__ tile_cnt, uint32_t ntiles) {
+  __shared__ uint32_t wt[kNT / kWave];
+  const uint64_t base = uint64_t(blockIdx.x) * kIdxTile;
+  uint32_t c = 0;
+#pragma unroll 4
+  for (int k = 0; k < kIdxIPT; ++k) {
+    const uint64_t i = base + uint64_t(k) * kNT + threadIdx.x;
+    if (i < n) c += (i == 0 || key(a[i]) != key(a[i - 1])) ? 1u : 0u;
+  }
+  c = wave_reduce(c, OpSum{});
+  if ((threadIdx.x & (kWave - 1)) == 0) wt[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kNT / kWave; ++i) t += wt[i];
+    tile_cnt[blockIdx.x] = t;
+    if (blockIdx.x == ntiles - 1) tile_cnt[ntiles] = 0;
+  }
+}
+
+// Compressed CSC index entries: the r-th run start overall, lying in
+// sub-partition q, lands at r + q (one sentinel slot per earlier q).
+template <class R>
+__global__ __launch_bounds__(kNT) void k_cindex_write(const R* __restrict__ a, uint64_t n, KeyQ<true> key,
+                                                      const uint32_t* __restrict__ tile_off,
+                                                      const eg_graph::Meta* __restrict__ meta,
+                                                      CIdx* __restrict__ out) {
+  __shared__ uint32_t wt[kNT / kWave];
+  const uint64_t base = uint64_t(blockIdx.x) * kIdxTile;
+  uint32_t run = tile_off[blockIdx.x];
+  for (int k = 0; k < kIdxIPT; ++k) {
+    const uint64_t i = base + uint64_t(k) * kNT + threadIdx.x;
+    bool f = false;
+    R cur{};
+    if (i < n) {
+      cur = a[i];
+      f = (i == 0) || key(cur) != key(a[i - 1]);
+    }
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan<kNT>(f ? 1u : 0u, 0u, OpSum{}, wt, &tot);
+    if (f) {
+      const uint32_t q = key.qmap[cur.src >> kPageShift];
+      CIdx c;
+      c.index = i - meta->q_off_csc[q];
+      c.vertex = cur.dst;
+      out[uint64_t(run) + ex + q] = c;
+    }
+    run += tot;
+  }
+}
+
+// Number of run starts in [0, x): tile prefix + the partial tile.
+template <class R, class KeyFn>
+__device__ uint64_t runs_before(const R* a, uint64_t x, KeyFn key, const uint32_t* tile_off, uint32_t* wt) {
+  const uint64_t t = x / kIdxTile;
+  const uint64_t b = t * kIdxTile;
+  uint32_t c = 0;
+  for (uint64_t i = b + threadIdx.x; i < x; i += kNT) c += (i == 0 || key(a[i]) != key(a[i - 1])) ? 1u : 0u;
+  c = wave_reduce(c, OpSum{});
+  if ((threadIdx.x & (kWave - 1)) == 0) wt[threadIdx.x / kWave] = c;
+  __syncthreads();
+  uint64_t tot = tile_off[t];
+  for (int i = 0; i < kNT / kWave; ++i) tot += wt[i];
+  __syncthreads();
+  return tot;
+}
+
+// One block per sub-partition: where its index starts, its entry count and
+// the sentinel {n_edges, V} (core/graph.hpp:1186-1187; R8 for empty ones).
+template <class R>
+__global__ __launch_bounds__(kNT) void k_cindex_sentinel(const R* __restrict__ a, KeyQ<true> key,
+                                                         const uint32_t* __restrict__ tile_off, uint32_t V,
+                                                         eg_graph::Meta* __restrict__ meta, CIdx* __restrict__ out) {
+  __shared__ uint32_t wt[kNT / kWave];
+  const int q = blockIdx.x;
+  const uint64_t b = meta->q_off_csc[q], e = meta->q_off_csc[q + 1];
+  const uint64_t rb = runs_before(a, b, key, tile_off, wt);
+  const uint64_t re = runs_before(a, e, key, tile_off, wt);
+  if (threadIdx.x == 0) {
+    CIdx c;
+    c.index = e - b;
+    c.vertex = V;
+    out[re + q] = c;
+    meta->cidx_off[q] = rb + q;
+    meta->cidx_cnt[q] = re - rb + 1;
+  }
+}
+
+// Dense CSR index (core/graph.hpp:1312-1353) in global positions: a run of
+// source u in sub-partition q writes index[q][u] = first and
+// index[q][u+1] = last+1, and sets bit u of bitmap[q].
+template <class R>
+__global__ __launch_bounds__(kNT) void k_csr_runs(const R* __restrict__ a, uint64_t n, KeyQ<false> key, uint32_t V,
+                                                  uint64_t* __restrict__ index, uint64_t* __restrict__ bitmap) {
+  const uint64_t words = (V >> 6) + 1;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    const R cur = a[i];
+    const uint64_t kc = key(cur);
+    const uint32_t q = uint32_t(kc >> key.vbits);
+    if (i == 0 || kc != key(a[i - 1])) {
+      index[uint64_t(q) * (V + 1) + cur.src] = i;
+      atomicOr((unsigned long long*)&bitmap[q * words + (cur.src >> 6)], 1ull << (cur.src & 63));
+    }
+    if (i == n - 1 || kc != key(a[i + 1])) index[uint64_t(q) * (V + 1) + cur.src + 1] = i + 1;
+  }
+}
+
+__global__ void k_csr_localize(uint64_t* __restrict__ index, uint32_t V, int Q, const eg_graph::Meta* __restrict__ meta) {
+  const uint64_t n = uint64_t(Q) * (V + 1);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    index[i] -= meta->q_off_csr[i / (V + 1)];
+}
+
+unsigned grid_for(uint64_t n, uint64_t per_block = kNT, uint64_t cap = 65536) {
+  return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, per_block), cap)));
+}
+
+template <class R>
+int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) {
+  const uint32_t V = g->V;
+  const int P = g->P, S = g->S, Q = g->Q;
+  const uint64_t E = sym ? 2 * E0 : E0;
+  const uint64_t rb = sizeof(R);
+  g->built = false;
+  g->E0 = E0;
+  g->E = E;
+  g->symmetrized = sym;
+  g->vbits = std::max(1, counting_bits(V - 1));
+  g->qbits = counting_bits(uint64_t(Q - 1));
+  const uint64_t pages = g->pages;
+
+  // --- buffers (grow-only; reused by repeated builds) ---
+  EG_TRY(g->out_deg.resize(uint64_t(V) * 4));
+  EG_TRY(g->in_deg.resize(uint64_t(V) * 4));
+  EG_TRY(g->page_prefix.resize((pages + 1) * 8));
+  EG_TRY(g->meta.resize(sizeof(eg_graph::Meta)));
+  EG_TRY(g->qmap.resize(pages * 2));
+  EG_TRY(g->csc_list.resize(std::max<uint64_t>(E, 1) * rb));
+  EG_TRY(g->csr_list.resize(std::max<uint64_t>(E, 1) * rb));
+  const uint64_t cidx_cap = std::min<uint64_t>(E, uint64_t(Q) * V) + Q;
+  EG_TRY(g->csc_cidx.resize(cidx_cap * sizeof(CIdx)));
+  const uint64_t idx_n = uint64_t(Q) * (V + 1);
+  const uint64_t words = (V >> 6) + 1;
+  EG_TRY(g->csr_index.resize(idx_n * 8));
+  EG_TRY(g->csr_bitmap.resize(uint64_t(Q) * words * 8));
+  EG_TRY(g->t1.resize(std::max<uint64_t>(E, 1) * rb));
+  EG_TRY(g->t2.resize(std::max<uint64_t>(E, 1) * rb));
+  const uint64_t hist_n = std::max<uint64_t>(radix_hist_elems(E), 1);
+  EG_TRY(g->hist.resize(hist_n * 4));
+  const uint64_t scan_n = std::max<uint64_t>({scan_tmp_elems(hist_n), scan_tmp_elems(idx_n), scan_tmp_elems(pages + 1),
+                                              scan_tmp_elems(ceil_div(E, kIdxTile) + 1)});
+  EG_TRY(g->scan_tmp.resize(scan_n * 8));
+  EG_TRY(g->tile_cnt.resize((ceil_div(E, kIdxTile) + 2) * 4));
+  EG_TRY(g->err.resize(4));
+  if (sym) EG_TRY(g->sym.resize(std::max<uint64_t>(E, 1) * rb));
+
+  eg_graph::Meta* meta = g->dmeta();
+  uint32_t* err = g->err.as<uint32_t>();
+  uint32_t* out_deg = g->out_deg.as<uint32_t>();
+  uint32_t* in_deg = g->in_deg.as<uint32_t>();
+  uint16_t* qmap = g->qmap.as<uint16_t>();
+
+  // --- degree + id check; the only mid-build synchronisation ---
+  EG_TRY(fill<uint32_t>(err, 1, 0u, s));
+  EG_TRY(fill<uint32_t>(out_deg, V, 0u, s));
+  EG_TRY(fill<uint32_t>(in_deg, V, 0u, s));
+  if (E0) {
+    hipLaunchKernelGGL((k_degree<R>), dim3(grid_for(E0, kNT * 8, 16384)), dim3(kNT), 0, s, d_in, E0, V, sym, out_deg,
+                       in_deg, err);
+    EG_HIP(hipGetLastError());
+  }
+  uint32_t h_err = 0;
+  EG_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  if (h_err) return fail(EG_ERR_RANGE, "edge list holds a vertex id >= vertices_num (%u)", V);
+  if (sym) EG_HIP(hipMemcpyAsync(in_deg, out_deg, uint64_t(V) * 4, hipMemcpyDeviceToDevice, s));
+
+  const R* L = d_in;
+  if (sym) {
+    hipLaunchKernelGGL((k_symmetrize<R>), dim3(grid_for(E, kNT * 4, 65536)), dim3(kNT), 0, s, d_in, g->sym.as<R>(),
+                       E0, P);
+    EG_HIP(hipGetLastError());
+    L = g->sym.as<R>();
+  }
+
+  // --- CPBT ---
+  uint64_t* prefix = g->page_prefix.as<uint64_t>();
+  hipLaunchKernelGGL(k_page_weights, dim3(unsigned(pages)), dim3(kNT), 0, s, out_deg, V, uint32_t(1 + (P - 1) / 16),
+                     uint32_t(P - 1), g->m, g->eta, prefix, uint32_t(pages));
+  EG_HIP(hipGetLastError());
+  EG_TRY(exclusive_scan_u64(prefix, prefix, pages + 1, g->scan_tmp.as<uint64_t>(), s));
+  EG_HIP(hipMemsetAsync(meta, 0, sizeof(eg_graph::Meta), s));
+  hipLaunchKernelGGL(k_cpbt, dim3(1), dim3(kWave), 0, s, prefix, uint32_t(pages), V, P, S, meta);
+  EG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_qmap, dim3(grid_for(pages)), dim3(kNT), 0, s, meta, uint32_t(pages), P, S, qmap);
+  EG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_q_counts, dim3(grid_for(V, kNT, 4096)), dim3(kNT), 0, s, out_deg, in_deg, V, qmap, meta);
+  EG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_q_offsets, dim3(1), dim3(kWave), 0, s, meta, Q);
+  EG_HIP(hipGetLastError());
+
+  uint32_t* hist = g->hist.as<uint32_t>();
+  uint32_t* stmp32 = g->scan_tmp.as<uint32_t>();
+  R* t1 = g->t1.as<R>();
+  R* t2 = g->t2.as<R>();
+  const uint64_t base = E0 / P;
+  const uint64_t mult = sym ? 2 : 1;
+
+  // --- both directions ---
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool pull = dir == 0;
+    R* out = pull ? g->csc_list.as<R>() : g->csr_list.as<R>();
+    const R* src_list = L;
+    if (P > 1) {  // shuffle: stable sort of every rank slice by the owner id
+      for (int r = 0; r < P; ++r) {
+        const uint64_t off = mult * base * r;
+        const uint64_t n_r = mult * (base + (r == P - 1 ? E0 % P : 0));
+        if (pull)
+          EG_TRY(radix_sort(L + off, t1 + off, t2 + off, n_r, KeySrc{}, g->vbits, hist, stmp32, s));
+        else
+          EG_TRY(radix_sort(L + off, t1 + off, t2 + off, n_r, KeyDst{}, g->vbits, hist, stmp32, s));
+      }
+      src_list = t1;
+    }
+    const int kbits = g->qbits + g->vbits;
+    if (pull)
+      EG_TRY(radix_sort(src_list, out, P > 1 ? t2 : t1, E, KeyQ<true>{qmap, g->vbits}, kbits, hist, stmp32, s));
+    else
+      EG_TRY(radix_sort(src_list, out, P > 1 ? t2 : t1, E, KeyQ<false>{qmap, g->vbits}, kbits, hist, stmp32, s));
+  }
+
+  // --- compressed CSC index ---
+  {
+    const R* a = g->csc_list.as<R>();
+    KeyQ<true> key{qmap, g->vbits};
+    const uint32_t ntiles = uint32_t(std::max<uint64_t>(ceil_div(E, kIdxTile), 1));
+    uint32_t* tc = g->tile_cnt.as<uint32_t>();
+    CIdx* cidx = g->csc_cidx.as<CIdx>();
+    if (E) {
+      hipLaunchKernelGGL((k_run_count<R, KeyQ<true>>), dim3(ntiles), dim3(kNT), 0, s, a, E, key, tc, ntiles);
+      EG_HIP(hipGetLastError());
+      EG_TRY(exclusive_scan_u32(tc, tc, uint64_t(ntiles) + 1, stmp32, s));
+      hipLaunchKernelGGL((k_cindex_write<R>), dim3(ntiles), dim3(kNT), 0, s, a, E, key, (const uint32_t*)tc, meta,
+                         cidx);
+      EG_HIP(hipGetLastError());
+    } else {
+      EG_TRY(fill<uint32_t>(tc, 2, 0u, s));
+    }
+    hipLaunchKernelGGL((k_cindex_sentinel<R>), dim3(Q), dim3(kNT), 0, s, a, key, (const uint32_t*)tc, V, meta, cidx);
+    EG_HIP(hipGetLastError());
+  }
+
+  // --- dense CSR index + bitmap ---
+  {
+    uint64_t* index = g->csr_index.as<uint64_t>();
+    uint64_t* bitmap = g->csr_bitmap.as<uint64_t>();
+    EG_TRY(fill<uint64_t>(index, idx_n, uint64_t(0), s));
+    EG_TRY(fill<uint64_t>(bitmap, uint64_t(Q) * words, uint64_t(0), s));
+    if (E) {
+      hipLaunchKernelGGL((k_csr_runs<R>), dim3(grid_for(E, kNT * 8, 65536)), dim3(kNT), 0, s, g->csr_list.as<R>(), E,
+                         KeyQ<false>{qmap, g->vbits}, V, index, bitmap);
+      EG_HIP(hipGetLastError());
+    }
+    EG_TRY(inclusive_max_scan_u64(index, index, idx_n, g->scan_tmp.as<uint64_t>(), s));
+    hipLaunchKernelGGL(k_csr_localize, dim3(grid_for(idx_n, kNT * 8, 65536)), dim3(kNT), 0, s, index, V, Q, meta);
+    EG_HIP(hipGetLastError());
+  }
+
+  // --- small tables back to the host ---
+  eg_graph::Meta h;
+  EG_HIP(hipMemcpyAsync(&h, meta, sizeof(h), hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  g->page_offset.assign(h.page_offset, h.page_offset + P + 1);
+  g->partition_offset.assign(h.partition_offset, h.partition_offset + P + 1);
+  g->local_partition_offset.assign(h.local_partition_offset, h.local_partition_offset + P * (S + 1));
+  g->q_off_csc.assign(h.q_off_csc, h.q_off_csc + Q + 1);
+  g->q_off_csr.assign(h.q_off_csr, h.q_off_csr + Q + 1);
+  g->cidx_off.assign(h.cidx_off, h.cidx_off + Q);
+  g->cidx_cnt.assign(h.cidx_cnt, h.cidx_cnt + Q);
+  if (g->q_off_csc[Q] != E || g->q_off_csr[Q] != E)
+    return fail(EG_ERR_STATE, "internal: sub-partition edge counts do not add up to E");
+  g->built = true;
+  return EG_OK;
+}
+
+}  // namespace
+}  // namespace eg
+
+using namespace eg;
+
+extern "C" {
+
+int eg_graph_create(uint32_t V, int P, int S, uint64_t eta, uint32_t m, int rec_bytes, eg_graph** out) {
+  if (!out) return fail(EG_ERR_ARG, "eg_graph_create: out is NULL");
+  *out = nullptr;
+  if (V == 0) return fail(EG_ERR_ARG, "vertices_num must be positive");
+  if (P < 1 || S < 1 || S > 8 || P * S > 256) return fail(EG_ERR_ARG, "need 1 <= P, 1 <= S <= 8, P*S <= 256");
+  if (rec_bytes != 8 && rec_bytes != 12) return fail(EG_ERR_ARG, "rec_bytes must be 8 or 12");
+  eg_graph* g = new (std::nothrow) eg_graph();
+  if (!g) return fail(EG_ERR_NOMEM, "host allocation failed");
+  g->V = V;
+  g->P = P;
+  g->S = S;
+  g->Q = P * S;
+  g->eta = eta;
+  g->m = m;
+  g->rec = rec_bytes;
+  g->pages = uint32_t(ceil_div(V, kPageSize));
+  *out = g;
+  return EG_OK;
+}
+
+int eg_graph_destroy(eg_graph* g) {
+  delete g;
+  return EG_OK;
+}
+
+int eg_graph_build(eg_graph* g, const void* d_edges, uint64_t E0, int symmetrize, void* stream) {
+  if (!g) return fail(EG_ERR_ARG, "eg_graph_build: graph is NULL");
+  if (!d_edges && E0) return fail(EG_ERR_ARG, "eg_graph_build: edge list is NULL");
+  const uint64_t E = symmetrize ? 2 * E0 : E0;
+  if (E >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "edge count %llu exceeds 2^32-1", (unsigned long long)E);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g->rec == 8) return build_impl<Rec8>(g, static_cast<const Rec8*>(d_edges), E0, symmetrize ? 1 : 0, s);
+  return build_impl<Rec12>(g, static_cast<const Rec12*>(d_edges), E0, symmetrize ? 1 : 0, s);
+}
+
+int eg_graph_get_info(const eg_graph* g, eg_graph_info* out) {
+  if (!g || !out) return fail(EG_ERR_ARG, "eg_graph_get_info: NULL argument");
+  out->vertices_num = g->V;
+  out->partitions = g->P;
+  out->sockets = g->S;
+  out->record_bytes = g->rec;
+  out->edges_num = g->E;
+  out->pages_num = g->pages;
+  out->symmetrized = g->symmetrized;
+  return EG_OK;
+}
+
+static int need_built(const eg_graph* g) {
+  if (!g) return fail(EG_ERR_ARG, "graph is NULL");
+  if (!g->built) return fail(EG_ERR_STATE, "graph has not been built");
+  return EG_OK;
+}
+
+int eg_graph_partition_offset(const eg_graph* g, uint32_t* h_out) {
+  EG_TRY(need_built(g));
+  if (!h_out) return fail(EG_ERR_ARG, "h_out is NULL");
+  std::memcpy(h_out, g->partition_offset.data(), sizeof(uint32_t) * (g->P + 1));
+  return EG_OK;
+}
+
+int eg_graph_local_partition_offset(const eg_graph* g, int p, uint32_t* h_out) {
+  EG_TRY(need_built(g));
+  if (!h_out || p < 0 || p >= g->P) return fail(EG_ERR_ARG, "bad partition %d", p);
+  std::memcpy(h_out, g->local_partition_offset.data() + p * (g->S + 1), sizeof(uint32_t) * (g->S + 1));
+  return EG_OK;
+}
+
+int eg_graph_degree(const eg_graph* g, const uint32_t** d_out_degree, const uint32_t** d_in_degree) {
+  EG_TRY(need_built(g));
+  if (d_out_degree) *d_out_degree = g->out_deg.as<uint32_t>();
+  if (d_in_degree) *d_in_degree = g->in_deg.as<uint32_t>();
+  return EG_OK;
+}
+
+int eg_graph_page_prefix(const eg_graph* g, const uint64_t** d_prefix) {
+  EG_TRY(need_built(g));
+  if (!d_prefix) return fail(EG_ERR_ARG, "d_prefix is NULL");
+  *d_prefix = g->page_prefix.as<uint64_t>();
+  return EG_OK;
+}
+
+int eg_graph_csc(const eg_graph* g, int p, int s, const void** d_list, uint64_t* n_edges, const void** d_cindex,
+                 uint64_t* n_cindex) {
+  EG_TRY(need_built(g));
+  if (p < 0 || p >= g->P || s < 0 || s >= g->S) return fail(EG_ERR_ARG, "bad (partition, socket) (%d, %d)", p, s);
+  const int q = p * g->S + s;
+  if (d_list) *d_list = static_cast<const char*>(g->csc_list.ptr) + g->q_off_csc[q] * g->rec;
+  if (n_edges) *n_edges = g->q_off_csc[q + 1] - g->q_off_csc[q];
+  if (d_cindex) *d_cindex = g->csc_cidx.as<CIdx>() + g->cidx_off[q];
+  if (n_cindex) *n_cindex = g->cidx_cnt[q];
+  return EG_OK;
+}
+
+int eg_graph_csr(const eg_graph* g, int p, int s, const void** d_list, uint64_t* n_edges, const uint64_t** d_index,
+                 const uint64_t** d_bitmap) {
+  EG_TRY(need_built(g));
+  if (p < 0 || p >= g->P || s < 0 || s >= g->S) return fail(EG_ERR_ARG, "bad (partition, socket) (%d, %d)", p, s);
+  const int q = p * g->S + s;
+  if (d_list) *d_list = static_cast<const char*>(g->csr_list.ptr) + g->q_off_csr[q] * g->rec;
+  if (n_edges) *n_edges = g->q_off_csr[q + 1] - g->q_off_csr[q];
+  if (d_index) *d_index = g->csr_index.as<uint64_t>() + uint64_t(q) * (g->V + 1);
+  if (d_bitmap) *d_bitmap = g->csr_bitmap.as<uint64_t>() + uint64_t(q) * ((g->V >> 6) + 1);
+  return EG_OK;
+}
+
+}  // extern "C"

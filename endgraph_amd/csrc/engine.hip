@@ -1,0 +1,593 @@
+// Gemini's edge-processing engine (core/graph.hpp:1376-1817) on one GPU,
+// specialised per toolkit signal/slot pair (toolkits/*.cpp): the
+// std::function lambdas of the reference cannot cross a C ABI, so each
+// process_edges / process_vertices pair used by a toolkit is a kernel here.
+//
+// Engine view (built once per graph build by eg_engine_prepare):
+//   pull side  the CSC lists of all sub-partitions back to back as a source
+//              array (4 B/edge), a row-start bit per edge and the destination
+//              of each row (a row = one compressed-index entry).  Dense-mode
+//              kernels walk edges 64 at a time per wave and reduce rows with
+//              a segmented wave scan; only rows cut by a wave boundary (or
+//              present in several sub-partitions) use atomics.
+//   push side  one CSR merged over the sub-partitions (u32 offsets).
+// Mode switch (core/graph.hpp:1483-1498, R3/R4): iterative => dense (pull);
+// rooted => sparse (push) under the reference policy; the auto policy and
+// undirected programs use sparse iff sum of out-degrees of the active set
+// < E/20.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "graph_state.hpp"
+
+namespace eg {
+namespace {
+
+constexpr int kNT = 256;
+
+__device__ __forceinline__ bool bm_get(const uint64_t* bm, uint32_t v) { return (bm[v >> 6] >> (v & 63)) & 1ull; }
+__device__ __forceinline__ void bm_set(uint64_t* bm, uint32_t v) {
+  atomicOr((unsigned long long*)&bm[v >> 6], 1ull << (v & 63));
+}
+
+unsigned grid_for(uint64_t n, uint64_t per_block = kNT, uint64_t cap = 65536) {
+  uint64_t b = ceil_div(n, per_block);
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return unsigned(b);
+}
+
+template <class T>
+__global__ void k_fill(T* __restrict__ p, uint64_t n, T v) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+template <class T>
+int fill(T* p, uint64_t n, T v, hipStream_t s) {
+  if (!n) return EG_OK;
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(n, kNT, 8192)), dim3(kNT), 0, s, p, n, v);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+// ------------------------------------------------------------- prepare --
+// SoA source (and weight) of every CSC edge, and the row-start bit per edge.
+template <class R>
+__global__ __launch_bounds__(kNT) void k_prep_csc(const R* __restrict__ lst, uint64_t E, uint32_t* __restrict__ src,
+                                                  float* __restrict__ w, uint64_t* __restrict__ rowflag,
+                                                  const uint16_t* __restrict__ qmap) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t nwords = ceil_div(E, 64);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t wd = gw; wd < nwords; wd += nw) {
+    const uint64_t e = wd * 64 + lane;
+    bool head = false;
+    if (e < E) {
+      const R r = lst[e];
+      src[e] = r.src;
+      if constexpr (sizeof(R) == 12) w[e] = reinterpret_cast<const Rec12&>(r).w;
+      if (e == 0) {
+        head = true;
+      } else {
+        const R p = lst[e - 1];
+        head = p.dst != r.dst || qmap[p.src >> kPageShift] != qmap[r.src >> kPageShift];
+      }
+    }
+    const uint64_t m = __ballot(head);
+    if (lane == 0) rowflag[wd] = m;
+  }
+}
+
+__global__ void k_popc_words(const uint64_t* __restrict__ words, uint64_t n, uint32_t* __restrict__ cnt) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    cnt[i] = uint32_t(__popcll(words[i]));
+}
+
+// Row r starts at edge e: row_start[r] = e, row_dst[r] = dst of e.
+template <class R>
+__global__ __launch_bounds__(kNT) void k_prep_rows(const R* __restrict__ lst, uint64_t E,
+                                                   const uint64_t* __restrict__ rowflag,
+                                                   const uint32_t* __restrict__ wordrank,
+                                                   uint32_t* __restrict__ row_start, uint32_t* __restrict__ row_dst) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t nwords = ceil_div(E, 64);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t wd = gw; wd < nwords; wd += nw) {
+    const uint64_t f = rowflag[wd];
+    if ((f >> lane) & 1ull) {
+      const uint64_t e = wd * 64 + lane;
+      const uint32_t r = wordrank[wd] + uint32_t(__popcll(f & ((1ull << lane) - 1)));
+      row_start[r] = uint32_t(e);
+      row_dst[r] = lst[e].dst;
+    }
+  }
+}
+
+// Merged CSR: edge at position li of sub-partition q's list (source u) goes to
+// out_ptr[u] + (edges of u in sub-partitions < q) + (li - index_q[u]).
+template <class R>
+__global__ void k_prep_csr(const R* __restrict__ lst, uint64_t E, const uint64_t* __restrict__ index, uint32_t V, int Q,
+                           const uint64_t* __restrict__ q_off, const uint32_t* __restrict__ out_ptr,
+                           uint32_t* __restrict__ dst, float* __restrict__ w) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += uint64_t(gridDim.x) * blockDim.x) {
+    int q = 0;
+    while (q + 1 < Q && q_off[q + 1] <= i) ++q;
+    const R r = lst[i];
+    const uint32_t u = r.src;
+    uint64_t before = 0;
+    for (int qq = 0; qq < q; ++qq) {
+      const uint64_t* ix = index + uint64_t(qq) * (V + 1);
+      before += ix[u + 1] - ix[u];
+    }
+    const uint64_t li = i - q_off[q];
+    const uint64_t pos = uint64_t(out_ptr[u]) + before + (li - index[uint64_t(q) * (V + 1) + u]);
+    dst[pos] = r.dst;
+    if constexpr (sizeof(R) == 12) w[pos] = reinterpret_cast<const Rec12&>(r).w;
+  }
+}
+
+__global__ void k_u32_from_u64(const uint64_t* __restrict__ a, uint32_t* __restrict__ b, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    b[i] = uint32_t(a[i]);
+}
+
+template <class R>
+__global__ void k_soa_csr(const R* __restrict__ lst, uint64_t E, uint32_t* __restrict__ dst, float* __restrict__ w) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += uint64_t(gridDim.x) * blockDim.x) {
+    const R r = lst[i];
+    dst[i] = r.dst;
+    if constexpr (sizeof(R) == 12) w[i] = reinterpret_cast<const Rec12&>(r).w;
+  }
+}
+
+template <class R>
+int prepare_impl(eg_graph* g, hipStream_t s) {
+  const uint64_t E = g->E;
+  const uint32_t V = g->V;
+  const int Q = g->Q;
+  const uint64_t nwords = ceil_div(E, 64);
+  const uint64_t words_v = (V >> 6) + 1;
+  EG_TRY(g->e_csc_src.resize(std::max<uint64_t>(E, 1) * 4));
+  EG_TRY(g->e_rowflag.resize((nwords + 1) * 8));
+  EG_TRY(g->e_wordrank.resize((nwords + 2) * 4));
+  EG_TRY(g->e_out_ptr.resize((uint64_t(V) + 1) * 4));
+  EG_TRY(g->e_out_dst.resize(std::max<uint64_t>(E, 1) * 4));
+  if (sizeof(R) == 12) {
+    EG_TRY(g->e_csc_w.resize(std::max<uint64_t>(E, 1) * 4));
+    EG_TRY(g->e_out_w.resize(std::max<uint64_t>(E, 1) * 4));
+  }
+  EG_TRY(g->e_scratch.resize(uint64_t(V) * 4 + 4096));
+  for (auto& b : g->e_bm) EG_TRY(b.resize(words_v * 8));
+  uint64_t scan_n = std::max<uint64_t>(scan_tmp_elems(nwords + 1), scan_tmp_elems(uint64_t(V) + 1));
+  if (g->scan_tmp.bytes < scan_n * 8) EG_TRY(g->scan_tmp.resize(scan_n * 8));
+
+  const R* csc = g->csc_list.as<R>();
+  const R* csr = g->csr_list.as<R>();
+  uint32_t* wordrank = g->e_wordrank.as<uint32_t>();
+  uint64_t* rowflag = g->e_rowflag.as<uint64_t>();
+  if (E) {
+    hipLaunchKernelGGL((k_prep_csc<R>), dim3(grid_for(nwords, kNT / kWave, 65536)), dim3(kNT), 0, s, csc, E,
+                       g->e_csc_src.as<uint32_t>(), g->e_csc_w.as<float>(), rowflag, g->qmap.as<uint16_t>());
+    EG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_popc_words, dim3(grid_for(nwords, kNT, 65536)), dim3(kNT), 0, s, rowflag, nwords, wordrank);
+    EG_HIP(hipGetLastError());
+  }
+  EG_TRY(fill<uint32_t>(wordrank + nwords, 1, 0u, s));
+  EG_TRY(exclusive_scan_u32(wordrank, wordrank, nwords + 1, g->scan_tmp.as<uint32_t>(), s));
+  uint32_t R_h = 0;
+  EG_HIP(hipMemcpyAsync(&R_h, wordrank + nwords, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  g->R = R_h;
+  EG_TRY(g->e_row_start.resize((uint64_t(R_h) + 1) * 4));
+  EG_TRY(g->e_row_dst.resize((uint64_t(R_h) + 1) * 4));
+  if (E) {
+    hipLaunchKernelGGL((k_prep_rows<R>), dim3(grid_for(nwords, kNT / kWave, 65536)), dim3(kNT), 0, s, csc, E, rowflag,
+                       (const uint32_t*)wordrank, g->e_row_start.as<uint32_t>(), g->e_row_dst.as<uint32_t>());
+    EG_HIP(hipGetLastError());
+  }
+  EG_TRY(fill<uint32_t>(g->e_row_start.as<uint32_t>() + R_h, 1, uint32_t(E), s));
+
+  uint32_t* out_ptr = g->e_out_ptr.as<uint32_t>();
+  if (Q == 1) {
+    hipLaunchKernelGGL(k_u32_from_u64, dim3(grid_for(uint64_t(V) + 1, kNT, 65536)), dim3(kNT), 0, s,
+                       (const uint64_t*)g->csr_index.as<uint64_t>(), out_ptr, uint64_t(V) + 1);
+    EG_HIP(hipGetLastError());
+    if (E) {
+      hipLaunchKernelGGL((k_soa_csr<R>), dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, csr, E,
+                         g->e_out_dst.as<uint32_t>(), g->e_out_w.as<float>());
+      EG_HIP(hipGetLastError());
+    }
+  } else {
+    // out_ptr = exclusive prefix of the out-degree
+    EG_HIP(hipMemcpyAsync(out_ptr, g->out_deg.ptr, uint64_t(V) * 4, hipMemcpyDeviceToDevice, s));
+    EG_TRY(fill<uint32_t>(out_ptr + V, 1, 0u, s));
+    EG_TRY(exclusive_scan_u32(out_ptr, out_ptr, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
+    if (E) {
+      hipLaunchKernelGGL((k_prep_csr<R>), dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, csr, E,
+                         (const uint64_t*)g->csr_index.as<uint64_t>(), V, Q, g->dmeta()->q_off_csr,
+                         (const uint32_t*)out_ptr, g->e_out_dst.as<uint32_t>(), g->e_out_w.as<float>());
+      EG_HIP(hipGetLastError());
+    }
+  }
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+int ensure_engine(eg_graph* g, hipStream_t s) {
+  if (!g) return fail(EG_ERR_ARG, "graph is NULL");
+  if (!g->built) return fail(EG_ERR_STATE, "graph has not been built");
+  if (g->eng_ready) return EG_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = g->rec == 8 ? prepare_impl<Rec8>(g, s) : prepare_impl<Rec12>(g, s);
+  if (rc != EG_OK) return rc;
+  g->eng_prepare_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  g->eng_ready = true;
+  return EG_OK;
+}
+
+// ----------------------------------------------------- dense row reduce --
+// Walks CSC edges 64 per step (wave w: words [w*K, (w+1)*K)), gathers one
+// value per edge with `val(e)`, reduces rows with a segmented wave scan and
+// hands each finished row's partial to `emit(row, value, shared)`; shared is
+// true for the rows cut by the wave's range ends (they need atomics).
+constexpr int kRowWordsPerWave = 8;
+
+// Compensated (Neumaier) pair for the FP32 pull sums.
+struct F2 {
+  float s, c;
+};
+__device__ __forceinline__ F2 f2_add(F2 a, F2 b) {
+  const float s = __fadd_rn(a.s, b.s);
+  const float bp = __fsub_rn(s, a.s);
+  const float err = __fadd_rn(__fsub_rn(a.s, __fsub_rn(s, bp)), __fsub_rn(b.s, bp));
+  return F2{s, __fadd_rn(__fadd_rn(a.c, b.c), err)};
+}
+struct OpF2 {
+  __device__ __forceinline__ F2 operator()(F2 a, F2 b) const { return f2_add(a, b); }
+};
+
+}  // namespace
+
+template <>
+__device__ __forceinline__ F2 shfl_up_t<F2>(F2 v, unsigned d) {
+  return F2{__shfl_up(v.s, d, kWave), __shfl_up(v.c, d, kWave)};
+}
+template <>
+__device__ __forceinline__ F2 shfl_t<F2>(F2 v, int lane) {
+  return F2{__shfl(v.s, lane, kWave), __shfl(v.c, lane, kWave)};
+}
+
+namespace {
+
+template <class T, class Op>
+__device__ __forceinline__ T seg_scan(T v, bool head, Op op) {
+  const int lane = threadIdx.x & (kWave - 1);
+  bool h = head;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const T ov = shfl_up_t(v, d);
+    const bool oh = __shfl_up(int(h), d, kWave) != 0;
+    if (lane >= d && !h) {
+      v = op(ov, v);
+      h = oh;
+    }
+  }
+  return v;
+}
+
+template <class T, class Op, class Val, class Emit>
+__device__ __forceinline__ void row_reduce_wave(uint64_t E, const uint64_t* __restrict__ rowflag,
+                                                const uint32_t* __restrict__ wordrank, T identity, Op op, Val val,
+                                                Emit emit) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t nwords = ceil_div(E, 64);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t w0 = gw * kRowWordsPerWave;
+  if (w0 >= nwords) return;
+  const uint64_t w1 = w0 + kRowWordsPerWave < nwords ? w0 + kRowWordsPerWave : nwords;
+  const uint64_t lt = (1ull << lane) - 1;
+  T carry = identity;
+  bool carry_live = false;     // a row is open from earlier edges
+  bool carry_shared = true;    // ... and it started before this wave's range
+  uint32_t carry_row = 0;
+  for (uint64_t wd = w0; wd < w1; ++wd) {
+    const uint64_t f = rowflag[wd];
+    const uint64_t e = wd * 64 + lane;
+    T v = e < E ? val(e) : identity;
+    const bool head = (f >> lane) & 1ull;
+    const uint32_t base = wordrank[wd];
+    if (lane == 0 && !head) v = op(carry, v);
+    if ((f & 1ull) && carry_live && lane == 0) emit(carry_row, carry, carry_shared);
+    const T inc = seg_scan(v, head, op);
+    // tails: lanes whose successor starts a row (lane 63 stays open)
+    const bool tail = lane < 63 ? ((f >> (lane + 1)) & 1ull) : false;
+    const uint64_t heads_le = __popcll(f & (lt | (1ull << lane)));
+    const uint32_t row = base + uint32_t(heads_le) - 1;  // rows started before + own
+    const bool first_seg_open = heads_le == 0;           // segment continues the carried row
+    if (tail && e < E) emit(row, inc, first_seg_open ? carry_shared : false);
+    // new carry = lane 63's segment
+    const T c63 = shfl_t(inc, 63);
+    const uint64_t h63 = __popcll(f);
+    if (h63 == 0) {
+      // no head in this word: the row open before it continues through it
+      carry = c63;
+      carry_live = true;
+      carry_row = base - 1;
+    } else {
+      carry = c63;
+      carry_live = true;
+      carry_shared = false;
+      carry_row = base + uint32_t(h63) - 1;
+    }
+    if (wd * 64 + 64 > E) break;
+  }
+  if (lane == 0 && carry_live) emit(carry_row, carry, true);
+}
+
+// --------------------------------------------------------------- PageRank
+// dense_signal/dense_slot of toolkits/pagerank.cpp:64-75: next[dst] += sum
+// of curr[src] over the in-edges, partials of several sub-partitions added
+// with atomics.
+__global__ __launch_bounds__(kNT) void k_pr_pull_f64(uint64_t E, const uint32_t* __restrict__ src,
+                                                      const uint64_t* __restrict__ rowflag,
+                                                      const uint32_t* __restrict__ wordrank,
+                                                      const uint32_t* __restrict__ row_dst,
+                                                      const double* __restrict__ curr, double* __restrict__ next,
+                                                      int all_atomic) {
+  row_reduce_wave<double>(
+      E, rowflag, wordrank, 0.0, OpSum{}, [&](uint64_t e) { return curr[src[e]]; },
+      [&](uint32_t row, double v, bool shared) {
+        double* p = &next[row_dst[row]];
+        if (shared || all_atomic)
+          atomicAdd(p, v);
+        else
+          *p = v;
+      });
+}
+
+// FP32 gathers with compensated row sums; the row total is handed over as
+// FP64 (s + c) for the FP64 residual step.
+__global__ __launch_bounds__(kNT) void k_pr_pull_f32c(uint64_t E, const uint32_t* __restrict__ src,
+                                                       const uint64_t* __restrict__ rowflag,
+                                                       const uint32_t* __restrict__ wordrank,
+                                                       const uint32_t* __restrict__ row_dst,
+                                                       const float* __restrict__ curr, double* __restrict__ next,
+                                                       int all_atomic) {
+  row_reduce_wave<F2>(
+      E, rowflag, wordrank, F2{0.f, 0.f}, OpF2{}, [&](uint64_t e) { return F2{curr[src[e]], 0.f}; },
+      [&](uint32_t row, F2 v, bool shared) {
+        double* p = &next[row_dst[row]];
+        const double t = double(v.s) + double(v.c);
+        if (shared || all_atomic)
+          atomicAdd(p, t);
+        else
+          *p = t;
+      });
+}
+
+// process_vertices of toolkits/pagerank.cpp:36-47 (init) and :79-98 (apply).
+__global__ __launch_bounds__(kNT) void k_pr_init(const uint32_t* __restrict__ deg, uint32_t V, double* __restrict__ x,
+                                                  float* __restrict__ x32) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x) {
+    double c = 1.0;
+    if (deg[v] > 0) c = c / double(deg[v]);
+    x[v] = c;
+    if (x32) x32[v] = float(c);
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_pr_apply(const uint32_t* __restrict__ deg, uint32_t V,
+                                                   const double* __restrict__ curr, double* __restrict__ next,
+                                                   float* __restrict__ next32, int last, double* __restrict__ delta) {
+  __shared__ double wt[kNT / kWave];
+  const double d = 0.85;
+  double acc = 0;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x) {
+    const double y = __dadd_rn(1.0 - d, __dmul_rn(d, next[v]));
+    double x = y;
+    if (!last) {
+      const uint32_t dg = deg[v];
+      if (dg > 0) {
+        x = y / double(dg);
+        acc += fabs(x - curr[v]) * double(dg);
+      } else {
+        acc += fabs(x - curr[v]);
+      }
+    }
+    next[v] = x;
+    if (next32) next32[v] = float(x);
+  }
+  acc = wave_reduce(acc, OpSum{});
+  if ((threadIdx.x & (kWave - 1)) == 0) wt[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int i = 0; i < kNT / kWave; ++i) t += wt[i];
+    atomicAdd(delta, t);
+  }
+}
+
+// ----------------------------------------------------- frontier helpers --
+// Frontier list from a bitmap (ascending ids) via per-word counts + scan.
+__global__ void k_bm_count(const uint64_t* __restrict__ bm, uint64_t words, uint32_t* __restrict__ cnt) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += uint64_t(gridDim.x) * blockDim.x)
+    cnt[i] = uint32_t(__popcll(bm[i]));
+}
+__global__ void k_bm_list(const uint64_t* __restrict__ bm, uint64_t words, const uint32_t* __restrict__ off,
+                          uint32_t* __restrict__ list) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += uint64_t(gridDim.x) * blockDim.x) {
+    uint64_t w = bm[i];
+    uint32_t o = off[i];
+    while (w) {
+      const int b = __ffsll((unsigned long long)w) - 1;
+      list[o++] = uint32_t(i * 64 + b);
+      w &= w - 1;
+    }
+  }
+}
+
+// Level bookkeeping: visited |= out; depth of new vertices; counts of new
+// vertices and of their out-edges (for the next mode decision).
+__global__ __launch_bounds__(kNT) void k_level_end(const uint64_t* __restrict__ out, uint64_t* __restrict__ visited,
+                                                    uint64_t words, uint32_t* __restrict__ depth, uint32_t level,
+                                                    const uint32_t* __restrict__ deg,
+                                                    unsigned long long* __restrict__ counters) {
+  unsigned long long nv = 0, ne = 0;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += uint64_t(gridDim.x) * blockDim.x) {
+    uint64_t w = out[i];
+    if (!w) continue;
+    if (visited) visited[i] |= w;
+    nv += __popcll(w);
+    while (w) {
+      const int b = __ffsll((unsigned long long)w) - 1;
+      const uint32_t v = uint32_t(i * 64 + b);
+      if (depth) depth[v] = level;
+      ne += deg[v];
+      w &= w - 1;
+    }
+  }
+  nv = wave_reduce(nv, OpSum{});
+  ne = wave_reduce(ne, OpSum{});
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    if (nv) atomicAdd(&counters[0], nv);
+    if (ne) atomicAdd(&counters[1], ne);
+  }
+}
+
+// --------------------------------------------------------------- BFS ----
+// Sparse slot of toolkits/bfs.cpp:52-58 with R12: an unvisited out-neighbour
+// takes the smallest frontier id as parent (atomicMin; visited is not
+// modified during a level, so the rule is order-independent).
+__global__ __launch_bounds__(kNT) void k_bfs_push(const uint32_t* __restrict__ flist, uint32_t fn,
+                                                   const uint32_t* __restrict__ out_ptr,
+                                                   const uint32_t* __restrict__ out_dst,
+                                                   const uint64_t* __restrict__ visited, uint32_t* __restrict__ parent,
+                                                   uint64_t* __restrict__ active_out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t i = gw; i < fn; i += nw) {
+    const uint32_t u = flist[i];
+    const uint32_t b = out_ptr[u], e = out_ptr[u + 1];
+    for (uint32_t k = b + lane; k < e; k += kWave) {
+      const uint32_t v = out_dst[k];
+      if (!bm_get(visited, v) && parent[v] > u) {
+        atomicMin(&parent[v], u);
+        if (!bm_get(active_out, v)) bm_set(active_out, v);
+      }
+    }
+  }
+}
+
+// Dense signal of toolkits/bfs.cpp:60-68 with R12: for an unvisited row, the
+// smallest in-neighbour in the frontier (all in-edges are scanned).
+__global__ __launch_bounds__(kNT) void k_bfs_pull(uint64_t R, const uint32_t* __restrict__ row_start,
+                                                   const uint32_t* __restrict__ row_dst,
+                                                   const uint32_t* __restrict__ src,
+                                                   const uint64_t* __restrict__ frontier,
+                                                   const uint64_t* __restrict__ visited, uint32_t V,
+                                                   uint32_t* __restrict__ parent, uint64_t* __restrict__ active_out) {
+  for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < R; r += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t v = row_dst[r];
+    if (bm_get(visited, v)) continue;
+    uint32_t best = V;
+    for (uint32_t k = row_start[r], e = row_start[r + 1]; k < e; ++k) {
+      const uint32_t u = src[k];
+      if (u < best && bm_get(frontier, u)) best = u;
+    }
+    if (best < V) {
+      atomicMin(&parent[v], best);
+      bm_set(active_out, v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- CC -----
+// toolkits/cc.cpp:47-79.  Push: a smaller label flows to out-neighbours.
+__global__ __launch_bounds__(kNT) void k_cc_push(const uint32_t* __restrict__ flist, uint32_t fn,
+                                                  const uint32_t* __restrict__ out_ptr,
+                                                  const uint32_t* __restrict__ out_dst, uint32_t* __restrict__ label,
+                                                  uint64_t* __restrict__ active_out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t i = gw; i < fn; i += nw) {
+    const uint32_t u = flist[i];
+    const uint32_t msg = label[u];
+    const uint32_t b = out_ptr[u], e = out_ptr[u + 1];
+    for (uint32_t k = b + lane; k < e; k += kWave) {
+      const uint32_t v = out_dst[k];
+      if (msg < label[v]) {
+        atomicMin(&label[v], msg);
+        bm_set(active_out, v);
+      }
+    }
+  }
+}
+
+// Pull: min label over in-edges (starting from dst itself, cc.cpp:60-68).
+__global__ __launch_bounds__(kNT) void k_cc_pull(uint64_t E, const uint32_t* __restrict__ src,
+                                                  const uint64_t* __restrict__ rowflag,
+                                                  const uint32_t* __restrict__ wordrank,
+                                                  const uint32_t* __restrict__ row_dst, uint32_t* __restrict__ label,
+                                                  uint64_t* __restrict__ active_out) {
+  row_reduce_wave<uint32_t>(
+      E, rowflag, wordrank, 0xffffffffu, OpMin{}, [&](uint64_t e) { return label[src[e]]; },
+      [&](uint32_t row, uint32_t m, bool) {
+        const uint32_t v = row_dst[row];
+        if (m < v && m < label[v]) {
+          atomicMin(&label[v], m);
+          bm_set(active_out, v);
+        }
+      });
+}
+
+// --------------------------------------------------------------- SSSP ----
+// toolkits/sssp.cpp:44-58: relax = dist[src] + w (one fp32 add, round to
+// nearest); distances are non-negative, so the float order is the order of
+// their bit patterns and atomicMin on the bits is write_min.
+__global__ __launch_bounds__(kNT) void k_sssp_push(const uint32_t* __restrict__ flist, uint32_t fn,
+                                                    const uint32_t* __restrict__ out_ptr,
+                                                    const uint32_t* __restrict__ out_dst,
+                                                    const float* __restrict__ out_w, float* __restrict__ dist,
+                                                    uint64_t* __restrict__ active_out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t i = gw; i < fn; i += nw) {
+    const uint32_t u = flist[i];
+    const float du = dist[u];
+    const uint32_t b = out_ptr[u], e = out_ptr[u + 1];
+    for (uint32_t k = b + lane; k < e; k += kWave) {
+      const uint32_t v = out_dst[k];
+      const float nd = __fadd_rn(du, out_w[k]);
+      if (nd < dist[v]) {
+        const int old = atomicMin(reinterpret_cast<int*>(&dist[v]), __float_as_int(nd));
+        if (nd < __int_as_float(old)) bm_set(active_out, v);
+      }
+    }
+  }
+}
+
+// Pull: min over in-edges of dist[src] + w (sssp.cpp:60-73).
+__global__ __launch_bounds__(kNT) void k_sssp_pull(uint64_t E, const uint32_t* __restrict__ src,
+                                                    const float* __restrict__ w, const uint64_t* __restrict__ rowflag,
+                                                    const uint32_t* __restrict__ wordrank,
+                                                    const uint32_t* __restrict__ row_dst, float* __restrict__ dist,
+                                                    uint64_t* __restrict__ active_out) {
+  row_reduce_wave<float>(
+      E, rowflag, wordrank, 1e9f, OpMin{}, [&](uint64_t e) { return __fadd_rn(dist[src[e]], w[e]); },
+      [&](uint32_t row, float m, bool) {
+        const uint32_t v = row_dst[row];
+        if (m < 1e9f && m < dist[v]) {
+          const int old = atomicMin(reinterpret_cast<int*>(&dist[v]), __float_as_int(m));
+          if (m < __int_as_float(old)) bm_set(active_out, v);
+        }
+      });
+}
+
+}  // namespace
+}  // namespace eg

@@ -1,0 +1,73 @@
+// Device-resident state of one built graph (the eg_graph handle).
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+struct eg_graph {
+  // configuration (Graph(int eta), core/graph.hpp:202; P ranks and S sockets)
+  uint32_t V = 0;
+  int P = 1, S = 1, Q = 1;  // Q = P*S sub-partitions
+  uint64_t eta = 0;
+  uint32_t m = 1;
+  int rec = 8;  // 8 = EdgeType<Empty>, 12 = EdgeType<float>
+  uint32_t pages = 0;
+
+  // result of the last build
+  bool built = false;
+  int symmetrized = 0;
+  uint64_t E0 = 0;  // records in the input list
+  uint64_t E = 0;   // records after symmetrisation
+  int vbits = 1, qbits = 0;
+
+  eg::DevBuf out_deg, in_deg;        // u32[V]
+  eg::DevBuf page_prefix;            // u64[pages+1]: page weights, then their exclusive prefix
+  eg::DevBuf meta;                   // small device tables, see Meta
+  eg::DevBuf qmap;                   // u16[pages]: sub-partition of each page
+  eg::DevBuf csc_list, csr_list;     // records, sub-partitions back to back
+  eg::DevBuf csc_cidx;               // CIdx, per sub-partition: entries + sentinel
+  eg::DevBuf csr_index;              // u64[Q][V+1] (sub-partition-local positions)
+  eg::DevBuf csr_bitmap;             // u64[Q][(V>>6)+1]
+  eg::DevBuf sym;                    // symmetrised logical list
+  eg::DevBuf t1, t2;                 // sort ping-pong
+  eg::DevBuf hist, scan_tmp, tile_cnt, err;
+
+  // Engine view, derived from the built arrays on the first engine call
+  // (eg_engine_prepare).  Edge positions are u32: a build holds < 2^32 edges.
+  bool eng_ready = false;
+  uint64_t R = 0;                    // pull rows: compressed CSC entries of all sub-partitions
+  eg::DevBuf e_csc_src;              // u32[E]   source of every CSC edge (SoA of csc_list)
+  eg::DevBuf e_csc_w;                // f32[E]   weights (12-byte records only)
+  eg::DevBuf e_rowflag;              // u64[ceil(E/64)]: bit e set iff CSC edge e starts a row
+  eg::DevBuf e_wordrank;             // u32[ceil(E/64)+1]: rows started before word w
+  eg::DevBuf e_row_dst;              // u32[R]   destination of each row
+  eg::DevBuf e_row_start;            // u32[R+1] first edge of each row
+  eg::DevBuf e_out_ptr;              // u32[V+1] merged CSR over all sub-partitions
+  eg::DevBuf e_out_dst;              // u32[E]
+  eg::DevBuf e_out_w;                // f32[E]
+  eg::DevBuf e_scratch;              // frontier list, counters
+  eg::DevBuf e_bm[3];                // three bitmaps of (V>>6)+1 words
+  double eng_prepare_ms = 0;
+
+  // host copies of the small tables
+  std::vector<uint32_t> page_offset, partition_offset;       // P+1
+  std::vector<uint32_t> local_partition_offset;              // P*(S+1)
+  std::vector<uint64_t> q_off_csc, q_off_csr;                // Q+1 (edges per sub-partition, prefix)
+  std::vector<uint64_t> cidx_off, cidx_cnt;                  // Q (entries incl. sentinel)
+
+  // Layout of `meta` (u64 words unless noted)
+  struct Meta {
+    uint64_t q_cnt_csc[256 + 1];
+    uint64_t q_cnt_csr[256 + 1];
+    uint64_t q_off_csc[256 + 1];
+    uint64_t q_off_csr[256 + 1];
+    uint64_t cidx_off[256];
+    uint64_t cidx_cnt[256];
+    uint32_t page_offset[256 + 1];
+    uint32_t partition_offset[256 + 1];
+    uint32_t local_page_offset[256 * 9];  // absolute page ids, P*(S+1)
+    uint32_t local_partition_offset[256 * 9];
+  };
+  Meta* dmeta() const { return meta.as<Meta>(); }
+};

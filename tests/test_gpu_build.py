@@ -1,0 +1,92 @@
+"""GPU parity of the build: every array bit-exact against the oracle."""
+import numpy as np
+import pytest
+
+import helpers as H
+from parity import assert_build_equal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eg():
+    import endgraph_amd
+    endgraph_amd.lib()
+    return endgraph_amd
+
+
+def _gpu_build(eg, edges, V, P, S, eta, m, sym):
+    rec = edges.dtype.itemsize
+    buf = eg.DeviceBuffer.from_array(edges)
+    g = eg.Graph(V, P, S, eta, m, rec)
+    g.build(buf.ptr, edges.shape[0], sym)
+    return g, buf
+
+
+@pytest.mark.parametrize("P,S,sym,weighted", [
+    (1, 1, False, False), (1, 2, False, False), (2, 1, False, True), (3, 1, True, False),
+    (8, 1, False, False), (3, 2, True, True), (8, 3, False, False), (5, 1, False, False),
+])
+def test_build_random_skewed(eg, oracle_lib, P, S, sym, weighted):
+    print(f"build random P={P} S={S} sym={sym} weighted={weighted}", flush=True)
+    rng = np.random.default_rng(1000 + P * 10 + S)
+    V = 3 * 4096 + 1000 if P < 8 else 12 * 4096 + 77
+    edges = H.random_edges(rng, V, 50000 + 13, weighted=weighted)
+    g, _ = _gpu_build(eg, edges, V, P, S, 3, 20, sym)
+    og = oracle_lib.OracleGraph(edges, V, P=P, S=S, eta=3, m=20, symmetrize=sym, threads=8)
+    assert_build_equal(g, og)
+
+
+@pytest.mark.parametrize("scale,P,sym", [(14, 1, False), (16, 1, False), (16, 8, False), (15, 4, True)])
+def test_build_rmat(eg, oracle_lib, scale, P, sym):
+    print(f"build rmat scale={scale} P={P} sym={sym}", flush=True)
+    V, E = 1 << scale, 16 << scale
+    buf = eg.DeviceBuffer(E * 8)
+    eg.gen_rmat(buf.ptr, E, scale, seed=1)
+    edges = buf.download(eg.REC8, E)
+    g = eg.Graph(V, P, 1, 1, 20, 8)
+    g.build(buf.ptr, E, sym)
+    og = oracle_lib.OracleGraph(edges, V, P=P, S=1, eta=1, m=20, symmetrize=sym, threads=8)
+    assert_build_equal(g, og)
+
+
+def test_build_edge_cases(eg, oracle_lib):
+    rng = np.random.default_rng(5)
+    # fewer edges than ranks, V = 1, self loops only, all edges in one page
+    cases = []
+    e = np.zeros(3, H.REC8); e["src"] = [0, 1, 2]; e["dst"] = [2, 2, 0]
+    cases.append((e, 5000, 4, 2, False))
+    e = np.zeros(7, H.REC8)
+    cases.append((e, 1, 1, 1, True))
+    e = np.zeros(400, H.REC8); e["src"] = rng.integers(0, 50, 400); e["dst"] = e["src"]
+    cases.append((e, 9 * 4096 + 5, 3, 3, False))
+    e = H.random_edges(rng, 70000, 4000, skew=False)
+    cases.append((e, 70000, 8, 1, True))
+    for edges, V, P, S, sym in cases:
+        print(f"edge case V={V} E={edges.shape[0]} P={P} S={S} sym={sym}", flush=True)
+        g, _ = _gpu_build(eg, edges, V, P, S, 2, 1, sym)
+        og = oracle_lib.OracleGraph(edges, V, P=P, S=S, eta=2, m=1, symmetrize=sym, threads=4)
+        assert_build_equal(g, og)
+
+
+def test_build_rejects_out_of_range_ids(eg):
+    e = np.zeros(10, H.REC8)
+    e["src"] = np.arange(10)
+    e["dst"] = 10  # == V
+    buf = eg.DeviceBuffer.from_array(e)
+    g = eg.Graph(10, 1, 1, 0, 1, 8)
+    with pytest.raises(eg.EndGraphError) as ei:
+        g.build(buf.ptr, 10)
+    assert ei.value.status == -2
+
+
+def test_rebuild_reuses_handle(eg, oracle_lib):
+    rng = np.random.default_rng(9)
+    V = 20000
+    g = eg.Graph(V, 2, 1, 1, 1, 8)
+    for E in (30000, 10000):  # second build smaller than the first
+        edges = H.random_edges(rng, V, E)
+        buf = eg.DeviceBuffer.from_array(edges)
+        g.build(buf.ptr, E)
+        og = oracle_lib.OracleGraph(edges, V, P=2, S=1, eta=1, m=1, threads=4)
+        assert_build_equal(g, og)
