@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Headline benchmark: CSR+CSC build Medges/s on one MI355X, with BFS GTEPS and
+PageRank GTEPS (FP32 compensated vs FP64) on the same graph.
+
+Workload (default): R-MAT scale 24, edge factor 16 (16,777,216 vertices,
+268,435,456 uint32 <src,dst> records), a/b/c/d = .57/.19/.19/.05, seed 2,
+split into P = 8 CPBT partitions (eta = 1, m = 1, S = 1) on one GPU.  A step
+is one full build from the device-resident edge list: degree histogram, CPBT
+boundaries, per-rank shuffle sort, two-level CSC and CSR with their indexes
+(the reference's timed prep region).  Inputs are generated on the GPU before
+the timed region.
+
+Prints one JSON line (rank 0).  Multi-GPU runs are not part of this build.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+WORKLOADS = {
+    # name: (description, V, E, generator, P, S, eta, m)
+    "rmat24": ("R-MAT scale 24, edge factor 16, a/b/c/d .57/.19/.19/.05, seed 2; P=8 CPBT partitions, S=1",
+               1 << 24, 1 << 28, ("rmat", 24, 2), 8, 1, 1, 1),
+    "twitter": ("twitter-shaped power law (out alpha 1.9, in alpha 2.1), seed 3; P=1, S=1",
+                41_652_230, 1_468_365_182, ("powerlaw", 3), 1, 1, 1, 20),
+    "rmat16": ("R-MAT scale 16, edge factor 16, seed 1; P=1, S=1",
+               1 << 16, 1 << 20, ("rmat", 16, 1), 1, 1, 1, 20),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def median_time(fn, runs=3, warm=1):
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-engine", action="store_true", help="skip the BFS / PageRank figures")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-edges", type=int, default=1 << 25)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one MI355X only; multi-GPU runs (--gpus > 1) are not part of it",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import endgraph_amd as eg
+
+    desc, V, E, gen, P, S, eta, m = WORKLOADS[args.workload]
+    log(f"[bench] workload {args.workload}: V={V} E={E} P={P} S={S}")
+    edges = eg.DeviceBuffer(E * 8)
+    if gen[0] == "rmat":
+        eg.gen_rmat(edges.ptr, E, gen[1], seed=gen[2])
+    else:
+        eg.gen_powerlaw(edges.ptr, E, V, seed=gen[1])
+    eg.synchronize()
+    log("[bench] edges generated")
+
+    g = eg.Graph(V, P, S, eta, m, 8)
+    for i in range(args.warmup):
+        g.build(edges.ptr, E)
+        log(f"[bench] warmup build {i} done")
+    eg.profile_reset()
+    eg.profile_enable(True)
+    eg.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g.build(edges.ptr, E)  # synchronises at the end of every build
+    eg.synchronize()
+    t = time.perf_counter() - t0
+    eg.profile_enable(False)
+    ms_per_step = 1e3 * t / args.steps
+    medges = E * args.steps / t / 1e6
+    log(f"[bench] build: {ms_per_step:.2f} ms/step, {medges:.1f} Medges/s")
+
+    kernels = {}
+    for k in ("radix_downsweep", "radix_upsweep", "scan", "degree", "cindex", "csr_index"):
+        ms, n, b = eg.profile_read(k)
+        if n:
+            kernels[k] = {"ms_total": ms, "launches": n, "avg_ms": ms / n, "bytes_per_launch": b / n,
+                          "gbs": (b / n) / (ms / n * 1e-3) / 1e9}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+    dk = kernels[dom]
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(dk["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dk["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_launch": int(dk["bytes_per_launch"]), "avg_launch_ms": round(dk["avg_ms"], 4),
+                "launches": dk["launches"]}
+
+    extra = {"kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in d.items()}
+                         for k, d in kernels.items()}}
+
+    if not args.no_engine:
+        prep_ms = g.prepare_engine()
+        extra["engine_prepare_ms"] = round(prep_ms, 2)
+        deg = g.out_degree()
+        root = int(np.argmax(deg))
+        parent = eg.DeviceBuffer(V * 4)
+        depth = eg.DeviceBuffer(V * 4)
+        res = {}
+
+        def run_bfs():
+            res["bfs"] = g.bfs(root, eg.POLICY_AUTO, parent, depth, download=False)
+
+        tb = median_time(run_bfs)
+        levels, trav = res["bfs"]
+        extra["bfs"] = {"root": root, "policy": "auto", "levels": levels, "edges_traversed": trav,
+                        "ms": round(tb * 1e3, 3), "gteps": round(trav / tb / 1e9, 3)}
+        log(f"[bench] bfs: {tb * 1e3:.2f} ms, {trav / tb / 1e9:.2f} GTEPS")
+        iters = 20
+        out64 = eg.DeviceBuffer(V * 8)
+        out32 = eg.DeviceBuffer(V * 8)
+        t64 = median_time(lambda: g.pagerank(iters, eg.PR_FP64, out64))
+        t32 = median_time(lambda: g.pagerank(iters, eg.PR_FP32C, out32))
+        x64 = out64.download(np.float64, V)
+        x32 = out32.download(np.float64, V)
+        l1 = float(np.abs(x32 - x64).sum() / np.abs(x64).sum())
+        extra["pagerank"] = {"iterations": iters, "gteps_fp64": round(iters * E / t64 / 1e9, 3),
+                             "gteps_fp32c": round(iters * E / t32 / 1e9, 3), "speedup_fp32c": round(t64 / t32, 3),
+                             "l1_fp32c_vs_fp64_gpu": l1, "ms_fp64": round(t64 * 1e3, 2),
+                             "ms_fp32c": round(t32 * 1e3, 2)}
+        log(f"[bench] pagerank: fp64 {iters * E / t64 / 1e9:.2f} GTEPS, fp32c {iters * E / t32 / 1e9:.2f} GTEPS, "
+            f"L1 {l1:.2e}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        n = min(args.cpu_sample_edges, E)
+        sample = edges.download(eg.REC8, n)
+        og = oracle.OracleGraph(sample, V, P=P, S=S, eta=eta, m=m, threads=threads)
+        secs = og.build_seconds
+        del og
+        cpu = {"value": round(n / secs / 1e6, 2), "unit": "Medges/s", "cores": threads, "kind": "port",
+               "sample": f"first {n:,} of the {E:,} records, same V/P/S/eta/m; oracle/cpu_ref.cpp build "
+                         f"(timed region = split_graph equivalent), {secs:.2f} s"}
+        log(f"[bench] cpu baseline: {cpu['value']} Medges/s on {threads} threads")
+
+    line = {
+        "metric": "CSR+CSC build Medges/s; PageRank GTEPS FP32-vs-FP64 at L1<=1e-6; BFS GTEPS",
+        "value": round(medges, 2),
+        "unit": "Medges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": desc, "vertices": V, "edges": E, "partitions": P, "sockets": S, "eta": eta,
+                   "m": m, "record_bytes": 8},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -78,6 +78,24 @@ def lib():
         "eg_graph_page_prefix": ([vp, pp], i32),
         "eg_graph_csc": ([vp, i32, i32, pp, C.POINTER(u64), pp, C.POINTER(u64)], i32),
         "eg_graph_csr": ([vp, i32, i32, pp, C.POINTER(u64), pp, pp], i32),
+        "eg_profile_enable": ([i32], i32),
+        "eg_profile_reset": ([], i32),
+        "eg_profile_read": ([C.c_char_p, C.POINTER(f64), C.POINTER(u64), C.POINTER(u64)], i32),
+        "eg_engine_prepare": ([vp, C.POINTER(f64), vp], i32),
+        "eg_bitmap_clear": ([vp, u32, vp], i32),
+        "eg_bitmap_fill": ([vp, u32, vp], i32),
+        "eg_bitmap_set_bit": ([vp, u32, vp], i32),
+        "eg_pr_init": ([vp, vp, vp, vp], i32),
+        "eg_pr_pull": ([vp, i32, vp, vp, vp], i32),
+        "eg_pr_apply": ([vp, vp, vp, vp, i32, vp, vp], i32),
+        "eg_pagerank": ([vp, i32, i32, vp, vp, vp], i32),
+        "eg_bfs_level": ([vp, i32, u32, vp, vp, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64),
+                          C.POINTER(i32), vp], i32),
+        "eg_bfs": ([vp, u32, i32, vp, vp, C.POINTER(u32), C.POINTER(u64), vp], i32),
+        "eg_cc_iteration": ([vp, i32, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64), vp], i32),
+        "eg_cc": ([vp, i32, vp, C.POINTER(u32), vp], i32),
+        "eg_sssp_iteration": ([vp, i32, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64), vp], i32),
+        "eg_sssp": ([vp, u32, i32, vp, C.POINTER(u32), vp], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -140,6 +158,25 @@ def to_host(d_ptr: int, dtype, count: int, stream=None) -> np.ndarray:
     if count:
         check(lib().eg_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(d_ptr), out.nbytes, _stream(stream)))
     return out
+
+
+POLICY_REFERENCE, POLICY_AUTO, POLICY_PUSH, POLICY_PULL = 0, 1, 2, 3
+PR_FP64, PR_FP32C = 0, 1
+
+
+def profile_enable(on: bool = True):
+    check(lib().eg_profile_enable(int(bool(on))))
+
+
+def profile_reset():
+    check(lib().eg_profile_reset())
+
+
+def profile_read(kernel: str):
+    """(total_ms, launches, algorithmic bytes) of a timed kernel class."""
+    ms, n, b = C.c_double(), C.c_uint64(), C.c_uint64()
+    check(lib().eg_profile_read(kernel.encode(), C.byref(ms), C.byref(n), C.byref(b)))
+    return ms.value, n.value, b.value
 
 
 def synchronize(stream=None):
@@ -250,3 +287,43 @@ class Graph:
         lst, n, idx, bm = self.csr_ptrs(p, s)
         return (to_host(lst, self.rec_dtype, n), to_host(idx, np.uint64, self.V + 1),
                 to_host(bm, np.uint64, (self.V >> 6) + 1))
+
+    # ----------------------------------------------------------- engine --
+    def prepare_engine(self, stream=None) -> float:
+        ms = C.c_double()
+        check(lib().eg_engine_prepare(self._h, C.byref(ms), _stream(stream)))
+        return ms.value
+
+    def pagerank(self, iterations: int, precision: int = PR_FP64, out: DeviceBuffer = None, stream=None):
+        """toolkits/pagerank.cpp compute(): final ranks (FP64) and delta(i) per iteration."""
+        own = out is None
+        if own:
+            out = DeviceBuffer(self.V * 8)
+        deltas = np.zeros(iterations + 1, np.float64)
+        check(lib().eg_pagerank(self._h, iterations, precision, C.c_void_p(out.ptr),
+                                deltas.ctypes.data_as(C.c_void_p), _stream(stream)))
+        return (out.download(np.float64, self.V) if own else out), deltas
+
+    def bfs(self, root: int, policy: int = POLICY_REFERENCE, parent: DeviceBuffer = None,
+            depth: DeviceBuffer = None, download: bool = True, stream=None):
+        parent = parent or DeviceBuffer(self.V * 4)
+        depth = depth or DeviceBuffer(self.V * 4)
+        levels, edges = C.c_uint32(), C.c_uint64()
+        check(lib().eg_bfs(self._h, root, policy, C.c_void_p(parent.ptr), C.c_void_p(depth.ptr), C.byref(levels),
+                           C.byref(edges), _stream(stream)))
+        if not download:
+            return levels.value, edges.value
+        return parent.download(np.uint32, self.V), depth.download(np.uint32, self.V), levels.value, edges.value
+
+    def cc(self, policy: int = POLICY_REFERENCE, label: DeviceBuffer = None, download: bool = True, stream=None):
+        label = label or DeviceBuffer(self.V * 4)
+        it = C.c_uint32()
+        check(lib().eg_cc(self._h, policy, C.c_void_p(label.ptr), C.byref(it), _stream(stream)))
+        return (label.download(np.uint32, self.V) if download else None), it.value
+
+    def sssp(self, root: int, policy: int = POLICY_REFERENCE, dist: DeviceBuffer = None, download: bool = True,
+             stream=None):
+        dist = dist or DeviceBuffer(self.V * 4)
+        it = C.c_uint32()
+        check(lib().eg_sssp(self._h, root, policy, C.c_void_p(dist.ptr), C.byref(it), _stream(stream)))
+        return (dist.download(np.float32, self.V) if download else None), it.value

@@ -1,6 +1,7 @@
 // C-ABI glue: error reporting, owned device buffers, memory helpers.
 #include <cstdarg>
 #include <cstring>
+#include <vector>
 
 #include "common.hpp"
 
@@ -42,6 +43,68 @@ void DevBuf::release() {
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
+}
+
+namespace {
+struct ProfRec {
+  hipEvent_t a, b;
+  int kernel;
+  uint64_t bytes;
+};
+bool g_prof = false;
+hipEvent_t g_pending = nullptr;
+std::vector<ProfRec> g_recs;
+std::vector<hipEvent_t> g_pool;
+struct ProfAgg {
+  double ms = 0;
+  uint64_t launches = 0, bytes = 0;
+};
+ProfAgg g_agg[PK_COUNT];
+const char* kProfNames[PK_COUNT] = {"radix_upsweep", "radix_downsweep", "scan", "degree",
+                                    "cindex", "csr_index", "pr_pull", "bfs"};
+
+hipEvent_t take_event() {
+  if (!g_pool.empty()) {
+    hipEvent_t e = g_pool.back();
+    g_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void drain() {
+  for (auto& r : g_recs) {
+    float ms = 0;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      g_agg[r.kernel].ms += ms;
+      g_agg[r.kernel].launches += 1;
+      g_agg[r.kernel].bytes += r.bytes;
+    }
+    g_pool.push_back(r.a);
+    g_pool.push_back(r.b);
+  }
+  g_recs.clear();
+}
+}  // namespace
+
+bool prof_on() { return g_prof; }
+
+void prof_begin(hipStream_t s) {
+  if (!g_prof) return;
+  g_pending = take_event();
+  if (g_pending) (void)hipEventRecord(g_pending, s);
+}
+
+void prof_end(int kernel, uint64_t bytes, hipStream_t s) {
+  if (!g_prof || !g_pending) return;
+  hipEvent_t b = take_event();
+  if (!b) return;
+  (void)hipEventRecord(b, s);
+  g_recs.push_back(ProfRec{g_pending, b, kernel, bytes});
+  g_pending = nullptr;
+  if (g_recs.size() > 4096) drain();
 }
 
 }  // namespace eg
@@ -87,6 +150,32 @@ int eg_memcpy_d2d(void* d_dst, const void* d_src, uint64_t bytes, void* stream) 
   if (!bytes) return EG_OK;
   EG_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
   return EG_OK;
+}
+
+int eg_profile_enable(int on) {
+  if (!on) drain();
+  g_prof = on != 0;
+  return EG_OK;
+}
+
+int eg_profile_reset(void) {
+  drain();
+  for (auto& a : g_agg) a = ProfAgg{};
+  return EG_OK;
+}
+
+int eg_profile_read(const char* kernel, double* total_ms, uint64_t* launches, uint64_t* bytes) {
+  if (!kernel) return fail(EG_ERR_ARG, "eg_profile_read: kernel name is NULL");
+  drain();
+  for (int k = 0; k < PK_COUNT; ++k) {
+    if (std::strcmp(kernel, kProfNames[k]) == 0) {
+      if (total_ms) *total_ms = g_agg[k].ms;
+      if (launches) *launches = g_agg[k].launches;
+      if (bytes) *bytes = g_agg[k].bytes;
+      return EG_OK;
+    }
+  }
+  return fail(EG_ERR_ARG, "unknown kernel class '%s'", kernel);
 }
 
 int eg_stream_synchronize(void* stream) {

@@ -405,6 +405,7 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   const uint64_t E = sym ? 2 * E0 : E0;
   const uint64_t rb = sizeof(R);
   g->built = false;
+  g->eng_ready = false;
   g->E0 = E0;
   g->E = E;
   g->symmetrized = sym;
@@ -448,8 +449,10 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   EG_TRY(fill<uint32_t>(out_deg, V, 0u, s));
   EG_TRY(fill<uint32_t>(in_deg, V, 0u, s));
   if (E0) {
+    prof_begin(s);
     hipLaunchKernelGGL((k_degree<R>), dim3(grid_for(E0, kNT * 8, 16384)), dim3(kNT), 0, s, d_in, E0, V, sym, out_deg,
                        in_deg, err);
+    prof_end(PK_DEGREE, E0 * rb + 2ull * V * 4, s);
     EG_HIP(hipGetLastError());
   }
   uint32_t h_err = 0;
@@ -523,8 +526,10 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
       hipLaunchKernelGGL((k_run_count<R, KeyQ<true>>), dim3(ntiles), dim3(kNT), 0, s, a, E, key, tc, ntiles);
       EG_HIP(hipGetLastError());
       EG_TRY(exclusive_scan_u32(tc, tc, uint64_t(ntiles) + 1, stmp32, s));
+      prof_begin(s);
       hipLaunchKernelGGL((k_cindex_write<R>), dim3(ntiles), dim3(kNT), 0, s, a, E, key, (const uint32_t*)tc, meta,
                          cidx);
+      prof_end(PK_CINDEX, E * rb, s);
       EG_HIP(hipGetLastError());
     } else {
       EG_TRY(fill<uint32_t>(tc, 2, 0u, s));
@@ -540,8 +545,10 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
     EG_TRY(fill<uint64_t>(index, idx_n, uint64_t(0), s));
     EG_TRY(fill<uint64_t>(bitmap, uint64_t(Q) * words, uint64_t(0), s));
     if (E) {
+      prof_begin(s);
       hipLaunchKernelGGL((k_csr_runs<R>), dim3(grid_for(E, kNT * 8, 65536)), dim3(kNT), 0, s, g->csr_list.as<R>(), E,
                          KeyQ<false>{qmap, g->vbits}, V, index, bitmap);
+      prof_end(PK_CSR_INDEX, E * rb, s);
       EG_HIP(hipGetLastError());
     }
     EG_TRY(inclusive_max_scan_u64(index, index, idx_n, g->scan_tmp.as<uint64_t>(), s));

@@ -118,7 +118,7 @@ __device__ __forceinline__ T shfl_t(T v, int lane) { return __shfl(v, lane, kWav
 constexpr uint32_t kPageSize = 4096;  // core/constants.hpp:21 PAGESIZE
 constexpr int kPageShift = 12;
 
-inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 inline int counting_bits(uint64_t n) {  // core/graph.hpp:1087-1092
   int i = 0;
@@ -137,6 +137,25 @@ struct DevBuf {
   template <class T>
   T* as() const { return static_cast<T*>(ptr); }
 };
+
+// ------------------------------------------------------------ profiling --
+// Optional HIP-event timing of individual kernel launches (eg_profile_*):
+// prof_begin/prof_end bracket one launch on its own stream; bytes are the
+// launch's algorithmic traffic.
+enum ProfKernel {
+  PK_RADIX_UPSWEEP = 0,
+  PK_RADIX_DOWNSWEEP,
+  PK_SCAN,
+  PK_DEGREE,
+  PK_CINDEX,
+  PK_CSR_INDEX,
+  PK_PR_PULL,
+  PK_BFS,
+  PK_COUNT
+};
+bool prof_on();
+void prof_begin(hipStream_t s);
+void prof_end(int kernel, uint64_t bytes, hipStream_t s);
 
 // ------------------------------------------------------------- scans -----
 // Device-wide exclusive prefix sums (reduce-then-scan, three launches).

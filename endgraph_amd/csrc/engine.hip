@@ -43,6 +43,11 @@ __global__ void k_fill(T* __restrict__ p, uint64_t n, T v) {
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
     p[i] = v;
 }
+__global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = uint32_t(i);
+}
+
 template <class T>
 int fill(T* p, uint64_t n, T v, hipStream_t s) {
   if (!n) return EG_OK;
@@ -429,6 +434,11 @@ __global__ void k_bm_list(const uint64_t* __restrict__ bm, uint64_t words, const
   }
 }
 
+__global__ void k_bm_or(uint64_t* __restrict__ dst, const uint64_t* __restrict__ src, uint64_t words) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += uint64_t(gridDim.x) * blockDim.x)
+    dst[i] |= src[i];
+}
+
 // Level bookkeeping: visited |= out; depth of new vertices; counts of new
 // vertices and of their out-edges (for the next mode decision).
 __global__ __launch_bounds__(kNT) void k_level_end(const uint64_t* __restrict__ out, uint64_t* __restrict__ visited,
@@ -441,11 +451,12 @@ __global__ __launch_bounds__(kNT) void k_level_end(const uint64_t* __restrict__ 
     if (!w) continue;
     if (visited) visited[i] |= w;
     nv += __popcll(w);
+    if (!depth && !deg) continue;
     while (w) {
       const int b = __ffsll((unsigned long long)w) - 1;
       const uint32_t v = uint32_t(i * 64 + b);
       if (depth) depth[v] = level;
-      ne += deg[v];
+      if (deg) ne += deg[v];
       w &= w - 1;
     }
   }
@@ -591,3 +602,422 @@ __global__ __launch_bounds__(kNT) void k_sssp_pull(uint64_t E, const uint32_t* _
 
 }  // namespace
 }  // namespace eg
+
+// ================================================================ host ===
+namespace eg {
+namespace {
+
+struct Scratch {
+  uint32_t* flist;                 // frontier list, V entries
+  uint32_t* wcnt;                  // per-word counts, words+1
+  unsigned long long* counters;    // [0] new vertices, [1] their out-edges
+  double* delta;                   // PageRank residuals, iterations+1
+};
+
+uint64_t words_of(uint32_t V) { return (uint64_t(V) >> 6) + 1; }
+
+uint64_t scratch_bytes(uint32_t V, int iters) {
+  return uint64_t(V) * 4 + 256 + (words_of(V) + 1) * 4 + 256 + 64 + uint64_t(iters + 2) * 8 + 256;
+}
+
+int get_scratch(eg_graph* g, int iters, Scratch* sc) {
+  const uint64_t need = scratch_bytes(g->V, iters);
+  if (g->e_scratch.bytes < need) EG_TRY(g->e_scratch.resize(need));
+  char* p = g->e_scratch.as<char>();
+  auto align = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+  uint64_t o = 0;
+  sc->flist = reinterpret_cast<uint32_t*>(p + o);
+  o = align(o + uint64_t(g->V) * 4);
+  sc->wcnt = reinterpret_cast<uint32_t*>(p + o);
+  o = align(o + (words_of(g->V) + 1) * 4);
+  sc->counters = reinterpret_cast<unsigned long long*>(p + o);
+  o = align(o + 64);
+  sc->delta = reinterpret_cast<double*>(p + o);
+  return EG_OK;
+}
+
+// ascending list of the set bits of bm; returns the count on the host
+int make_list(eg_graph* g, const uint64_t* bm, Scratch& sc, uint32_t* h_n, hipStream_t s) {
+  const uint64_t words = words_of(g->V);
+  hipLaunchKernelGGL(k_bm_count, dim3(grid_for(words, kNT, 65536)), dim3(kNT), 0, s, bm, words, sc.wcnt);
+  EG_HIP(hipGetLastError());
+  EG_TRY(fill<uint32_t>(sc.wcnt + words, 1, 0u, s));
+  EG_TRY(exclusive_scan_u32(sc.wcnt, sc.wcnt, words + 1, g->scan_tmp.as<uint32_t>(), s));
+  hipLaunchKernelGGL(k_bm_list, dim3(grid_for(words, kNT, 65536)), dim3(kNT), 0, s, bm, words,
+                     (const uint32_t*)sc.wcnt, sc.flist);
+  EG_HIP(hipGetLastError());
+  EG_HIP(hipMemcpyAsync(h_n, sc.wcnt + words, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+// visited |= out (if given), depth of out = level (if given); returns the
+// number of vertices in out and the sum of their out-degrees.
+int level_end(eg_graph* g, const uint64_t* out, uint64_t* visited, uint32_t* depth, uint32_t level, Scratch& sc,
+              uint64_t* h_nv, uint64_t* h_ne, hipStream_t s) {
+  const uint64_t words = words_of(g->V);
+  EG_TRY(fill<unsigned long long>(sc.counters, 2, 0ull, s));
+  hipLaunchKernelGGL(k_level_end, dim3(grid_for(words, kNT, 8192)), dim3(kNT), 0, s, out, visited, words, depth, level,
+                     (const uint32_t*)g->out_deg.as<uint32_t>(), sc.counters);
+  EG_HIP(hipGetLastError());
+  unsigned long long h[2];
+  EG_HIP(hipMemcpyAsync(h, sc.counters, 16, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  if (h_nv) *h_nv = h[0];
+  if (h_ne) *h_ne = h[1];
+  return EG_OK;
+}
+
+unsigned push_grid(uint64_t fn) { return grid_for(fn, kNT / kWave, 16384); }
+unsigned row_grid(uint64_t E) { return grid_for(ceil_div(ceil_div(E, 64), kRowWordsPerWave), kNT / kWave, 1u << 30); }
+
+// core/graph.hpp:1483-1498: rooted programs are sparse under the reference
+// policy; undirected ones (and every program under auto) switch on the
+// active out-edge count (:1491, R4).
+bool choose_sparse(const eg_graph* g, int policy, uint64_t active_edges, bool rooted) {
+  if (policy == EG_POLICY_PUSH) return true;
+  if (policy == EG_POLICY_PULL) return false;
+  if (policy == EG_POLICY_REFERENCE && rooted) return true;
+  return active_edges < g->E / 20;
+}
+
+int pr_pull(eg_graph* g, int precision, const void* x, double* next, hipStream_t s) {
+  EG_TRY(fill<double>(next, g->V, 0.0, s));
+  if (!g->E) return EG_OK;
+  const int all_atomic = g->Q > 1;
+  prof_begin(s);
+  if (precision == EG_PR_FP64)
+    hipLaunchKernelGGL(k_pr_pull_f64, dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E, g->e_csc_src.as<uint32_t>(),
+                       g->e_rowflag.as<uint64_t>(), g->e_wordrank.as<uint32_t>(), g->e_row_dst.as<uint32_t>(),
+                       static_cast<const double*>(x), next, all_atomic);
+  else
+    hipLaunchKernelGGL(k_pr_pull_f32c, dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E, g->e_csc_src.as<uint32_t>(),
+                       g->e_rowflag.as<uint64_t>(), g->e_wordrank.as<uint32_t>(), g->e_row_dst.as<uint32_t>(),
+                       static_cast<const float*>(x), next, all_atomic);
+  // algorithmic bytes: source id + gathered value per edge, row start bits,
+  // destination id + partial per row
+  const uint64_t xb = precision == EG_PR_FP64 ? 8 : 4;
+  prof_end(PK_PR_PULL, g->E * (4 + xb) + g->E / 8 + g->R * (4 + 8), s);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+}  // namespace
+}  // namespace eg
+
+using namespace eg;
+
+extern "C" {
+
+int eg_engine_prepare(eg_graph* g, double* h_ms, void* stream) {
+  EG_TRY(ensure_engine(g, static_cast<hipStream_t>(stream)));
+  if (h_ms) *h_ms = g->eng_prepare_ms;
+  return EG_OK;
+}
+
+// ---- device helpers ----
+int eg_device_fill(void* d_ptr, uint64_t n, const void* h_value, int value_bytes, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!d_ptr || !h_value) return fail(EG_ERR_ARG, "eg_device_fill: NULL argument");
+  switch (value_bytes) {
+    case 1: return fill<uint8_t>(static_cast<uint8_t*>(d_ptr), n, *static_cast<const uint8_t*>(h_value), s);
+    case 2: return fill<uint16_t>(static_cast<uint16_t*>(d_ptr), n, *static_cast<const uint16_t*>(h_value), s);
+    case 4: return fill<uint32_t>(static_cast<uint32_t*>(d_ptr), n, *static_cast<const uint32_t*>(h_value), s);
+    case 8: return fill<uint64_t>(static_cast<uint64_t*>(d_ptr), n, *static_cast<const uint64_t*>(h_value), s);
+    default: return fail(EG_ERR_ARG, "eg_device_fill: value_bytes must be 1, 2, 4 or 8");
+  }
+}
+
+int eg_device_iota(uint32_t* d_ptr, uint32_t n, void* stream) {
+  if (!d_ptr && n) return fail(EG_ERR_ARG, "eg_device_iota: NULL array");
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(n, kNT, 65536)), dim3(kNT), 0, static_cast<hipStream_t>(stream), d_ptr, n);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+// ---- bitmaps (core/bitmap.hpp:20-68) ----
+int eg_bitmap_or(uint64_t* d_dst, const uint64_t* d_src, uint32_t V, void* stream) {
+  const uint64_t words = words_of(V);
+  hipLaunchKernelGGL(k_bm_or, dim3(grid_for(words, kNT, 65536)), dim3(kNT), 0, static_cast<hipStream_t>(stream), d_dst,
+                     d_src, words);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+int eg_bitmap_count(const uint64_t* d_bm, uint32_t V, uint64_t* h_count, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!h_count) return fail(EG_ERR_ARG, "eg_bitmap_count: h_count is NULL");
+  DevBuf c;
+  EG_TRY(c.resize(16));
+  EG_TRY(fill<unsigned long long>(c.as<unsigned long long>(), 2, 0ull, s));
+  hipLaunchKernelGGL(k_level_end, dim3(grid_for(words_of(V), kNT, 8192)), dim3(kNT), 0, s, d_bm, nullptr, words_of(V),
+                     nullptr, 0u, nullptr, c.as<unsigned long long>());
+  EG_HIP(hipGetLastError());
+  unsigned long long h[2];
+  EG_HIP(hipMemcpyAsync(h, c.ptr, 16, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  *h_count = h[0];
+  return EG_OK;
+}
+
+int eg_bitmap_degree_sum(eg_graph* g, const uint64_t* d_bm, uint64_t* h_sum, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  if (!h_sum) return fail(EG_ERR_ARG, "eg_bitmap_degree_sum: h_sum is NULL");
+  Scratch sc;
+  EG_TRY(get_scratch(g, 0, &sc));
+  return level_end(g, d_bm, nullptr, nullptr, 0, sc, nullptr, h_sum, s);
+}
+
+int eg_bitmap_clear(uint64_t* d_bm, uint32_t V, void* stream) {
+  return fill<uint64_t>(d_bm, words_of(V), 0ull, static_cast<hipStream_t>(stream));
+}
+
+int eg_bitmap_fill(uint64_t* d_bm, uint32_t V, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t full = uint64_t(V) >> 6;
+  EG_TRY(fill<uint64_t>(d_bm, full, ~0ull, s));
+  const uint64_t tail = (V & 63) ? ((1ull << (V & 63)) - 1) : 0ull;  // bits [full*64, V)
+  EG_TRY(fill<uint64_t>(d_bm + full, 1, tail, s));
+  return EG_OK;
+}
+
+int eg_bitmap_set_bit(uint64_t* d_bm, uint32_t v, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint64_t w = 0;
+  EG_HIP(hipMemcpyAsync(&w, d_bm + (v >> 6), 8, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  w |= 1ull << (v & 63);
+  EG_HIP(hipMemcpyAsync(d_bm + (v >> 6), &w, 8, hipMemcpyHostToDevice, s));
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+// ---- PageRank (toolkits/pagerank.cpp) ----
+int eg_pr_init(eg_graph* g, double* d_x, float* d_x32, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  hipLaunchKernelGGL(k_pr_init, dim3(grid_for(g->V, kNT, 65536)), dim3(kNT), 0, s,
+                     (const uint32_t*)g->out_deg.as<uint32_t>(), g->V, d_x, d_x32);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+int eg_pr_pull(eg_graph* g, int precision, const void* d_x, double* d_next, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  if (precision != EG_PR_FP64 && precision != EG_PR_FP32C) return fail(EG_ERR_ARG, "unknown precision %d", precision);
+  return pr_pull(g, precision, d_x, d_next, s);
+}
+
+int eg_pr_apply(eg_graph* g, const double* d_x, double* d_next, float* d_next32, int last, double* d_delta,
+                void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  if (!d_delta) return fail(EG_ERR_ARG, "d_delta is NULL");
+  hipLaunchKernelGGL(k_pr_apply, dim3(grid_for(g->V, kNT, 4096)), dim3(kNT), 0, s,
+                     (const uint32_t*)g->out_deg.as<uint32_t>(), g->V, d_x, d_next, d_next32, last, d_delta);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+int eg_pagerank(eg_graph* g, int iterations, int precision, double* d_out, double* h_deltas, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  if (iterations < 1 || !d_out) return fail(EG_ERR_ARG, "need iterations >= 1 and an output array");
+  if (precision != EG_PR_FP64 && precision != EG_PR_FP32C) return fail(EG_ERR_ARG, "unknown precision %d", precision);
+  Scratch sc;
+  EG_TRY(get_scratch(g, iterations, &sc));
+  const uint64_t V = g->V;
+  EG_TRY(g->e_pr.resize(V * 8 + V * 4 + 512));
+  double* a = d_out;                     // current x (FP64)
+  double* b = g->e_pr.as<double>();      // next
+  float* x32 = precision == EG_PR_FP32C ? reinterpret_cast<float*>(g->e_pr.as<char>() + ((V * 8 + 255) & ~255ull))
+                                        : nullptr;
+  EG_TRY(fill<double>(sc.delta, uint64_t(iterations) + 1, 0.0, s));
+  hipLaunchKernelGGL(k_pr_init, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
+                     (const uint32_t*)g->out_deg.as<uint32_t>(), g->V, a, x32);
+  EG_HIP(hipGetLastError());
+  for (int it = 0; it < iterations; ++it) {
+    EG_TRY(pr_pull(g, precision, precision == EG_PR_FP64 ? static_cast<const void*>(a) : x32, b, s));
+    hipLaunchKernelGGL(k_pr_apply, dim3(grid_for(V, kNT, 4096)), dim3(kNT), 0, s,
+                       (const uint32_t*)g->out_deg.as<uint32_t>(), g->V, (const double*)a, b, x32,
+                       it == iterations - 1 ? 1 : 0, sc.delta + it + 1);
+    EG_HIP(hipGetLastError());
+    std::swap(a, b);
+  }
+  if (a != d_out) EG_HIP(hipMemcpyAsync(d_out, a, V * 8, hipMemcpyDeviceToDevice, s));
+  if (h_deltas) {
+    EG_HIP(hipMemcpyAsync(h_deltas, sc.delta, sizeof(double) * (iterations + 1), hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    h_deltas[0] = 1.0;  // sum of 1 over all vertices / V (pagerank.cpp:36-47)
+    for (int it = 1; it <= iterations; ++it) h_deltas[it] /= double(g->V);
+  }
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+// ---- BFS (toolkits/bfs.cpp) ----
+int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontier, uint64_t* d_next,
+                 uint64_t* d_visited, uint32_t* d_parent, uint32_t* d_depth, uint64_t frontier_edges,
+                 uint64_t* h_new_vertices, uint64_t* h_new_edges, int* h_was_sparse, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  Scratch sc;
+  EG_TRY(get_scratch(g, 0, &sc));
+  const bool sparse = choose_sparse(g, policy, frontier_edges, true);
+  EG_TRY(fill<uint64_t>(d_next, words_of(g->V), 0ull, s));
+  prof_begin(s);
+  if (sparse) {
+    uint32_t fn = 0;
+    EG_TRY(make_list(g, d_frontier, sc, &fn, s));
+    if (fn)
+      hipLaunchKernelGGL(k_bfs_push, dim3(push_grid(fn)), dim3(kNT), 0, s, (const uint32_t*)sc.flist, fn,
+                         (const uint32_t*)g->e_out_ptr.as<uint32_t>(), (const uint32_t*)g->e_out_dst.as<uint32_t>(),
+                         (const uint64_t*)d_visited, d_parent, d_next);
+    prof_end(PK_BFS, frontier_edges * 4 + uint64_t(fn) * 8, s);
+  } else if (g->R) {
+    hipLaunchKernelGGL(k_bfs_pull, dim3(grid_for(g->R, kNT, 65536)), dim3(kNT), 0, s, g->R,
+                       (const uint32_t*)g->e_row_start.as<uint32_t>(), (const uint32_t*)g->e_row_dst.as<uint32_t>(),
+                       (const uint32_t*)g->e_csc_src.as<uint32_t>(), d_frontier, (const uint64_t*)d_visited, g->V,
+                       d_parent, d_next);
+    prof_end(PK_BFS, g->R * 12, s);
+  }
+  EG_HIP(hipGetLastError());
+  if (h_was_sparse) *h_was_sparse = sparse ? 1 : 0;
+  return level_end(g, d_next, d_visited, d_depth, level + 1, sc, h_new_vertices, h_new_edges, s);
+}
+
+int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t* d_depth, uint32_t* h_levels,
+           uint64_t* h_edges, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  if (root >= g->V) return fail(EG_ERR_ARG, "root %u >= vertices_num", root);
+  if (!d_parent) return fail(EG_ERR_ARG, "d_parent is NULL");
+  const uint32_t V = g->V;
+  uint64_t* fr = g->e_bm[0].as<uint64_t>();
+  uint64_t* nx = g->e_bm[1].as<uint64_t>();
+  uint64_t* vis = g->e_bm[2].as<uint64_t>();
+  EG_TRY(fill<uint32_t>(d_parent, V, V, s));
+  EG_TRY(fill<uint32_t>(d_parent + root, 1, root, s));
+  if (d_depth) {
+    EG_TRY(fill<uint32_t>(d_depth, V, 0xffffffffu, s));
+    EG_TRY(fill<uint32_t>(d_depth + root, 1, 0u, s));
+  }
+  EG_TRY(fill<uint64_t>(fr, words_of(V), 0ull, s));
+  EG_TRY(fill<uint64_t>(fr + (root >> 6), 1, 1ull << (root & 63), s));
+  EG_TRY(fill<uint64_t>(vis, words_of(V), 0ull, s));
+  EG_TRY(fill<uint64_t>(vis + (root >> 6), 1, 1ull << (root & 63), s));
+  uint32_t deg_root = 0;
+  EG_HIP(hipMemcpyAsync(&deg_root, g->out_deg.as<uint32_t>() + root, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  uint64_t nv = 1, ne = deg_root, traversed = deg_root;
+  uint32_t level = 0;
+  while (nv > 0) {
+    EG_TRY(eg_bfs_level(g, policy, level, fr, nx, vis, d_parent, d_depth, ne, &nv, &ne, nullptr, stream));
+    traversed += ne;
+    ++level;
+    std::swap(fr, nx);
+  }
+  if (h_levels) *h_levels = level;
+  if (h_edges) *h_edges = traversed;
+  return EG_OK;
+}
+
+// ---- CC (toolkits/cc.cpp) ----
+int eg_cc_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t* d_next, uint32_t* d_label,
+                    uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  Scratch sc;
+  EG_TRY(get_scratch(g, 0, &sc));
+  EG_TRY(fill<uint64_t>(d_next, words_of(g->V), 0ull, s));
+  if (choose_sparse(g, policy, active_edges, false)) {
+    uint32_t fn = 0;
+    EG_TRY(make_list(g, d_active, sc, &fn, s));
+    if (fn)
+      hipLaunchKernelGGL(k_cc_push, dim3(push_grid(fn)), dim3(kNT), 0, s, (const uint32_t*)sc.flist, fn,
+                         (const uint32_t*)g->e_out_ptr.as<uint32_t>(), (const uint32_t*)g->e_out_dst.as<uint32_t>(),
+                         d_label, d_next);
+  } else if (g->E) {
+    hipLaunchKernelGGL(k_cc_pull, dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E,
+                       (const uint32_t*)g->e_csc_src.as<uint32_t>(), (const uint64_t*)g->e_rowflag.as<uint64_t>(),
+                       (const uint32_t*)g->e_wordrank.as<uint32_t>(), (const uint32_t*)g->e_row_dst.as<uint32_t>(),
+                       d_label, d_next);
+  }
+  EG_HIP(hipGetLastError());
+  return level_end(g, d_next, nullptr, nullptr, 0, sc, h_new_active, h_new_edges, s);
+}
+
+int eg_cc(eg_graph* g, int policy, uint32_t* d_label, uint32_t* h_iterations, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  if (!d_label) return fail(EG_ERR_ARG, "d_label is NULL");
+  uint64_t* a = g->e_bm[0].as<uint64_t>();
+  uint64_t* b = g->e_bm[1].as<uint64_t>();
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(g->V, kNT, 65536)), dim3(kNT), 0, s, d_label, g->V);
+  EG_HIP(hipGetLastError());
+  EG_TRY(eg_bitmap_fill(a, g->V, stream));
+  uint64_t nv = g->V, ne = g->E;
+  uint32_t it = 0;
+  while (nv > 0) {
+    EG_TRY(eg_cc_iteration(g, policy, a, b, d_label, ne, &nv, &ne, stream));
+    std::swap(a, b);
+    ++it;
+  }
+  if (h_iterations) *h_iterations = it;
+  return EG_OK;
+}
+
+// ---- SSSP (toolkits/sssp.cpp) ----
+int eg_sssp_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t* d_next, float* d_dist,
+                      uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  if (g->rec != 12) return fail(EG_ERR_STATE, "SSSP needs a graph of EdgeType<float> (12-byte) records");
+  Scratch sc;
+  EG_TRY(get_scratch(g, 0, &sc));
+  EG_TRY(fill<uint64_t>(d_next, words_of(g->V), 0ull, s));
+  if (choose_sparse(g, policy, active_edges, true)) {
+    uint32_t fn = 0;
+    EG_TRY(make_list(g, d_active, sc, &fn, s));
+    if (fn)
+      hipLaunchKernelGGL(k_sssp_push, dim3(push_grid(fn)), dim3(kNT), 0, s, (const uint32_t*)sc.flist, fn,
+                         (const uint32_t*)g->e_out_ptr.as<uint32_t>(), (const uint32_t*)g->e_out_dst.as<uint32_t>(),
+                         (const float*)g->e_out_w.as<float>(), d_dist, d_next);
+  } else if (g->E) {
+    hipLaunchKernelGGL(k_sssp_pull, dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E,
+                       (const uint32_t*)g->e_csc_src.as<uint32_t>(), (const float*)g->e_csc_w.as<float>(),
+                       (const uint64_t*)g->e_rowflag.as<uint64_t>(), (const uint32_t*)g->e_wordrank.as<uint32_t>(),
+                       (const uint32_t*)g->e_row_dst.as<uint32_t>(), d_dist, d_next);
+  }
+  EG_HIP(hipGetLastError());
+  return level_end(g, d_next, nullptr, nullptr, 0, sc, h_new_active, h_new_edges, s);
+}
+
+int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_iterations, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  if (g->rec != 12) return fail(EG_ERR_STATE, "SSSP needs a graph of EdgeType<float> (12-byte) records");
+  if (root >= g->V || !d_dist) return fail(EG_ERR_ARG, "bad root or NULL d_dist");
+  uint64_t* a = g->e_bm[0].as<uint64_t>();
+  uint64_t* b = g->e_bm[1].as<uint64_t>();
+  EG_TRY(fill<float>(d_dist, g->V, 1e9f, s));
+  EG_TRY(fill<float>(d_dist + root, 1, 0.0f, s));
+  EG_TRY(fill<uint64_t>(a, words_of(g->V), 0ull, s));
+  EG_TRY(fill<uint64_t>(a + (root >> 6), 1, 1ull << (root & 63), s));
+  uint32_t deg_root = 0;
+  EG_HIP(hipMemcpyAsync(&deg_root, g->out_deg.as<uint32_t>() + root, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  uint64_t nv = 1, ne = deg_root;
+  uint32_t it = 0;
+  while (nv > 0) {
+    EG_TRY(eg_sssp_iteration(g, policy, a, b, d_dist, ne, &nv, &ne, stream));
+    std::swap(a, b);
+    ++it;
+  }
+  if (h_iterations) *h_iterations = it;
+  return EG_OK;
+}
+
+}  // extern "C"

@@ -48,6 +48,7 @@ struct eg_graph {
   eg::DevBuf e_out_w;                // f32[E]
   eg::DevBuf e_scratch;              // frontier list, counters
   eg::DevBuf e_bm[3];                // three bitmaps of (V>>6)+1 words
+  eg::DevBuf e_pr;                   // PageRank work vectors
   double eng_prepare_ms = 0;
 
   // host copies of the small tables

@@ -166,12 +166,19 @@ int radix_sort(const R* in, R* out, R* tmp, uint64_t n, KeyFn key, int key_bits,
   for (int p = 0; p < plan.passes; ++p) {
     R* dst = ((plan.passes - 1 - p) % 2 == 0) ? out : tmp;
     const uint64_t nh = (uint64_t(1) << plan.bits[p]) * ntiles;
+    prof_begin(s);
     hipLaunchKernelGGL((k_radix_upsweep<R, KeyFn>), dim3(ntiles), dim3(kSortNT), 0, s, src, n, key, plan.shift[p],
                        plan.bits[p], hist, ntiles);
+    prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
     EG_HIP(hipGetLastError());
+    prof_begin(s);
     EG_TRY(exclusive_scan_u32(hist, hist, nh, scan_tmp, s));
+    prof_end(PK_SCAN, nh * 8, s);
+    prof_begin(s);
     hipLaunchKernelGGL((k_radix_downsweep<R, KeyFn>), dim3(ntiles), dim3(kSortNT), 0, s, src, dst, n, key,
                        plan.shift[p], plan.bits[p], (const uint32_t*)hist, ntiles);
+    // algorithmic traffic: every record read once and written once
+    prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
     EG_HIP(hipGetLastError());
     src = dst;
   }

@@ -51,6 +51,21 @@ int eg_memcpy_h2d(void* d_dst, const void* h_src, uint64_t bytes, void* stream);
 int eg_memcpy_d2h(void* h_dst, const void* d_src, uint64_t bytes, void* stream); /* synchronises */
 int eg_memcpy_d2d(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
 int eg_stream_synchronize(void* stream);
+/* fill_vertex_array (core/graph.hpp:289-294) for any 1/2/4/8-byte element:
+ * n copies of *h_value.  eg_device_iota: d[i] = i (cc.cpp:35-41's label). */
+int eg_device_fill(void* d_ptr, uint64_t n, const void* h_value, int value_bytes, void* stream);
+int eg_device_iota(uint32_t* d_ptr, uint32_t n, void* stream);
+
+/* ---- kernel timing (no reference counterpart; the reference prints
+ *      wall-clock timers only, core/graph.hpp:1367-1371) -------------------
+ * While enabled, selected launches are bracketed by HIP events on their own
+ * stream.  eg_profile_read returns, for a kernel class ("radix_upsweep",
+ * "radix_downsweep", "scan", "degree", "cindex", "csr_index", "pr_pull",
+ * "bfs"), the summed event time, the launch count and the summed algorithmic
+ * bytes of those launches.  Not thread-safe. */
+int eg_profile_enable(int on);
+int eg_profile_reset(void);
+int eg_profile_read(const char* kernel, double* total_ms, uint64_t* launches, uint64_t* bytes);
 
 /* ---- synthetic inputs (no reference counterpart: the reference reads
  *      binary edge files, README.md:40-41; these write the same records) -- */
@@ -113,6 +128,79 @@ int eg_graph_csc(const eg_graph* g, int p, int s, const void** d_list, uint64_t*
  * defined: index[u] == index[u+1] for u without edges. */
 int eg_graph_csr(const eg_graph* g, int p, int s, const void** d_list, uint64_t* n_edges, const uint64_t** d_index,
                  const uint64_t** d_bitmap);
+
+/* ---- engine (Gemini process_vertices / process_edges) --------------------
+ * The reference's process_edges takes four std::function lambdas
+ * (core/graph.hpp:1473); a C ABI cannot carry them, so every signal/slot pair
+ * the toolkits use is one entry point.  Mode policy for process_edges
+ * (core/graph.hpp:1483-1498, repaired names):
+ *   EG_POLICY_REFERENCE  iterative programs dense, rooted programs sparse,
+ *                        undirected (CC) sparse iff active out-edges < E/20
+ *   EG_POLICY_AUTO       sparse iff active out-edges < E/20 for every program
+ *   EG_POLICY_PUSH / EG_POLICY_PULL force one mode.
+ * Bitmaps are core/bitmap.hpp's layout: u64 words, (V>>6)+1 of them. */
+enum eg_policy { EG_POLICY_REFERENCE = 0, EG_POLICY_AUTO = 1, EG_POLICY_PUSH = 2, EG_POLICY_PULL = 3 };
+enum eg_precision { EG_PR_FP64 = 0, EG_PR_FP32C = 1 };
+
+/* Derives the engine's view of the built arrays (SoA source/target ids,
+ * row-start bits, a CSR merged over sub-partitions).  Runs on the first
+ * engine call after a build; calling it directly separates its cost
+ * (the reference does the same work inside prep: tuned_chunks_dense,
+ * core/graph.hpp:1191-1228).  h_ms (optional) gets its wall time. */
+int eg_engine_prepare(eg_graph* g, double* h_ms, void* stream);
+
+/* BitmapType::clear / fill / set_bit (core/bitmap.hpp:36-65). */
+int eg_bitmap_clear(uint64_t* d_bm, uint32_t V, void* stream);
+int eg_bitmap_fill(uint64_t* d_bm, uint32_t V, void* stream);
+int eg_bitmap_set_bit(uint64_t* d_bm, uint32_t v, void* stream); /* synchronises */
+/* d_dst |= d_src (bfs.cpp:80-86's visited update); number of set bits
+ * (process_vertices returning 1 per active vertex); sum of out-degrees over
+ * the set bits (the mode switch input, core/graph.hpp:1484-1490). */
+int eg_bitmap_or(uint64_t* d_dst, const uint64_t* d_src, uint32_t V, void* stream);
+int eg_bitmap_count(const uint64_t* d_bm, uint32_t V, uint64_t* h_count, void* stream); /* synchronises */
+int eg_bitmap_degree_sum(eg_graph* g, const uint64_t* d_bm, uint64_t* h_sum, void* stream); /* synchronises */
+
+/* PageRank, toolkits/pagerank.cpp:26-103.  eg_pr_init: process_vertices
+ * x = 1/deg (1 if deg == 0), also written as float to d_x32 if non-NULL.
+ * eg_pr_pull: process_edges dense step, d_next = sum over in-edges of x[src]
+ * (EG_PR_FP64: d_x is double*; EG_PR_FP32C: d_x is float* and the per-vertex
+ * sums are compensated FP32, handed over in FP64).  eg_pr_apply:
+ * process_vertices next = 0.15 + 0.85*next, divided by deg unless last;
+ * adds sum |next - x| * deg to *d_delta (device double, not divided by V);
+ * writes a float copy to d_next32 if non-NULL.
+ * eg_pagerank runs the whole loop; h_deltas (optional) gets iterations+1
+ * values, delta(i) of the reference (already divided by V). */
+int eg_pr_init(eg_graph* g, double* d_x, float* d_x32, void* stream);
+int eg_pr_pull(eg_graph* g, int precision, const void* d_x, double* d_next, void* stream);
+int eg_pr_apply(eg_graph* g, const double* d_x, double* d_next, float* d_next32, int last, double* d_delta,
+                void* stream);
+int eg_pagerank(eg_graph* g, int iterations, int precision, double* d_out, double* h_deltas, void* stream);
+
+/* BFS, toolkits/bfs.cpp:22-112.  Parent rule (no reference counterpart: the
+ * reference's CAS race leaves it open): the smallest frontier id among a
+ * vertex's in-neighbours.  eg_bfs_level runs one process_edges +
+ * process_vertices step: d_next = newly reached vertices, parents set,
+ * d_visited |= d_next, d_depth (optional) = level+1 for them;
+ * frontier_edges = sum of out-degrees of the frontier (mode switch input);
+ * returns the new vertex count and their out-edge sum.  eg_bfs runs from
+ * root; h_edges gets the number of out-edges of reached vertices. */
+int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontier, uint64_t* d_next,
+                 uint64_t* d_visited, uint32_t* d_parent, uint32_t* d_depth, uint64_t frontier_edges,
+                 uint64_t* h_new_vertices, uint64_t* h_new_edges, int* h_was_sparse, void* stream);
+int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t* d_depth, uint32_t* h_levels,
+           uint64_t* h_edges, void* stream);
+
+/* CC, toolkits/cc.cpp:22-112 (graph built with symmetrize=1): label =
+ * smallest id of the component.  One iteration or the whole loop. */
+int eg_cc_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t* d_next, uint32_t* d_label,
+                    uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream);
+int eg_cc(eg_graph* g, int policy, uint32_t* d_label, uint32_t* h_iterations, void* stream);
+
+/* SSSP, toolkits/sssp.cpp:24-108 (12-byte records): float distances, 1e9
+ * for unreached, relax = dist[src] + w in one round-to-nearest fp32 add. */
+int eg_sssp_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t* d_next, float* d_dist,
+                      uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream);
+int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_iterations, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,108 @@
+"""GPU parity of the engine: BFS parents/depths, CC labels and SSSP distances
+bit-exact against the oracle; PageRank within a stated L1 bound."""
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+PR_FP64_TOL = 1e-12    # relative L1, FP64 GPU vs FP64 oracle (only summation order differs)
+PR_FP32C_TOL = 1e-6    # relative L1, compensated FP32 vs FP64 oracle
+
+
+@pytest.fixture(scope="module")
+def eg():
+    import endgraph_amd
+    endgraph_amd.lib()
+    return endgraph_amd
+
+
+def _pair(eg, oracle_lib, edges, V, P, S, sym=False, eta=1, m=1):
+    buf = eg.DeviceBuffer.from_array(edges)
+    g = eg.Graph(V, P, S, eta, m, edges.dtype.itemsize)
+    g.build(buf.ptr, edges.shape[0], sym)
+    og = oracle_lib.OracleGraph(edges, V, P=P, S=S, eta=eta, m=m, symmetrize=sym, threads=8)
+    return g, og, buf
+
+
+def _rmat(eg, scale, ef=16, seed=1, rec=8, wlo=0.0, whi=1.0):
+    V, E = 1 << scale, ef << scale
+    buf = eg.DeviceBuffer(E * rec)
+    eg.gen_rmat(buf.ptr, E, scale, seed=seed, rec_bytes=rec, w_lo=wlo, w_hi=whi)
+    return buf.download(eg.REC8 if rec == 8 else eg.REC12, E), V
+
+
+@pytest.mark.parametrize("scale,P,S", [(12, 1, 1), (14, 1, 1), (14, 8, 1), (13, 3, 2)])
+@pytest.mark.parametrize("policy", [0, 1])
+def test_bfs_bit_exact(eg, oracle_lib, scale, P, S, policy):
+    print(f"bfs scale={scale} P={P} S={S} policy={policy}", flush=True)
+    edges, V = _rmat(eg, scale)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, P, S)
+    root = int(np.argmax(og.out_degree()))
+    parent, depth, levels, _ = g.bfs(root, policy)
+    o_parent, o_depth, o_levels = og.bfs(root, policy)
+    assert np.array_equal(depth, o_depth)
+    assert np.array_equal(parent, o_parent)
+    assert levels == o_levels
+
+
+def test_bfs_pull_only_matches(eg, oracle_lib):
+    edges, V = _rmat(eg, 13, seed=4)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, 2, 1)
+    root = int(np.argmax(og.out_degree()))
+    parent, depth, _, _ = g.bfs(root, eg.POLICY_PULL)
+    o_parent, o_depth, _ = og.bfs(root, 0)
+    assert np.array_equal(depth, o_depth) and np.array_equal(parent, o_parent)
+
+
+@pytest.mark.parametrize("scale,P,S", [(12, 1, 1), (14, 4, 1), (13, 2, 2)])
+@pytest.mark.parametrize("policy", [0, 2, 3])
+def test_cc_bit_exact(eg, oracle_lib, scale, P, S, policy):
+    print(f"cc scale={scale} P={P} S={S} policy={policy}", flush=True)
+    edges, V = _rmat(eg, scale, ef=2, seed=7)  # sparse: many components
+    g, og, _ = _pair(eg, oracle_lib, edges, V, P, S, sym=True)
+    label, _ = g.cc(policy)
+    o_label, _ = og.cc()
+    assert np.array_equal(label, o_label)
+
+
+@pytest.mark.parametrize("scale,P,S", [(12, 1, 1), (14, 1, 1), (13, 4, 1), (13, 2, 2)])
+@pytest.mark.parametrize("policy", [0, 1, 3])
+def test_sssp_bit_exact(eg, oracle_lib, scale, P, S, policy):
+    print(f"sssp scale={scale} P={P} S={S} policy={policy}", flush=True)
+    edges, V = _rmat(eg, scale, rec=12)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, P, S)
+    root = int(np.argmax(og.out_degree()))
+    dist, _ = g.sssp(root, policy)
+    o_dist, _ = og.sssp(root)
+    assert np.array_equal(dist.view(np.uint32), o_dist.view(np.uint32))
+
+
+@pytest.mark.parametrize("scale,P,S", [(12, 1, 1), (15, 1, 1), (14, 8, 1), (13, 2, 2)])
+def test_pagerank_fp64_and_fp32c(eg, oracle_lib, scale, P, S):
+    print(f"pagerank scale={scale} P={P} S={S}", flush=True)
+    edges, V = _rmat(eg, scale)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, P, S, m=20)
+    ref, ref_d = og.pagerank(20)
+    x64, d64 = g.pagerank(20, eg.PR_FP64)
+    x32, d32 = g.pagerank(20, eg.PR_FP32C)
+    l1_64 = np.abs(x64 - ref).sum() / np.abs(ref).sum()
+    l1_32 = np.abs(x32 - ref).sum() / np.abs(ref).sum()
+    print(f"  relative L1: fp64 {l1_64:.3e}  fp32c {l1_32:.3e}", flush=True)
+    assert l1_64 <= PR_FP64_TOL
+    assert l1_32 <= PR_FP32C_TOL
+    assert np.allclose(d64, ref_d, rtol=1e-9, atol=1e-15)
+
+
+def test_pagerank_dense_small_closed_form(eg, oracle_lib):
+    """Tiny graph: GPU FP64 PageRank against a dense matrix power iteration."""
+    rng = np.random.default_rng(3)
+    V = 300
+    edges = H.random_edges(rng, V, 3000)
+    buf = eg.DeviceBuffer.from_array(edges)
+    g = eg.Graph(V, 1, 1, 0, 1, 8)
+    g.build(buf.ptr, edges.shape[0])
+    x, _ = g.pagerank(15, eg.PR_FP64)
+    exp = H.pagerank_dense(V, edges["src"].astype(np.int64), edges["dst"].astype(np.int64), 15)
+    assert np.abs(x - exp).sum() / np.abs(exp).sum() < 1e-13

@@ -1,0 +1,85 @@
+"""The C++ drop-in toolkits (endgraph_amd/toolkits) on a binary edge file,
+checked against the oracle through their printed results."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "endgraph_amd", "toolkits", "_build")
+
+
+def run(tool, *args):
+    exe = os.path.join(BIN, tool)
+    if not os.path.exists(exe):
+        pytest.fail(f"toolkit {tool} not built (make -C endgraph_amd/toolkits)")
+    print(f"run {tool} {' '.join(map(str, args))}", flush=True)
+    p = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr + p.stdout
+    return p.stdout
+
+
+@pytest.fixture(scope="module")
+def graph_file(tmp_path_factory):
+    rng = np.random.default_rng(21)
+    V = 5000
+    edges = H.random_edges(rng, V, 40000, weighted=False)
+    wedges = H.random_edges(rng, V, 40000, weighted=True)
+    d = tmp_path_factory.mktemp("g")
+    f8, f12 = d / "g.bin", d / "gw.bin"
+    edges.tofile(f8)
+    wedges.tofile(f12)
+    return V, edges, wedges, str(f8), str(f12)
+
+
+@pytest.mark.parametrize("P,S,policy", [(1, 1, "reference"), (3, 2, "auto")])
+def test_bfs_toolkit(oracle_lib, graph_file, P, S, policy):
+    V, edges, _, f8, _ = graph_file
+    og = oracle_lib.OracleGraph(edges, V, P=P, S=S, eta=1, m=1)
+    root = int(np.argmax(og.out_degree()))
+    out = run("bfs", f8, V, root, 1, P, S, policy)
+    found = [int(x) for x in re.findall(r"found_vertices = (\d+)", out)]
+    parent, depth, _ = og.bfs(root)
+    assert found and all(f == int((parent < V).sum()) for f in found)
+
+
+@pytest.mark.parametrize("P,S", [(1, 1), (4, 1)])
+def test_cc_toolkit(oracle_lib, graph_file, P, S):
+    V, edges, _, f8, _ = graph_file
+    og = oracle_lib.OracleGraph(edges[:3000], V, P=P, S=S, eta=1, m=1, symmetrize=True)
+    small = os.path.join(os.path.dirname(f8), f"small{P}.bin")
+    edges[:3000].tofile(small)
+    out = run("cc", small, V, 0, 1, P, S)
+    comps = [int(x) for x in re.findall(r"components = (\d+)", out)]
+    label, _ = og.cc()
+    assert comps and all(c == len(np.unique(label)) for c in comps)
+
+
+def test_sssp_toolkit(oracle_lib, graph_file):
+    V, _, wedges, _, f12 = graph_file
+    og = oracle_lib.OracleGraph(wedges, V, P=2, S=1, eta=1, m=1)
+    root = int(np.argmax(og.out_degree()))
+    out = run("sssp", f12, V, root, 1, 2, 1)
+    dist, _ = og.sssp(root)
+    reach = dist < 1e9
+    vmax = int(np.flatnonzero(reach & (dist == dist[reach].max()))[0])
+    m = re.findall(r"distance\[(\d+)\]=([0-9.]+)", out)
+    assert m and all(int(v) == vmax and abs(float(x) - float(dist[vmax])) < 1e-5 for v, x in m)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_pagerank_toolkit(oracle_lib, graph_file, prec):
+    V, edges, _, f8, _ = graph_file
+    og = oracle_lib.OracleGraph(edges, V, P=2, S=1, eta=1, m=10)
+    pr, deltas = og.pagerank(10)
+    out = run("pagerank", f8, V, 10, 1, 2, 1, prec)
+    sums = [float(x) for x in re.findall(r"pr_sum=([0-9.]+)", out)]
+    assert sums and all(abs(s - pr.sum()) <= 1e-6 * pr.sum() + 1e-6 for s in sums)
+    d = [float(x) for x in re.findall(r"delta\(\d+\)=([0-9.]+)", out)[:10]]
+    assert np.allclose(d, deltas[:10], rtol=1e-5, atol=1e-6)
