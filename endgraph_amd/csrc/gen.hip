@@ -4,6 +4,7 @@
 // list is identical whatever the launch shape, and any slice can be
 // regenerated on the host for checking.
 #include <cmath>
+#include <vector>
 
 #include "common.hpp"
 
@@ -105,6 +106,28 @@ __global__ void k_powerlaw(void* out, uint64_t E, uint32_t V, int half, double b
   }
 }
 
+// Erdos-Renyi batch: record e of graph g (ebase[g] <= e < ebase[g+1]) joins
+// two ids drawn uniformly from [0, V_g) (32-bit draw scaled by V_g).
+template <int RB>
+__global__ void k_er_batch(void* out, uint64_t E, const uint64_t* __restrict__ ebase,
+                           const uint32_t* __restrict__ nv, int n, uint64_t key) {
+  for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += uint64_t(gridDim.x) * blockDim.x) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (ebase[mid] <= e)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint64_t V = nv[lo];
+    const uint64_t r = draw(key, e, 0);
+    const uint32_t src = uint32_t(((r & 0xffffffffull) * V) >> 32);
+    const uint32_t dst = uint32_t(((r >> 32) * V) >> 32);
+    store<RB>(out, e, src, dst, RB == 12 ? weight_of(key, e, 0.f, 1.f) : 0.f);
+  }
+}
+
 uint64_t threshold(double p) {
   double t = std::floor(p * 4294967296.0);
   if (t < 0) t = 0;
@@ -182,6 +205,36 @@ int eg_gen_powerlaw(void* d_edges, uint64_t E, uint32_t V, double alpha_out, dou
   else
     hipLaunchKernelGGL(k_powerlaw<12>, dim3(gen_grid(E)), dim3(256), 0, s, d_edges, E, V, half, bo, co, bi, ci, key);
   EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+int eg_gen_er_batch(void* d_edges, int n_graphs, const uint32_t* h_vertices, const uint64_t* h_edges, uint64_t seed,
+                    int rec_bytes, void* stream) {
+  if (n_graphs < 1 || !h_vertices || !h_edges) return fail(EG_ERR_ARG, "eg_gen_er_batch: bad graph list");
+  if (rec_bytes != 8 && rec_bytes != 12) return fail(EG_ERR_ARG, "rec_bytes must be 8 or 12");
+  std::vector<uint64_t> eb(n_graphs + 1, 0);
+  for (int g = 0; g < n_graphs; ++g) {
+    if (h_vertices[g] == 0 && h_edges[g] > 0) return fail(EG_ERR_ARG, "graph %d has edges but no vertices", g);
+    eb[g + 1] = eb[g] + h_edges[g];
+  }
+  const uint64_t E = eb[n_graphs];
+  if (!E) return EG_OK;
+  if (!d_edges) return fail(EG_ERR_ARG, "eg_gen_er_batch: d_edges is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevBuf d_eb, d_nv;
+  EG_TRY(d_eb.resize((n_graphs + 1) * 8));
+  EG_TRY(d_nv.resize(n_graphs * 4));
+  EG_HIP(hipMemcpyAsync(d_eb.ptr, eb.data(), (n_graphs + 1) * 8, hipMemcpyHostToDevice, s));
+  EG_HIP(hipMemcpyAsync(d_nv.ptr, h_vertices, n_graphs * 4, hipMemcpyHostToDevice, s));
+  const uint64_t key = mix64(seed);
+  if (rec_bytes == 8)
+    hipLaunchKernelGGL(k_er_batch<8>, dim3(gen_grid(E)), dim3(256), 0, s, d_edges, E, d_eb.as<uint64_t>(),
+                       d_nv.as<uint32_t>(), n_graphs, key);
+  else
+    hipLaunchKernelGGL(k_er_batch<12>, dim3(gen_grid(E)), dim3(256), 0, s, d_edges, E, d_eb.as<uint64_t>(),
+                       d_nv.as<uint32_t>(), n_graphs, key);
+  EG_HIP(hipGetLastError());
+  EG_HIP(hipStreamSynchronize(s));  // the size tables are freed on return
   return EG_OK;
 }
 

@@ -81,6 +81,12 @@ int eg_gen_rmat(void* d_edges, uint64_t E, uint32_t scale, double a, double b, d
 int eg_gen_powerlaw(void* d_edges, uint64_t E, uint32_t V, double alpha_out, double alpha_in, double max_out,
                     double max_in, uint64_t seed, int rec_bytes, void* stream);
 
+/* Erdos-Renyi batch: n_graphs graphs back to back, graph g with
+ * h_vertices[g] vertices and h_edges[g] records of uniform local ids
+ * (rec_bytes 12 adds a weight uniform in [0,1)).  Synchronises. */
+int eg_gen_er_batch(void* d_edges, int n_graphs, const uint32_t* h_vertices, const uint64_t* h_edges, uint64_t seed,
+                    int rec_bytes, void* stream);
+
 /* ---- graph construction ------------------------------------------------ */
 /* Graph<EdgeData>(eta) (core/graph.hpp:202) with the partition/socket
  * counts of the MPI world / NUMA topology made explicit: P partitions (ranks)
@@ -128,6 +134,20 @@ int eg_graph_csc(const eg_graph* g, int p, int s, const void** d_list, uint64_t*
  * defined: index[u] == index[u+1] for u without edges. */
 int eg_graph_csr(const eg_graph* g, int p, int s, const void** d_list, uint64_t* n_edges, const uint64_t** d_index,
                  const uint64_t** d_bitmap);
+
+/* ---- batched CSR construction (no reference counterpart: one process of
+ *      the reference builds one graph) -------------------------------------
+ * n_graphs independent graphs stored back to back in d_edges (graph g: local
+ * ids in [0, h_vertices[g]), h_edges[g] records) are built together; graph
+ * g's result equals prepare_push_mode's CSR at P = S = 1
+ * (core/graph.hpp:1248-1355): records ordered by (src, input position) and a
+ * dense u64 row pointer of h_vertices[g]+1 entries.  Synchronises. */
+typedef struct eg_batch eg_batch;
+int eg_batch_csr_build(const void* d_edges, int rec_bytes, int n_graphs, const uint32_t* h_vertices,
+                       const uint64_t* h_edges, eg_batch** out, void* stream);
+int eg_batch_csr_get(const eg_batch* b, int g, const void** d_list, uint64_t* n_edges, const uint64_t** d_rowptr,
+                     uint32_t* n_vertices);
+int eg_batch_destroy(eg_batch* b);
 
 /* ---- engine (Gemini process_vertices / process_edges) --------------------
  * The reference's process_edges takes four std::function lambdas
