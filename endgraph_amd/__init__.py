@@ -78,6 +78,15 @@ def lib():
         "eg_graph_page_prefix": ([vp, pp], i32),
         "eg_graph_csc": ([vp, i32, i32, pp, C.POINTER(u64), pp, C.POINTER(u64)], i32),
         "eg_graph_csr": ([vp, i32, i32, pp, C.POINTER(u64), pp, pp], i32),
+        "eg_gen_er_batch": ([vp, i32, vp, vp, u64, i32, vp], i32),
+        "eg_batch_csr_build": ([vp, i32, i32, vp, vp, pp, vp], i32),
+        "eg_batch_csr_get": ([vp, i32, pp, C.POINTER(u64), pp, C.POINTER(u32)], i32),
+        "eg_batch_destroy": ([vp], i32),
+        "eg_device_fill": ([vp, u64, vp, i32, vp], i32),
+        "eg_device_iota": ([vp, u32, vp], i32),
+        "eg_bitmap_or": ([vp, vp, u32, vp], i32),
+        "eg_bitmap_count": ([vp, u32, C.POINTER(u64), vp], i32),
+        "eg_bitmap_degree_sum": ([vp, vp, C.POINTER(u64), vp], i32),
         "eg_profile_enable": ([i32], i32),
         "eg_profile_reset": ([], i32),
         "eg_profile_read": ([C.c_char_p, C.POINTER(f64), C.POINTER(u64), C.POINTER(u64)], i32),
@@ -193,6 +202,41 @@ def gen_powerlaw(d_edges: int, E: int, V: int, alpha_out=1.9, alpha_in=2.1, max_
                  seed=3, rec_bytes=8, stream=None):
     check(lib().eg_gen_powerlaw(C.c_void_p(d_edges), E, V, alpha_out, alpha_in, max_out, max_in, seed, rec_bytes,
                                 _stream(stream)))
+
+
+def gen_er_batch(d_edges: int, vertices, edges, seed=4, rec_bytes=8, stream=None):
+    v = np.ascontiguousarray(vertices, dtype=np.uint32)
+    e = np.ascontiguousarray(edges, dtype=np.uint64)
+    check(lib().eg_gen_er_batch(C.c_void_p(d_edges), int(v.size), v.ctypes.data_as(C.c_void_p),
+                                e.ctypes.data_as(C.c_void_p), seed, rec_bytes, _stream(stream)))
+
+
+class BatchCSR:
+    """Many independent graphs built together (eg_batch_csr_build); graph g's
+    CSR equals prepare_push_mode's at P = S = 1."""
+
+    def __init__(self, d_edges: int, vertices, edges, rec_bytes: int = 8, stream=None):
+        v = np.ascontiguousarray(vertices, dtype=np.uint32)
+        e = np.ascontiguousarray(edges, dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().eg_batch_csr_build(C.c_void_p(d_edges), rec_bytes, int(v.size), v.ctypes.data_as(C.c_void_p),
+                                       e.ctypes.data_as(C.c_void_p), C.byref(h), _stream(stream)))
+        self._h, self.n, self.rec = h, int(v.size), rec_bytes
+        self.rec_dtype = REC8 if rec_bytes == 8 else REC12
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().eg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def get(self, g: int):
+        lst, rp = C.c_void_p(), C.c_void_p()
+        n, nv = C.c_uint64(), C.c_uint32()
+        check(lib().eg_batch_csr_get(self._h, g, C.byref(lst), C.byref(n), C.byref(rp), C.byref(nv)))
+        return to_host(lst.value, self.rec_dtype, n.value), to_host(rp.value, np.uint64, nv.value + 1)
 
 
 # ----------------------------------------------------------------- graph --

@@ -66,38 +66,49 @@ __global__ void k_symmetrize(const R* __restrict__ in, R* __restrict__ out, uint
   }
 }
 
-// Degree histogram (core/graph.hpp:689-707) for both directions; ids >= V are
-// skipped and flagged.  For a symmetrised build it reads the input once and
-// counts each edge at both endpoints.
-template <class R>
-__global__ __launch_bounds__(kNT) void k_degree(const R* __restrict__ e, uint64_t n, uint32_t V, int sym,
-                                                 uint32_t* __restrict__ out_deg, uint32_t* __restrict__ in_deg,
-                                                 uint32_t* __restrict__ err) {
-  bool bad = false;
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
-    const R r = e[i];
-    if (r.src >= V || r.dst >= V) {
-      bad = true;
-      continue;
+// Degree histogram (core/graph.hpp:689-707, R5) from an array sorted by
+// the counted endpoint (within rank slices: a key that continues across a
+// slice boundary only splits its count in two).  A wave adds, per run of
+// equal keys inside its 64 records, the run's length: one atomic per run
+// and wave, at ascending addresses.
+template <class R, bool kSrc>
+__global__ __launch_bounds__(kNT) void k_run_degree(const R* __restrict__ a, uint64_t n, uint32_t* __restrict__ deg) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t nwords = ceil_div(n, 64);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  for (uint64_t wd = gw; wd < nwords; wd += nw) {
+    const uint64_t e = wd * 64 + lane;
+    const bool valid = e < n;
+    uint32_t k = 0;
+    bool head = false;
+    if (valid) {
+      k = kSrc ? a[e].src : a[e].dst;
+      head = e == 0 || k != (kSrc ? a[e - 1].src : a[e - 1].dst);
     }
-    atomicAdd(&out_deg[r.src], 1u);
-    if (sym)
-      atomicAdd(&out_deg[r.dst], 1u);
-    else
-      atomicAdd(&in_deg[r.dst], 1u);
+    const uint64_t H = __ballot(head);
+    const bool end = valid && (e == n - 1 || lane == 63 || ((H >> (lane + 1)) & 1ull));
+    if (end) {
+      const uint64_t hb = H & le;
+      const int start = hb ? 63 - __clzll((long long)hb) : 0;
+      atomicAdd(&deg[k], uint32_t(lane - start + 1));
+    }
   }
-  if (__ballot(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(err, 1u);
 }
 
 // page_amount (core/graph.hpp:718-741): one block per page.  The weight
 // (1+(P-1)/16)*deg + eta*(P-1 + m*deg) keeps the reference's integer types:
 // the first product and the bracket are 32-bit unsigned, the rest 64-bit.
-__global__ __launch_bounds__(kNT) void k_page_weights(const uint32_t* __restrict__ deg, uint32_t V, uint32_t f1,
+// Also writes each page's out- and in-degree sums (edges owned per page).
+__global__ __launch_bounds__(kNT) void k_page_weights(const uint32_t* __restrict__ deg,
+                                                       const uint32_t* __restrict__ in_deg, uint32_t V, uint32_t f1,
                                                        uint32_t pm1, uint32_t m, uint64_t eta,
-                                                       uint64_t* __restrict__ page_w, uint32_t pages) {
-  __shared__ uint64_t wt[kNT / kWave];
+                                                       uint64_t* __restrict__ page_w, uint32_t pages,
+                                                       uint64_t* __restrict__ page_out, uint64_t* __restrict__ page_in) {
+  __shared__ uint64_t wt[3][kNT / kWave];
   const uint32_t pg = blockIdx.x;
-  uint64_t acc = 0;
+  uint64_t acc = 0, so = 0, si = 0;
   for (uint32_t k = threadIdx.x; k < kPageSize; k += kNT) {
     const uint32_t v = pg * kPageSize + k;
     if (v < V) {
@@ -105,15 +116,29 @@ __global__ __launch_bounds__(kNT) void k_page_weights(const uint32_t* __restrict
       const uint32_t a = f1 * d;
       const uint32_t b = pm1 + m * d;
       acc += uint64_t(a) + eta * uint64_t(b);
+      so += d;
+      si += in_deg[v];
     }
   }
   acc = wave_reduce(acc, OpSum{});
-  if ((threadIdx.x & (kWave - 1)) == 0) wt[threadIdx.x / kWave] = acc;
+  so = wave_reduce(so, OpSum{});
+  si = wave_reduce(si, OpSum{});
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    wt[0][threadIdx.x / kWave] = acc;
+    wt[1][threadIdx.x / kWave] = so;
+    wt[2][threadIdx.x / kWave] = si;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint64_t t = 0;
-    for (int i = 0; i < kNT / kWave; ++i) t += wt[i];
+    uint64_t t = 0, to = 0, ti = 0;
+    for (int i = 0; i < kNT / kWave; ++i) {
+      t += wt[0][i];
+      to += wt[1][i];
+      ti += wt[2][i];
+    }
     page_w[pg] = t;
+    page_out[pg] = to;
+    page_in[pg] = ti;
     if (pg == pages - 1) page_w[pages] = 0;  // slot that becomes the total after the scan
   }
 }
@@ -216,26 +241,23 @@ __global__ void k_qmap(const eg_graph::Meta* __restrict__ meta, uint32_t pages, 
   qmap[pg] = uint16_t(p * S + s);
 }
 
-// Edges per sub-partition: CSC owner = src (out-degree), CSR owner = dst
-// (in-degree).  A wave covers 64 consecutive ids of one page, so one q.
-__global__ __launch_bounds__(kNT) void k_q_counts(const uint32_t* __restrict__ out_deg,
-                                                   const uint32_t* __restrict__ in_deg, uint32_t V,
-                                                   const uint16_t* __restrict__ qmap, eg_graph::Meta* meta) {
-  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < V; base += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t v = base + threadIdx.x;
-    uint64_t a = 0, b = 0;
-    if (v < V) {
-      a = out_deg[v];
-      b = in_deg[v];
-    }
-    a = wave_reduce(a, OpSum{});
-    b = wave_reduce(b, OpSum{});
-    const uint64_t v0 = base + (threadIdx.x & ~uint64_t(kWave - 1));
-    if ((threadIdx.x & (kWave - 1)) == 0 && v0 < V) {
-      const int q = qmap[v0 >> kPageShift];
-      if (a) atomicAdd((unsigned long long*)&meta->q_cnt_csc[q], (unsigned long long)a);
-      if (b) atomicAdd((unsigned long long*)&meta->q_cnt_csr[q], (unsigned long long)b);
-    }
+// Edges per sub-partition (CSC owner = src: out-degree; CSR owner = dst:
+// in-degree) from the per-page sums; one block, LDS counters.
+__global__ __launch_bounds__(kNT) void k_q_counts(const uint64_t* __restrict__ page_out,
+                                                   const uint64_t* __restrict__ page_in, uint32_t pages,
+                                                   const uint16_t* __restrict__ qmap, int Q, eg_graph::Meta* meta) {
+  __shared__ unsigned long long co[256], ci[256];
+  for (int q = threadIdx.x; q < 256; q += kNT) co[q] = ci[q] = 0;
+  __syncthreads();
+  for (uint32_t pg = threadIdx.x; pg < pages; pg += kNT) {
+    const int q = qmap[pg];
+    atomicAdd(&co[q], (unsigned long long)page_out[pg]);
+    atomicAdd(&ci[q], (unsigned long long)page_in[pg]);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < Q; q += kNT) {
+    meta->q_cnt_csc[q] = co[q];
+    meta->q_cnt_csr[q] = ci[q];
   }
 }
 
@@ -253,6 +275,28 @@ __global__ void k_q_offsets(eg_graph::Meta* meta, int Q) {
 }
 
 // ----------------------------------------------------------- sort keys --
+// First sort of a build: key = one endpoint; a record with an id >= V sets
+// *err and sorts as V-1, so every access stays in bounds until the check.
+template <bool kSrc>
+struct KeyChecked {
+  uint32_t V;
+  uint32_t* err;
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const {
+    if (r.src >= V || r.dst >= V) atomicOr(err, 1u);
+    const uint32_t k = kSrc ? r.src : r.dst;
+    return k < V ? k : V - 1;
+  }
+};
+// Stable pass on the sub-partition of the owner only.
+template <bool kPull>
+struct KeyQOnly {
+  const uint16_t* qmap;
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const {
+    return qmap[(kPull ? r.src : r.dst) >> kPageShift];
+  }
+};
 struct KeySrc {
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.src; }
@@ -398,6 +442,37 @@ unsigned grid_for(uint64_t n, uint64_t per_block = kNT, uint64_t cap = 65536) {
   return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, per_block), cap)));
 }
 
+template <class R, class KeyFn>
+int rank_sorts(const R* L, R* out, R* tmp, uint64_t E0, int P, uint64_t mult, KeyFn key, int bits, uint32_t* hist,
+               uint32_t* stmp, hipStream_t s) {
+  const uint64_t base = E0 / P;
+  for (int r = 0; r < P; ++r) {
+    const uint64_t off = mult * base * r;
+    const uint64_t n_r = mult * (base + (r == P - 1 ? E0 % P : 0));
+    EG_TRY(radix_sort(L + off, out + off, tmp + off, n_r, key, bits, hist, stmp, s));
+  }
+  return EG_OK;
+}
+
+template <class R, bool kSrc>
+int run_degree(const R* a, uint64_t n, uint32_t* deg, hipStream_t s) {
+  if (!n) return EG_OK;
+  prof_begin(s);
+  hipLaunchKernelGGL((k_run_degree<R, kSrc>), dim3(grid_for(ceil_div(n, 64), kNT / kWave, 65536)), dim3(kNT), 0, s, a,
+                     n, deg);
+  prof_end(PK_DEGREE, n * sizeof(R), s);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+// Build order (every sort is stable; see the file comment for the target
+// orders):
+//   A = rank slices sorted by src   (P = 1: the whole list)   -> out-degree
+//   B = rank slices sorted by dst   (P = 1: the whole list)   -> in-degree
+//   page weights, CPBT, sub-partition map
+//   CSC = A sorted by (q(src), dst)  (P = 1: B sorted by q(src))
+//   CSR = B sorted by (q(dst), src)  (P = 1: A sorted by q(dst))
+// At P = 1 and S = 1 the q pass is the identity and A, B are the results.
 template <class R>
 int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) {
   const uint32_t V = g->V;
@@ -412,23 +487,25 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   g->vbits = std::max(1, counting_bits(V - 1));
   g->qbits = counting_bits(uint64_t(Q - 1));
   const uint64_t pages = g->pages;
+  const uint64_t Emax = std::max<uint64_t>(E, 1);
 
   // --- buffers (grow-only; reused by repeated builds) ---
   EG_TRY(g->out_deg.resize(uint64_t(V) * 4));
   EG_TRY(g->in_deg.resize(uint64_t(V) * 4));
-  EG_TRY(g->page_prefix.resize((pages + 1) * 8));
+  EG_TRY(g->page_prefix.resize((pages + 1) * 8 + 2 * pages * 8));
   EG_TRY(g->meta.resize(sizeof(eg_graph::Meta)));
   EG_TRY(g->qmap.resize(pages * 2));
-  EG_TRY(g->csc_list.resize(std::max<uint64_t>(E, 1) * rb));
-  EG_TRY(g->csr_list.resize(std::max<uint64_t>(E, 1) * rb));
+  EG_TRY(g->csc_list.resize(Emax * rb));
+  EG_TRY(g->csr_list.resize(Emax * rb));
   const uint64_t cidx_cap = std::min<uint64_t>(E, uint64_t(Q) * V) + Q;
   EG_TRY(g->csc_cidx.resize(cidx_cap * sizeof(CIdx)));
   const uint64_t idx_n = uint64_t(Q) * (V + 1);
   const uint64_t words = (V >> 6) + 1;
   EG_TRY(g->csr_index.resize(idx_n * 8));
   EG_TRY(g->csr_bitmap.resize(uint64_t(Q) * words * 8));
-  EG_TRY(g->t1.resize(std::max<uint64_t>(E, 1) * rb));
-  EG_TRY(g->t2.resize(std::max<uint64_t>(E, 1) * rb));
+  EG_TRY(g->t1.resize(Emax * rb));
+  EG_TRY(g->t2.resize(Emax * rb));
+  if (P > 1 || Q > 1) EG_TRY(g->t3.resize(Emax * rb));
   const uint64_t hist_n = std::max<uint64_t>(radix_hist_elems(E), 1);
   EG_TRY(g->hist.resize(hist_n * 4));
   const uint64_t scan_n = std::max<uint64_t>({scan_tmp_elems(hist_n), scan_tmp_elems(idx_n), scan_tmp_elems(pages + 1),
@@ -436,30 +513,16 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   EG_TRY(g->scan_tmp.resize(scan_n * 8));
   EG_TRY(g->tile_cnt.resize((ceil_div(E, kIdxTile) + 2) * 4));
   EG_TRY(g->err.resize(4));
-  if (sym) EG_TRY(g->sym.resize(std::max<uint64_t>(E, 1) * rb));
+  if (sym) EG_TRY(g->sym.resize(Emax * rb));
 
   eg_graph::Meta* meta = g->dmeta();
   uint32_t* err = g->err.as<uint32_t>();
   uint32_t* out_deg = g->out_deg.as<uint32_t>();
   uint32_t* in_deg = g->in_deg.as<uint32_t>();
   uint16_t* qmap = g->qmap.as<uint16_t>();
-
-  // --- degree + id check; the only mid-build synchronisation ---
-  EG_TRY(fill<uint32_t>(err, 1, 0u, s));
-  EG_TRY(fill<uint32_t>(out_deg, V, 0u, s));
-  EG_TRY(fill<uint32_t>(in_deg, V, 0u, s));
-  if (E0) {
-    prof_begin(s);
-    hipLaunchKernelGGL((k_degree<R>), dim3(grid_for(E0, kNT * 8, 16384)), dim3(kNT), 0, s, d_in, E0, V, sym, out_deg,
-                       in_deg, err);
-    prof_end(PK_DEGREE, E0 * rb + 2ull * V * 4, s);
-    EG_HIP(hipGetLastError());
-  }
-  uint32_t h_err = 0;
-  EG_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  if (h_err) return fail(EG_ERR_RANGE, "edge list holds a vertex id >= vertices_num (%u)", V);
-  if (sym) EG_HIP(hipMemcpyAsync(in_deg, out_deg, uint64_t(V) * 4, hipMemcpyDeviceToDevice, s));
+  uint32_t* hist = g->hist.as<uint32_t>();
+  uint32_t* stmp32 = g->scan_tmp.as<uint32_t>();
+  const uint64_t mult = sym ? 2 : 1;
 
   const R* L = d_in;
   if (sym) {
@@ -469,10 +532,32 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
     L = g->sym.as<R>();
   }
 
+  // --- A, B: first sorts (they also validate every id: the only mid-build
+  // synchronisation) ---
+  EG_TRY(fill<uint32_t>(err, 1, 0u, s));
+  R* A = g->t1.as<R>();
+  R* B = g->t2.as<R>();
+  R* T = P > 1 ? g->t3.as<R>() : g->csc_list.as<R>();  // ping-pong of the first sorts
+  EG_TRY(rank_sorts(L, A, T, E0, P, mult, KeyChecked<true>{V, err}, g->vbits, hist, stmp32, s));
+  EG_TRY(rank_sorts(L, B, T, E0, P, mult, KeyChecked<false>{V, err}, g->vbits, hist, stmp32, s));
+  uint32_t h_err = 0;
+  EG_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  if (h_err) return fail(EG_ERR_RANGE, "edge list holds a vertex id >= vertices_num (%u)", V);
+
+  // --- degrees from runs ---
+  EG_TRY(fill<uint32_t>(out_deg, V, 0u, s));
+  EG_TRY(fill<uint32_t>(in_deg, V, 0u, s));
+  EG_TRY((run_degree<R, true>(A, E, out_deg, s)));
+  EG_TRY((run_degree<R, false>(B, E, in_deg, s)));
+
   // --- CPBT ---
   uint64_t* prefix = g->page_prefix.as<uint64_t>();
-  hipLaunchKernelGGL(k_page_weights, dim3(unsigned(pages)), dim3(kNT), 0, s, out_deg, V, uint32_t(1 + (P - 1) / 16),
-                     uint32_t(P - 1), g->m, g->eta, prefix, uint32_t(pages));
+  uint64_t* page_out = prefix + pages + 1;
+  uint64_t* page_in = page_out + pages;
+  hipLaunchKernelGGL(k_page_weights, dim3(unsigned(pages)), dim3(kNT), 0, s, (const uint32_t*)out_deg,
+                     (const uint32_t*)in_deg, V, uint32_t(1 + (P - 1) / 16), uint32_t(P - 1), g->m, g->eta, prefix,
+                     uint32_t(pages), page_out, page_in);
   EG_HIP(hipGetLastError());
   EG_TRY(exclusive_scan_u64(prefix, prefix, pages + 1, g->scan_tmp.as<uint64_t>(), s));
   EG_HIP(hipMemsetAsync(meta, 0, sizeof(eg_graph::Meta), s));
@@ -480,39 +565,26 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   EG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_qmap, dim3(grid_for(pages)), dim3(kNT), 0, s, meta, uint32_t(pages), P, S, qmap);
   EG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_q_counts, dim3(grid_for(V, kNT, 4096)), dim3(kNT), 0, s, out_deg, in_deg, V, qmap, meta);
+  hipLaunchKernelGGL(k_q_counts, dim3(1), dim3(kNT), 0, s, (const uint64_t*)page_out, (const uint64_t*)page_in,
+                     uint32_t(pages), (const uint16_t*)qmap, Q, meta);
   EG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_q_offsets, dim3(1), dim3(kWave), 0, s, meta, Q);
   EG_HIP(hipGetLastError());
 
-  uint32_t* hist = g->hist.as<uint32_t>();
-  uint32_t* stmp32 = g->scan_tmp.as<uint32_t>();
-  R* t1 = g->t1.as<R>();
-  R* t2 = g->t2.as<R>();
-  const uint64_t base = E0 / P;
-  const uint64_t mult = sym ? 2 : 1;
-
-  // --- both directions ---
-  for (int dir = 0; dir < 2; ++dir) {
-    const bool pull = dir == 0;
-    R* out = pull ? g->csc_list.as<R>() : g->csr_list.as<R>();
-    const R* src_list = L;
-    if (P > 1) {  // shuffle: stable sort of every rank slice by the owner id
-      for (int r = 0; r < P; ++r) {
-        const uint64_t off = mult * base * r;
-        const uint64_t n_r = mult * (base + (r == P - 1 ? E0 % P : 0));
-        if (pull)
-          EG_TRY(radix_sort(L + off, t1 + off, t2 + off, n_r, KeySrc{}, g->vbits, hist, stmp32, s));
-        else
-          EG_TRY(radix_sort(L + off, t1 + off, t2 + off, n_r, KeyDst{}, g->vbits, hist, stmp32, s));
-      }
-      src_list = t1;
-    }
+  // --- second sorts ---
+  if (P > 1) {
     const int kbits = g->qbits + g->vbits;
-    if (pull)
-      EG_TRY(radix_sort(src_list, out, P > 1 ? t2 : t1, E, KeyQ<true>{qmap, g->vbits}, kbits, hist, stmp32, s));
-    else
-      EG_TRY(radix_sort(src_list, out, P > 1 ? t2 : t1, E, KeyQ<false>{qmap, g->vbits}, kbits, hist, stmp32, s));
+    EG_TRY(radix_sort((const R*)A, g->csc_list.as<R>(), g->t3.as<R>(), E, KeyQ<true>{qmap, g->vbits}, kbits, hist,
+                      stmp32, s));
+    EG_TRY(radix_sort((const R*)B, g->csr_list.as<R>(), g->t3.as<R>(), E, KeyQ<false>{qmap, g->vbits}, kbits, hist,
+                      stmp32, s));
+  } else if (Q == 1) {
+    g->csr_list.swap(g->t1);  // A is the CSR, B the CSC
+    g->csc_list.swap(g->t2);
+  } else {
+    const int qb = std::max(1, g->qbits);
+    EG_TRY(radix_sort((const R*)B, g->csc_list.as<R>(), g->t3.as<R>(), E, KeyQOnly<true>{qmap}, qb, hist, stmp32, s));
+    EG_TRY(radix_sort((const R*)A, g->csr_list.as<R>(), g->t3.as<R>(), E, KeyQOnly<false>{qmap}, qb, hist, stmp32, s));
   }
 
   // --- compressed CSC index ---

@@ -136,6 +136,17 @@ struct DevBuf {
   ~DevBuf() { release(); }
   template <class T>
   T* as() const { return static_cast<T*>(ptr); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  void swap(DevBuf& o) {
+    void* p = ptr;
+    uint64_t b = bytes;
+    ptr = o.ptr;
+    bytes = o.bytes;
+    o.ptr = p;
+    o.bytes = b;
+  }
 };
 
 // ------------------------------------------------------------ profiling --

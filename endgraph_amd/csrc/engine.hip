@@ -57,7 +57,8 @@ int fill(T* p, uint64_t n, T v, hipStream_t s) {
 }
 
 // ------------------------------------------------------------- prepare --
-// SoA source (and weight) of every CSC edge, and the row-start bit per edge.
+// SoA source (and weight) of every CSC edge, and the row-start bit per edge
+// (a row = one compressed-index entry: a destination within a sub-partition).
 template <class R>
 __global__ __launch_bounds__(kNT) void k_prep_csc(const R* __restrict__ lst, uint64_t E, uint32_t* __restrict__ src,
                                                   float* __restrict__ w, uint64_t* __restrict__ rowflag,
@@ -169,6 +170,11 @@ int prepare_impl(eg_graph* g, hipStream_t s) {
   uint64_t scan_n = std::max<uint64_t>(scan_tmp_elems(nwords + 1), scan_tmp_elems(uint64_t(V) + 1));
   if (g->scan_tmp.bytes < scan_n * 8) EG_TRY(g->scan_tmp.resize(scan_n * 8));
 
+  // Pull side: the sub-partitions' CSC lists as built, one row per
+  // (sub-partition, destination).  Keeping the sub-partitions apart keeps
+  // each row's gathers inside one source range (better cache reuse than one
+  // merged row per destination, measured); rows of one destination in
+  // several sub-partitions combine with atomics.
   const R* csc = g->csc_list.as<R>();
   const R* csr = g->csr_list.as<R>();
   uint32_t* wordrank = g->e_wordrank.as<uint32_t>();
@@ -684,7 +690,7 @@ bool choose_sparse(const eg_graph* g, int policy, uint64_t active_edges, bool ro
 int pr_pull(eg_graph* g, int precision, const void* x, double* next, hipStream_t s) {
   EG_TRY(fill<double>(next, g->V, 0.0, s));
   if (!g->E) return EG_OK;
-  const int all_atomic = g->Q > 1;
+  const int all_atomic = g->Q > 1;  // a destination has a row per sub-partition
   prof_begin(s);
   if (precision == EG_PR_FP64)
     hipLaunchKernelGGL(k_pr_pull_f64, dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E, g->e_csc_src.as<uint32_t>(),

@@ -30,7 +30,7 @@ struct eg_graph {
   eg::DevBuf csr_index;              // u64[Q][V+1] (sub-partition-local positions)
   eg::DevBuf csr_bitmap;             // u64[Q][(V>>6)+1]
   eg::DevBuf sym;                    // symmetrised logical list
-  eg::DevBuf t1, t2;                 // sort ping-pong
+  eg::DevBuf t1, t2, t3;             // first-sort results and sort ping-pong
   eg::DevBuf hist, scan_tmp, tile_cnt, err;
 
   // Engine view, derived from the built arrays on the first engine call
