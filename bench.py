@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
     ap.add_argument("--no-engine", action="store_true", help="skip the BFS / PageRank figures")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default sort variant")
     ap.add_argument("--cpu-sample-edges", type=int, default=1 << 25)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     args = ap.parse_args()
@@ -85,6 +86,12 @@ def main():
     log("[bench] edges generated")
 
     g = eg.Graph(V, P, S, eta, m, 8)
+    variant = None
+    if not args.no_autotune:
+        sc, nt, bits, ms = g.autotune_sort(edges.ptr, E)  # untimed: every variant builds twice
+        variant = {"scheme": ["three-launch", "one-pass"][sc], "threads": nt, "tile_records": nt * 16,
+                   "digit_bits": bits, "autotune_best_ms": round(ms, 2)}
+        log(f"[bench] autotuned sort: {variant}")
     for i in range(args.warmup):
         g.build(edges.ptr, E)
         log(f"[bench] warmup build {i} done")
@@ -181,7 +188,8 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": desc, "vertices": V, "edges": E, "partitions": P, "sockets": S, "eta": eta,
-                   "m": m, "record_bytes": 8},
+                   "m": m, "record_bytes": 8, "sort_variant": variant or dict(zip(("scheme", "threads", "digit_bits"),
+                                                                                 eg.get_sort_variant(8)))},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -87,6 +87,10 @@ def lib():
         "eg_bitmap_or": ([vp, vp, u32, vp], i32),
         "eg_bitmap_count": ([vp, u32, C.POINTER(u64), vp], i32),
         "eg_bitmap_degree_sum": ([vp, vp, C.POINTER(u64), vp], i32),
+        "eg_set_sort_variant": ([i32, i32, i32, i32], i32),
+        "eg_get_sort_variant": ([i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], i32),
+        "eg_autotune_sort": ([vp, vp, u64, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(f64), vp],
+                             i32),
         "eg_profile_enable": ([i32], i32),
         "eg_profile_reset": ([], i32),
         "eg_profile_read": ([C.c_char_p, C.POINTER(f64), C.POINTER(u64), C.POINTER(u64)], i32),
@@ -171,6 +175,17 @@ def to_host(d_ptr: int, dtype, count: int, stream=None) -> np.ndarray:
 
 POLICY_REFERENCE, POLICY_AUTO, POLICY_PUSH, POLICY_PULL = 0, 1, 2, 3
 PR_FP64, PR_FP32C = 0, 1
+
+
+def set_sort_variant(rec_bytes: int, scheme: int, threads: int, digit_bits: int = 8):
+    check(lib().eg_set_sort_variant(rec_bytes, scheme, threads, digit_bits))
+
+
+def get_sort_variant(rec_bytes: int = 8):
+    """(scheme, threads, digit_bits) of the radix sort used for this record size."""
+    sc, nt, b = C.c_int(), C.c_int(), C.c_int()
+    check(lib().eg_get_sort_variant(rec_bytes, C.byref(sc), C.byref(nt), C.byref(b)))
+    return sc.value, nt.value, b.value
 
 
 def profile_enable(on: bool = True):
@@ -266,6 +281,13 @@ class Graph:
     # load_directed / load_undirected_from_directed (core/graph.hpp:1358, 629)
     def build(self, d_edges: int, E0: int, symmetrize: bool = False, stream=None):
         check(lib().eg_graph_build(self._h, C.c_void_p(d_edges), E0, int(bool(symmetrize)), _stream(stream)))
+
+    def autotune_sort(self, d_edges: int, E0: int, symmetrize: bool = False, stream=None):
+        """Time every sort variant on this input; keep the fastest (graph left built)."""
+        sc, nt, b, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(lib().eg_autotune_sort(self._h, C.c_void_p(d_edges), E0, int(bool(symmetrize)), C.byref(sc),
+                                     C.byref(nt), C.byref(b), C.byref(ms), _stream(stream)))
+        return sc.value, nt.value, b.value, ms.value
 
     def load_directed(self, d_edges: int, E0: int, stream=None):
         self.build(d_edges, E0, False, stream)

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "radix.hpp"
 
 namespace eg {
 
@@ -91,6 +92,11 @@ void drain() {
 
 bool prof_on() { return g_prof; }
 
+// Sort variant per record size (8 or 12 bytes); defaults chosen by measurement
+// on RMAT-24, replaced by eg_set_sort_variant / eg_autotune_sort.
+static SortVariant g_variant8{kThreeLaunch, 512, 9}, g_variant12{kThreeLaunch, 512, 9};
+SortVariant& sort_variant(int rec_bytes) { return rec_bytes <= 8 ? g_variant8 : g_variant12; }
+
 void prof_begin(hipStream_t s) {
   if (!g_prof) return;
   g_pending = take_event();
@@ -176,6 +182,26 @@ int eg_profile_read(const char* kernel, double* total_ms, uint64_t* launches, ui
     }
   }
   return fail(EG_ERR_ARG, "unknown kernel class '%s'", kernel);
+}
+
+int eg_set_sort_variant(int rec_bytes, int scheme, int threads, int digit_bits) {
+  if (rec_bytes != 8 && rec_bytes != 12) return fail(EG_ERR_ARG, "rec_bytes must be 8 or 12");
+  if (scheme != kThreeLaunch && scheme != kOnePass) return fail(EG_ERR_ARG, "scheme must be 0 or 1");
+  if (threads != 256 && threads != 512 && threads != 1024) return fail(EG_ERR_ARG, "threads must be 256, 512 or 1024");
+  if (rec_bytes == 12 && threads == 1024) return fail(EG_ERR_ARG, "12-byte records need threads <= 512 (LDS)");
+  if (digit_bits != 8 && digit_bits != 9) return fail(EG_ERR_ARG, "digit_bits must be 8 or 9");
+  if (digit_bits == 9 && threads != 512) return fail(EG_ERR_ARG, "9-bit digits need 512 threads");
+  sort_variant(rec_bytes) = SortVariant{scheme, threads, digit_bits};
+  return EG_OK;
+}
+
+int eg_get_sort_variant(int rec_bytes, int* scheme, int* threads, int* digit_bits) {
+  if (rec_bytes != 8 && rec_bytes != 12) return fail(EG_ERR_ARG, "rec_bytes must be 8 or 12");
+  const SortVariant v = sort_variant(rec_bytes);
+  if (scheme) *scheme = v.scheme;
+  if (threads) *threads = v.nt;
+  if (digit_bits) *digit_bits = v.bits;
+  return EG_OK;
 }
 
 int eg_stream_synchronize(void* stream) {

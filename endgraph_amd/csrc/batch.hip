@@ -12,7 +12,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "sort.hpp"
+#include "radix.hpp"
 
 struct eg_batch {
   int n = 0;
@@ -23,6 +23,7 @@ struct eg_batch {
   eg::DevBuf relabel, t2, out;         // records
   eg::DevBuf rowptr;                   // u64[Vtot + n]: graph g's pointers at vbase[g] + g
   eg::DevBuf hist, scan_tmp;
+  eg::RxWork sortw;
   uint32_t max_v = 0;
 };
 
@@ -117,9 +118,7 @@ int batch_build(eg_batch* b, const R* d_in, hipStream_t s) {
   EG_TRY(b->t2.resize(std::max<uint64_t>(E, 1) * sizeof(R)));
   EG_TRY(b->out.resize(std::max<uint64_t>(E, 1) * sizeof(R)));
   EG_TRY(b->rowptr.resize(nptr * 8));
-  const uint64_t hn = std::max<uint64_t>(radix_hist_elems(E), 1);
-  EG_TRY(b->hist.resize(hn * 4));
-  EG_TRY(b->scan_tmp.resize(std::max(scan_tmp_elems(hn), scan_tmp_elems(nptr)) * 8));
+  EG_TRY(b->scan_tmp.resize(scan_tmp_elems(nptr) * 8));
   DevBuf err;
   EG_TRY(err.resize(4));
   EG_HIP(hipMemsetAsync(err.ptr, 0, 4, s));
@@ -135,8 +134,10 @@ int batch_build(eg_batch* b, const R* d_in, hipStream_t s) {
   EG_HIP(hipStreamSynchronize(s));
   if (h_err) return fail(EG_ERR_RANGE, "a record's vertex id is outside its graph");
   const int bits = std::max(1, counting_bits(b->Vtot - 1));
-  EG_TRY(radix_sort(static_cast<const R*>(b->relabel.as<R>()), b->out.as<R>(), b->t2.as<R>(), E, KeyGlobalSrc{}, bits,
-                    b->hist.as<uint32_t>(), b->scan_tmp.as<uint32_t>(), s));
+  EG_TRY(b->sortw.resize_segs(1));
+  EG_HIP(hipMemsetAsync(b->sortw.err.ptr, 0, 4, s));
+  EG_TRY(rx_sort(static_cast<const R*>(b->relabel.as<R>()), b->out.as<R>(), b->t2.as<R>(), std::vector<uint64_t>{0, E},
+                 KeyGlobalSrc{}, bits, b->sortw, s));
   uint64_t* rp = b->rowptr.as<uint64_t>();
   EG_HIP(hipMemsetAsync(rp, 0, nptr * 8, s));
   if (E) {
@@ -150,7 +151,10 @@ int batch_build(eg_batch* b, const R* d_in, hipStream_t s) {
   hipLaunchKernelGGL((k_batch_localize<R>), dim3(grid_for(std::max(E, nptr))), dim3(kNT), 0, s, b->out.as<R>(), E, rp,
                      nptr, vb, eb, b->n);
   EG_HIP(hipGetLastError());
+  uint32_t sort_err = 0;
+  EG_HIP(hipMemcpyAsync(&sort_err, b->sortw.err.ptr, 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
+  if (sort_err) return fail(EG_ERR_HIP, "radix sort look-back exceeded its spin limit");
   return EG_OK;
 }
 

@@ -17,10 +17,10 @@
 //   indexes      compressed CSC index (:1147-1188), dense CSR index and
 //                bitmap (:1312-1353).
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 #include "graph_state.hpp"
-#include "sort.hpp"
 
 namespace eg {
 namespace {
@@ -68,34 +68,14 @@ __global__ void k_symmetrize(const R* __restrict__ in, R* __restrict__ out, uint
 
 // Degree histogram (core/graph.hpp:689-707, R5) from an array sorted by
 // the counted endpoint (within rank slices: a key that continues across a
-// slice boundary only splits its count in two).  A wave adds, per run of
-// equal keys inside its 64 records, the run's length: one atomic per run
-// and wave, at ascending addresses.
-template <class R, bool kSrc>
-__global__ __launch_bounds__(kNT) void k_run_degree(const R* __restrict__ a, uint64_t n, uint32_t* __restrict__ deg) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t nwords = ceil_div(n, 64);
-  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-  for (uint64_t wd = gw; wd < nwords; wd += nw) {
-    const uint64_t e = wd * 64 + lane;
-    const bool valid = e < n;
-    uint32_t k = 0;
-    bool head = false;
-    if (valid) {
-      k = kSrc ? a[e].src : a[e].dst;
-      head = e == 0 || k != (kSrc ? a[e - 1].src : a[e - 1].dst);
-    }
-    const uint64_t H = __ballot(head);
-    const bool end = valid && (e == n - 1 || lane == 63 || ((H >> (lane + 1)) & 1ull));
-    if (end) {
-      const uint64_t hb = H & le;
-      const int start = hb ? 63 - __clzll((long long)hb) : 0;
-      atomicAdd(&deg[k], uint32_t(lane - start + 1));
-    }
-  }
-}
+// slice boundary only splits its count in two).  Each wave owns 512
+// consecutive records and adds, per run inside them, the run's length: one
+// atomic per run and chunk, at ascending addresses.
+template <bool kSrc>
+struct KeyEnd {
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const { return kSrc ? r.src : r.dst; }
+};
 
 // page_amount (core/graph.hpp:718-741): one block per page.  The weight
 // (1+(P-1)/16)*deg + eta*(P-1 + m*deg) keeps the reference's integer types:
@@ -319,74 +299,122 @@ struct KeyQ {
 };
 
 // ------------------------------------------------------------- indexes --
-constexpr int kIdxIPT = 16;
-constexpr int kIdxTile = kNT * kIdxIPT;
+// Run-boundary kernels over a sorted list: wave g owns the 512 consecutive
+// records [g*512, g*512+512) in 8 rounds of 64.  All loads are issued before
+// use, neighbours come from wave shuffles (or one extra load at each chunk
+// end), and the sub-partition and vertex are the composite key's two fields.
+constexpr int kCR = 8;                 // rounds per wave
+constexpr int kChunk = kCR * kWave;    // records per wave
+constexpr uint64_t kNoKey = ~0ull;     // never a real key
 
-// Count run starts (key changes) per tile.
+struct ChunkKeys {
+  uint64_t k[kCR];
+  uint64_t prev, next;  // keys just before / after the chunk (kNoKey at the ends)
+};
+
 template <class R, class KeyFn>
-__global__ __launch_bounds__(kNT) void k_run_count(const R* __restrict__ a, uint64_t n, KeyFn key,
-                                                   uint32_t* __restrict__ tile_cnt, uint32_t ntiles) {
-  __shared__ uint32_t wt[kNT / kWave];
-  const uint64_t base = uint64_t(blockIdx.x) * kIdxTile;
-  uint32_t c = 0;
-#pragma unroll 4
-  for (int k = 0; k < kIdxIPT; ++k) {
-    const uint64_t i = base + uint64_t(k) * kNT + threadIdx.x;
-    if (i < n) c += (i == 0 || key(a[i]) != key(a[i - 1])) ? 1u : 0u;
+__device__ __forceinline__ void chunk_keys(const R* __restrict__ a, uint64_t n, uint64_t c0, KeyFn key,
+                                           ChunkKeys& c) {
+  const int lane = threadIdx.x & (kWave - 1);
+  R r[kCR];
+#pragma unroll
+  for (int j = 0; j < kCR; ++j) {
+    const uint64_t i = c0 + uint64_t(j) * kWave + lane;
+    r[j] = a[i < n ? i : n - 1];
   }
-  c = wave_reduce(c, OpSum{});
-  if ((threadIdx.x & (kWave - 1)) == 0) wt[threadIdx.x / kWave] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int i = 0; i < kNT / kWave; ++i) t += wt[i];
-    tile_cnt[blockIdx.x] = t;
-    if (blockIdx.x == ntiles - 1) tile_cnt[ntiles] = 0;
-  }
+  const R rp = a[c0 > 0 ? c0 - 1 : 0];
+  const uint64_t nx = c0 + kChunk;
+  const R rn = a[nx < n ? nx : n - 1];
+#pragma unroll
+  for (int j = 0; j < kCR; ++j) c.k[j] = key(r[j]);
+  c.prev = c0 > 0 ? key(rp) : kNoKey;
+  c.next = nx < n ? key(rn) : kNoKey;
 }
 
-// Compressed CSC index entries: the r-th run start overall, lying in
+// head: first record of a run (key differs from the previous record)
+__device__ __forceinline__ bool chunk_head(const ChunkKeys& c, int j, uint64_t i, uint64_t n) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t up = shfl_up_t(c.k[j], 1);
+  const uint64_t last_prev = j > 0 ? shfl_t(c.k[j > 0 ? j - 1 : 0], kWave - 1) : c.prev;
+  const uint64_t pk = lane > 0 ? up : last_prev;
+  return i < n && pk != c.k[j];
+}
+
+// tail: last record of a run (key differs from the next record)
+__device__ __forceinline__ bool chunk_tail(const ChunkKeys& c, int j, uint64_t i, uint64_t n) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t dn = __shfl_down(c.k[j], 1, kWave);
+  const uint64_t first_next = j < kCR - 1 ? shfl_t(c.k[j < kCR - 1 ? j + 1 : j], 0) : c.next;
+  const uint64_t nk = lane < kWave - 1 ? dn : first_next;
+  return i < n && (i == n - 1 || nk != c.k[j]);
+}
+
+#define EG_FOR_CHUNKS(n)                                                                        \
+  const uint64_t n_chunks_ = ceil_div((n), kChunk);                                             \
+  const uint64_t gw_ = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;                \
+  const uint64_t nw_ = uint64_t(gridDim.x) * blockDim.x / kWave;                                \
+  for (uint64_t chunk = gw_; chunk < n_chunks_; chunk += nw_)
+
+// Run heads per wave chunk (for the compressed index).
+template <class R, class KeyFn>
+__global__ __launch_bounds__(kNT) void k_run_count(const R* __restrict__ a, uint64_t n, KeyFn key,
+                                                   uint32_t* __restrict__ cnt, uint64_t nchunks) {
+  const int lane = threadIdx.x & (kWave - 1);
+  EG_FOR_CHUNKS(n) {
+    const uint64_t c0 = chunk * kChunk;
+    ChunkKeys c;
+    chunk_keys(a, n, c0, key, c);
+    uint32_t h = 0;
+#pragma unroll
+    for (int j = 0; j < kCR; ++j) h += uint32_t(__popcll(__ballot(chunk_head(c, j, c0 + uint64_t(j) * kWave + lane, n))));
+    if (lane == 0) cnt[chunk] = h;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt[nchunks] = 0;
+}
+
+// Compressed CSC index entries: the r-th run head overall, lying in
 // sub-partition q, lands at r + q (one sentinel slot per earlier q).
 template <class R>
 __global__ __launch_bounds__(kNT) void k_cindex_write(const R* __restrict__ a, uint64_t n, KeyQ<true> key,
-                                                      const uint32_t* __restrict__ tile_off,
+                                                      const uint32_t* __restrict__ off,
                                                       const eg_graph::Meta* __restrict__ meta,
                                                       CIdx* __restrict__ out) {
-  __shared__ uint32_t wt[kNT / kWave];
-  const uint64_t base = uint64_t(blockIdx.x) * kIdxTile;
-  uint32_t run = tile_off[blockIdx.x];
-  for (int k = 0; k < kIdxIPT; ++k) {
-    const uint64_t i = base + uint64_t(k) * kNT + threadIdx.x;
-    bool f = false;
-    R cur{};
-    if (i < n) {
-      cur = a[i];
-      f = (i == 0) || key(cur) != key(a[i - 1]);
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t lt = (1ull << lane) - 1;
+  const uint64_t vmask = (1ull << key.vbits) - 1;
+  EG_FOR_CHUNKS(n) {
+    const uint64_t c0 = chunk * kChunk;
+    ChunkKeys c;
+    chunk_keys(a, n, c0, key, c);
+    uint64_t run = off[chunk];
+#pragma unroll
+    for (int j = 0; j < kCR; ++j) {
+      const uint64_t i = c0 + uint64_t(j) * kWave + lane;
+      const bool h = chunk_head(c, j, i, n);
+      const uint64_t H = __ballot(h);
+      if (h) {
+        const uint32_t q = uint32_t(c.k[j] >> key.vbits);
+        CIdx e;
+        e.index = i - meta->q_off_csc[q];
+        e.vertex = uint32_t(c.k[j] & vmask);
+        out[run + __popcll(H & lt) + q] = e;
+      }
+      run += __popcll(H);
     }
-    uint32_t tot;
-    const uint32_t ex = block_exclusive_scan<kNT>(f ? 1u : 0u, 0u, OpSum{}, wt, &tot);
-    if (f) {
-      const uint32_t q = key.qmap[cur.src >> kPageShift];
-      CIdx c;
-      c.index = i - meta->q_off_csc[q];
-      c.vertex = cur.dst;
-      out[uint64_t(run) + ex + q] = c;
-    }
-    run += tot;
   }
 }
 
-// Number of run starts in [0, x): tile prefix + the partial tile.
+// Number of run heads in [0, x): chunk prefix + the partial chunk (one block).
 template <class R, class KeyFn>
-__device__ uint64_t runs_before(const R* a, uint64_t x, KeyFn key, const uint32_t* tile_off, uint32_t* wt) {
-  const uint64_t t = x / kIdxTile;
-  const uint64_t b = t * kIdxTile;
+__device__ uint64_t runs_before(const R* a, uint64_t n, uint64_t x, KeyFn key, const uint32_t* off, uint32_t* wt) {
+  const uint64_t ch = x / kChunk;
+  const uint64_t b = ch * kChunk;
   uint32_t c = 0;
   for (uint64_t i = b + threadIdx.x; i < x; i += kNT) c += (i == 0 || key(a[i]) != key(a[i - 1])) ? 1u : 0u;
   c = wave_reduce(c, OpSum{});
   if ((threadIdx.x & (kWave - 1)) == 0) wt[threadIdx.x / kWave] = c;
   __syncthreads();
-  uint64_t tot = tile_off[t];
+  uint64_t tot = off[ch];
   for (int i = 0; i < kNT / kWave; ++i) tot += wt[i];
   __syncthreads();
   return tot;
@@ -395,14 +423,14 @@ __device__ uint64_t runs_before(const R* a, uint64_t x, KeyFn key, const uint32_
 // One block per sub-partition: where its index starts, its entry count and
 // the sentinel {n_edges, V} (core/graph.hpp:1186-1187; R8 for empty ones).
 template <class R>
-__global__ __launch_bounds__(kNT) void k_cindex_sentinel(const R* __restrict__ a, KeyQ<true> key,
-                                                         const uint32_t* __restrict__ tile_off, uint32_t V,
+__global__ __launch_bounds__(kNT) void k_cindex_sentinel(const R* __restrict__ a, uint64_t n, KeyQ<true> key,
+                                                         const uint32_t* __restrict__ off, uint32_t V,
                                                          eg_graph::Meta* __restrict__ meta, CIdx* __restrict__ out) {
   __shared__ uint32_t wt[kNT / kWave];
   const int q = blockIdx.x;
   const uint64_t b = meta->q_off_csc[q], e = meta->q_off_csc[q + 1];
-  const uint64_t rb = runs_before(a, b, key, tile_off, wt);
-  const uint64_t re = runs_before(a, e, key, tile_off, wt);
+  const uint64_t rb = runs_before(a, n, b, key, off, wt);
+  const uint64_t re = runs_before(a, n, e, key, off, wt);
   if (threadIdx.x == 0) {
     CIdx c;
     c.index = e - b;
@@ -415,51 +443,214 @@ __global__ __launch_bounds__(kNT) void k_cindex_sentinel(const R* __restrict__ a
 
 // Dense CSR index (core/graph.hpp:1312-1353) in global positions: a run of
 // source u in sub-partition q writes index[q][u] = first and
-// index[q][u+1] = last+1, and sets bit u of bitmap[q].
+// index[q][u+1] = last+1; gaps are filled by the max-scan afterwards.
 template <class R>
 __global__ __launch_bounds__(kNT) void k_csr_runs(const R* __restrict__ a, uint64_t n, KeyQ<false> key, uint32_t V,
-                                                  uint64_t* __restrict__ index, uint64_t* __restrict__ bitmap) {
-  const uint64_t words = (V >> 6) + 1;
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
-    const R cur = a[i];
-    const uint64_t kc = key(cur);
-    const uint32_t q = uint32_t(kc >> key.vbits);
-    if (i == 0 || kc != key(a[i - 1])) {
-      index[uint64_t(q) * (V + 1) + cur.src] = i;
-      atomicOr((unsigned long long*)&bitmap[q * words + (cur.src >> 6)], 1ull << (cur.src & 63));
+                                                  uint64_t* __restrict__ index) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t vmask = (1ull << key.vbits) - 1;
+  EG_FOR_CHUNKS(n) {
+    const uint64_t c0 = chunk * kChunk;
+    ChunkKeys c;
+    chunk_keys(a, n, c0, key, c);
+#pragma unroll
+    for (int j = 0; j < kCR; ++j) {
+      const uint64_t i = c0 + uint64_t(j) * kWave + lane;
+      const uint64_t q = c.k[j] >> key.vbits;
+      const uint32_t u = uint32_t(c.k[j] & vmask);
+      if (chunk_head(c, j, i, n)) index[q * (V + 1) + u] = i;
+      if (chunk_tail(c, j, i, n)) index[q * (V + 1) + u + 1] = i + 1;
     }
-    if (i == n - 1 || kc != key(a[i + 1])) index[uint64_t(q) * (V + 1) + cur.src + 1] = i + 1;
   }
 }
 
-__global__ void k_csr_localize(uint64_t* __restrict__ index, uint32_t V, int Q, const eg_graph::Meta* __restrict__ meta) {
-  const uint64_t n = uint64_t(Q) * (V + 1);
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-    index[i] -= meta->q_off_csr[i / (V + 1)];
+template <class R, bool kSrc>
+__global__ __launch_bounds__(kNT) void k_run_degree(const R* __restrict__ a, uint64_t n, uint32_t* __restrict__ deg) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  EG_FOR_CHUNKS(n) {
+    const uint64_t c0 = chunk * kChunk;
+    ChunkKeys c;
+    chunk_keys(a, n, c0, KeyEnd<kSrc>{}, c);
+    uint64_t H[kCR];
+#pragma unroll
+    for (int j = 0; j < kCR; ++j) H[j] = __ballot(chunk_head(c, j, c0 + uint64_t(j) * kWave + lane, n));
+#pragma unroll
+    for (int j = 0; j < kCR; ++j) {
+      const uint64_t i = c0 + uint64_t(j) * kWave + lane;
+      const bool tail = chunk_tail(c, j, i, n) || (lane == 63 && j == kCR - 1 && i < n);
+      if (!tail) continue;
+      // latest head at or before (j, lane) inside the chunk; chunk start if none
+      int sj = -1, sl = 0;
+      if (H[j] & le) {
+        sj = j;
+        sl = 63 - __clzll((long long)(H[j] & le));
+      } else {
+        for (int jj = j - 1; jj >= 0; --jj)
+          if (H[jj]) {
+            sj = jj;
+            sl = 63 - __clzll((long long)H[jj]);
+            break;
+          }
+      }
+      const uint32_t start = sj < 0 ? 0u : uint32_t(sj * kWave + sl);
+      atomicAdd(&deg[uint32_t(c.k[j])], uint32_t(j * kWave + lane) - start + 1u);
+    }
+  }
+}
+
+// Sub-partition-local positions, and the bitmap: u has edges in q iff
+// index[q][u+1] > index[q][u].  A wave takes one bitmap word (64 ids) per
+// step; the ballot of its lanes is the word.
+__global__ __launch_bounds__(kNT) void k_csr_localize(const uint64_t* __restrict__ gpos, uint64_t* __restrict__ index,
+                                                      uint64_t* __restrict__ bitmap, uint32_t V, int Q,
+                                                      const eg_graph::Meta* __restrict__ meta) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t words = (V >> 6) + 1;
+  const uint64_t total = uint64_t(Q) * words;
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t t = gw; t < total; t += nw) {
+    const uint64_t q = t / words, wd = t % words;
+    const uint64_t u = wd * 64 + lane;
+    const uint64_t* row = gpos + q * (uint64_t(V) + 1);
+    bool present = false;
+    if (u <= V) {
+      const uint64_t v = row[u];
+      if (u < V) present = row[u + 1] > v;
+      index[q * (uint64_t(V) + 1) + u] = v - meta->q_off_csr[q];
+    }
+    const uint64_t m = __ballot(present);
+    if (lane == 0) bitmap[q * words + wd] = m;
+  }
+}
+
+// --- P > 1: regroup the rank slices by sub-partition ---------------------
+// A rank slice sorted by its owner id is already grouped by sub-partition
+// (q is monotone in the vertex id).  Block (r, q) = the records of rank r
+// owned by q.  Copying the blocks in (q, r) order and then sorting each q
+// stably by the other endpoint gives the order (q, key, r, owner, j): the
+// composite (q, key) sort with the q digit done by a copy.
+template <class R, bool kSrc>
+__global__ void k_block_bounds(const R* __restrict__ a, const uint64_t* __restrict__ rank_start, int P, int Q, int S,
+                               const eg_graph::Meta* __restrict__ meta, uint64_t* __restrict__ bpos) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P * (Q + 1)) return;
+  const int r = t / (Q + 1), q = t % (Q + 1);
+  const uint64_t lo0 = rank_start[r], hi0 = rank_start[r + 1];
+  if (q == 0 || q == Q) {
+    bpos[t] = q == 0 ? lo0 : hi0;
+    return;
+  }
+  const int p = q / S, sk = q % S;
+  const uint32_t vb = meta->local_partition_offset[p * (S + 1) + sk];  // first vertex of q
+  uint64_t lo = lo0, hi = hi0;  // first record whose owner >= vb
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if ((kSrc ? a[mid].src : a[mid].dst) < vb)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  bpos[t] = lo;
+}
+
+// (destination start, source start) of every block in (q, r) order, plus the
+// end sentinel; one thread (P*Q is small).
+__global__ void k_block_table(const uint64_t* __restrict__ bpos, int P, int Q, uint64_t* __restrict__ tab_dst,
+                              uint64_t* __restrict__ tab_src) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint64_t d = 0;
+  int b = 0;
+  for (int q = 0; q < Q; ++q)
+    for (int r = 0; r < P; ++r) {
+      const uint64_t s0 = bpos[r * (Q + 1) + q], s1 = bpos[r * (Q + 1) + q + 1];
+      tab_dst[b] = d;
+      tab_src[b] = s0;
+      d += s1 - s0;
+      ++b;
+    }
+  tab_dst[b] = d;
+}
+
+// A wave copies kSpan consecutive destination records: one binary search for
+// the span, then per 512-record chunk a fast path when the chunk lies inside
+// one block (the usual case: blocks hold millions of records).
+constexpr int kSpanChunks = 8;
+
+template <class R>
+__global__ __launch_bounds__(kNT) void k_block_copy(const R* __restrict__ in, R* __restrict__ out, uint64_t n,
+                                                    const uint64_t* __restrict__ tab_dst,
+                                                    const uint64_t* __restrict__ tab_src, int nblk) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t span = uint64_t(kSpanChunks) * kChunk;
+  const uint64_t nspan = ceil_div(n, span);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t sp = gw; sp < nspan; sp += nw) {
+    const uint64_t s0 = sp * span;
+    int b = 0, hi = nblk;  // last block starting at or before s0
+    while (hi - b > 1) {
+      const int mid = (b + hi) >> 1;
+      if (tab_dst[mid] <= s0)
+        b = mid;
+      else
+        hi = mid;
+    }
+    for (int ch = 0; ch < kSpanChunks; ++ch) {
+      const uint64_t c0 = s0 + uint64_t(ch) * kChunk;
+      if (c0 >= n) break;
+      while (b + 1 < nblk && tab_dst[b + 1] <= c0) ++b;  // wave-uniform
+      const uint64_t cend = c0 + kChunk < n ? c0 + kChunk : n;
+      const bool one_block = b + 1 >= nblk || tab_dst[b + 1] >= cend;
+      uint64_t src[kCR];
+#pragma unroll
+      for (int j = 0; j < kCR; ++j) {
+        const uint64_t i = c0 + uint64_t(j) * kWave + lane;
+        int bb = b;
+        if (!one_block)
+          while (bb + 1 < nblk && tab_dst[bb + 1] <= i) ++bb;
+        src[j] = i < n ? tab_src[bb] + (i - tab_dst[bb]) : tab_src[b];
+      }
+      R r[kCR];
+#pragma unroll
+      for (int j = 0; j < kCR; ++j) r[j] = in[src[j]];
+#pragma unroll
+      for (int j = 0; j < kCR; ++j) {
+        const uint64_t i = c0 + uint64_t(j) * kWave + lane;
+        if (i < n) out[i] = r[j];
+      }
+    }
+  }
 }
 
 unsigned grid_for(uint64_t n, uint64_t per_block = kNT, uint64_t cap = 65536) {
   return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, per_block), cap)));
 }
 
+// Stable sort of every rank slice (one segment each) by key.
 template <class R, class KeyFn>
-int rank_sorts(const R* L, R* out, R* tmp, uint64_t E0, int P, uint64_t mult, KeyFn key, int bits, uint32_t* hist,
-               uint32_t* stmp, hipStream_t s) {
+int rank_sorts(eg_graph* g, const R* L, R* out, R* tmp, uint64_t E0, int P, uint64_t mult, KeyFn key, int bits,
+               hipStream_t s) {
   const uint64_t base = E0 / P;
-  for (int r = 0; r < P; ++r) {
-    const uint64_t off = mult * base * r;
-    const uint64_t n_r = mult * (base + (r == P - 1 ? E0 % P : 0));
-    EG_TRY(radix_sort(L + off, out + off, tmp + off, n_r, key, bits, hist, stmp, s));
-  }
-  return EG_OK;
+  std::vector<uint64_t> seg(P + 1);
+  for (int r = 0; r < P; ++r) seg[r] = mult * base * r;
+  seg[P] = mult * E0;
+  return rx_sort(L, out, tmp, seg, key, bits, g->sortw, s);
 }
+
+template <class R, class KeyFn>
+int whole_sort(eg_graph* g, const R* in, R* out, R* tmp, uint64_t n, KeyFn key, int bits, hipStream_t s) {
+  return rx_sort(in, out, tmp, std::vector<uint64_t>{0, n}, key, bits, g->sortw, s);
+}
+
+unsigned chunk_grid(uint64_t n) { return grid_for(ceil_div(n, kChunk), kNT / kWave, 65536); }
 
 template <class R, bool kSrc>
 int run_degree(const R* a, uint64_t n, uint32_t* deg, hipStream_t s) {
   if (!n) return EG_OK;
   prof_begin(s);
-  hipLaunchKernelGGL((k_run_degree<R, kSrc>), dim3(grid_for(ceil_div(n, 64), kNT / kWave, 65536)), dim3(kNT), 0, s, a,
-                     n, deg);
+  hipLaunchKernelGGL((k_run_degree<R, kSrc>), dim3(chunk_grid(n)), dim3(kNT), 0, s, a, n, deg);
   prof_end(PK_DEGREE, n * sizeof(R), s);
   EG_HIP(hipGetLastError());
   return EG_OK;
@@ -506,12 +697,11 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   EG_TRY(g->t1.resize(Emax * rb));
   EG_TRY(g->t2.resize(Emax * rb));
   if (P > 1 || Q > 1) EG_TRY(g->t3.resize(Emax * rb));
-  const uint64_t hist_n = std::max<uint64_t>(radix_hist_elems(E), 1);
-  EG_TRY(g->hist.resize(hist_n * 4));
+  const uint64_t hist_n = 1;
   const uint64_t scan_n = std::max<uint64_t>({scan_tmp_elems(hist_n), scan_tmp_elems(idx_n), scan_tmp_elems(pages + 1),
-                                              scan_tmp_elems(ceil_div(E, kIdxTile) + 1)});
+                                              scan_tmp_elems(ceil_div(E, kChunk) + 1)});
   EG_TRY(g->scan_tmp.resize(scan_n * 8));
-  EG_TRY(g->tile_cnt.resize((ceil_div(E, kIdxTile) + 2) * 4));
+  EG_TRY(g->tile_cnt.resize((ceil_div(E, kChunk) + 2) * 4));
   EG_TRY(g->err.resize(4));
   if (sym) EG_TRY(g->sym.resize(Emax * rb));
 
@@ -520,7 +710,6 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   uint32_t* out_deg = g->out_deg.as<uint32_t>();
   uint32_t* in_deg = g->in_deg.as<uint32_t>();
   uint16_t* qmap = g->qmap.as<uint16_t>();
-  uint32_t* hist = g->hist.as<uint32_t>();
   uint32_t* stmp32 = g->scan_tmp.as<uint32_t>();
   const uint64_t mult = sym ? 2 : 1;
 
@@ -535,11 +724,13 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   // --- A, B: first sorts (they also validate every id: the only mid-build
   // synchronisation) ---
   EG_TRY(fill<uint32_t>(err, 1, 0u, s));
+  EG_TRY(g->sortw.resize_segs(1));
+  EG_TRY(fill<uint32_t>(g->sortw.err.as<uint32_t>(), 1, 0u, s));
   R* A = g->t1.as<R>();
   R* B = g->t2.as<R>();
   R* T = P > 1 ? g->t3.as<R>() : g->csc_list.as<R>();  // ping-pong of the first sorts
-  EG_TRY(rank_sorts(L, A, T, E0, P, mult, KeyChecked<true>{V, err}, g->vbits, hist, stmp32, s));
-  EG_TRY(rank_sorts(L, B, T, E0, P, mult, KeyChecked<false>{V, err}, g->vbits, hist, stmp32, s));
+  EG_TRY(rank_sorts(g, L, A, T, E0, P, mult, KeyChecked<true>{V, err}, g->vbits, s));
+  EG_TRY(rank_sorts(g, L, B, T, E0, P, mult, KeyChecked<false>{V, err}, g->vbits, s));
   uint32_t h_err = 0;
   EG_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
@@ -573,65 +764,108 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
 
   // --- second sorts ---
   if (P > 1) {
-    const int kbits = g->qbits + g->vbits;
-    EG_TRY(radix_sort((const R*)A, g->csc_list.as<R>(), g->t3.as<R>(), E, KeyQ<true>{qmap, g->vbits}, kbits, hist,
-                      stmp32, s));
-    EG_TRY(radix_sort((const R*)B, g->csr_list.as<R>(), g->t3.as<R>(), E, KeyQ<false>{qmap, g->vbits}, kbits, hist,
-                      stmp32, s));
+    // sub-partition sizes to the host: they delimit the segmented sorts
+    std::vector<uint64_t> qoff(2 * (Q + 1));
+    EG_HIP(hipMemcpyAsync(qoff.data(), meta->q_off_csc, (Q + 1) * 8, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipMemcpyAsync(qoff.data() + Q + 1, meta->q_off_csr, (Q + 1) * 8, hipMemcpyDeviceToHost, s));
+    std::vector<uint64_t> rank_start(P + 1);
+    for (int r = 0; r < P; ++r) rank_start[r] = mult * (E0 / P) * r;
+    rank_start[P] = E;
+    EG_TRY(g->blk.resize((uint64_t(P) + 1) * 8 + uint64_t(P) * (Q + 1) * 8 + 2 * (uint64_t(P) * Q + 1) * 8 + 64));
+    uint64_t* d_rank = g->blk.as<uint64_t>();
+    uint64_t* bpos = d_rank + P + 1;
+    uint64_t* tab_dst = bpos + uint64_t(P) * (Q + 1);
+    uint64_t* tab_src = tab_dst + uint64_t(P) * Q + 1;
+    EG_HIP(hipMemcpyAsync(d_rank, rank_start.data(), (P + 1) * 8, hipMemcpyHostToDevice, s));
+    EG_HIP(hipStreamSynchronize(s));
+    const std::vector<uint64_t> seg_csc(qoff.begin(), qoff.begin() + Q + 1);
+    const std::vector<uint64_t> seg_csr(qoff.begin() + Q + 1, qoff.end());
+    R* X = g->t3.as<R>();
+    for (int dir = 0; dir < 2; ++dir) {
+      const bool pull = dir == 0;
+      R* in = pull ? A : B;
+      const unsigned bb = unsigned(ceil_div(uint64_t(P) * (Q + 1), kNT));
+      if (pull)
+        hipLaunchKernelGGL((k_block_bounds<R, true>), dim3(bb), dim3(kNT), 0, s, (const R*)in, (const uint64_t*)d_rank,
+                           P, Q, S, (const eg_graph::Meta*)meta, bpos);
+      else
+        hipLaunchKernelGGL((k_block_bounds<R, false>), dim3(bb), dim3(kNT), 0, s, (const R*)in,
+                           (const uint64_t*)d_rank, P, Q, S, (const eg_graph::Meta*)meta, bpos);
+      EG_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_block_table, dim3(1), dim3(kWave), 0, s, (const uint64_t*)bpos, P, Q, tab_dst, tab_src);
+      EG_HIP(hipGetLastError());
+      hipLaunchKernelGGL((k_block_copy<R>), dim3(grid_for(ceil_div(E, uint64_t(kSpanChunks) * kChunk), kNT / kWave,
+                                                           65536)),
+                         dim3(kNT), 0, s, (const R*)in, X, E, (const uint64_t*)tab_dst, (const uint64_t*)tab_src,
+                         P * Q);
+      EG_HIP(hipGetLastError());
+      // the slice array is free now: it is the ping-pong of the segmented sort
+      if (pull)
+        EG_TRY(rx_sort((const R*)X, g->csc_list.as<R>(), in, seg_csc, KeyDst{}, g->vbits, g->sortw, s));
+      else
+        EG_TRY(rx_sort((const R*)X, g->csr_list.as<R>(), in, seg_csr, KeySrc{}, g->vbits, g->sortw, s));
+    }
   } else if (Q == 1) {
     g->csr_list.swap(g->t1);  // A is the CSR, B the CSC
     g->csc_list.swap(g->t2);
   } else {
     const int qb = std::max(1, g->qbits);
-    EG_TRY(radix_sort((const R*)B, g->csc_list.as<R>(), g->t3.as<R>(), E, KeyQOnly<true>{qmap}, qb, hist, stmp32, s));
-    EG_TRY(radix_sort((const R*)A, g->csr_list.as<R>(), g->t3.as<R>(), E, KeyQOnly<false>{qmap}, qb, hist, stmp32, s));
+    EG_TRY(whole_sort(g, (const R*)B, g->csc_list.as<R>(), g->t3.as<R>(), E, KeyQOnly<true>{qmap}, qb, s));
+    EG_TRY(whole_sort(g, (const R*)A, g->csr_list.as<R>(), g->t3.as<R>(), E, KeyQOnly<false>{qmap}, qb, s));
   }
 
   // --- compressed CSC index ---
   {
     const R* a = g->csc_list.as<R>();
     KeyQ<true> key{qmap, g->vbits};
-    const uint32_t ntiles = uint32_t(std::max<uint64_t>(ceil_div(E, kIdxTile), 1));
+    const uint64_t nch = ceil_div(E, kChunk);
     uint32_t* tc = g->tile_cnt.as<uint32_t>();
     CIdx* cidx = g->csc_cidx.as<CIdx>();
     if (E) {
-      hipLaunchKernelGGL((k_run_count<R, KeyQ<true>>), dim3(ntiles), dim3(kNT), 0, s, a, E, key, tc, ntiles);
+      hipLaunchKernelGGL((k_run_count<R, KeyQ<true>>), dim3(chunk_grid(E)), dim3(kNT), 0, s, a, E, key, tc, nch);
       EG_HIP(hipGetLastError());
-      EG_TRY(exclusive_scan_u32(tc, tc, uint64_t(ntiles) + 1, stmp32, s));
+      EG_TRY(exclusive_scan_u32(tc, tc, nch + 1, stmp32, s));
       prof_begin(s);
-      hipLaunchKernelGGL((k_cindex_write<R>), dim3(ntiles), dim3(kNT), 0, s, a, E, key, (const uint32_t*)tc, meta,
-                         cidx);
+      hipLaunchKernelGGL((k_cindex_write<R>), dim3(chunk_grid(E)), dim3(kNT), 0, s, a, E, key, (const uint32_t*)tc,
+                         meta, cidx);
       prof_end(PK_CINDEX, E * rb, s);
       EG_HIP(hipGetLastError());
     } else {
       EG_TRY(fill<uint32_t>(tc, 2, 0u, s));
     }
-    hipLaunchKernelGGL((k_cindex_sentinel<R>), dim3(Q), dim3(kNT), 0, s, a, key, (const uint32_t*)tc, V, meta, cidx);
+    hipLaunchKernelGGL((k_cindex_sentinel<R>), dim3(Q), dim3(kNT), 0, s, a, E, key, (const uint32_t*)tc, V, meta,
+                       cidx);
     EG_HIP(hipGetLastError());
   }
 
-  // --- dense CSR index + bitmap ---
+  // --- dense CSR index + bitmap: run ends as global positions into gpos,
+  // max-scan fills the gaps, localize writes the index and the bitmap ---
   {
     uint64_t* index = g->csr_index.as<uint64_t>();
     uint64_t* bitmap = g->csr_bitmap.as<uint64_t>();
-    EG_TRY(fill<uint64_t>(index, idx_n, uint64_t(0), s));
-    EG_TRY(fill<uint64_t>(bitmap, uint64_t(Q) * words, uint64_t(0), s));
+    EG_TRY(g->idx_tmp.resize(idx_n * 8));
+    uint64_t* gpos = g->idx_tmp.as<uint64_t>();
+    EG_TRY(fill<uint64_t>(gpos, idx_n, uint64_t(0), s));
     if (E) {
       prof_begin(s);
-      hipLaunchKernelGGL((k_csr_runs<R>), dim3(grid_for(E, kNT * 8, 65536)), dim3(kNT), 0, s, g->csr_list.as<R>(), E,
-                         KeyQ<false>{qmap, g->vbits}, V, index, bitmap);
+      hipLaunchKernelGGL((k_csr_runs<R>), dim3(chunk_grid(E)), dim3(kNT), 0, s, g->csr_list.as<R>(), E,
+                         KeyQ<false>{qmap, g->vbits}, V, gpos);
       prof_end(PK_CSR_INDEX, E * rb, s);
       EG_HIP(hipGetLastError());
     }
-    EG_TRY(inclusive_max_scan_u64(index, index, idx_n, g->scan_tmp.as<uint64_t>(), s));
-    hipLaunchKernelGGL(k_csr_localize, dim3(grid_for(idx_n, kNT * 8, 65536)), dim3(kNT), 0, s, index, V, Q, meta);
+    EG_TRY(inclusive_max_scan_u64(gpos, gpos, idx_n, g->scan_tmp.as<uint64_t>(), s));
+    hipLaunchKernelGGL(k_csr_localize, dim3(grid_for(uint64_t(Q) * words, kNT / kWave, 65536)), dim3(kNT), 0, s,
+                       (const uint64_t*)gpos, index, bitmap, V, Q, meta);
     EG_HIP(hipGetLastError());
   }
 
   // --- small tables back to the host ---
   eg_graph::Meta h;
+  uint32_t sort_err = 0;
   EG_HIP(hipMemcpyAsync(&h, meta, sizeof(h), hipMemcpyDeviceToHost, s));
+  EG_HIP(hipMemcpyAsync(&sort_err, g->sortw.err.ptr, 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
+  if (sort_err) return fail(EG_ERR_HIP, "radix sort look-back exceeded its spin limit");
   g->page_offset.assign(h.page_offset, h.page_offset + P + 1);
   g->partition_offset.assign(h.partition_offset, h.partition_offset + P + 1);
   g->local_partition_offset.assign(h.local_partition_offset, h.local_partition_offset + P * (S + 1));
@@ -685,6 +919,47 @@ int eg_graph_build(eg_graph* g, const void* d_edges, uint64_t E0, int symmetrize
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (g->rec == 8) return build_impl<Rec8>(g, static_cast<const Rec8*>(d_edges), E0, symmetrize ? 1 : 0, s);
   return build_impl<Rec12>(g, static_cast<const Rec12*>(d_edges), E0, symmetrize ? 1 : 0, s);
+}
+
+int eg_autotune_sort(eg_graph* g, const void* d_edges, uint64_t E0, int symmetrize, int* h_scheme, int* h_threads,
+                     int* h_bits, double* h_ms, void* stream) {
+  if (!g) return fail(EG_ERR_ARG, "eg_autotune_sort: graph is NULL");
+  struct Cand {
+    int scheme, nt, bits;
+  };
+  std::vector<Cand> cands = {{kThreeLaunch, 256, 8}, {kThreeLaunch, 512, 8}, {kThreeLaunch, 512, 9},
+                             {kOnePass, 512, 8},     {kOnePass, 512, 9}};
+  if (g->rec == 8) {
+    cands.push_back({kThreeLaunch, 1024, 8});
+    cands.push_back({kOnePass, 1024, 8});
+  }
+  const SortVariant keep = sort_variant(g->rec);
+  double best = 1e300;
+  SortVariant pick = keep;
+  for (const Cand& c : cands) {
+    sort_variant(g->rec) = SortVariant{c.scheme, c.nt, c.bits};
+    double t = 0;
+    for (int rep = 0; rep < 2; ++rep) {  // the first build may allocate
+      const auto t0 = std::chrono::steady_clock::now();
+      const int rc = eg_graph_build(g, d_edges, E0, symmetrize, stream);
+      if (rc != EG_OK) {
+        sort_variant(g->rec) = keep;
+        return rc;
+      }
+      t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (t < best) {
+      best = t;
+      pick = SortVariant{c.scheme, c.nt, c.bits};
+    }
+  }
+  sort_variant(g->rec) = pick;
+  const int rc = eg_graph_build(g, d_edges, E0, symmetrize, stream);  // leave the graph built
+  if (h_scheme) *h_scheme = pick.scheme;
+  if (h_threads) *h_threads = pick.nt;
+  if (h_bits) *h_bits = pick.bits;
+  if (h_ms) *h_ms = best;
+  return rc;
 }
 
 int eg_graph_get_info(const eg_graph* g, eg_graph_info* out) {

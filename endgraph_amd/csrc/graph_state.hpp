@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "radix.hpp"
 
 struct eg_graph {
   // configuration (Graph(int eta), core/graph.hpp:202; P ranks and S sockets)
@@ -29,9 +30,12 @@ struct eg_graph {
   eg::DevBuf csc_cidx;               // CIdx, per sub-partition: entries + sentinel
   eg::DevBuf csr_index;              // u64[Q][V+1] (sub-partition-local positions)
   eg::DevBuf csr_bitmap;             // u64[Q][(V>>6)+1]
+  eg::DevBuf blk;                    // block tables of the P > 1 regrouping
+  eg::DevBuf idx_tmp;                // u64[Q][V+1] global positions before localizing
   eg::DevBuf sym;                    // symmetrised logical list
   eg::DevBuf t1, t2, t3;             // first-sort results and sort ping-pong
   eg::DevBuf hist, scan_tmp, tile_cnt, err;
+  eg::RxWork sortw;                  // segmented radix sort workspace
 
   // Engine view, derived from the built arrays on the first engine call
   // (eg_engine_prepare).  Edge positions are u32: a build holds < 2^32 edges.

@@ -1,0 +1,496 @@
+// Segmented stable LSD radix sort of edge records on gfx950.
+//
+// The reference sorts with load_balance_radix_sort (core/graph.hpp:465-584):
+// a stable LSD sort whose result depends only on stability, so any stable sort
+// by the same key gives the identical order.  A sort here covers nseg
+// independent contiguous segments (one per rank slice, or one) and takes
+// ceil(key_bits/8) balanced digit passes.  Two pass schemes, each with the
+// tile size as a template parameter (NT threads x 16 records):
+//
+//   three-launch   upsweep: per-tile digit histogram (LDS atomics) into a
+//                  (segment, digit, tile)-major table; device scan of the
+//                  table; downsweep: re-read, rank, scatter.
+//   one-pass       a histogram of all passes up front, then per pass one
+//                  launch: tiles taken by ticket publish their digit counts
+//                  and look back over earlier tiles of their segment
+//                  (decoupled look-back, bounded wait) before scattering.
+//
+// Ranking inside a tile is stable: wave w owns a contiguous run of the tile,
+// handled in rounds of 64 records; the lanes holding the same digit find each
+// other with one ballot per digit bit, and per-wave running counters in LDS
+// order the rounds.  The tile is then staged in LDS in digit order, so each
+// digit's run is written contiguously (bigger tiles => longer runs => fewer
+// partially written lines).  Variants are chosen per record size by
+// select_sort_variant (see eg_autotune_sort).
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace eg {
+
+constexpr int kRxIPT = 16;
+constexpr int kRxMaxBits = 9;  // widest digit any variant uses
+constexpr int kRxMaxPasses = 8;
+constexpr uint32_t kRxSpinLimit = 1u << 22;
+constexpr uint64_t kFlagAgg = 1, kFlagInc = 2;
+
+enum SortScheme { kThreeLaunch = 0, kOnePass = 1 };
+
+struct RxPlan {
+  int passes = 0;
+  int shift[kRxMaxPasses] = {0};
+  int bits[kRxMaxPasses] = {0};
+};
+
+inline RxPlan rx_plan(int key_bits, int max_bits) {
+  RxPlan p;
+  if (key_bits <= 0) key_bits = 1;
+  p.passes = (key_bits + max_bits - 1) / max_bits;
+  const int d = (key_bits + p.passes - 1) / p.passes;
+  for (int i = 0; i < p.passes; ++i) {
+    p.shift[i] = i * d;
+    p.bits[i] = d < key_bits - i * d ? d : key_bits - i * d;
+  }
+  return p;
+}
+
+struct RxPassArgs {
+  int npass;
+  int shift[kRxMaxPasses];
+  int bits[kRxMaxPasses];
+};
+
+__device__ __forceinline__ uint64_t rx_pack(uint32_t epoch, uint64_t flag, uint32_t v) {
+  return (uint64_t(epoch) << 34) | (flag << 32) | v;
+}
+
+// segment of global tile t: the last s with tile_start[s] <= t (skips empty
+// segments, whose tile ranges are empty)
+__device__ __forceinline__ int rx_seg_of(const uint32_t* tile_start, int nseg, uint32_t t) {
+  int lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_start[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+struct RxTile {
+  int s;          // segment
+  uint32_t lt;    // tile index inside the segment
+  uint32_t nt;    // tiles of the segment
+  uint64_t base;  // first record
+  uint64_t end;   // segment end
+  uint32_t count;
+};
+
+template <int TILE>
+__device__ __forceinline__ RxTile rx_tile(const uint64_t* seg, const uint32_t* ts, int nseg, uint32_t t) {
+  RxTile x;
+  x.s = rx_seg_of(ts, nseg, t);
+  x.lt = t - ts[x.s];
+  x.nt = ts[x.s + 1] - ts[x.s];
+  x.base = seg[x.s] + uint64_t(x.lt) * TILE;
+  x.end = seg[x.s + 1];
+  x.count = (x.end - x.base) < uint64_t(TILE) ? uint32_t(x.end - x.base) : uint32_t(TILE);
+  return x;
+}
+
+// Stable in-tile ranking shared by both schemes.  On return: item/dig/rank per
+// record of this thread, wcnt[w][d] = records of digit d in waves < w, *tot
+// (threads d < nb) = tile count of digit d.  The 16 loads are issued before
+// any of them is used (invalid lanes re-read the tile's first record), so a
+// wave has all its loads in flight at once.
+template <int NT, int BM, class R, class KeyFn>
+__device__ __forceinline__ void rx_rank(const R* __restrict__ in, const RxTile& x, KeyFn key, int shift, int bits,
+                                        R (&item)[kRxIPT], uint32_t (&dig)[kRxIPT], uint32_t (&rank)[kRxIPT],
+                                        uint32_t (*wcnt)[1 << BM], uint32_t* tot) {
+  constexpr int W = NT / kWave;
+  const int nb = 1 << bits;
+  const uint32_t mask = uint32_t(nb - 1);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+  const uint32_t first = uint32_t(w) * (kRxIPT * kWave) + lane;
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) {
+    const uint32_t li = first + uint32_t(k) * kWave;
+    item[k] = in[x.base + (li < x.count ? li : 0u)];
+  }
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) dig[k] = uint32_t(key(item[k]) >> shift) & mask;
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) {
+    const bool valid = first + uint32_t(k) * kWave < x.count;
+    const uint32_t d = dig[k];
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bb = 0; bb < BM; ++bb) {
+      if (bb < bits) {  // wave-uniform
+        const bool set = (d >> bb) & 1u;
+        const uint64_t bal = __ballot(set);
+        peers &= set ? bal : ~bal;
+      }
+    }
+    const uint32_t below = uint32_t(__popcll(peers & lt_mask));
+    const uint32_t total = uint32_t(__popcll(peers));
+    const uint32_t prior = wcnt[w][d];
+    rank[k] = prior + below;
+    if (valid && below + 1 == total) wcnt[w][d] = prior + total;  // highest peer updates
+  }
+  __syncthreads();
+  uint32_t t = 0;
+  for (int d = threadIdx.x; d < nb; d += NT) {
+#pragma unroll
+    for (int ww = 0; ww < W; ++ww) {
+      const uint32_t c = wcnt[ww][d];
+      wcnt[ww][d] = t;
+      t += c;
+    }
+  }
+  *tot = t;
+}
+
+// Stage in digit order and write every digit's run contiguously (LDS reads,
+// then digits, then stores, so the key lookups of a thread overlap).
+template <int NT, int BM, class R, class KeyFn>
+__device__ __forceinline__ void rx_scatter(R* __restrict__ out, const RxTile& x, KeyFn key, int shift, int bits,
+                                           const R (&item)[kRxIPT], const uint32_t (&dig)[kRxIPT],
+                                           const uint32_t (&rank)[kRxIPT], uint32_t (*wcnt)[1 << BM],
+                                           const uint32_t* goff, const uint32_t* dstart, R* stage) {
+  const uint32_t mask = uint32_t((1 << bits) - 1);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const uint32_t first = uint32_t(w) * (kRxIPT * kWave) + lane;
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k)
+    if (first + uint32_t(k) * kWave < x.count) stage[dstart[dig[k]] + wcnt[w][dig[k]] + rank[k]] = item[k];
+  __syncthreads();
+  R r[kRxIPT];
+  uint32_t d[kRxIPT];
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) {
+    const uint32_t lp = uint32_t(k) * NT + threadIdx.x;
+    r[k] = stage[lp < x.count ? lp : 0u];
+  }
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) d[k] = uint32_t(key(r[k]) >> shift) & mask;
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) {
+    const uint32_t lp = uint32_t(k) * NT + threadIdx.x;
+    if (lp < x.count) {
+      const uint64_t pos = uint64_t(goff[d[k]]) + (lp - dstart[d[k]]);
+      if (pos < x.end) out[pos] = r[k];  // always true unless a look-back gave up
+    }
+  }
+}
+
+// dstart[d] = records of digits < d in the tile (tot: thread d's count for
+// d < nb; a thread may own digits d and d + NT when nb > NT).
+template <int NT>
+__device__ __forceinline__ void rx_dstart(int nb, uint32_t tot, uint32_t* wt, uint32_t* dstart) {
+  uint32_t btot;
+  const uint32_t ex = block_exclusive_scan<NT>(int(threadIdx.x) < nb ? tot : 0u, 0u, OpSum{}, wt, &btot);
+  if (int(threadIdx.x) < nb) dstart[threadIdx.x] = ex;
+  __syncthreads();
+}
+
+// ---------------------------------------------------------- three-launch --
+// table index of (tile t of segment s, digit d): ts[s]*nb + d*nt + lt
+template <int NT, int BM, class R, class KeyFn>
+__global__ __launch_bounds__(NT) void k_rx_up(const R* __restrict__ in, const uint64_t* __restrict__ seg,
+                                              const uint32_t* __restrict__ ts, int nseg, KeyFn key, int shift,
+                                              int bits, uint32_t* __restrict__ hist) {
+  constexpr int TILE = NT * kRxIPT;
+  __shared__ uint32_t h[1 << BM];
+  const int nb = 1 << bits;
+  for (int i = threadIdx.x; i < nb; i += NT) h[i] = 0;
+  __syncthreads();
+  const RxTile x = rx_tile<TILE>(seg, ts, nseg, blockIdx.x);
+  const uint32_t mask = uint32_t(nb - 1);
+  R r[kRxIPT];
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) {
+    const uint32_t li = uint32_t(k) * NT + threadIdx.x;
+    r[k] = in[x.base + (li < x.count ? li : 0u)];
+  }
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) {
+    const uint32_t d = uint32_t(key(r[k]) >> shift) & mask;
+    if (uint32_t(k) * NT + threadIdx.x < x.count) atomicAdd(&h[d], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < nb; d += NT) hist[uint64_t(ts[x.s]) * nb + uint64_t(d) * x.nt + x.lt] = h[d];
+}
+
+template <int NT, int BM, class R, class KeyFn>
+__global__ __launch_bounds__(NT) void k_rx_down(const R* __restrict__ in, R* __restrict__ out,
+                                                const uint64_t* __restrict__ seg, const uint32_t* __restrict__ ts,
+                                                int nseg, KeyFn key, int shift, int bits,
+                                                const uint32_t* __restrict__ offs) {
+  constexpr int TILE = NT * kRxIPT;
+  constexpr int W = NT / kWave;
+  __shared__ uint32_t wcnt[W][1 << BM];
+  __shared__ uint32_t goff[1 << BM];
+  __shared__ uint32_t dstart[1 << BM];
+  __shared__ uint32_t wt[W];
+  __shared__ R stage[TILE];
+  const int nb = 1 << bits;
+  for (int i = threadIdx.x; i < W * nb; i += NT) wcnt[i / nb][i % nb] = 0;
+  const RxTile x = rx_tile<TILE>(seg, ts, nseg, blockIdx.x);
+  for (int d = threadIdx.x; d < nb; d += NT) goff[d] = offs[uint64_t(ts[x.s]) * nb + uint64_t(d) * x.nt + x.lt];
+  __syncthreads();
+  R item[kRxIPT];
+  uint32_t dig[kRxIPT], rank[kRxIPT];
+  uint32_t tot;
+  rx_rank<NT, BM>(in, x, key, shift, bits, item, dig, rank, wcnt, &tot);
+  rx_dstart<NT>(nb, tot, wt, dstart);
+  rx_scatter<NT, BM>(out, x, key, shift, bits, item, dig, rank, wcnt, goff, dstart, stage);
+}
+
+// -------------------------------------------------------------- one-pass --
+template <int NT, int BM, class R, class KeyFn>
+__global__ __launch_bounds__(NT) void k_rx_hist_all(const R* __restrict__ in, const uint64_t* __restrict__ seg,
+                                                    const uint32_t* __restrict__ ts, int nseg, KeyFn key,
+                                                    RxPassArgs pa, uint32_t* __restrict__ ghist) {
+  constexpr int TILE = NT * kRxIPT;
+  __shared__ uint32_t h[kRxMaxPasses][1 << BM];
+  for (int i = threadIdx.x; i < kRxMaxPasses * (1 << BM); i += NT) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const RxTile x = rx_tile<TILE>(seg, ts, nseg, blockIdx.x);
+  R r[kRxIPT];
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) {
+    const uint32_t li = uint32_t(k) * NT + threadIdx.x;
+    r[k] = in[x.base + (li < x.count ? li : 0u)];
+  }
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) {
+    if (uint32_t(k) * NT + threadIdx.x < x.count) {
+      const uint64_t kv = key(r[k]);
+      for (int p = 0; p < pa.npass; ++p) atomicAdd(&h[p][uint32_t(kv >> pa.shift[p]) & ((1u << pa.bits[p]) - 1)], 1u);
+    }
+  }
+  __syncthreads();
+  for (int p = 0; p < pa.npass; ++p) {
+    const int nb = 1 << pa.bits[p];
+    uint32_t* g = ghist + (uint64_t(x.s) * kRxMaxPasses + p) * (1 << kRxMaxBits);
+    for (int d = threadIdx.x; d < nb; d += NT)
+      if (h[p][d]) atomicAdd(&g[d], h[p][d]);
+  }
+}
+
+template <int NT, int BM, class R, class KeyFn>
+__global__ __launch_bounds__(NT) void k_rx_onepass(const R* __restrict__ in, R* __restrict__ out,
+                                                   const uint64_t* __restrict__ seg, const uint32_t* __restrict__ ts,
+                                                   int nseg, KeyFn key, int shift, int bits,
+                                                   const uint32_t* __restrict__ dbase, uint64_t* __restrict__ status,
+                                                   uint32_t* __restrict__ ticket, uint32_t epoch,
+                                                   uint32_t* __restrict__ err) {
+  constexpr int TILE = NT * kRxIPT;
+  constexpr int W = NT / kWave;
+  __shared__ uint32_t wcnt[W][1 << BM];
+  __shared__ uint32_t goff[1 << BM];
+  __shared__ uint32_t dstart[1 << BM];
+  __shared__ uint32_t wt[W];
+  __shared__ uint32_t s_tile;
+  __shared__ R stage[TILE];
+  const int nb = 1 << bits;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+  for (int i = threadIdx.x; i < W * nb; i += NT) wcnt[i / nb][i % nb] = 0;
+  __syncthreads();
+  const uint32_t t = s_tile;
+  const RxTile x = rx_tile<TILE>(seg, ts, nseg, t);
+  const uint32_t* db = dbase + uint64_t(x.s) * kRxMaxPasses * (1 << kRxMaxBits);
+  R item[kRxIPT];
+  uint32_t dig[kRxIPT], rank[kRxIPT];
+  uint32_t tot;
+  rx_rank<NT, BM>(in, x, key, shift, bits, item, dig, rank, wcnt, &tot);
+  // A status word is its own payload (epoch | flag | count): relaxed
+  // device-scope atomics suffice; release/acquire would write back or
+  // invalidate the L2 on every publish and poll.
+  for (int d = threadIdx.x; d < nb; d += NT) {
+    uint64_t* my = status + uint64_t(t) * (1 << kRxMaxBits) + d;
+    if (x.lt == 0) {
+      __hip_atomic_store(my, rx_pack(epoch, kFlagInc, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      goff[d] = db[d];
+    } else {
+      __hip_atomic_store(my, rx_pack(epoch, kFlagAgg, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t excl = 0, spins = 0;
+      uint32_t j = t - 1;
+      const uint32_t first = t - x.lt;
+      while (true) {
+        const uint64_t sw =
+            __hip_atomic_load(status + uint64_t(j) * (1 << kRxMaxBits) + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t fl = (sw >> 32) & 3ull;
+        if (uint32_t(sw >> 34) != epoch || fl == 0) {
+          if (++spins > kRxSpinLimit) {
+            atomicOr(err, 1u);
+            break;
+          }
+          continue;
+        }
+        excl += uint32_t(sw);
+        if (fl == kFlagInc || j == first) break;
+        --j;
+      }
+      __hip_atomic_store(my, rx_pack(epoch, kFlagInc, excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      goff[d] = db[d] + excl;
+    }
+  }
+  rx_dstart<NT>(nb, tot, wt, dstart);
+  rx_scatter<NT, BM>(out, x, key, shift, bits, item, dig, rank, wcnt, goff, dstart, stage);
+}
+
+namespace {  // internal linkage: this header is included by several sources
+
+__global__ void k_rx_base(uint32_t* __restrict__ ghist, const uint64_t* __restrict__ seg, int nseg, int npass) {
+  const int s = blockIdx.x;
+  if (s >= nseg || int(threadIdx.x) >= npass) return;
+  uint32_t* g = ghist + (uint64_t(s) * kRxMaxPasses + threadIdx.x) * (1 << kRxMaxBits);
+  uint32_t run = uint32_t(seg[s]);
+  for (int d = 0; d < (1 << kRxMaxBits); ++d) {
+    const uint32_t c = g[d];
+    g[d] = run;
+    run += c;
+  }
+}
+
+__global__ void k_rx_zero(uint32_t* __restrict__ p, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host --
+// Sort variant: scheme x tile threads.  LDS per block ~ NT*16*rec bytes, so
+// 12-byte records stop at 512 threads (96 KB), 8-byte ones at 1024 (128 KB).
+struct SortVariant {
+  int scheme = kThreeLaunch;
+  int nt = 256;
+  int bits = 8;  // widest digit: 8, or 9 with 512 threads
+};
+
+// process-wide choice per record size (eg_set_sort_variant / autotuner)
+SortVariant& sort_variant(int rec_bytes);
+
+// Device workspace (grow-only, owned by the caller).
+struct RxWork {
+  DevBuf table, ghist, status, ticket, segs, err, scan;
+  uint32_t epoch = 0;
+  int resize_segs(int nseg) {
+    EG_TRY(segs.resize((uint64_t(nseg) + 1) * 12 + 16));
+    EG_TRY(ticket.resize(kRxMaxPasses * 4));
+    EG_TRY(err.resize(4));
+    return EG_OK;
+  }
+};
+
+template <int NT, int BM, class R, class KeyFn>
+int rx_sort_nt(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, int scheme,
+               RxWork& w, hipStream_t s) {
+  static_assert((1 << BM) <= NT, "one digit per thread in the count/scan steps");
+  static_assert(NT * kRxIPT * sizeof(R) + (NT / kWave) * (4 << BM) + (8 << BM) <= 160 * 1024, "LDS of one block");
+  constexpr int TILE = NT * kRxIPT;
+  const int nseg = int(seg.size()) - 1;
+  std::vector<uint32_t> ts(nseg + 1, 0);
+  for (int k = 0; k < nseg; ++k) ts[k + 1] = ts[k] + uint32_t(ceil_div(seg[k + 1] - seg[k], TILE));
+  const uint32_t tiles = ts[nseg];
+  if (tiles == 0) return EG_OK;
+  EG_TRY(w.resize_segs(nseg));
+  uint64_t* d_seg = w.segs.as<uint64_t>();
+  uint32_t* d_ts = reinterpret_cast<uint32_t*>(d_seg + nseg + 1);
+  EG_HIP(hipMemcpyAsync(d_seg, seg.data(), (nseg + 1) * 8, hipMemcpyHostToDevice, s));
+  EG_HIP(hipMemcpyAsync(d_ts, ts.data(), (nseg + 1) * 4, hipMemcpyHostToDevice, s));
+  const uint64_t n = seg[nseg] - seg[0];
+  const RxPlan plan = rx_plan(key_bits, BM);
+  const R* src = in;
+  if (scheme == kThreeLaunch) {
+    const uint64_t tn = uint64_t(tiles) << kRxMaxBits;
+    EG_TRY(w.table.resize(tn * 4));
+    if (w.scan.bytes < scan_tmp_elems(tn) * 4) EG_TRY(w.scan.resize(scan_tmp_elems(tn) * 4));
+    for (int p = 0; p < plan.passes; ++p) {
+      R* dst = ((plan.passes - 1 - p) % 2 == 0) ? out : tmp;
+      prof_begin(s);
+      hipLaunchKernelGGL((k_rx_up<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, (const uint64_t*)d_seg,
+                         (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p], w.table.as<uint32_t>());
+      prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
+      EG_HIP(hipGetLastError());
+      EG_TRY(exclusive_scan_u32(w.table.as<uint32_t>(), w.table.as<uint32_t>(), uint64_t(tiles) << plan.bits[p],
+                                w.scan.as<uint32_t>(), s));
+      prof_begin(s);
+      hipLaunchKernelGGL((k_rx_down<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, (const uint64_t*)d_seg,
+                         (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p],
+                         (const uint32_t*)w.table.as<uint32_t>());
+      prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
+      EG_HIP(hipGetLastError());
+      src = dst;
+    }
+    return EG_OK;
+  }
+  // one-pass
+  EG_TRY(w.ghist.resize(uint64_t(nseg) * kRxMaxPasses * (1 << kRxMaxBits) * 4));
+  const uint64_t sb = uint64_t(tiles) * (1 << kRxMaxBits) * 8;
+  if (w.status.bytes < sb) {
+    EG_TRY(w.status.resize(sb));
+    EG_HIP(hipMemsetAsync(w.status.ptr, 0, w.status.bytes, s));  // epoch 0 is never used
+  }
+  RxPassArgs pa;
+  pa.npass = plan.passes;
+  for (int p = 0; p < kRxMaxPasses; ++p) {
+    pa.shift[p] = plan.shift[p];
+    pa.bits[p] = plan.bits[p];
+  }
+  const uint64_t nh = uint64_t(nseg) * kRxMaxPasses * (1 << kRxMaxBits);
+  hipLaunchKernelGGL(k_rx_zero, dim3(unsigned(std::min<uint64_t>(ceil_div(nh, 256), 4096))), dim3(256), 0, s,
+                     w.ghist.as<uint32_t>(), nh);
+  hipLaunchKernelGGL(k_rx_zero, dim3(1), dim3(64), 0, s, w.ticket.as<uint32_t>(), uint64_t(kRxMaxPasses));
+  prof_begin(s);
+  hipLaunchKernelGGL((k_rx_hist_all<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, in, (const uint64_t*)d_seg,
+                     (const uint32_t*)d_ts, nseg, key, pa, w.ghist.as<uint32_t>());
+  prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
+  EG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_rx_base, dim3(nseg), dim3(64), 0, s, w.ghist.as<uint32_t>(), (const uint64_t*)d_seg, nseg,
+                     plan.passes);
+  EG_HIP(hipGetLastError());
+  for (int p = 0; p < plan.passes; ++p) {
+    R* dst = ((plan.passes - 1 - p) % 2 == 0) ? out : tmp;
+    const uint32_t epoch = ++w.epoch;
+    prof_begin(s);
+    hipLaunchKernelGGL((k_rx_onepass<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, (const uint64_t*)d_seg,
+                       (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p],
+                       (const uint32_t*)w.ghist.as<uint32_t>() + uint64_t(p) * (1 << kRxMaxBits),
+                       w.status.as<uint64_t>(), w.ticket.as<uint32_t>() + p, epoch, w.err.as<uint32_t>());
+    prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
+    EG_HIP(hipGetLastError());
+    src = dst;
+  }
+  return EG_OK;
+}
+
+// Sort every segment [seg[k], seg[k+1]) of `in` stably by key into the same
+// positions of `out` (tmp = ping-pong; in, out, tmp distinct; in unchanged).
+// The caller zeroes w.err before and checks it after synchronising.
+template <class R, class KeyFn>
+int rx_sort(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w,
+            hipStream_t s) {
+  const int nseg = int(seg.size()) - 1;
+  if (nseg < 1 || seg.back() == seg.front()) return EG_OK;
+  if (seg.front() != 0) return fail(EG_ERR_ARG, "internal: sort segments must start at 0");
+  if (seg.back() >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "sort of more than 2^32-1 records");
+  const SortVariant v = sort_variant(int(sizeof(R)));
+  if (v.nt == 256) return rx_sort_nt<256, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
+  if constexpr (sizeof(R) <= 8) {
+    if (v.nt >= 1024) return rx_sort_nt<1024, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
+  }
+  if (v.bits >= 9) return rx_sort_nt<512, 9>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
+  return rx_sort_nt<512, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
+}
+
+}  // namespace eg

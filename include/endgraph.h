@@ -67,6 +67,21 @@ int eg_profile_enable(int on);
 int eg_profile_reset(void);
 int eg_profile_read(const char* kernel, double* total_ms, uint64_t* launches, uint64_t* bytes);
 
+/* ---- radix-sort variant (no reference counterpart) ------------------------
+ * Every stable sort of the build is a segmented LSD radix sort with two pass
+ * schemes (0: upsweep/scan/downsweep, 1: one pass per digit with decoupled
+ * look-back), a tile of threads x 16 records (threads 256, 512 or 1024;
+ * 1024 only for 8-byte records) and digits of up to 8 or 9 bits (9 only with
+ * 512 threads).  Process-wide per record size; chosen by eg_autotune_sort or
+ * set here.  Results never depend on the variant. */
+int eg_set_sort_variant(int rec_bytes, int scheme, int threads, int digit_bits);
+int eg_get_sort_variant(int rec_bytes, int* scheme, int* threads, int* digit_bits);
+/* Builds g from the given list once per candidate variant (twice each, the
+ * second timed), keeps the fastest for g's record size and leaves g built
+ * with it.  Synchronises. */
+int eg_autotune_sort(eg_graph* g, const void* d_edges, uint64_t E0, int symmetrize, int* h_scheme, int* h_threads,
+                     int* h_digit_bits, double* h_ms, void* stream);
+
 /* ---- synthetic inputs (no reference counterpart: the reference reads
  *      binary edge files, README.md:40-41; these write the same records) -- */
 /* R-MAT: 2^scale vertices, E records, quadrant probabilities a,b,c

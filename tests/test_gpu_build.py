@@ -90,3 +90,43 @@ def test_rebuild_reuses_handle(eg, oracle_lib):
         g.build(buf.ptr, E)
         og = oracle_lib.OracleGraph(edges, V, P=2, S=1, eta=1, m=1, threads=4)
         assert_build_equal(g, og)
+
+
+VARIANTS = [(256, 8), (512, 8), (512, 9), (1024, 8)]
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+@pytest.mark.parametrize("threads,bits", VARIANTS)
+@pytest.mark.parametrize("rec", [8, 12])
+def test_build_every_sort_variant(eg, oracle_lib, scheme, threads, bits, rec):
+    if rec == 12 and threads == 1024:
+        pytest.skip("12-byte records use at most 512 threads (LDS)")
+    print(f"sort variant scheme={scheme} threads={threads} bits={bits} rec={rec}", flush=True)
+    rng = np.random.default_rng(77)
+    V = 9 * 4096 + 100
+    edges = H.random_edges(rng, V, 70001, weighted=(rec == 12))
+    keep = eg.get_sort_variant(rec)
+    eg.set_sort_variant(rec, scheme, threads, bits)
+    try:
+        for P, S, sym in ((1, 1, False), (4, 2, True)):
+            g, _ = _gpu_build(eg, edges, V, P, S, 2, 5, sym)
+            og = oracle_lib.OracleGraph(edges, V, P=P, S=S, eta=2, m=5, symmetrize=sym, threads=8)
+            assert_build_equal(g, og)
+    finally:
+        eg.set_sort_variant(rec, *keep)
+
+
+def test_autotune_picks_a_variant_and_stays_exact(eg, oracle_lib):
+    rng = np.random.default_rng(78)
+    V = 50000
+    edges = H.random_edges(rng, V, 200000)
+    buf = eg.DeviceBuffer.from_array(edges)
+    g = eg.Graph(V, 3, 1, 1, 1, 8)
+    keep = eg.get_sort_variant(8)
+    try:
+        scheme, threads, bits, ms = g.autotune_sort(buf.ptr, edges.shape[0])
+        assert scheme in (0, 1) and threads in (256, 512, 1024) and bits in (8, 9) and ms > 0
+        og = oracle_lib.OracleGraph(edges, V, P=3, S=1, eta=1, m=1, threads=8)
+        assert_build_equal(g, og)
+    finally:
+        eg.set_sort_variant(8, *keep)
