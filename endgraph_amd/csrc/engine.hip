@@ -231,6 +231,7 @@ int ensure_engine(eg_graph* g, hipStream_t s) {
   if (!g) return fail(EG_ERR_ARG, "graph is NULL");
   if (!g->built) return fail(EG_ERR_STATE, "graph has not been built");
   if (g->eng_ready) return EG_OK;
+  if (g->E >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "the engine's edge offsets are 32-bit: need E < 2^32");
   auto t0 = std::chrono::steady_clock::now();
   int rc = g->rec == 8 ? prepare_impl<Rec8>(g, s) : prepare_impl<Rec12>(g, s);
   if (rc != EG_OK) return rc;
@@ -290,50 +291,67 @@ __device__ __forceinline__ T seg_scan(T v, bool head, Op op) {
 }
 
 template <class T, class Op, class Val, class Emit>
-__device__ __forceinline__ void row_reduce_wave(uint64_t E, const uint64_t* __restrict__ rowflag,
+__device__ __forceinline__ void row_reduce_wave(uint64_t E, const uint32_t* __restrict__ src,
+                                                const uint64_t* __restrict__ rowflag,
                                                 const uint32_t* __restrict__ wordrank, T identity, Op op, Val val,
                                                 Emit emit) {
+  constexpr int K = kRowWordsPerWave;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t nwords = ceil_div(E, 64);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const uint64_t w0 = gw * kRowWordsPerWave;
+  const uint64_t w0 = gw * K;
   if (w0 >= nwords) return;
-  const uint64_t w1 = w0 + kRowWordsPerWave < nwords ? w0 + kRowWordsPerWave : nwords;
+  const int nwd = w0 + K < nwords ? K : int(nwords - w0);
+  // all row bits, sources and gathered values of the wave's range first, so
+  // every load of the range is in flight before the first use
+  uint64_t f[K];
+  uint32_t base[K], sv[K];
+  T v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint64_t wd = w0 + k;
+    const uint64_t e = wd * 64 + lane;
+    const bool ok = k < nwd;
+    f[k] = ok ? rowflag[wd] : 0ull;
+    base[k] = ok ? wordrank[wd] : 0u;
+    sv[k] = ok && e < E ? src[e] : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint64_t e = (w0 + k) * 64 + lane;
+    v[k] = k < nwd && e < E ? val(e, sv[k]) : identity;
+  }
   const uint64_t lt = (1ull << lane) - 1;
   T carry = identity;
   bool carry_live = false;     // a row is open from earlier edges
   bool carry_shared = true;    // ... and it started before this wave's range
   uint32_t carry_row = 0;
-  for (uint64_t wd = w0; wd < w1; ++wd) {
-    const uint64_t f = rowflag[wd];
-    const uint64_t e = wd * 64 + lane;
-    T v = e < E ? val(e) : identity;
-    const bool head = (f >> lane) & 1ull;
-    const uint32_t base = wordrank[wd];
-    if (lane == 0 && !head) v = op(carry, v);
-    if ((f & 1ull) && carry_live && lane == 0) emit(carry_row, carry, carry_shared);
-    const T inc = seg_scan(v, head, op);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k >= nwd) break;  // wave-uniform
+    const uint64_t e = (w0 + k) * 64 + lane;
+    T x = v[k];
+    const bool head = (f[k] >> lane) & 1ull;
+    if (lane == 0 && !head) x = op(carry, x);
+    if ((f[k] & 1ull) && carry_live && lane == 0) emit(carry_row, carry, carry_shared);
+    const T inc = seg_scan(x, head, op);
     // tails: lanes whose successor starts a row (lane 63 stays open)
-    const bool tail = lane < 63 ? ((f >> (lane + 1)) & 1ull) : false;
-    const uint64_t heads_le = __popcll(f & (lt | (1ull << lane)));
-    const uint32_t row = base + uint32_t(heads_le) - 1;  // rows started before + own
-    const bool first_seg_open = heads_le == 0;           // segment continues the carried row
+    const bool tail = lane < 63 ? ((f[k] >> (lane + 1)) & 1ull) : false;
+    const uint64_t heads_le = __popcll(f[k] & (lt | (1ull << lane)));
+    const uint32_t row = base[k] + uint32_t(heads_le) - 1;  // rows started before + own
+    const bool first_seg_open = heads_le == 0;              // segment continues the carried row
     if (tail && e < E) emit(row, inc, first_seg_open ? carry_shared : false);
     // new carry = lane 63's segment
     const T c63 = shfl_t(inc, 63);
-    const uint64_t h63 = __popcll(f);
+    const uint64_t h63 = __popcll(f[k]);
+    carry = c63;
+    carry_live = true;
     if (h63 == 0) {
-      // no head in this word: the row open before it continues through it
-      carry = c63;
-      carry_live = true;
-      carry_row = base - 1;
+      carry_row = base[k] - 1;  // no head in this word: the open row continues
     } else {
-      carry = c63;
-      carry_live = true;
       carry_shared = false;
-      carry_row = base + uint32_t(h63) - 1;
+      carry_row = base[k] + uint32_t(h63) - 1;
     }
-    if (wd * 64 + 64 > E) break;
   }
   if (lane == 0 && carry_live) emit(carry_row, carry, true);
 }
@@ -349,7 +367,7 @@ __global__ __launch_bounds__(kNT) void k_pr_pull_f64(uint64_t E, const uint32_t*
                                                       const double* __restrict__ curr, double* __restrict__ next,
                                                       int all_atomic) {
   row_reduce_wave<double>(
-      E, rowflag, wordrank, 0.0, OpSum{}, [&](uint64_t e) { return curr[src[e]]; },
+      E, src, rowflag, wordrank, 0.0, OpSum{}, [&](uint64_t, uint32_t s) { return curr[s]; },
       [&](uint32_t row, double v, bool shared) {
         double* p = &next[row_dst[row]];
         if (shared || all_atomic)
@@ -368,7 +386,7 @@ __global__ __launch_bounds__(kNT) void k_pr_pull_f32c(uint64_t E, const uint32_t
                                                        const float* __restrict__ curr, double* __restrict__ next,
                                                        int all_atomic) {
   row_reduce_wave<F2>(
-      E, rowflag, wordrank, F2{0.f, 0.f}, OpF2{}, [&](uint64_t e) { return F2{curr[src[e]], 0.f}; },
+      E, src, rowflag, wordrank, F2{0.f, 0.f}, OpF2{}, [&](uint64_t, uint32_t s) { return F2{curr[s], 0.f}; },
       [&](uint32_t row, F2 v, bool shared) {
         double* p = &next[row_dst[row]];
         const double t = double(v.s) + double(v.c);
@@ -474,29 +492,125 @@ __global__ __launch_bounds__(kNT) void k_level_end(const uint64_t* __restrict__ 
   }
 }
 
-// --------------------------------------------------------------- BFS ----
+// ------------------------------------------------------ sparse (push) ----
+// Edge-balanced frontier expansion: block b takes the 2048 consecutive slots
+// [b*2048, (b+1)*2048) of the frontier's concatenated out-adjacency
+// (foff = exclusive prefix of the frontier degrees, foff[fn] = total); each
+// slot finds its frontier vertex by binary search, in an LDS copy of the
+// block's window of foff when it fits.  A hub's edges spread over many
+// blocks instead of one wave.
+constexpr int kExpSlots = kNT * 8;
+constexpr int kExpWin = kExpSlots + 2;
+
+__device__ __forceinline__ uint32_t last_le_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t x) {
+  // largest i in [lo, hi) with a[i] <= x (a non-decreasing, a[lo] <= x)
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (a[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kNT) void k_expand(const uint32_t* __restrict__ flist, const uint32_t* __restrict__ foff,
+                                                uint32_t fn, const uint32_t* __restrict__ out_ptr, Op op) {
+  __shared__ uint32_t win[kExpWin];
+  __shared__ uint32_t s_lo, s_hi;
+  const uint32_t total = foff[fn];
+  const uint32_t t0 = blockIdx.x * uint32_t(kExpSlots);
+  if (t0 >= total) return;  // block-uniform
+  const uint32_t t1 = uint32_t(uint64_t(t0) + kExpSlots < total ? uint64_t(t0) + kExpSlots : total);
+  if (threadIdx.x == 0) {
+    s_lo = last_le_u32(foff, 0, fn, t0);
+    s_hi = last_le_u32(foff, s_lo, fn, t1 - 1);
+  }
+  __syncthreads();
+  const uint32_t lo = s_lo, hi = s_hi;  // vertices holding the block's slots
+  const bool in_lds = hi - lo + 2 <= uint32_t(kExpWin);
+  if (in_lds)
+    for (uint32_t i = threadIdx.x; i < hi - lo + 2; i += kNT) win[i] = foff[lo + i];
+  __syncthreads();
+  uint32_t u[8], e[8];
+  bool ok[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const uint64_t t64 = uint64_t(t0) + uint32_t(k) * kNT + threadIdx.x;
+    ok[k] = t64 < t1;
+    const uint32_t t = uint32_t(t64);
+    uint32_t i, start;
+    if (in_lds) {
+      i = lo + last_le_u32(win, 0, hi - lo + 1, ok[k] ? t : t0);
+      start = win[i - lo];
+    } else {
+      i = last_le_u32(foff, lo, hi + 1, ok[k] ? t : t0);
+      start = foff[i];
+    }
+    u[k] = flist[i];
+    e[k] = out_ptr[u[k]] + ((ok[k] ? t : t0) - start);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (ok[k]) op(u[k], e[k]);
+}
+
 // Sparse slot of toolkits/bfs.cpp:52-58 with R12: an unvisited out-neighbour
 // takes the smallest frontier id as parent (atomicMin; visited is not
 // modified during a level, so the rule is order-independent).
-__global__ __launch_bounds__(kNT) void k_bfs_push(const uint32_t* __restrict__ flist, uint32_t fn,
-                                                   const uint32_t* __restrict__ out_ptr,
-                                                   const uint32_t* __restrict__ out_dst,
-                                                   const uint64_t* __restrict__ visited, uint32_t* __restrict__ parent,
-                                                   uint64_t* __restrict__ active_out) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  for (uint64_t i = gw; i < fn; i += nw) {
-    const uint32_t u = flist[i];
-    const uint32_t b = out_ptr[u], e = out_ptr[u + 1];
-    for (uint32_t k = b + lane; k < e; k += kWave) {
-      const uint32_t v = out_dst[k];
-      if (!bm_get(visited, v) && parent[v] > u) {
-        atomicMin(&parent[v], u);
-        if (!bm_get(active_out, v)) bm_set(active_out, v);
-      }
+struct BfsPushOp {
+  const uint32_t* out_dst;
+  const uint64_t* visited;
+  uint32_t* parent;
+  uint64_t* next;
+  __device__ __forceinline__ void operator()(uint32_t u, uint32_t e) const {
+    const uint32_t v = out_dst[e];
+    if (!bm_get(visited, v) && parent[v] > u) {
+      atomicMin(&parent[v], u);
+      if (!bm_get(next, v)) bm_set(next, v);
     }
   }
+};
+
+// toolkits/cc.cpp:47-58: a smaller label flows to out-neighbours.
+struct CcPushOp {
+  const uint32_t* out_dst;
+  uint32_t* label;
+  uint64_t* next;
+  __device__ __forceinline__ void operator()(uint32_t u, uint32_t e) const {
+    const uint32_t v = out_dst[e];
+    const uint32_t msg = label[u];
+    if (msg < label[v]) {
+      atomicMin(&label[v], msg);
+      bm_set(next, v);
+    }
+  }
+};
+
+// toolkits/sssp.cpp:44-58: relax = dist[src] + w (one fp32 add, round to
+// nearest); distances are non-negative, so the float order is the order of
+// their bit patterns and atomicMin on the bits is write_min.
+struct SsspPushOp {
+  const uint32_t* out_dst;
+  const float* out_w;
+  float* dist;
+  uint64_t* next;
+  __device__ __forceinline__ void operator()(uint32_t u, uint32_t e) const {
+    const uint32_t v = out_dst[e];
+    const float nd = __fadd_rn(dist[u], out_w[e]);
+    if (nd < dist[v]) {
+      const int old = atomicMin(reinterpret_cast<int*>(&dist[v]), __float_as_int(nd));
+      if (nd < __int_as_float(old)) bm_set(next, v);
+    }
+  }
+};
+
+// frontier degrees for the expansion's prefix
+__global__ void k_front_deg(const uint32_t* __restrict__ flist, uint32_t fn, const uint32_t* __restrict__ out_ptr,
+                            uint32_t* __restrict__ fdeg) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i <= fn; i += uint64_t(gridDim.x) * blockDim.x)
+    fdeg[i] = i < fn ? out_ptr[flist[i] + 1] - out_ptr[flist[i]] : 0u;
 }
 
 // Dense signal of toolkits/bfs.cpp:60-68 with R12: for an unvisited row, the
@@ -523,28 +637,6 @@ __global__ __launch_bounds__(kNT) void k_bfs_pull(uint64_t R, const uint32_t* __
 }
 
 // ---------------------------------------------------------------- CC -----
-// toolkits/cc.cpp:47-79.  Push: a smaller label flows to out-neighbours.
-__global__ __launch_bounds__(kNT) void k_cc_push(const uint32_t* __restrict__ flist, uint32_t fn,
-                                                  const uint32_t* __restrict__ out_ptr,
-                                                  const uint32_t* __restrict__ out_dst, uint32_t* __restrict__ label,
-                                                  uint64_t* __restrict__ active_out) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  for (uint64_t i = gw; i < fn; i += nw) {
-    const uint32_t u = flist[i];
-    const uint32_t msg = label[u];
-    const uint32_t b = out_ptr[u], e = out_ptr[u + 1];
-    for (uint32_t k = b + lane; k < e; k += kWave) {
-      const uint32_t v = out_dst[k];
-      if (msg < label[v]) {
-        atomicMin(&label[v], msg);
-        bm_set(active_out, v);
-      }
-    }
-  }
-}
-
 // Pull: min label over in-edges (starting from dst itself, cc.cpp:60-68).
 __global__ __launch_bounds__(kNT) void k_cc_pull(uint64_t E, const uint32_t* __restrict__ src,
                                                   const uint64_t* __restrict__ rowflag,
@@ -552,7 +644,7 @@ __global__ __launch_bounds__(kNT) void k_cc_pull(uint64_t E, const uint32_t* __r
                                                   const uint32_t* __restrict__ row_dst, uint32_t* __restrict__ label,
                                                   uint64_t* __restrict__ active_out) {
   row_reduce_wave<uint32_t>(
-      E, rowflag, wordrank, 0xffffffffu, OpMin{}, [&](uint64_t e) { return label[src[e]]; },
+      E, src, rowflag, wordrank, 0xffffffffu, OpMin{}, [&](uint64_t, uint32_t s) { return label[s]; },
       [&](uint32_t row, uint32_t m, bool) {
         const uint32_t v = row_dst[row];
         if (m < v && m < label[v]) {
@@ -563,32 +655,6 @@ __global__ __launch_bounds__(kNT) void k_cc_pull(uint64_t E, const uint32_t* __r
 }
 
 // --------------------------------------------------------------- SSSP ----
-// toolkits/sssp.cpp:44-58: relax = dist[src] + w (one fp32 add, round to
-// nearest); distances are non-negative, so the float order is the order of
-// their bit patterns and atomicMin on the bits is write_min.
-__global__ __launch_bounds__(kNT) void k_sssp_push(const uint32_t* __restrict__ flist, uint32_t fn,
-                                                    const uint32_t* __restrict__ out_ptr,
-                                                    const uint32_t* __restrict__ out_dst,
-                                                    const float* __restrict__ out_w, float* __restrict__ dist,
-                                                    uint64_t* __restrict__ active_out) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  for (uint64_t i = gw; i < fn; i += nw) {
-    const uint32_t u = flist[i];
-    const float du = dist[u];
-    const uint32_t b = out_ptr[u], e = out_ptr[u + 1];
-    for (uint32_t k = b + lane; k < e; k += kWave) {
-      const uint32_t v = out_dst[k];
-      const float nd = __fadd_rn(du, out_w[k]);
-      if (nd < dist[v]) {
-        const int old = atomicMin(reinterpret_cast<int*>(&dist[v]), __float_as_int(nd));
-        if (nd < __int_as_float(old)) bm_set(active_out, v);
-      }
-    }
-  }
-}
-
 // Pull: min over in-edges of dist[src] + w (sssp.cpp:60-73).
 __global__ __launch_bounds__(kNT) void k_sssp_pull(uint64_t E, const uint32_t* __restrict__ src,
                                                     const float* __restrict__ w, const uint64_t* __restrict__ rowflag,
@@ -596,7 +662,7 @@ __global__ __launch_bounds__(kNT) void k_sssp_pull(uint64_t E, const uint32_t* _
                                                     const uint32_t* __restrict__ row_dst, float* __restrict__ dist,
                                                     uint64_t* __restrict__ active_out) {
   row_reduce_wave<float>(
-      E, rowflag, wordrank, 1e9f, OpMin{}, [&](uint64_t e) { return __fadd_rn(dist[src[e]], w[e]); },
+      E, src, rowflag, wordrank, 1e9f, OpMin{}, [&](uint64_t e, uint32_t s) { return __fadd_rn(dist[s], w[e]); },
       [&](uint32_t row, float m, bool) {
         const uint32_t v = row_dst[row];
         if (m < 1e9f && m < dist[v]) {
@@ -615,6 +681,7 @@ namespace {
 
 struct Scratch {
   uint32_t* flist;                 // frontier list, V entries
+  uint32_t* foff;                  // prefix of the frontier degrees, V+1 entries
   uint32_t* wcnt;                  // per-word counts, words+1
   unsigned long long* counters;    // [0] new vertices, [1] their out-edges
   double* delta;                   // PageRank residuals, iterations+1
@@ -623,7 +690,8 @@ struct Scratch {
 uint64_t words_of(uint32_t V) { return (uint64_t(V) >> 6) + 1; }
 
 uint64_t scratch_bytes(uint32_t V, int iters) {
-  return uint64_t(V) * 4 + 256 + (words_of(V) + 1) * 4 + 256 + 64 + uint64_t(iters + 2) * 8 + 256;
+  return uint64_t(V) * 4 + 256 + (uint64_t(V) + 1) * 4 + 256 + (words_of(V) + 1) * 4 + 256 + 64 +
+         uint64_t(iters + 2) * 8 + 256;
 }
 
 int get_scratch(eg_graph* g, int iters, Scratch* sc) {
@@ -634,6 +702,8 @@ int get_scratch(eg_graph* g, int iters, Scratch* sc) {
   uint64_t o = 0;
   sc->flist = reinterpret_cast<uint32_t*>(p + o);
   o = align(o + uint64_t(g->V) * 4);
+  sc->foff = reinterpret_cast<uint32_t*>(p + o);
+  o = align(o + (uint64_t(g->V) + 1) * 4);
   sc->wcnt = reinterpret_cast<uint32_t*>(p + o);
   o = align(o + (words_of(g->V) + 1) * 4);
   sc->counters = reinterpret_cast<unsigned long long*>(p + o);
@@ -642,8 +712,9 @@ int get_scratch(eg_graph* g, int iters, Scratch* sc) {
   return EG_OK;
 }
 
-// ascending list of the set bits of bm; returns the count on the host
-int make_list(eg_graph* g, const uint64_t* bm, Scratch& sc, uint32_t* h_n, hipStream_t s) {
+// Ascending list of the set bits of bm and the prefix of their out-degrees;
+// returns the count and the total degree on the host.
+int make_frontier(eg_graph* g, const uint64_t* bm, Scratch& sc, uint32_t* h_n, uint32_t* h_total, hipStream_t s) {
   const uint64_t words = words_of(g->V);
   hipLaunchKernelGGL(k_bm_count, dim3(grid_for(words, kNT, 65536)), dim3(kNT), 0, s, bm, words, sc.wcnt);
   EG_HIP(hipGetLastError());
@@ -654,6 +725,23 @@ int make_list(eg_graph* g, const uint64_t* bm, Scratch& sc, uint32_t* h_n, hipSt
   EG_HIP(hipGetLastError());
   EG_HIP(hipMemcpyAsync(h_n, sc.wcnt + words, 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
+  const uint32_t fn = *h_n;
+  hipLaunchKernelGGL(k_front_deg, dim3(grid_for(uint64_t(fn) + 1, kNT, 65536)), dim3(kNT), 0, s,
+                     (const uint32_t*)sc.flist, fn, (const uint32_t*)g->e_out_ptr.as<uint32_t>(), sc.foff);
+  EG_HIP(hipGetLastError());
+  EG_TRY(exclusive_scan_u32(sc.foff, sc.foff, uint64_t(fn) + 1, g->scan_tmp.as<uint32_t>(), s));
+  EG_HIP(hipMemcpyAsync(h_total, sc.foff + fn, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+template <class Op>
+int expand(eg_graph* g, Scratch& sc, uint32_t fn, uint32_t total, Op op, hipStream_t s) {
+  if (!fn || !total) return EG_OK;
+  hipLaunchKernelGGL((k_expand<Op>), dim3(unsigned(ceil_div(total, kExpSlots))), dim3(kNT), 0, s,
+                     (const uint32_t*)sc.flist, (const uint32_t*)sc.foff, fn,
+                     (const uint32_t*)g->e_out_ptr.as<uint32_t>(), op);
+  EG_HIP(hipGetLastError());
   return EG_OK;
 }
 
@@ -674,7 +762,6 @@ int level_end(eg_graph* g, const uint64_t* out, uint64_t* visited, uint32_t* dep
   return EG_OK;
 }
 
-unsigned push_grid(uint64_t fn) { return grid_for(fn, kNT / kWave, 16384); }
 unsigned row_grid(uint64_t E) { return grid_for(ceil_div(ceil_div(E, 64), kRowWordsPerWave), kNT / kWave, 1u << 30); }
 
 // core/graph.hpp:1483-1498: rooted programs are sparse under the reference
@@ -875,13 +962,11 @@ int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_fron
   EG_TRY(fill<uint64_t>(d_next, words_of(g->V), 0ull, s));
   prof_begin(s);
   if (sparse) {
-    uint32_t fn = 0;
-    EG_TRY(make_list(g, d_frontier, sc, &fn, s));
-    if (fn)
-      hipLaunchKernelGGL(k_bfs_push, dim3(push_grid(fn)), dim3(kNT), 0, s, (const uint32_t*)sc.flist, fn,
-                         (const uint32_t*)g->e_out_ptr.as<uint32_t>(), (const uint32_t*)g->e_out_dst.as<uint32_t>(),
-                         (const uint64_t*)d_visited, d_parent, d_next);
-    prof_end(PK_BFS, frontier_edges * 4 + uint64_t(fn) * 8, s);
+    uint32_t fn = 0, total = 0;
+    EG_TRY(make_frontier(g, d_frontier, sc, &fn, &total, s));
+    EG_TRY(expand(g, sc, fn, total,
+                  BfsPushOp{g->e_out_dst.as<uint32_t>(), (const uint64_t*)d_visited, d_parent, d_next}, s));
+    prof_end(PK_BFS, uint64_t(total) * 4 + uint64_t(fn) * 8, s);
   } else if (g->R) {
     hipLaunchKernelGGL(k_bfs_pull, dim3(grid_for(g->R, kNT, 65536)), dim3(kNT), 0, s, g->R,
                        (const uint32_t*)g->e_row_start.as<uint32_t>(), (const uint32_t*)g->e_row_dst.as<uint32_t>(),
@@ -939,12 +1024,9 @@ int eg_cc_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t*
   EG_TRY(get_scratch(g, 0, &sc));
   EG_TRY(fill<uint64_t>(d_next, words_of(g->V), 0ull, s));
   if (choose_sparse(g, policy, active_edges, false)) {
-    uint32_t fn = 0;
-    EG_TRY(make_list(g, d_active, sc, &fn, s));
-    if (fn)
-      hipLaunchKernelGGL(k_cc_push, dim3(push_grid(fn)), dim3(kNT), 0, s, (const uint32_t*)sc.flist, fn,
-                         (const uint32_t*)g->e_out_ptr.as<uint32_t>(), (const uint32_t*)g->e_out_dst.as<uint32_t>(),
-                         d_label, d_next);
+    uint32_t fn = 0, total = 0;
+    EG_TRY(make_frontier(g, d_active, sc, &fn, &total, s));
+    EG_TRY(expand(g, sc, fn, total, CcPushOp{g->e_out_dst.as<uint32_t>(), d_label, d_next}, s));
   } else if (g->E) {
     hipLaunchKernelGGL(k_cc_pull, dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E,
                        (const uint32_t*)g->e_csc_src.as<uint32_t>(), (const uint64_t*)g->e_rowflag.as<uint64_t>(),
@@ -985,12 +1067,10 @@ int eg_sssp_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_
   EG_TRY(get_scratch(g, 0, &sc));
   EG_TRY(fill<uint64_t>(d_next, words_of(g->V), 0ull, s));
   if (choose_sparse(g, policy, active_edges, true)) {
-    uint32_t fn = 0;
-    EG_TRY(make_list(g, d_active, sc, &fn, s));
-    if (fn)
-      hipLaunchKernelGGL(k_sssp_push, dim3(push_grid(fn)), dim3(kNT), 0, s, (const uint32_t*)sc.flist, fn,
-                         (const uint32_t*)g->e_out_ptr.as<uint32_t>(), (const uint32_t*)g->e_out_dst.as<uint32_t>(),
-                         (const float*)g->e_out_w.as<float>(), d_dist, d_next);
+    uint32_t fn = 0, total = 0;
+    EG_TRY(make_frontier(g, d_active, sc, &fn, &total, s));
+    EG_TRY(expand(g, sc, fn, total,
+                  SsspPushOp{g->e_out_dst.as<uint32_t>(), g->e_out_w.as<float>(), d_dist, d_next}, s));
   } else if (g->E) {
     hipLaunchKernelGGL(k_sssp_pull, dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E,
                        (const uint32_t*)g->e_csc_src.as<uint32_t>(), (const float*)g->e_csc_w.as<float>(),
