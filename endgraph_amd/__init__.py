@@ -102,6 +102,7 @@ def lib():
         "eg_pr_pull": ([vp, i32, vp, vp, vp], i32),
         "eg_pr_apply": ([vp, vp, vp, vp, i32, vp, vp], i32),
         "eg_pagerank": ([vp, i32, i32, vp, vp, vp], i32),
+        "eg_pagerank_view_prepare": ([vp, C.POINTER(f64), vp], i32),
         "eg_bfs_level": ([vp, i32, u32, vp, vp, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64),
                           C.POINTER(i32), vp], i32),
         "eg_bfs": ([vp, u32, i32, vp, vp, C.POINTER(u32), C.POINTER(u64), vp], i32),
@@ -175,6 +176,9 @@ def to_host(d_ptr: int, dtype, count: int, stream=None) -> np.ndarray:
 
 POLICY_REFERENCE, POLICY_AUTO, POLICY_PUSH, POLICY_PULL = 0, 1, 2, 3
 PR_FP64, PR_FP32C = 0, 1
+# layout flags OR-ed into the precision of Graph.pagerank (defaults: FP64
+# plain, FP32C degree-ordered)
+PR_ORDERED, PR_PLAIN = 0x10, 0x20
 
 
 def set_sort_variant(rec_bytes: int, scheme: int, threads: int, digit_bits: int = 8):
@@ -358,6 +362,12 @@ class Graph:
     def prepare_engine(self, stream=None) -> float:
         ms = C.c_double()
         check(lib().eg_engine_prepare(self._h, C.byref(ms), _stream(stream)))
+        return ms.value
+
+    def prepare_pagerank_view(self, stream=None) -> float:
+        """Derive the degree-ordered PageRank view now; returns its wall ms."""
+        ms = C.c_double(0)
+        check(lib().eg_pagerank_view_prepare(self._h, C.byref(ms), _stream(stream)))
         return ms.value
 
     def pagerank(self, iterations: int, precision: int = PR_FP64, out: DeviceBuffer = None, stream=None):

@@ -672,6 +672,7 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   const uint64_t rb = sizeof(R);
   g->built = false;
   g->eng_ready = false;
+  g->pv_ready = false;
   g->E0 = E0;
   g->E = E;
   g->symmetrized = sym;

@@ -58,7 +58,8 @@ int fill(T* p, uint64_t n, T v, hipStream_t s) {
 
 // ------------------------------------------------------------- prepare --
 // SoA source (and weight) of every CSC edge, and the row-start bit per edge
-// (a row = one compressed-index entry: a destination within a sub-partition).
+// (a row = one compressed-index entry: a destination within a sub-partition;
+// without qmap, one destination).
 template <class R>
 __global__ __launch_bounds__(kNT) void k_prep_csc(const R* __restrict__ lst, uint64_t E, uint32_t* __restrict__ src,
                                                   float* __restrict__ w, uint64_t* __restrict__ rowflag,
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(kNT) void k_prep_csc(const R* __restrict__ lst, uin
         head = true;
       } else {
         const R p = lst[e - 1];
-        head = p.dst != r.dst || qmap[p.src >> kPageShift] != qmap[r.src >> kPageShift];
+        head = p.dst != r.dst || (qmap && qmap[p.src >> kPageShift] != qmap[r.src >> kPageShift]);
       }
     }
     const uint64_t m = __ballot(head);
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(kNT) void k_prep_rows(const R* __restrict__ lst, ui
     if ((f >> lane) & 1ull) {
       const uint64_t e = wd * 64 + lane;
       const uint32_t r = wordrank[wd] + uint32_t(__popcll(f & ((1ull << lane) - 1)));
-      row_start[r] = uint32_t(e);
+      if (row_start) row_start[r] = uint32_t(e);
       row_dst[r] = lst[e].dst;
     }
   }
@@ -240,12 +241,147 @@ int ensure_engine(eg_graph* g, hipStream_t s) {
   return EG_OK;
 }
 
+// ------------------------------------------ degree-ordered PageRank view --
+// The pull gathers x[src] once per in-edge, so a source is read out-degree
+// times.  Renumbering vertices by descending out-degree (ties by id) packs
+// the most-read values into the fewest cache lines: on a twitter-shaped graph
+// the top 2^20 sources hold 85% of the edges, 4 MiB of FP32 values, where
+// the original ids spread them over one cache line each.  Within a row the
+// edges are ordered by new source id, so the 64 lanes of a gather step read
+// ascending, mostly neighbouring values (one cache line serves many lanes in
+// the long rows of high in-degree).  Each row adds the same terms as the
+// built CSC; only their order differs, which moves an FP64 row sum by a few
+// ulps and a compensated FP32 row sum not at all at the FP32 rounding.
+struct KeyDstField {
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
+};
+struct KeyDstSrc {
+  int vb;  // bits of a vertex id
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const { return (uint64_t(r.dst) << vb) | r.src; }
+};
+
+// record v = (v, E - deg[v]): ascending key = descending degree
+__global__ void k_deg_keys(const uint32_t* __restrict__ deg, uint32_t V, uint64_t E, Rec8* __restrict__ rec) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
+    rec[v] = Rec8{uint32_t(v), uint32_t(E - deg[v])};
+}
+
+__global__ void k_view_perm(const Rec8* __restrict__ sorted, uint32_t V, const uint32_t* __restrict__ deg,
+                            uint32_t* __restrict__ perm, uint32_t* __restrict__ inv, uint32_t* __restrict__ vdeg) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < V; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t u = sorted[i].src;
+    inv[i] = u;
+    perm[u] = uint32_t(i);
+    vdeg[i] = deg[u];
+  }
+}
+
+template <class R>
+__global__ void k_view_relabel(const R* __restrict__ csc, uint64_t E, const uint32_t* __restrict__ perm,
+                               Rec8* __restrict__ out) {
+  for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += uint64_t(gridDim.x) * blockDim.x) {
+    const R r = csc[e];
+    out[e] = Rec8{perm[r.src], perm[r.dst]};
+  }
+}
+
+template <class R>
+int view_impl(eg_graph* g, hipStream_t s) {
+  const uint64_t E = g->E;
+  const uint32_t V = g->V;
+  const uint64_t nwords = ceil_div(E, 64);
+  EG_TRY(g->pv_perm.resize(uint64_t(V) * 4));
+  EG_TRY(g->pv_inv.resize(uint64_t(V) * 4));
+  EG_TRY(g->pv_deg.resize(uint64_t(V) * 4));
+  EG_TRY(g->pv_src.resize(std::max<uint64_t>(E, 1) * 4));
+  EG_TRY(g->pv_rowflag.resize((nwords + 1) * 8));
+  EG_TRY(g->pv_wordrank.resize((nwords + 2) * 4));
+  const uint64_t n = std::max<uint64_t>(E, V);
+  DevBuf a, b, c;  // sort input, output, ping-pong
+  EG_TRY(a.resize(std::max<uint64_t>(n, 1) * 8));
+  EG_TRY(b.resize(std::max<uint64_t>(n, 1) * 8));
+  EG_TRY(c.resize(std::max<uint64_t>(n, 1) * 8));
+  Rec8* ra = a.as<Rec8>();
+  Rec8* rb = b.as<Rec8>();
+  RxWork w;
+  EG_TRY(w.resize_segs(1));
+  EG_HIP(hipMemsetAsync(w.err.ptr, 0, 4, s));
+  // new ids: stable sort of (v, E - deg) by the second field
+  hipLaunchKernelGGL(k_deg_keys, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
+                     (const uint32_t*)g->out_deg.as<uint32_t>(), V, E, ra);
+  EG_HIP(hipGetLastError());
+  EG_TRY(rx_sort((const Rec8*)ra, rb, c.as<Rec8>(), std::vector<uint64_t>{0, V}, KeyDstField{},
+                 std::max(1, counting_bits(E)), w, s));
+  hipLaunchKernelGGL(k_view_perm, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s, (const Rec8*)rb, V,
+                     (const uint32_t*)g->out_deg.as<uint32_t>(), g->pv_perm.as<uint32_t>(), g->pv_inv.as<uint32_t>(),
+                     g->pv_deg.as<uint32_t>());
+  EG_HIP(hipGetLastError());
+  uint32_t* wordrank = g->pv_wordrank.as<uint32_t>();
+  uint64_t* rowflag = g->pv_rowflag.as<uint64_t>();
+  if (E) {
+    // relabelled CSC sorted by (new destination, new source)
+    hipLaunchKernelGGL((k_view_relabel<R>), dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, g->csc_list.as<R>(), E,
+                       (const uint32_t*)g->pv_perm.as<uint32_t>(), ra);
+    EG_HIP(hipGetLastError());
+    const int vb = std::max(1, counting_bits(V - 1));
+    EG_TRY(rx_sort((const Rec8*)ra, rb, c.as<Rec8>(), std::vector<uint64_t>{0, E}, KeyDstSrc{vb}, 2 * vb, w, s));
+    hipLaunchKernelGGL((k_prep_csc<Rec8>), dim3(grid_for(nwords, kNT / kWave, 65536)), dim3(kNT), 0, s,
+                       (const Rec8*)rb, E, g->pv_src.as<uint32_t>(), (float*)nullptr, rowflag,
+                       (const uint16_t*)nullptr);
+    EG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_popc_words, dim3(grid_for(nwords, kNT, 65536)), dim3(kNT), 0, s, rowflag, nwords, wordrank);
+    EG_HIP(hipGetLastError());
+  }
+  EG_TRY(fill<uint32_t>(wordrank + nwords, 1, 0u, s));
+  EG_TRY(exclusive_scan_u32(wordrank, wordrank, nwords + 1, g->scan_tmp.as<uint32_t>(), s));
+  uint32_t R_h = 0;
+  EG_HIP(hipMemcpyAsync(&R_h, wordrank + nwords, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  g->pv_R = R_h;
+  EG_TRY(g->pv_row_dst.resize((uint64_t(R_h) + 1) * 4));
+  if (E) {
+    hipLaunchKernelGGL((k_prep_rows<Rec8>), dim3(grid_for(nwords, kNT / kWave, 65536)), dim3(kNT), 0, s,
+                       (const Rec8*)rb, E, (const uint64_t*)rowflag, (const uint32_t*)wordrank, (uint32_t*)nullptr,
+                       g->pv_row_dst.as<uint32_t>());
+    EG_HIP(hipGetLastError());
+  }
+  uint32_t sort_err = 0;
+  EG_HIP(hipMemcpyAsync(&sort_err, w.err.ptr, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+  if (sort_err) return fail(EG_ERR_HIP, "radix sort look-back exceeded its spin limit");
+  return EG_OK;
+}
+
+int ensure_view(eg_graph* g, hipStream_t s) {
+  EG_TRY(ensure_engine(g, s));
+  if (g->pv_ready) return EG_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = g->rec == 8 ? view_impl<Rec8>(g, s) : view_impl<Rec12>(g, s);
+  if (rc != EG_OK) return rc;
+  g->pv_prepare_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  g->pv_ready = true;
+  return EG_OK;
+}
+
+__global__ void k_gather_by(const uint32_t* __restrict__ idx, uint32_t V, const double* __restrict__ from,
+                            double* __restrict__ to) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
+    to[v] = from[idx[v]];
+}
+
 // ----------------------------------------------------- dense row reduce --
-// Walks CSC edges 64 per step (wave w: words [w*K, (w+1)*K)), gathers one
+// Walks CSC edges 64 per step (a wave takes words [w0, w0+K)), gathers one
 // value per edge with `val(e)`, reduces rows with a segmented wave scan and
 // hands each finished row's partial to `emit(row, value, shared)`; shared is
 // true for the rows cut by the wave's range ends (they need atomics).
-constexpr int kRowWordsPerWave = 8;
+constexpr int kRowWordsPerWave = 8;  // 4 and 16 measured slower on the pull
+
+// first word of the calling wave's range in a one-range-per-wave grid
+__device__ __forceinline__ uint64_t first_word() {
+  return (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave * kRowWordsPerWave;
+}
 
 // Compensated (Neumaier) pair for the FP32 pull sums.
 struct F2 {
@@ -291,15 +427,13 @@ __device__ __forceinline__ T seg_scan(T v, bool head, Op op) {
 }
 
 template <class T, class Op, class Val, class Emit>
-__device__ __forceinline__ void row_reduce_wave(uint64_t E, const uint32_t* __restrict__ src,
+__device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const uint32_t* __restrict__ src,
                                                 const uint64_t* __restrict__ rowflag,
                                                 const uint32_t* __restrict__ wordrank, T identity, Op op, Val val,
                                                 Emit emit) {
   constexpr int K = kRowWordsPerWave;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t nwords = ceil_div(E, 64);
-  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const uint64_t w0 = gw * K;
   if (w0 >= nwords) return;
   const int nwd = w0 + K < nwords ? K : int(nwords - w0);
   // all row bits, sources and gathered values of the wave's range first, so
@@ -367,7 +501,7 @@ __global__ __launch_bounds__(kNT) void k_pr_pull_f64(uint64_t E, const uint32_t*
                                                       const double* __restrict__ curr, double* __restrict__ next,
                                                       int all_atomic) {
   row_reduce_wave<double>(
-      E, src, rowflag, wordrank, 0.0, OpSum{}, [&](uint64_t, uint32_t s) { return curr[s]; },
+      first_word(), E, src, rowflag, wordrank, 0.0, OpSum{}, [&](uint64_t, uint32_t s) { return curr[s]; },
       [&](uint32_t row, double v, bool shared) {
         double* p = &next[row_dst[row]];
         if (shared || all_atomic)
@@ -386,7 +520,7 @@ __global__ __launch_bounds__(kNT) void k_pr_pull_f32c(uint64_t E, const uint32_t
                                                        const float* __restrict__ curr, double* __restrict__ next,
                                                        int all_atomic) {
   row_reduce_wave<F2>(
-      E, src, rowflag, wordrank, F2{0.f, 0.f}, OpF2{}, [&](uint64_t, uint32_t s) { return F2{curr[s], 0.f}; },
+      first_word(), E, src, rowflag, wordrank, F2{0.f, 0.f}, OpF2{}, [&](uint64_t, uint32_t s) { return F2{curr[s], 0.f}; },
       [&](uint32_t row, F2 v, bool shared) {
         double* p = &next[row_dst[row]];
         const double t = double(v.s) + double(v.c);
@@ -644,7 +778,7 @@ __global__ __launch_bounds__(kNT) void k_cc_pull(uint64_t E, const uint32_t* __r
                                                   const uint32_t* __restrict__ row_dst, uint32_t* __restrict__ label,
                                                   uint64_t* __restrict__ active_out) {
   row_reduce_wave<uint32_t>(
-      E, src, rowflag, wordrank, 0xffffffffu, OpMin{}, [&](uint64_t, uint32_t s) { return label[s]; },
+      first_word(), E, src, rowflag, wordrank, 0xffffffffu, OpMin{}, [&](uint64_t, uint32_t s) { return label[s]; },
       [&](uint32_t row, uint32_t m, bool) {
         const uint32_t v = row_dst[row];
         if (m < v && m < label[v]) {
@@ -662,7 +796,7 @@ __global__ __launch_bounds__(kNT) void k_sssp_pull(uint64_t E, const uint32_t* _
                                                     const uint32_t* __restrict__ row_dst, float* __restrict__ dist,
                                                     uint64_t* __restrict__ active_out) {
   row_reduce_wave<float>(
-      E, src, rowflag, wordrank, 1e9f, OpMin{}, [&](uint64_t e, uint32_t s) { return __fadd_rn(dist[s], w[e]); },
+      first_word(), E, src, rowflag, wordrank, 1e9f, OpMin{}, [&](uint64_t e, uint32_t s) { return __fadd_rn(dist[s], w[e]); },
       [&](uint32_t row, float m, bool) {
         const uint32_t v = row_dst[row];
         if (m < 1e9f && m < dist[v]) {
@@ -774,23 +908,39 @@ bool choose_sparse(const eg_graph* g, int policy, uint64_t active_edges, bool ro
   return active_edges < g->E / 20;
 }
 
-int pr_pull(eg_graph* g, int precision, const void* x, double* next, hipStream_t s) {
-  EG_TRY(fill<double>(next, g->V, 0.0, s));
-  if (!g->E) return EG_OK;
-  const int all_atomic = g->Q > 1;  // a destination has a row per sub-partition
+// The pull's edge arrays: the built CSC (plain) or the degree-ordered view.
+struct PullView {
+  uint64_t E, R;
+  const uint32_t *src, *wordrank, *row_dst;
+  const uint64_t* rowflag;
+  int all_atomic;  // a destination has a row per sub-partition
+};
+
+PullView plain_view(const eg_graph* g) {
+  return PullView{g->E, g->R, g->e_csc_src.as<uint32_t>(), g->e_wordrank.as<uint32_t>(), g->e_row_dst.as<uint32_t>(),
+                  g->e_rowflag.as<uint64_t>(), g->Q > 1};
+}
+
+PullView ordered_view(const eg_graph* g) {
+  return PullView{g->E, g->pv_R, g->pv_src.as<uint32_t>(), g->pv_wordrank.as<uint32_t>(),
+                  g->pv_row_dst.as<uint32_t>(), g->pv_rowflag.as<uint64_t>(), 0};
+}
+
+int pr_pull(const PullView& pv, uint32_t V, int precision, const void* x, double* next, hipStream_t s) {
+  EG_TRY(fill<double>(next, V, 0.0, s));
+  if (!pv.E) return EG_OK;
   prof_begin(s);
-  if (precision == EG_PR_FP64)
-    hipLaunchKernelGGL(k_pr_pull_f64, dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E, g->e_csc_src.as<uint32_t>(),
-                       g->e_rowflag.as<uint64_t>(), g->e_wordrank.as<uint32_t>(), g->e_row_dst.as<uint32_t>(),
-                       static_cast<const double*>(x), next, all_atomic);
-  else
-    hipLaunchKernelGGL(k_pr_pull_f32c, dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E, g->e_csc_src.as<uint32_t>(),
-                       g->e_rowflag.as<uint64_t>(), g->e_wordrank.as<uint32_t>(), g->e_row_dst.as<uint32_t>(),
-                       static_cast<const float*>(x), next, all_atomic);
+  if (precision == EG_PR_FP64) {
+    hipLaunchKernelGGL(k_pr_pull_f64, dim3(row_grid(pv.E)), dim3(kNT), 0, s, pv.E, pv.src, pv.rowflag, pv.wordrank,
+                       pv.row_dst, static_cast<const double*>(x), next, pv.all_atomic);
+  } else {
+    hipLaunchKernelGGL(k_pr_pull_f32c, dim3(row_grid(pv.E)), dim3(kNT), 0, s, pv.E, pv.src, pv.rowflag, pv.wordrank,
+                       pv.row_dst, static_cast<const float*>(x), next, pv.all_atomic);
+  }
   // algorithmic bytes: source id + gathered value per edge, row start bits,
   // destination id + partial per row
   const uint64_t xb = precision == EG_PR_FP64 ? 8 : 4;
-  prof_end(PK_PR_PULL, g->E * (4 + xb) + g->E / 8 + g->R * (4 + 8), s);
+  prof_end(PK_PR_PULL, pv.E * (4 + xb) + pv.E / 8 + pv.R * (4 + 8), s);
   EG_HIP(hipGetLastError());
   return EG_OK;
 }
@@ -900,7 +1050,7 @@ int eg_pr_pull(eg_graph* g, int precision, const void* d_x, double* d_next, void
   hipStream_t s = static_cast<hipStream_t>(stream);
   EG_TRY(ensure_engine(g, s));
   if (precision != EG_PR_FP64 && precision != EG_PR_FP32C) return fail(EG_ERR_ARG, "unknown precision %d", precision);
-  return pr_pull(g, precision, d_x, d_next, s);
+  return pr_pull(plain_view(g), g->V, precision, d_x, d_next, s);
 }
 
 int eg_pr_apply(eg_graph* g, const double* d_x, double* d_next, float* d_next32, int last, double* d_delta,
@@ -918,28 +1068,42 @@ int eg_pagerank(eg_graph* g, int iterations, int precision, double* d_out, doubl
   hipStream_t s = static_cast<hipStream_t>(stream);
   EG_TRY(ensure_engine(g, s));
   if (iterations < 1 || !d_out) return fail(EG_ERR_ARG, "need iterations >= 1 and an output array");
-  if (precision != EG_PR_FP64 && precision != EG_PR_FP32C) return fail(EG_ERR_ARG, "unknown precision %d", precision);
+  const int prec = precision & 0xF;
+  const int layout = precision & ~0xF;
+  if (prec != EG_PR_FP64 && prec != EG_PR_FP32C) return fail(EG_ERR_ARG, "unknown precision %d", prec);
+  if (layout != 0 && layout != EG_PR_ORDERED && layout != EG_PR_PLAIN)
+    return fail(EG_ERR_ARG, "precision flags: at most one of EG_PR_ORDERED, EG_PR_PLAIN");
+  const bool ordered = layout == EG_PR_ORDERED || (layout == 0 && prec == EG_PR_FP32C);
+  if (ordered) EG_TRY(ensure_view(g, s));
+  const PullView pv = ordered ? ordered_view(g) : plain_view(g);
+  const uint32_t* deg = ordered ? g->pv_deg.as<uint32_t>() : g->out_deg.as<uint32_t>();
   Scratch sc;
   EG_TRY(get_scratch(g, iterations, &sc));
   const uint64_t V = g->V;
-  EG_TRY(g->e_pr.resize(V * 8 + V * 4 + 512));
-  double* a = d_out;                     // current x (FP64)
-  double* b = g->e_pr.as<double>();      // next
-  float* x32 = precision == EG_PR_FP32C ? reinterpret_cast<float*>(g->e_pr.as<char>() + ((V * 8 + 255) & ~255ull))
-                                        : nullptr;
+  const uint64_t vb = (V * 8 + 255) & ~255ull;
+  EG_TRY(g->e_pr.resize(2 * vb + V * 4 + 256));
+  // ordered: both FP64 vectors live in the view's ids, d_out gets the
+  // original order at the end
+  double* a = ordered ? g->e_pr.as<double>() : d_out;  // current x (FP64)
+  double* b = reinterpret_cast<double*>(g->e_pr.as<char>() + (ordered ? vb : 0));  // next
+  float* x32 = prec == EG_PR_FP32C ? reinterpret_cast<float*>(g->e_pr.as<char>() + 2 * vb) : nullptr;
   EG_TRY(fill<double>(sc.delta, uint64_t(iterations) + 1, 0.0, s));
-  hipLaunchKernelGGL(k_pr_init, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
-                     (const uint32_t*)g->out_deg.as<uint32_t>(), g->V, a, x32);
+  hipLaunchKernelGGL(k_pr_init, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s, deg, g->V, a, x32);
   EG_HIP(hipGetLastError());
   for (int it = 0; it < iterations; ++it) {
-    EG_TRY(pr_pull(g, precision, precision == EG_PR_FP64 ? static_cast<const void*>(a) : x32, b, s));
-    hipLaunchKernelGGL(k_pr_apply, dim3(grid_for(V, kNT, 4096)), dim3(kNT), 0, s,
-                       (const uint32_t*)g->out_deg.as<uint32_t>(), g->V, (const double*)a, b, x32,
+    EG_TRY(pr_pull(pv, g->V, prec, prec == EG_PR_FP64 ? static_cast<const void*>(a) : x32, b, s));
+    hipLaunchKernelGGL(k_pr_apply, dim3(grid_for(V, kNT, 4096)), dim3(kNT), 0, s, deg, g->V, (const double*)a, b, x32,
                        it == iterations - 1 ? 1 : 0, sc.delta + it + 1);
     EG_HIP(hipGetLastError());
     std::swap(a, b);
   }
-  if (a != d_out) EG_HIP(hipMemcpyAsync(d_out, a, V * 8, hipMemcpyDeviceToDevice, s));
+  if (ordered) {
+    hipLaunchKernelGGL(k_gather_by, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
+                       (const uint32_t*)g->pv_perm.as<uint32_t>(), g->V, (const double*)a, d_out);
+    EG_HIP(hipGetLastError());
+  } else if (a != d_out) {
+    EG_HIP(hipMemcpyAsync(d_out, a, V * 8, hipMemcpyDeviceToDevice, s));
+  }
   if (h_deltas) {
     EG_HIP(hipMemcpyAsync(h_deltas, sc.delta, sizeof(double) * (iterations + 1), hipMemcpyDeviceToHost, s));
     EG_HIP(hipStreamSynchronize(s));
@@ -947,6 +1111,12 @@ int eg_pagerank(eg_graph* g, int iterations, int precision, double* d_out, doubl
     for (int it = 1; it <= iterations; ++it) h_deltas[it] /= double(g->V);
   }
   EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+int eg_pagerank_view_prepare(eg_graph* g, double* h_ms, void* stream) {
+  EG_TRY(ensure_view(g, static_cast<hipStream_t>(stream)));
+  if (h_ms) *h_ms = g->pv_prepare_ms;
   return EG_OK;
 }
 

@@ -53,6 +53,18 @@ struct eg_graph {
   eg::DevBuf e_scratch;              // frontier list, counters
   eg::DevBuf e_bm[3];                // three bitmaps of (V>>6)+1 words
   eg::DevBuf e_pr;                   // PageRank work vectors
+  // degree-ordered PageRank view: vertices renumbered by descending
+  // out-degree (ties by id), one row per destination in new-id order
+  bool pv_ready = false;
+  uint32_t pv_R = 0;
+  double pv_prepare_ms = 0;
+  eg::DevBuf pv_perm;                // u32[V]  old id -> new id
+  eg::DevBuf pv_inv;                 // u32[V]  new id -> old id
+  eg::DevBuf pv_deg;                 // u32[V]  out-degree by new id
+  eg::DevBuf pv_src;                 // u32[E]  new source id of every edge, rows in new-id order
+  eg::DevBuf pv_rowflag;             // u64[ceil(E/64)] row-start bits
+  eg::DevBuf pv_wordrank;            // u32[ceil(E/64)+1]
+  eg::DevBuf pv_row_dst;             // u32[R]  new destination id of each row
   double eng_prepare_ms = 0;
 
   // host copies of the small tables

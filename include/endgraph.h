@@ -176,6 +176,14 @@ int eg_batch_destroy(eg_batch* b);
  * Bitmaps are core/bitmap.hpp's layout: u64 words, (V>>6)+1 of them. */
 enum eg_policy { EG_POLICY_REFERENCE = 0, EG_POLICY_AUTO = 1, EG_POLICY_PUSH = 2, EG_POLICY_PULL = 3 };
 enum eg_precision { EG_PR_FP64 = 0, EG_PR_FP32C = 1 };
+/* Layout flags for eg_pagerank, OR-ed into the precision (no reference
+ * counterpart).  EG_PR_ORDERED runs the pull on a degree-ordered view:
+ * vertices renumbered by descending out-degree so the most-gathered values
+ * share cache lines; each destination's in-edges are ordered by new source
+ * id (the same terms as the CSC row, summed in another order).
+ * EG_PR_PLAIN runs it on the built CSC in original ids.  Defaults: FP64
+ * plain, FP32C ordered.  The view is derived on first use after a build. */
+enum eg_pr_layout { EG_PR_ORDERED = 0x10, EG_PR_PLAIN = 0x20 };
 
 /* Derives the engine's view of the built arrays (SoA source/target ids,
  * row-start bits, a CSR merged over sub-partitions).  Runs on the first
@@ -203,13 +211,17 @@ int eg_bitmap_degree_sum(eg_graph* g, const uint64_t* d_bm, uint64_t* h_sum, voi
  * process_vertices next = 0.15 + 0.85*next, divided by deg unless last;
  * adds sum |next - x| * deg to *d_delta (device double, not divided by V);
  * writes a float copy to d_next32 if non-NULL.
- * eg_pagerank runs the whole loop; h_deltas (optional) gets iterations+1
- * values, delta(i) of the reference (already divided by V). */
+ * eg_pagerank runs the whole loop (precision may carry an eg_pr_layout
+ * flag; d_out is in original ids either way); h_deltas (optional) gets
+ * iterations+1 values, delta(i) of the reference (already divided by V).
+ * eg_pagerank_view_prepare derives the degree-ordered view now (h_ms gets
+ * its wall time) so that its one-time cost is kept out of a timed run. */
 int eg_pr_init(eg_graph* g, double* d_x, float* d_x32, void* stream);
 int eg_pr_pull(eg_graph* g, int precision, const void* d_x, double* d_next, void* stream);
 int eg_pr_apply(eg_graph* g, const double* d_x, double* d_next, float* d_next32, int last, double* d_delta,
                 void* stream);
 int eg_pagerank(eg_graph* g, int iterations, int precision, double* d_out, double* h_deltas, void* stream);
+int eg_pagerank_view_prepare(eg_graph* g, double* h_ms, void* stream);
 
 /* BFS, toolkits/bfs.cpp:22-112.  Parent rule (no reference counterpart: the
  * reference's CAS race leaves it open): the smallest frontier id among a

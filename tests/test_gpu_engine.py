@@ -85,14 +85,47 @@ def test_pagerank_fp64_and_fp32c(eg, oracle_lib, scale, P, S):
     edges, V = _rmat(eg, scale)
     g, og, _ = _pair(eg, oracle_lib, edges, V, P, S, m=20)
     ref, ref_d = og.pagerank(20)
-    x64, d64 = g.pagerank(20, eg.PR_FP64)
-    x32, d32 = g.pagerank(20, eg.PR_FP32C)
-    l1_64 = np.abs(x64 - ref).sum() / np.abs(ref).sum()
-    l1_32 = np.abs(x32 - ref).sum() / np.abs(ref).sum()
-    print(f"  relative L1: fp64 {l1_64:.3e}  fp32c {l1_32:.3e}", flush=True)
-    assert l1_64 <= PR_FP64_TOL
-    assert l1_32 <= PR_FP32C_TOL
-    assert np.allclose(d64, ref_d, rtol=1e-9, atol=1e-15)
+    runs = {
+        "fp64 plain": (eg.PR_FP64, PR_FP64_TOL),
+        "fp64 ordered": (eg.PR_FP64 | eg.PR_ORDERED, PR_FP64_TOL),
+        "fp32c ordered": (eg.PR_FP32C, PR_FP32C_TOL),
+        "fp32c plain": (eg.PR_FP32C | eg.PR_PLAIN, PR_FP32C_TOL),
+    }
+    for name, (prec, tol) in runs.items():
+        x, d = g.pagerank(20, prec)
+        l1 = np.abs(x - ref).sum() / np.abs(ref).sum()
+        print(f"  relative L1 {name}: {l1:.3e}", flush=True)
+        assert l1 <= tol, name
+        if prec & 0xF == eg.PR_FP64:
+            assert np.allclose(d, ref_d, rtol=1e-9, atol=1e-15), name
+
+
+def test_pagerank_layout_flags_rejected(eg):
+    edges = np.zeros(4, dtype=eg.REC8)
+    edges["src"] = [0, 1, 2, 3]
+    edges["dst"] = [1, 2, 3, 0]
+    buf = eg.DeviceBuffer.from_array(edges)
+    g = eg.Graph(4, 1, 1, 1, 1, 8)
+    g.build(buf.ptr, 4)
+    with pytest.raises(eg.EndGraphError):
+        g.pagerank(3, eg.PR_FP64 | eg.PR_ORDERED | eg.PR_PLAIN)
+    with pytest.raises(eg.EndGraphError):
+        g.pagerank(3, 5)
+
+
+def test_pagerank_ordered_view_rebuilt(eg, oracle_lib):
+    """The degree-ordered view follows a rebuild of the same handle."""
+    rng = np.random.default_rng(11)
+    V = 5000
+    g = eg.Graph(V, 1, 1, 1, 20, 8)
+    for k in range(2):
+        edges = H.random_edges(rng, V, 40000 + 7000 * k, skew=True)
+        buf = eg.DeviceBuffer.from_array(edges)
+        g.build(buf.ptr, edges.shape[0])
+        og = oracle_lib.OracleGraph(edges, V, P=1, S=1, eta=1, m=20, threads=8)
+        ref, _ = og.pagerank(10)
+        x, _ = g.pagerank(10, eg.PR_FP32C)
+        assert np.abs(x - ref).sum() / np.abs(ref).sum() <= PR_FP32C_TOL
 
 
 def test_pagerank_dense_small_closed_form(eg, oracle_lib):
