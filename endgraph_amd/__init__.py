@@ -110,6 +110,8 @@ def lib():
         "eg_cc": ([vp, i32, vp, C.POINTER(u32), vp], i32),
         "eg_sssp_iteration": ([vp, i32, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64), vp], i32),
         "eg_sssp": ([vp, u32, i32, vp, C.POINTER(u32), vp], i32),
+        "eg_bfs_multi": ([vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(u32), vp], i32),
+        "eg_sssp_multi": ([vp, vp, i32, i32, vp, C.POINTER(u32), vp], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -396,6 +398,33 @@ class Graph:
         it = C.c_uint32()
         check(lib().eg_cc(self._h, policy, C.c_void_p(label.ptr), C.byref(it), _stream(stream)))
         return (label.download(np.uint32, self.V) if download else None), it.value
+
+    def bfs_multi(self, roots, policy: int = POLICY_REFERENCE, parent: DeviceBuffer = None,
+                  depth: DeviceBuffer = None, download: bool = True, stream=None):
+        """BFS from up to 64 roots in one launch; parent/depth are (V, 64) vertex-major, column r = roots[r]."""
+        r = np.ascontiguousarray(roots, dtype=np.uint32)
+        parent = parent or DeviceBuffer(self.V * 64 * 4)
+        depth = depth or DeviceBuffer(self.V * 64 * 4)
+        levels = np.zeros(r.size, np.uint32)
+        edges = np.zeros(r.size, np.uint64)
+        steps = C.c_uint32()
+        check(lib().eg_bfs_multi(self._h, r.ctypes.data_as(C.c_void_p), r.size, policy, C.c_void_p(parent.ptr),
+                                 C.c_void_p(depth.ptr), levels.ctypes.data_as(C.c_void_p),
+                                 edges.ctypes.data_as(C.c_void_p), C.byref(steps), _stream(stream)))
+        if not download:
+            return levels, edges, steps.value
+        return (parent.download(np.uint32, self.V * 64).reshape(self.V, 64),
+                depth.download(np.uint32, self.V * 64).reshape(self.V, 64), levels, edges, steps.value)
+
+    def sssp_multi(self, roots, policy: int = POLICY_REFERENCE, dist: DeviceBuffer = None, download: bool = True,
+                   stream=None):
+        """SSSP from up to 64 roots in one launch; dist is (V, 64) vertex-major, column r = roots[r]."""
+        r = np.ascontiguousarray(roots, dtype=np.uint32)
+        dist = dist or DeviceBuffer(self.V * 64 * 4)
+        steps = C.c_uint32()
+        check(lib().eg_sssp_multi(self._h, r.ctypes.data_as(C.c_void_p), r.size, policy, C.c_void_p(dist.ptr),
+                                  C.byref(steps), _stream(stream)))
+        return (dist.download(np.float32, self.V * 64).reshape(self.V, 64) if download else None), steps.value
 
     def sssp(self, root: int, policy: int = POLICY_REFERENCE, dist: DeviceBuffer = None, download: bool = True,
              stream=None):

@@ -673,6 +673,7 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   g->built = false;
   g->eng_ready = false;
   g->pv_ready = false;
+  g->tv_ready = false;
   g->E0 = E0;
   g->E = E;
   g->symmetrized = sym;

@@ -946,6 +946,9 @@ int pr_pull(const PullView& pv, uint32_t V, int precision, const void* x, double
 }
 
 }  // namespace
+
+int engine_ensure(eg_graph* g, hipStream_t s) { return ensure_engine(g, s); }
+
 }  // namespace eg
 
 using namespace eg;

@@ -65,6 +65,14 @@ struct eg_graph {
   eg::DevBuf pv_rowflag;             // u64[ceil(E/64)] row-start bits
   eg::DevBuf pv_wordrank;            // u32[ceil(E/64)+1]
   eg::DevBuf pv_row_dst;             // u32[R]  new destination id of each row
+  // traversal view for the batched (64-root) programs: the CSC merged over
+  // sub-partitions with each destination's in-edges sorted by source
+  bool tv_ready = false;
+  eg::DevBuf tv_ptr;                 // u32[V+1] row pointer (prefix of in-degree)
+  eg::DevBuf tv_src;                 // u32[E]
+  eg::DevBuf tv_w;                   // f32[E] (12-byte records only)
+  eg::DevBuf e_multi;                // batched traversal masks, work items, counters
+  eg::DevBuf e_multi_roots;          // roots and per-root statistics
   double eng_prepare_ms = 0;
 
   // host copies of the small tables
@@ -88,3 +96,8 @@ struct eg_graph {
   };
   Meta* dmeta() const { return meta.as<Meta>(); }
 };
+
+namespace eg {
+// Derives the engine's view after a build (first engine call; engine.hip).
+int engine_ensure(eg_graph* g, hipStream_t s);
+}  // namespace eg

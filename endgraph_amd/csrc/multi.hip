@@ -1,0 +1,592 @@
+// Batched traversals: BFS and SSSP from up to 64 roots at once (no
+// reference counterpart: toolkits/bfs.cpp and toolkits/sssp.cpp run one
+// root per process).  The 64 roots are the 64 lanes of a CDNA wavefront:
+// per-vertex state is one u64 mask (bit r = root r) and per-(vertex, root)
+// results are stored vertex-major, x[v*64 + r], so that a wave serving one
+// vertex reads or writes its 64 roots as one 256-byte row.
+//
+// A traversal runs as steps of three launches each, queued four steps at a
+// time between host checks (the device decides the direction of each step,
+// so no step waits for the host):
+//   push       sparse steps expand work items (a frontier vertex and a
+//              4096-edge chunk of its out-edges, so hubs spread over waves);
+//   pull       dense steps: a wave per destination walks its in-edges 64 at
+//              a time (rows sorted by source) and, per root, takes the first
+//              frontier source it meets;
+//   end        new bits -> visited / next frontier, results, work items and
+//              the next step's counters.
+// Only one of push / pull works in a step (the rule of
+// core/graph.hpp:1483-1498 with the policies of the single-root programs,
+// evaluated on the device from the counters); a step queued after the
+// frontier emptied returns at once.
+// Results equal those of the single-root programs bit for bit: BFS parents
+// follow the same min-id rule (the smallest frontier in-neighbour of the
+// level that reaches v), depths are levels, SSSP distances are the unique
+// fixpoint of dist[v] = min(dist[v], dist[u] + w) with fp32 adds.
+#include <algorithm>
+#include <vector>
+
+#include "graph_state.hpp"
+
+namespace eg {
+namespace {
+
+constexpr int kNT = 256;
+constexpr uint32_t kItemEdges = 4096;  // out-edges per push work item
+constexpr uint32_t kUnreached = 0xffffffffu;
+constexpr float kInfDist = 1e9f;       // toolkits/sssp.cpp INF
+
+unsigned grid_for(uint64_t n, uint64_t per = kNT, uint64_t cap = 65536) {
+  uint64_t b = ceil_div(n, per);
+  return unsigned(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+// Counters of one level: vertices with new bits, their out-degree sum and
+// the number of push work items they produced.
+struct MCount {
+  unsigned long long nv, ne;
+  unsigned int items, pad;
+};
+
+struct OpOr {
+  __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return a | b; }
+};
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int lane) {
+  return (uint64_t(__shfl(uint32_t(v >> 32), lane, kWave)) << 32) | __shfl(uint32_t(v), lane, kWave);
+}
+__device__ __forceinline__ uint64_t wave_or(uint64_t v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const uint32_t hi = __shfl_xor(uint32_t(v >> 32), d, kWave), lo = __shfl_xor(uint32_t(v), d, kWave);
+    v |= (uint64_t(hi) << 32) | lo;
+  }
+  return v;
+}
+
+__device__ __forceinline__ bool sparse_level(int policy, unsigned long long active_edges, uint64_t E) {
+  if (policy == EG_POLICY_PUSH) return true;
+  if (policy == EG_POLICY_PULL) return false;
+  if (policy == EG_POLICY_REFERENCE) return true;  // rooted programs
+  return active_edges < E / 20;
+}
+
+template <class T>
+__global__ void k_fill_t(T* __restrict__ p, uint64_t n, T v) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+template <class T>
+int fill_t(T* p, uint64_t n, T v, hipStream_t s) {
+  if (!n) return EG_OK;
+  hipLaunchKernelGGL((k_fill_t<T>), dim3(grid_for(n, kNT, 65536)), dim3(kNT), 0, s, p, n, v);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+// ------------------------------------------------------ traversal view --
+struct KeyDstSrc {
+  int vb;
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const { return (uint64_t(r.dst) << vb) | r.src; }
+};
+
+template <class R>
+__global__ void k_tv_soa(const R* __restrict__ lst, uint64_t E, uint32_t* __restrict__ src, float* __restrict__ w) {
+  for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += uint64_t(gridDim.x) * blockDim.x) {
+    const R r = lst[e];
+    src[e] = r.src;
+    if constexpr (sizeof(R) == 12) w[e] = reinterpret_cast<const Rec12&>(r).w;
+  }
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ a, uint32_t* __restrict__ b, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    b[i] = a[i];
+}
+
+template <class R>
+int tview_impl(eg_graph* g, hipStream_t s) {
+  const uint64_t E = g->E;
+  const uint32_t V = g->V;
+  EG_TRY(g->tv_ptr.resize((uint64_t(V) + 1) * 4));
+  EG_TRY(g->tv_src.resize(std::max<uint64_t>(E, 1) * 4));
+  if (sizeof(R) == 12) EG_TRY(g->tv_w.resize(std::max<uint64_t>(E, 1) * 4));
+  uint32_t* ptr = g->tv_ptr.as<uint32_t>();
+  // rows sorted by destination: row v starts at the prefix of the in-degree
+  hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)g->in_deg.as<uint32_t>(), ptr,
+                     uint64_t(V));
+  EG_HIP(hipGetLastError());
+  EG_TRY(fill_t<uint32_t>(ptr + V, 1, 0u, s));
+  EG_TRY(exclusive_scan_u32(ptr, ptr, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
+  if (E) {
+    DevBuf b, c;
+    EG_TRY(b.resize(E * sizeof(R)));
+    EG_TRY(c.resize(E * sizeof(R)));
+    RxWork w;
+    EG_TRY(w.resize_segs(1));
+    EG_HIP(hipMemsetAsync(w.err.ptr, 0, 4, s));
+    const int vb = std::max(1, counting_bits(V - 1));
+    EG_TRY(rx_sort(g->csc_list.as<R>(), b.as<R>(), c.as<R>(), std::vector<uint64_t>{0, E}, KeyDstSrc{vb}, 2 * vb, w,
+                   s));
+    hipLaunchKernelGGL((k_tv_soa<R>), dim3(grid_for(E)), dim3(kNT), 0, s, (const R*)b.as<R>(), E,
+                       g->tv_src.as<uint32_t>(), g->tv_w.as<float>());
+    EG_HIP(hipGetLastError());
+    uint32_t err = 0;
+    EG_HIP(hipMemcpyAsync(&err, w.err.ptr, 4, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+    if (err) return fail(EG_ERR_HIP, "radix sort look-back exceeded its spin limit");
+  }
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+int ensure_tview(eg_graph* g, hipStream_t s) {
+  EG_TRY(engine_ensure(g, s));
+  if (g->tv_ready) return EG_OK;
+  EG_TRY(g->rec == 8 ? tview_impl<Rec8>(g, s) : tview_impl<Rec12>(g, s));
+  g->tv_ready = true;
+  return EG_OK;
+}
+
+// ------------------------------------------------------------ shared state --
+struct MultiState {
+  uint64_t *vis, *fr, *nx;  // u64[V] masks (SSSP uses fr as the active set, vis unused)
+  uint64_t* items[2];       // push work items (v << 32 | chunk), ping-pong by step parity
+  MCount* cnt;              // ring of 3 by step: read at s, written at s-1, cleared at s-2
+  uint32_t* steps;          // steps that had a nonempty frontier
+};
+
+uint64_t item_cap(const eg_graph* g) { return uint64_t(g->V) + g->E / kItemEdges + 1; }
+
+int multi_state(eg_graph* g, MultiState* st) {
+  const uint64_t V = g->V;
+  const uint64_t ic = item_cap(g);
+  const uint64_t bytes = 3 * V * 8 + 2 * ic * 8 + 3 * sizeof(MCount) + 256;
+  EG_TRY(g->e_multi.resize(bytes));
+  char* p = g->e_multi.as<char>();
+  st->vis = reinterpret_cast<uint64_t*>(p);
+  st->fr = st->vis + V;
+  st->nx = st->fr + V;
+  st->items[0] = st->nx + V;
+  st->items[1] = st->items[0] + ic;
+  st->cnt = reinterpret_cast<MCount*>(st->items[1] + ic);
+  st->steps = reinterpret_cast<uint32_t*>(st->cnt + 3);
+  return EG_OK;
+}
+
+// Roots: lane r < n sets bit r of its root; work items and counters of the
+// first step come from the distinct roots.  One wave.
+__global__ void k_multi_roots(const uint32_t* __restrict__ roots, int n, const uint32_t* __restrict__ deg,
+                              uint64_t* __restrict__ vis, uint64_t* __restrict__ fr, uint64_t* __restrict__ items,
+                              MCount* __restrict__ cnt, uint32_t* __restrict__ parent, uint32_t* __restrict__ depth,
+                              float* __restrict__ dist) {
+  const int r = threadIdx.x;
+  if (r >= n) return;
+  const uint32_t v = roots[r];
+  if (vis) atomicOr((unsigned long long*)&vis[v], 1ull << r);
+  atomicOr((unsigned long long*)&fr[v], 1ull << r);
+  if (parent) parent[uint64_t(v) * 64 + r] = v;
+  if (depth) depth[uint64_t(v) * 64 + r] = 0;
+  if (dist) dist[uint64_t(v) * 64 + r] = 0.0f;
+  bool first = true;  // the lowest lane holding this root adds its work items
+  for (int q = 0; q < r; ++q) first = first && roots[q] != v;
+  if (!first) return;
+  const uint32_t d = deg[v];
+  atomicAdd(&cnt[0].nv, 1ull);
+  atomicAdd(&cnt[0].ne, (unsigned long long)d);
+  const uint32_t k = uint32_t(ceil_div(d, kItemEdges));
+  const uint32_t at = atomicAdd(&cnt[0].items, k);
+  for (uint32_t j = 0; j < k; ++j) items[at + j] = (uint64_t(v) << 32) | j;
+}
+
+__device__ __forceinline__ void append_items(uint32_t v, uint32_t d, MCount* nc, uint64_t* items) {
+  const uint32_t k = uint32_t(ceil_div(d, kItemEdges));
+  if (!k) return;
+  const uint32_t at = atomicAdd(&nc->items, k);
+  for (uint32_t j = 0; j < k; ++j) items[at + j] = (uint64_t(v) << 32) | j;
+}
+
+__device__ __forceinline__ void add_counts(unsigned long long nv, unsigned long long ne, MCount* nc) {
+  nv = wave_reduce(nv, OpSum{});
+  ne = wave_reduce(ne, OpSum{});
+  if ((threadIdx.x & (kWave - 1)) == 0 && nv) {
+    atomicAdd(&nc->nv, nv);
+    atomicAdd(&nc->ne, ne);
+  }
+}
+
+// Step bookkeeping shared by the level-end kernels: clears the counters of
+// step s+2 (last read at step s-1) and records step s as run.
+__device__ __forceinline__ void step_bookkeeping(const MultiState& st, uint32_t step, bool live) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st.cnt[(step + 2) % 3] = MCount{0, 0, 0, 0};
+    if (live) *st.steps = step + 1;
+  }
+}
+
+// ------------------------------------------------------------------ BFS --
+struct BfsArgs {
+  uint32_t V;
+  uint64_t E;
+  int policy;
+  uint64_t lanes;  // bits of the live roots
+  const uint32_t *out_ptr, *out_dst, *deg, *in_ptr, *in_src;
+  MultiState st;
+  uint32_t *parent, *depth;
+};
+
+// Sparse step (bfs.cpp:52-58 per root): unvisited out-neighbours of the
+// frontier take the smallest frontier id as parent.
+__global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t step) {
+  const MCount c = a.st.cnt[step % 3];
+  if (!c.nv || !sparse_level(a.policy, c.ne, a.E)) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t* items = a.st.items[step & 1];
+  for (uint64_t i = gw; i < c.items; i += nw) {
+    const uint64_t it = items[i];
+    const uint32_t u = uint32_t(it >> 32), ch = uint32_t(it);
+    const uint64_t fm = a.st.fr[u];
+    const uint32_t e0 = a.out_ptr[u] + ch * kItemEdges;
+    const uint32_t e1 = min(e0 + kItemEdges, a.out_ptr[u + 1]);
+    for (uint32_t e = e0 + lane; e < e1; e += kWave) {
+      const uint32_t v = a.out_dst[e];
+      uint64_t bits = fm & ~a.st.vis[v];
+      if (!bits) continue;
+      atomicOr((unsigned long long*)&a.st.nx[v], bits);
+      while (bits) {
+        const int r = __ffsll((unsigned long long)bits) - 1;
+        atomicMin(&a.parent[uint64_t(v) * 64 + r], u);
+        bits &= bits - 1;
+      }
+    }
+  }
+}
+
+// Dense step (bfs.cpp:60-68 per root): a wave per destination walks its
+// in-edges in source order; the first frontier source of a root (the
+// smallest id) is its parent; the row stops once every unvisited root of v
+// is found.
+__global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t step) {
+  const MCount c = a.st.cnt[step % 3];
+  if (!c.nv || sparse_level(a.policy, c.ne, a.E)) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t v = gw; v < a.V; v += nw) {
+    const uint64_t need = a.lanes & ~a.st.vis[v];
+    if (!need) continue;
+    uint64_t found = 0;
+    const uint32_t b1 = a.in_ptr[v + 1];
+    for (uint32_t e = a.in_ptr[v]; e < b1 && found != need; e += kWave) {
+      const bool ok = e + lane < b1;
+      const uint32_t u = ok ? a.in_src[e + lane] : 0u;
+      const uint64_t f = ok ? (a.st.fr[u] & need) : 0ull;
+      uint64_t todo = wave_or(f) & ~found;
+      while (todo) {
+        const int r = __ffsll((unsigned long long)todo) - 1;
+        const uint64_t m = __ballot((f >> r) & 1ull);
+        const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
+        if (lane == r) a.parent[v * 64 + r] = p;
+        found |= 1ull << r;
+        todo &= todo - 1;
+      }
+    }
+    if (found && lane == 0) a.st.nx[v] = found;
+  }
+}
+
+// Level end (bfs.cpp:70-90 per root): visited |= next, depth = level + 1,
+// next becomes the frontier; counters and work items of the next step.
+__global__ __launch_bounds__(kNT) void k_mbfs_end(BfsArgs a, uint32_t step) {
+  const bool live = a.st.cnt[step % 3].nv != 0;
+  step_bookkeeping(a.st, step, live);
+  if (!live) return;
+  MCount* nc = &a.st.cnt[(step + 1) % 3];
+  uint64_t* nitems = a.st.items[(step + 1) & 1];
+  unsigned long long nv = 0, ne = 0;
+  const uint64_t nt = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < a.V; v += nt) {
+    const uint64_t nb = a.st.nx[v];
+    if (nb) {
+      a.st.vis[v] |= nb;
+      a.st.nx[v] = 0;
+      for (uint64_t b = nb; b; b &= b - 1) a.depth[v * 64 + (__ffsll((unsigned long long)b) - 1)] = step + 1;
+      const uint32_t d = a.deg[v];
+      ++nv;
+      ne += d;
+      append_items(uint32_t(v), d, nc, nitems);
+    }
+    if (a.st.fr[v] != nb) a.st.fr[v] = nb;
+  }
+  add_counts(nv, ne, nc);
+}
+
+// ----------------------------------------------------------------- SSSP --
+struct SsspArgs {
+  uint32_t V;
+  uint64_t E;
+  int policy;
+  uint64_t lanes;
+  const uint32_t *out_ptr, *out_dst, *deg, *in_ptr, *in_src;
+  const float *out_w, *in_w;
+  MultiState st;
+  float* dist;
+};
+
+// Sparse round (sssp.cpp:44-58 per root): relax dist[u] + w into the
+// out-neighbours; write_min on the bits of non-negative floats.
+__global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t step) {
+  const MCount c = a.st.cnt[step % 3];
+  if (!c.nv || !sparse_level(a.policy, c.ne, a.E)) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t* items = a.st.items[step & 1];
+  for (uint64_t i = gw; i < c.items; i += nw) {
+    const uint64_t it = items[i];
+    const uint32_t u = uint32_t(it >> 32), ch = uint32_t(it);
+    const uint64_t am = a.st.fr[u];
+    const uint32_t e0 = a.out_ptr[u] + ch * kItemEdges;
+    const uint32_t e1 = min(e0 + kItemEdges, a.out_ptr[u + 1]);
+    for (uint32_t e = e0 + lane; e < e1; e += kWave) {
+      const uint32_t v = a.out_dst[e];
+      const float w = a.out_w[e];
+      for (uint64_t b = am; b; b &= b - 1) {
+        const int r = __ffsll((unsigned long long)b) - 1;
+        const float nd = __fadd_rn(a.dist[uint64_t(u) * 64 + r], w);
+        float* p = &a.dist[uint64_t(v) * 64 + r];
+        if (nd < *p) {
+          const int old = atomicMin(reinterpret_cast<int*>(p), __float_as_int(nd));
+          if (nd < __int_as_float(old)) atomicOr((unsigned long long*)&a.st.nx[v], 1ull << r);
+        }
+      }
+    }
+  }
+}
+
+// Dense round (sssp.cpp:60-73 per root): min over active in-neighbours of
+// dist[u] + w; lane r is root r, so each in-edge reads its source's 64
+// distances as one row.
+__global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t step) {
+  const MCount c = a.st.cnt[step % 3];
+  if (!c.nv || sparse_level(a.policy, c.ne, a.E)) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t v = gw; v < a.V; v += nw) {
+    const uint32_t b1 = a.in_ptr[v + 1];
+    float best = kInfDist;
+    for (uint32_t e = a.in_ptr[v]; e < b1; e += kWave) {
+      const bool ok = e + lane < b1;
+      const uint32_t u = ok ? a.in_src[e + lane] : 0u;
+      const float w = ok ? a.in_w[e + lane] : 0.0f;
+      const uint64_t am = ok ? a.st.fr[u] : 0ull;
+      if (!wave_or(am)) continue;
+      const int cnt = int(min<uint32_t>(kWave, b1 - e));
+      for (int j = 0; j < cnt; ++j) {
+        const uint64_t aj = shfl64(am, j);
+        if (!aj) continue;  // wave-uniform
+        const uint32_t uj = __shfl(u, j, kWave);
+        const float wj = __shfl(w, j, kWave);
+        if ((aj >> lane) & 1ull) best = fminf(best, __fadd_rn(a.dist[uint64_t(uj) * 64 + lane], wj));
+      }
+    }
+    float* p = &a.dist[v * 64 + lane];
+    const bool lower = ((a.lanes >> lane) & 1ull) && best < *p;
+    if (lower) *p = best;
+    const uint64_t m = __ballot(lower);
+    if (m && lane == 0) a.st.nx[v] = m;
+  }
+}
+
+// Round end: the lowered vertices are the next active set.
+__global__ __launch_bounds__(kNT) void k_msssp_end(SsspArgs a, uint32_t step) {
+  const bool live = a.st.cnt[step % 3].nv != 0;
+  step_bookkeeping(a.st, step, live);
+  if (!live) return;
+  MCount* nc = &a.st.cnt[(step + 1) % 3];
+  uint64_t* nitems = a.st.items[(step + 1) & 1];
+  unsigned long long nv = 0, ne = 0;
+  const uint64_t nt = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < a.V; v += nt) {
+    const uint64_t nb = a.st.nx[v];
+    if (nb) {
+      a.st.nx[v] = 0;
+      const uint32_t d = a.deg[v];
+      ++nv;
+      ne += d;
+      append_items(uint32_t(v), d, nc, nitems);
+    }
+    if (a.st.fr[v] != nb) a.st.fr[v] = nb;
+  }
+  add_counts(nv, ne, nc);
+}
+
+// Per-root statistics of a finished BFS: levels (max depth + 1) and edges
+// traversed (out-degree sum of the reached vertices), as eg_bfs reports.
+__global__ __launch_bounds__(kNT) void k_bfs_multi_stats(const uint32_t* __restrict__ depth, uint32_t V,
+                                                         const uint32_t* __restrict__ deg, uint32_t* __restrict__ maxd,
+                                                         unsigned long long* __restrict__ edges) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  uint32_t md = 0;
+  unsigned long long ed = 0;
+  for (uint64_t v = gw; v < V; v += nw) {
+    const uint32_t d = depth[v * 64 + lane];
+    if (d != kUnreached) {
+      md = max(md, d);
+      ed += deg[v];
+    }
+  }
+  atomicMax(&maxd[lane], md);
+  atomicAdd(&edges[lane], ed);
+}
+
+int check_roots(const eg_graph* g, const uint32_t* h_roots, int n) {
+  if (n < 1 || n > 64 || !h_roots) return fail(EG_ERR_ARG, "need 1..64 roots");
+  for (int r = 0; r < n; ++r)
+    if (h_roots[r] >= g->V) return fail(EG_ERR_ARG, "root %u >= vertices_num", h_roots[r]);
+  return EG_OK;
+}
+
+int start_multi(eg_graph* g, const uint32_t* h_roots, int n, MultiState* st, uint32_t* parent, uint32_t* depth,
+                float* dist, hipStream_t s) {
+  EG_TRY(multi_state(g, st));
+  const uint64_t V = g->V;
+  EG_TRY(fill_t<uint64_t>(st->vis, 3 * V, 0ull, s));  // vis, fr, nx
+  EG_TRY(fill_t<unsigned long long>(reinterpret_cast<unsigned long long*>(st->cnt), 3 * sizeof(MCount) / 8, 0ull, s));
+  EG_TRY(fill_t<uint32_t>(st->steps, 1, 0u, s));
+  if (parent) EG_TRY(fill_t<uint32_t>(parent, V * 64, g->V, s));
+  if (depth) EG_TRY(fill_t<uint32_t>(depth, V * 64, kUnreached, s));
+  if (dist) EG_TRY(fill_t<float>(dist, V * 64, kInfDist, s));
+  EG_TRY(g->e_multi_roots.resize(64 * 4 + 64 * 4 + 64 * 8 + 256));
+  EG_HIP(hipMemcpyAsync(g->e_multi_roots.ptr, h_roots, size_t(n) * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_multi_roots, dim3(1), dim3(64), 0, s, (const uint32_t*)g->e_multi_roots.as<uint32_t>(), n,
+                     (const uint32_t*)g->out_deg.as<uint32_t>(), parent ? st->vis : nullptr, st->fr, st->items[0],
+                     st->cnt, parent, depth, dist);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+// Runs steps until a step has an empty frontier: kBatch steps are queued
+// between host checks; a queued step after the last live one returns at
+// once (its counters are zero).
+constexpr uint32_t kBatch = 4;
+
+template <class Args>
+int run_steps(eg_graph* g, const Args& a, void (*push)(Args, uint32_t), void (*pull)(Args, uint32_t),
+              void (*end)(Args, uint32_t), uint32_t* h_steps, hipStream_t s) {
+  const unsigned wave_grid = grid_for(g->V, kNT / kWave, 65536);
+  const unsigned push_grid = 8192;
+  const unsigned end_grid = grid_for(g->V, kNT, 16384);
+  const uint32_t max_steps = g->V + 1;  // a frontier is empty after at most V steps
+  uint32_t step = 0;
+  for (;;) {
+    for (uint32_t j = 0; j < kBatch && step < max_steps; ++j, ++step) {
+      hipLaunchKernelGGL(push, dim3(push_grid), dim3(kNT), 0, s, a, step);
+      hipLaunchKernelGGL(pull, dim3(wave_grid), dim3(kNT), 0, s, a, step);
+      hipLaunchKernelGGL(end, dim3(end_grid), dim3(kNT), 0, s, a, step);
+      EG_HIP(hipGetLastError());
+    }
+    unsigned long long next_nv = 0;
+    EG_HIP(hipMemcpyAsync(&next_nv, &a.st.cnt[step % 3].nv, 8, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    if (!next_nv || step >= max_steps) break;
+  }
+  EG_HIP(hipMemcpyAsync(h_steps, a.st.steps, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+}  // namespace
+}  // namespace eg
+
+using namespace eg;
+
+extern "C" {
+
+int eg_bfs_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, uint32_t* d_parent, uint32_t* d_depth,
+                 uint32_t* h_levels, uint64_t* h_edges, uint32_t* h_steps, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!g) return fail(EG_ERR_ARG, "graph is NULL");
+  EG_TRY(check_roots(g, h_roots, n_roots));
+  if (!d_parent || !d_depth) return fail(EG_ERR_ARG, "d_parent and d_depth (V*64 each) are required");
+  if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
+  EG_TRY(ensure_tview(g, s));
+  MultiState st;
+  EG_TRY(start_multi(g, h_roots, n_roots, &st, d_parent, d_depth, nullptr, s));
+  BfsArgs a;
+  a.V = g->V;
+  a.E = g->E;
+  a.policy = policy;
+  a.lanes = n_roots == 64 ? ~0ull : ((1ull << n_roots) - 1);
+  a.out_ptr = g->e_out_ptr.as<uint32_t>();
+  a.out_dst = g->e_out_dst.as<uint32_t>();
+  a.deg = g->out_deg.as<uint32_t>();
+  a.in_ptr = g->tv_ptr.as<uint32_t>();
+  a.in_src = g->tv_src.as<uint32_t>();
+  a.st = st;
+  a.parent = d_parent;
+  a.depth = d_depth;
+  uint32_t steps = 0;
+  prof_begin(s);
+  EG_TRY(run_steps<BfsArgs>(g, a, k_mbfs_push, k_mbfs_pull, k_mbfs_end, &steps, s));
+  prof_end(PK_BFS, 0, s);
+  // per-root levels and traversed edges
+  uint32_t* maxd = g->e_multi_roots.as<uint32_t>() + 64;
+  unsigned long long* edges = reinterpret_cast<unsigned long long*>(g->e_multi_roots.as<char>() + 512);
+  EG_TRY(fill_t<uint32_t>(maxd, 64, 0u, s));
+  EG_TRY(fill_t<unsigned long long>(edges, 64, 0ull, s));
+  hipLaunchKernelGGL(k_bfs_multi_stats, dim3(grid_for(g->V, kNT / kWave, 2048)), dim3(kNT), 0, s,
+                     (const uint32_t*)d_depth, g->V, (const uint32_t*)g->out_deg.as<uint32_t>(), maxd, edges);
+  EG_HIP(hipGetLastError());
+  uint32_t h_maxd[64];
+  unsigned long long h_ed[64];
+  EG_HIP(hipMemcpyAsync(h_maxd, maxd, sizeof(h_maxd), hipMemcpyDeviceToHost, s));
+  EG_HIP(hipMemcpyAsync(h_ed, edges, sizeof(h_ed), hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  for (int r = 0; r < n_roots; ++r) {
+    if (h_levels) h_levels[r] = h_maxd[r] + 1;
+    if (h_edges) h_edges[r] = h_ed[r];
+  }
+  if (h_steps) *h_steps = steps;
+  return EG_OK;
+}
+
+int eg_sssp_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, float* d_dist, uint32_t* h_steps,
+                  void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!g) return fail(EG_ERR_ARG, "graph is NULL");
+  if (g->rec != 12) return fail(EG_ERR_STATE, "SSSP needs a graph of EdgeType<float> (12-byte) records");
+  EG_TRY(check_roots(g, h_roots, n_roots));
+  if (!d_dist) return fail(EG_ERR_ARG, "d_dist (V*64 floats) is required");
+  if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
+  EG_TRY(ensure_tview(g, s));
+  MultiState st;
+  EG_TRY(start_multi(g, h_roots, n_roots, &st, nullptr, nullptr, d_dist, s));
+  SsspArgs a;
+  a.V = g->V;
+  a.E = g->E;
+  a.policy = policy;
+  a.lanes = n_roots == 64 ? ~0ull : ((1ull << n_roots) - 1);
+  a.out_ptr = g->e_out_ptr.as<uint32_t>();
+  a.out_dst = g->e_out_dst.as<uint32_t>();
+  a.out_w = g->e_out_w.as<float>();
+  a.deg = g->out_deg.as<uint32_t>();
+  a.in_ptr = g->tv_ptr.as<uint32_t>();
+  a.in_src = g->tv_src.as<uint32_t>();
+  a.in_w = g->tv_w.as<float>();
+  a.st = st;
+  a.dist = d_dist;
+  uint32_t steps = 0;
+  EG_TRY(run_steps<SsspArgs>(g, a, k_msssp_push, k_msssp_pull, k_msssp_end, &steps, s));
+  if (h_steps) *h_steps = steps;
+  return EG_OK;
+}
+
+}  // extern "C"

@@ -249,6 +249,23 @@ int eg_sssp_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_
                       uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream);
 int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_iterations, void* stream);
 
+/* Batched BFS and SSSP from n_roots (1..64) roots at once (no reference
+ * counterpart: bfs.cpp / sssp.cpp run one root per process); each level or
+ * relaxation round serves all roots, directions are decided on the device.
+ * Root r is lane r of a wavefront; results are vertex-major, 64 per vertex:
+ * d_parent[v*64 + r], d_depth[v*64 + r] (u32, V*64 each), d_dist[v*64 + r]
+ * (f32, V*64).  Each root's column equals eg_bfs / eg_sssp from that root bit
+ * for bit (same min-id parent rule, same policy meanings); lanes >= n_roots
+ * hold the unreached values (parent V, depth 0xffffffff, dist 1e9).
+ * h_levels / h_edges (optional, n_roots each) get what eg_bfs reports per
+ * root; h_steps (optional) the number of levels / rounds run.
+ * The first call after a build derives a source-sorted copy of the CSC
+ * (the pull side of the batched programs). */
+int eg_bfs_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, uint32_t* d_parent, uint32_t* d_depth,
+                 uint32_t* h_levels, uint64_t* h_edges, uint32_t* h_steps, void* stream);
+int eg_sssp_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, float* d_dist, uint32_t* h_steps,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif
