@@ -54,6 +54,79 @@ def median_time(fn, runs=3, warm=1):
     return statistics.median(ts)
 
 
+def batching_extras(eg):
+    """Batched programs: 64 seeded roots of BFS and SSSP on R-MAT scale 22 with
+    fp32 weights U[0,1) (one batched run against 64 single-root runs), and the
+    CSR build of 256 Erdos-Renyi graphs (|V| log-uniform in 1e3..1e6, average
+    degree U[8,32]) as one batch against one build per graph."""
+    out = {}
+    scale, V = 22, 1 << 22
+    E = 16 * V
+    buf = eg.DeviceBuffer(E * 12)
+    eg.gen_rmat(buf.ptr, E, scale, seed=4, rec_bytes=12, w_lo=0.0, w_hi=1.0)
+    g = eg.Graph(V, 1, 1, 1, 1, 12)
+    g.build(buf.ptr, E)
+    g.prepare_engine()
+    roots = np.random.default_rng(4).integers(0, V, 64).astype(np.uint32)
+    parent = eg.DeviceBuffer(V * 64 * 4)
+    depth = eg.DeviceBuffer(V * 64 * 4)
+    res = {}
+
+    def bfs_batch():
+        res["b"] = g.bfs_multi(roots, eg.POLICY_AUTO, parent, depth, download=False)
+
+    def bfs_single():
+        p1, d1 = eg.DeviceBuffer(V * 4), eg.DeviceBuffer(V * 4)
+        for r in roots:
+            g.bfs(int(r), eg.POLICY_AUTO, p1, d1, download=False)
+
+    tb = median_time(bfs_batch)
+    ts = median_time(bfs_single, runs=1)
+    levels, edges, steps = res["b"]
+    out["bfs_64_roots"] = {"workload": "R-MAT scale 22, edge factor 16, fp32 weights U[0,1), seed 4; 64 roots, seed 4",
+                           "ms_batched": round(tb * 1e3, 3), "ms_64_single": round(ts * 1e3, 3),
+                           "speedup": round(ts / tb, 2), "steps": int(steps),
+                           "gteps_batched": round(float(edges.sum()) / tb / 1e9, 3)}
+    log(f"[bench] bfs 64 roots: batched {tb * 1e3:.2f} ms vs single {ts * 1e3:.2f} ms")
+    del parent, depth
+    dist = eg.DeviceBuffer(V * 64 * 4)
+    tb = median_time(lambda: g.sssp_multi(roots, eg.POLICY_AUTO, dist, download=False))
+
+    def sssp_single():
+        d1 = eg.DeviceBuffer(V * 4)
+        for r in roots:
+            g.sssp(int(r), eg.POLICY_AUTO, d1, download=False)
+
+    ts = median_time(sssp_single, runs=1)
+    out["sssp_64_roots"] = {"ms_batched": round(tb * 1e3, 3), "ms_64_single": round(ts * 1e3, 3),
+                            "speedup": round(ts / tb, 2)}
+    log(f"[bench] sssp 64 roots: batched {tb * 1e3:.2f} ms vs single {ts * 1e3:.2f} ms")
+    del dist, g, buf
+    rng = np.random.default_rng(256)
+    n = 256
+    nv = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n)).astype(np.uint32)
+    ne = (nv.astype(np.float64) * rng.uniform(8, 32, n)).astype(np.uint64)
+    total = int(ne.sum())
+    ebuf = eg.DeviceBuffer(total * 8)
+    eg.gen_er_batch(ebuf.ptr, nv, ne, seed=5)
+    b = eg.BatchCSR(ebuf.ptr, nv, ne)
+    tb = median_time(lambda: b.rebuild(ebuf.ptr))
+    off = np.concatenate([[0], np.cumsum(ne)]).astype(np.int64)
+    singles = [eg.BatchCSR(ebuf.ptr + int(off[i]) * 8, nv[i:i + 1], ne[i:i + 1]) for i in range(n)]
+
+    def one_by_one():
+        for i in range(n):
+            singles[i].rebuild(ebuf.ptr + int(off[i]) * 8)
+
+    ts = median_time(one_by_one, runs=1)
+    out["er_256_csr"] = {"workload": "256 Erdos-Renyi graphs, |V| log-uniform 1e3..1e6, degree U[8,32], seed 256",
+                         "vertices": int(nv.sum()), "edges": total, "ms_batched": round(tb * 1e3, 3),
+                         "ms_one_by_one": round(ts * 1e3, 3), "speedup": round(ts / tb, 2),
+                         "medges_per_s_batched": round(total / tb / 1e6, 1)}
+    log(f"[bench] 256 ER CSR builds: batched {tb * 1e3:.2f} ms vs one by one {ts * 1e3:.2f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +135,7 @@ def main():
     ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
     ap.add_argument("--no-engine", action="store_true", help="skip the BFS / PageRank figures")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batching", action="store_true", help="skip the batched BFS/SSSP/CSR figures")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default sort variant")
     ap.add_argument("--cpu-sample-edges", type=int, default=1 << 25)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
@@ -167,6 +241,9 @@ def main():
             "speedup_fp32c_vs_fp64_ordered": round(t64o / t32, 3)}
         log(f"[bench] pagerank: fp64 {gt(t64)} GTEPS, fp32c {gt(t32)} GTEPS ({t64 / t32:.2f}x), L1 {l1:.2e}; "
             f"fp64 ordered {gt(t64o)}, fp32c plain {gt(t32p)}; view {view_ms:.1f} ms")
+
+    if not args.no_engine and not args.no_batching:
+        extra["batching"] = batching_extras(eg)
 
     cpu = None
     if not args.no_cpu_baseline:

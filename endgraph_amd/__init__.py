@@ -80,6 +80,7 @@ def lib():
         "eg_graph_csr": ([vp, i32, i32, pp, C.POINTER(u64), pp, pp], i32),
         "eg_gen_er_batch": ([vp, i32, vp, vp, u64, i32, vp], i32),
         "eg_batch_csr_build": ([vp, i32, i32, vp, vp, pp, vp], i32),
+        "eg_batch_csr_rebuild": ([vp, vp, vp], i32),
         "eg_batch_csr_get": ([vp, i32, pp, C.POINTER(u64), pp, C.POINTER(u32)], i32),
         "eg_batch_destroy": ([vp], i32),
         "eg_device_fill": ([vp, u64, vp, i32, vp], i32),
@@ -252,6 +253,10 @@ class BatchCSR:
 
     def __del__(self):
         self.close()
+
+    def rebuild(self, d_edges: int, stream=None):
+        """Build again from new records of the same sizes (no allocation)."""
+        check(lib().eg_batch_csr_rebuild(self._h, C.c_void_p(d_edges), _stream(stream)))
 
     def get(self, g: int):
         lst, rp = C.c_void_p(), C.c_void_p()

@@ -22,7 +22,7 @@ struct eg_batch {
   eg::DevBuf d_vbase, d_ebase;         // n+1 each (device)
   eg::DevBuf relabel, t2, out;         // records
   eg::DevBuf rowptr;                   // u64[Vtot + n]: graph g's pointers at vbase[g] + g
-  eg::DevBuf hist, scan_tmp;
+  eg::DevBuf hist, scan_tmp, err;
   eg::RxWork sortw;
   uint32_t max_v = 0;
 };
@@ -119,7 +119,7 @@ int batch_build(eg_batch* b, const R* d_in, hipStream_t s) {
   EG_TRY(b->out.resize(std::max<uint64_t>(E, 1) * sizeof(R)));
   EG_TRY(b->rowptr.resize(nptr * 8));
   EG_TRY(b->scan_tmp.resize(scan_tmp_elems(nptr) * 8));
-  DevBuf err;
+  DevBuf& err = b->err;
   EG_TRY(err.resize(4));
   EG_HIP(hipMemsetAsync(err.ptr, 0, 4, s));
   const uint64_t* vb = b->d_vbase.as<uint64_t>();
@@ -209,6 +209,14 @@ int eg_batch_csr_build(const void* d_edges, int rec_bytes, int n_graphs, const u
   }
   *out = b;
   return EG_OK;
+}
+
+int eg_batch_csr_rebuild(eg_batch* b, const void* d_edges, void* stream) {
+  if (!b) return fail(EG_ERR_ARG, "batch is NULL");
+  if (!d_edges && b->E) return fail(EG_ERR_ARG, "edge list is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return b->rec == 8 ? batch_build<Rec8>(b, static_cast<const Rec8*>(d_edges), s)
+                     : batch_build<Rec12>(b, static_cast<const Rec12*>(d_edges), s);
 }
 
 int eg_batch_csr_get(const eg_batch* b, int g, const void** d_list, uint64_t* n_edges, const uint64_t** d_rowptr,

@@ -71,6 +71,8 @@ struct eg_graph {
   eg::DevBuf tv_ptr;                 // u32[V+1] row pointer (prefix of in-degree)
   eg::DevBuf tv_src;                 // u32[E]
   eg::DevBuf tv_w;                   // f32[E] (12-byte records only)
+  eg::DevBuf tv_items;               // u64 (v << 32 | chunk): in-edge chunks of at most 512 edges
+  uint64_t tv_nitems = 0;
   eg::DevBuf e_multi;                // batched traversal masks, work items, counters
   eg::DevBuf e_multi_roots;          // roots and per-root statistics
   double eng_prepare_ms = 0;

@@ -33,6 +33,7 @@ namespace {
 
 constexpr int kNT = 256;
 constexpr uint32_t kItemEdges = 4096;  // out-edges per push work item
+constexpr uint32_t kPullEdges = 512;   // in-edges per pull work item (SSSP)
 constexpr uint32_t kUnreached = 0xffffffffu;
 constexpr float kInfDist = 1e9f;       // toolkits/sssp.cpp INF
 
@@ -41,10 +42,10 @@ unsigned grid_for(uint64_t n, uint64_t per = kNT, uint64_t cap = 65536) {
   return unsigned(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
-// Counters of one level: vertices with new bits, their out-degree sum and
-// the number of push work items they produced.
+// Counters of one level: vertices with new bits, their out-degree sum, the
+// roots that still have a frontier and the number of push work items.
 struct MCount {
-  unsigned long long nv, ne;
+  unsigned long long nv, ne, roots;
   unsigned int items, pad;
 };
 
@@ -106,6 +107,17 @@ __global__ void k_copy_u32(const uint32_t* __restrict__ a, uint32_t* __restrict_
     b[i] = a[i];
 }
 
+// Pull work items: in-edge chunks of at most kPullEdges edges per vertex.
+__global__ void k_tv_item_count(const uint32_t* __restrict__ in_deg, uint32_t V, uint32_t* __restrict__ cnt) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
+    cnt[v] = uint32_t(ceil_div(in_deg[v], kPullEdges));
+}
+
+__global__ void k_tv_items(const uint32_t* __restrict__ off, uint32_t V, uint64_t* __restrict__ items) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
+    for (uint32_t j = 0; j < off[v + 1] - off[v]; ++j) items[off[v] + j] = (v << 32) | j;
+}
+
 template <class R>
 int tview_impl(eg_graph* g, hipStream_t s) {
   const uint64_t E = g->E;
@@ -120,6 +132,25 @@ int tview_impl(eg_graph* g, hipStream_t s) {
   EG_HIP(hipGetLastError());
   EG_TRY(fill_t<uint32_t>(ptr + V, 1, 0u, s));
   EG_TRY(exclusive_scan_u32(ptr, ptr, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
+  {
+    DevBuf off;
+    EG_TRY(off.resize((uint64_t(V) + 1) * 4));
+    uint32_t* o = off.as<uint32_t>();
+    hipLaunchKernelGGL(k_tv_item_count, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)g->in_deg.as<uint32_t>(),
+                       V, o);
+    EG_HIP(hipGetLastError());
+    EG_TRY(fill_t<uint32_t>(o + V, 1, 0u, s));
+    EG_TRY(exclusive_scan_u32(o, o, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
+    uint32_t n = 0;
+    EG_HIP(hipMemcpyAsync(&n, o + V, 4, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    g->tv_nitems = n;
+    EG_TRY(g->tv_items.resize(std::max<uint64_t>(n, 1) * 8));
+    hipLaunchKernelGGL(k_tv_items, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)o, V,
+                       g->tv_items.as<uint64_t>());
+    EG_HIP(hipGetLastError());
+    EG_HIP(hipStreamSynchronize(s));  // `off` is freed on scope exit
+  }
   if (E) {
     DevBuf b, c;
     EG_TRY(b.resize(E * sizeof(R)));
@@ -193,6 +224,7 @@ __global__ void k_multi_roots(const uint32_t* __restrict__ roots, int n, const u
   bool first = true;  // the lowest lane holding this root adds its work items
   for (int q = 0; q < r; ++q) first = first && roots[q] != v;
   if (!first) return;
+  atomicOr(&cnt[0].roots, 1ull << r);
   const uint32_t d = deg[v];
   atomicAdd(&cnt[0].nv, 1ull);
   atomicAdd(&cnt[0].ne, (unsigned long long)d);
@@ -201,19 +233,74 @@ __global__ void k_multi_roots(const uint32_t* __restrict__ roots, int n, const u
   for (uint32_t j = 0; j < k; ++j) items[at + j] = (uint64_t(v) << 32) | j;
 }
 
-__device__ __forceinline__ void append_items(uint32_t v, uint32_t d, MCount* nc, uint64_t* items) {
-  const uint32_t k = uint32_t(ceil_div(d, kItemEdges));
-  if (!k) return;
-  const uint32_t at = atomicAdd(&nc->items, k);
-  for (uint32_t j = 0; j < k; ++j) items[at + j] = (uint64_t(v) << 32) | j;
-}
-
-__device__ __forceinline__ void add_counts(unsigned long long nv, unsigned long long ne, MCount* nc) {
+// Level end shared by both programs.  A wave owns the vertex groups
+// v0 = (gw + i*nw) * 64 ..; lane = vertex.  Pass 1 sums the work items and
+// counters of the wave's vertices, one atomic per wave reserves the items,
+// pass 2 writes them and hands each vertex with new bits to `visit`
+// (wave-uniform call per group: lanes without new bits pass nb = 0).
+// Counters reach memory once per block.
+template <class Visit>
+__device__ __forceinline__ void level_end(uint32_t V, const uint32_t* __restrict__ deg, uint64_t* __restrict__ nx,
+                                          uint64_t* __restrict__ fr, MCount* nc, uint64_t* __restrict__ items,
+                                          Visit visit) {
+  __shared__ unsigned long long s_nv[kNT / kWave], s_ne[kNT / kWave], s_roots[kNT / kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  unsigned long long nv = 0, ne = 0, roots = 0;
+  uint32_t ksum = 0;
+  for (uint64_t v0 = gw * kWave; v0 < V; v0 += nw * kWave) {
+    const uint64_t v = v0 + lane;
+    const uint64_t nb = v < V ? nx[v] : 0ull;
+    if (nb) {
+      const uint32_t d = deg[v];
+      roots |= nb;
+      ++nv;
+      ne += d;
+      ksum += uint32_t(ceil_div(d, kItemEdges));
+    }
+  }
+  const uint32_t inc = wave_inclusive_scan(ksum, OpSum{});
+  const uint32_t tot = __shfl(inc, kWave - 1, kWave);
+  uint32_t base = 0;
+  if (lane == 0 && tot) base = atomicAdd(&nc->items, tot);
+  uint32_t at = __shfl(base, 0, kWave) + inc - ksum;
+  for (uint64_t v0 = gw * kWave; v0 < V; v0 += nw * kWave) {
+    const uint64_t v = v0 + lane;
+    uint64_t nb = 0;
+    if (v < V) {
+      nb = nx[v];
+      if (nb) {
+        nx[v] = 0;
+        const uint32_t k = uint32_t(ceil_div(deg[v], kItemEdges));
+        for (uint32_t j = 0; j < k; ++j) items[at + j] = (v << 32) | j;
+        at += k;
+      }
+      if (fr[v] != nb) fr[v] = nb;
+    }
+    visit(v0, v, nb);
+  }
   nv = wave_reduce(nv, OpSum{});
   ne = wave_reduce(ne, OpSum{});
-  if ((threadIdx.x & (kWave - 1)) == 0 && nv) {
-    atomicAdd(&nc->nv, nv);
-    atomicAdd(&nc->ne, ne);
+  roots = wave_or(roots);
+  if (lane == 0) {
+    s_nv[threadIdx.x / kWave] = nv;
+    s_ne[threadIdx.x / kWave] = ne;
+    s_roots[threadIdx.x / kWave] = roots;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tv = 0, te = 0, tr = 0;
+    for (int i = 0; i < kNT / kWave; ++i) {
+      tv += s_nv[i];
+      te += s_ne[i];
+      tr |= s_roots[i];
+    }
+    if (tv) {
+      atomicAdd(&nc->nv, tv);
+      atomicAdd(&nc->ne, te);
+      atomicOr(&nc->roots, tr);
+    }
   }
 }
 
@@ -221,7 +308,7 @@ __device__ __forceinline__ void add_counts(unsigned long long nv, unsigned long 
 // step s+2 (last read at step s-1) and records step s as run.
 __device__ __forceinline__ void step_bookkeeping(const MultiState& st, uint32_t step, bool live) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st.cnt[(step + 2) % 3] = MCount{0, 0, 0, 0};
+    st.cnt[(step + 2) % 3] = MCount{0, 0, 0, 0, 0};
     if (live) *st.steps = step + 1;
   }
 }
@@ -233,6 +320,8 @@ struct BfsArgs {
   int policy;
   uint64_t lanes;  // bits of the live roots
   const uint32_t *out_ptr, *out_dst, *deg, *in_ptr, *in_src;
+  const uint64_t* pull_items;  // in-edge chunks (v << 32 | chunk)
+  uint64_t n_pull_items;
   MultiState st;
   uint32_t *parent, *depth;
 };
@@ -266,22 +355,27 @@ __global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t step) {
   }
 }
 
-// Dense step (bfs.cpp:60-68 per root): a wave per destination walks its
-// in-edges in source order; the first frontier source of a root (the
-// smallest id) is its parent; the row stops once every unvisited root of v
-// is found.
+// Dense step (bfs.cpp:60-68 per root): a wave takes an in-edge chunk (at
+// most 512 edges, in source order) of a destination; the first frontier
+// source of a root in the chunk is the chunk's smallest, write_min over the
+// chunks gives the smallest of the row.  Only roots that still have a
+// frontier are looked for, and a chunk stops once all of them are found.
 __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t step) {
   const MCount c = a.st.cnt[step % 3];
   if (!c.nv || sparse_level(a.policy, c.ne, a.E)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  for (uint64_t v = gw; v < a.V; v += nw) {
-    const uint64_t need = a.lanes & ~a.st.vis[v];
+  const uint64_t live_roots = a.lanes & c.roots;
+  for (uint64_t i = gw; i < a.n_pull_items; i += nw) {
+    const uint64_t it = a.pull_items[i];
+    const uint64_t v = it >> 32;
+    const uint64_t need = live_roots & ~a.st.vis[v];
     if (!need) continue;
+    const uint32_t e0 = a.in_ptr[v] + uint32_t(it) * kPullEdges;
+    const uint32_t b1 = min(e0 + kPullEdges, a.in_ptr[v + 1]);
     uint64_t found = 0;
-    const uint32_t b1 = a.in_ptr[v + 1];
-    for (uint32_t e = a.in_ptr[v]; e < b1 && found != need; e += kWave) {
+    for (uint32_t e = e0; e < b1 && found != need; e += kWave) {
       const bool ok = e + lane < b1;
       const uint32_t u = ok ? a.in_src[e + lane] : 0u;
       const uint64_t f = ok ? (a.st.fr[u] & need) : 0ull;
@@ -290,12 +384,12 @@ __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t step) {
         const int r = __ffsll((unsigned long long)todo) - 1;
         const uint64_t m = __ballot((f >> r) & 1ull);
         const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
-        if (lane == r) a.parent[v * 64 + r] = p;
+        if (lane == r) atomicMin(&a.parent[v * 64 + r], p);
         found |= 1ull << r;
         todo &= todo - 1;
       }
     }
-    if (found && lane == 0) a.st.nx[v] = found;
+    if (found && lane == 0) atomicOr((unsigned long long*)&a.st.nx[v], found);
   }
 }
 
@@ -304,25 +398,20 @@ __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t step) {
 __global__ __launch_bounds__(kNT) void k_mbfs_end(BfsArgs a, uint32_t step) {
   const bool live = a.st.cnt[step % 3].nv != 0;
   step_bookkeeping(a.st, step, live);
-  if (!live) return;
-  MCount* nc = &a.st.cnt[(step + 1) % 3];
-  uint64_t* nitems = a.st.items[(step + 1) & 1];
-  unsigned long long nv = 0, ne = 0;
-  const uint64_t nt = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < a.V; v += nt) {
-    const uint64_t nb = a.st.nx[v];
-    if (nb) {
-      a.st.vis[v] |= nb;
-      a.st.nx[v] = 0;
-      for (uint64_t b = nb; b; b &= b - 1) a.depth[v * 64 + (__ffsll((unsigned long long)b) - 1)] = step + 1;
-      const uint32_t d = a.deg[v];
-      ++nv;
-      ne += d;
-      append_items(uint32_t(v), d, nc, nitems);
-    }
-    if (a.st.fr[v] != nb) a.st.fr[v] = nb;
-  }
-  add_counts(nv, ne, nc);
+  if (!live) return;  // block-uniform
+  const int lane = threadIdx.x & (kWave - 1);
+  level_end(a.V, a.deg, a.st.nx, a.st.fr, &a.st.cnt[(step + 1) % 3], a.st.items[(step + 1) & 1],
+            [&](uint64_t v0, uint64_t v, uint64_t nb) {
+              if (nb) a.st.vis[v] |= nb;
+              // depth rows: lane = root, one 256-byte row per vertex
+              uint64_t rows = __ballot(nb != 0);
+              while (rows) {
+                const int j = __ffsll((unsigned long long)rows) - 1;
+                rows &= rows - 1;
+                const uint64_t bj = shfl64(nb, j);
+                if ((bj >> lane) & 1ull) a.depth[(v0 + j) * 64 + lane] = step + 1;
+              }
+            });
 }
 
 // ----------------------------------------------------------------- SSSP --
@@ -333,6 +422,8 @@ struct SsspArgs {
   uint64_t lanes;
   const uint32_t *out_ptr, *out_dst, *deg, *in_ptr, *in_src;
   const float *out_w, *in_w;
+  const uint64_t* pull_items;  // in-edge chunks (v << 32 | chunk)
+  uint64_t n_pull_items;
   MultiState st;
   float* dist;
 };
@@ -370,36 +461,54 @@ __global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t step) {
 
 // Dense round (sssp.cpp:60-73 per root): min over active in-neighbours of
 // dist[u] + w; lane r is root r, so each in-edge reads its source's 64
-// distances as one row.
+// distances as one row.  A wave takes an in-edge chunk of at most 512 edges
+// (hub rows spread over many waves) and lowers dist[v] with write_min.
 __global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t step) {
   const MCount c = a.st.cnt[step % 3];
   if (!c.nv || sparse_level(a.policy, c.ne, a.E)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  for (uint64_t v = gw; v < a.V; v += nw) {
-    const uint32_t b1 = a.in_ptr[v + 1];
+  for (uint64_t i = gw; i < a.n_pull_items; i += nw) {
+    const uint64_t it = a.pull_items[i];
+    const uint64_t v = it >> 32;
+    const uint32_t e0 = a.in_ptr[v] + uint32_t(it) * kPullEdges;
+    const uint32_t b1 = min(e0 + kPullEdges, a.in_ptr[v + 1]);
     float best = kInfDist;
-    for (uint32_t e = a.in_ptr[v]; e < b1; e += kWave) {
+    for (uint32_t e = e0; e < b1; e += kWave) {
       const bool ok = e + lane < b1;
       const uint32_t u = ok ? a.in_src[e + lane] : 0u;
       const float w = ok ? a.in_w[e + lane] : 0.0f;
       const uint64_t am = ok ? a.st.fr[u] : 0ull;
-      if (!wave_or(am)) continue;
-      const int cnt = int(min<uint32_t>(kWave, b1 - e));
-      for (int j = 0; j < cnt; ++j) {
-        const uint64_t aj = shfl64(am, j);
-        if (!aj) continue;  // wave-uniform
-        const uint32_t uj = __shfl(u, j, kWave);
-        const float wj = __shfl(w, j, kWave);
-        if ((aj >> lane) & 1ull) best = fminf(best, __fadd_rn(a.dist[uint64_t(uj) * 64 + lane], wj));
+      uint64_t edges = __ballot(am != 0);  // in-edges of the chunk with an active source
+      while (edges) {
+        // eight active in-edges at a time: all distance rows in flight
+        // before the first use
+        constexpr int kG = 8;
+        uint32_t uj[kG];
+        float wj[kG], dj[kG];
+        bool take[kG];
+#pragma unroll
+        for (int k = 0; k < kG; ++k) {
+          const int j = edges ? __ffsll((unsigned long long)edges) - 1 : 0;
+          const bool live = edges != 0;
+          edges &= edges - 1;
+          uj[k] = __shfl(u, j, kWave);
+          wj[k] = __shfl(w, j, kWave);
+          take[k] = live && ((shfl64(am, j) >> lane) & 1ull);
+        }
+#pragma unroll
+        for (int k = 0; k < kG; ++k) dj[k] = take[k] ? a.dist[uint64_t(uj[k]) * 64 + lane] : kInfDist;
+#pragma unroll
+        for (int k = 0; k < kG; ++k)
+          if (take[k]) best = fminf(best, __fadd_rn(dj[k], wj[k]));
       }
     }
     float* p = &a.dist[v * 64 + lane];
-    const bool lower = ((a.lanes >> lane) & 1ull) && best < *p;
-    if (lower) *p = best;
+    bool lower = ((a.lanes >> lane) & 1ull) && best < *p;
+    if (lower) lower = best < __int_as_float(atomicMin(reinterpret_cast<int*>(p), __float_as_int(best)));
     const uint64_t m = __ballot(lower);
-    if (m && lane == 0) a.st.nx[v] = m;
+    if (m && lane == 0) atomicOr((unsigned long long*)&a.st.nx[v], m);
   }
 }
 
@@ -407,23 +516,9 @@ __global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t step) {
 __global__ __launch_bounds__(kNT) void k_msssp_end(SsspArgs a, uint32_t step) {
   const bool live = a.st.cnt[step % 3].nv != 0;
   step_bookkeeping(a.st, step, live);
-  if (!live) return;
-  MCount* nc = &a.st.cnt[(step + 1) % 3];
-  uint64_t* nitems = a.st.items[(step + 1) & 1];
-  unsigned long long nv = 0, ne = 0;
-  const uint64_t nt = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < a.V; v += nt) {
-    const uint64_t nb = a.st.nx[v];
-    if (nb) {
-      a.st.nx[v] = 0;
-      const uint32_t d = a.deg[v];
-      ++nv;
-      ne += d;
-      append_items(uint32_t(v), d, nc, nitems);
-    }
-    if (a.st.fr[v] != nb) a.st.fr[v] = nb;
-  }
-  add_counts(nv, ne, nc);
+  if (!live) return;  // block-uniform
+  level_end(a.V, a.deg, a.st.nx, a.st.fr, &a.st.cnt[(step + 1) % 3], a.st.items[(step + 1) & 1],
+            [](uint64_t, uint64_t, uint64_t) {});
 }
 
 // Per-root statistics of a finished BFS: levels (max depth + 1) and edges
@@ -481,9 +576,9 @@ constexpr uint32_t kBatch = 4;
 template <class Args>
 int run_steps(eg_graph* g, const Args& a, void (*push)(Args, uint32_t), void (*pull)(Args, uint32_t),
               void (*end)(Args, uint32_t), uint32_t* h_steps, hipStream_t s) {
-  const unsigned wave_grid = grid_for(g->V, kNT / kWave, 65536);
+  const unsigned wave_grid = grid_for(g->tv_nitems, kNT / kWave, 65536);
   const unsigned push_grid = 8192;
-  const unsigned end_grid = grid_for(g->V, kNT, 16384);
+  const unsigned end_grid = grid_for(ceil_div(g->V, kWave), kNT / kWave, 1024);
   const uint32_t max_steps = g->V + 1;  // a frontier is empty after at most V steps
   uint32_t step = 0;
   for (;;) {
@@ -530,6 +625,8 @@ int eg_bfs_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, 
   a.deg = g->out_deg.as<uint32_t>();
   a.in_ptr = g->tv_ptr.as<uint32_t>();
   a.in_src = g->tv_src.as<uint32_t>();
+  a.pull_items = g->tv_items.as<uint64_t>();
+  a.n_pull_items = g->tv_nitems;
   a.st = st;
   a.parent = d_parent;
   a.depth = d_depth;
@@ -581,6 +678,8 @@ int eg_sssp_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy,
   a.in_ptr = g->tv_ptr.as<uint32_t>();
   a.in_src = g->tv_src.as<uint32_t>();
   a.in_w = g->tv_w.as<float>();
+  a.pull_items = g->tv_items.as<uint64_t>();
+  a.n_pull_items = g->tv_nitems;
   a.st = st;
   a.dist = d_dist;
   uint32_t steps = 0;

@@ -160,6 +160,9 @@ int eg_graph_csr(const eg_graph* g, int p, int s, const void** d_list, uint64_t*
 typedef struct eg_batch eg_batch;
 int eg_batch_csr_build(const void* d_edges, int rec_bytes, int n_graphs, const uint32_t* h_vertices,
                        const uint64_t* h_edges, eg_batch** out, void* stream);
+/* Builds again from new records of the same sizes, reusing the batch's
+ * device memory (no allocation after the first build).  Synchronises. */
+int eg_batch_csr_rebuild(eg_batch* b, const void* d_edges, void* stream);
 int eg_batch_csr_get(const eg_batch* b, int g, const void** d_list, uint64_t* n_edges, const uint64_t** d_rowptr,
                      uint32_t* n_vertices);
 int eg_batch_destroy(eg_batch* b);

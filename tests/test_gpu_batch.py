@@ -51,3 +51,23 @@ def test_batch_rejects_out_of_range(eg):
     with pytest.raises(eg.EndGraphError) as ei:
         eg.BatchCSR(buf.ptr, [4, 3], [2, 2])
     assert ei.value.status == -2
+
+
+def test_batch_rebuild_reuses_handle(eg):
+    """A rebuild from new records of the same sizes equals a fresh build."""
+    rng = np.random.default_rng(8)
+    n = 12
+    V = np.exp(rng.uniform(np.log(100), np.log(5000), n)).astype(np.uint32)
+    Ecnt = (V.astype(np.float64) * rng.uniform(8, 32, n)).astype(np.uint64)
+    total = int(Ecnt.sum())
+    a = eg.DeviceBuffer(total * 8)
+    c = eg.DeviceBuffer(total * 8)
+    eg.gen_er_batch(a.ptr, V, Ecnt, seed=1)
+    eg.gen_er_batch(c.ptr, V, Ecnt, seed=2)
+    b = eg.BatchCSR(a.ptr, V, Ecnt)
+    b.rebuild(c.ptr)
+    fresh = eg.BatchCSR(c.ptr, V, Ecnt)
+    for g in range(n):
+        l1, r1 = b.get(g)
+        l2, r2 = fresh.get(g)
+        assert np.array_equal(l1.view(np.uint8), l2.view(np.uint8)) and np.array_equal(r1, r2), g
