@@ -92,9 +92,10 @@ void drain() {
 
 bool prof_on() { return g_prof; }
 
-// Sort variant per record size (8 or 12 bytes); defaults chosen by measurement
-// on RMAT-24, replaced by eg_set_sort_variant / eg_autotune_sort.
-static SortVariant g_variant8{kThreeLaunch, 512, 9}, g_variant12{kThreeLaunch, 512, 9};
+// Sort variant per record size (8 or 12 bytes): the autotuner's choice on
+// RMAT-24 P=8 (three launches, 512 threads, 8-bit digits), replaced by
+// eg_set_sort_variant / eg_autotune_sort.
+static SortVariant g_variant8{kThreeLaunch, 512, 8}, g_variant12{kThreeLaunch, 512, 8};
 SortVariant& sort_variant(int rec_bytes) { return rec_bytes <= 8 ? g_variant8 : g_variant12; }
 
 void prof_begin(hipStream_t s) {

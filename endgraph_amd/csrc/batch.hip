@@ -4,11 +4,12 @@
 // P = 1, S = 1, core/graph.hpp:1248-1355: records ordered by (src, input
 // position), dense row pointer).
 //
-// The batch is treated as the disjoint union of its graphs: sources are
-// relabelled to global ids (graph base + local id), one stable LSD sort by the
-// global source orders every graph at once, and the sources are mapped back.
-// Row pointers come from run boundaries (first/last record of every source)
-// and an inclusive max-scan, as in the single-graph build.
+// The graphs stay in place as the segments of one segmented stable LSD sort
+// by local source id (a tile never spans two graphs; the digit count is set by
+// the largest graph).  Row pointers come from run boundaries (first/last
+// record of every source) written as global positions, one inclusive
+// max-scan over all pointer arrays (an empty leading row of graph g reads the
+// previous graph's end, ebase[g]) and a final shift to local positions.
 #include <algorithm>
 #include <vector>
 
@@ -18,27 +19,29 @@ struct eg_batch {
   int n = 0;
   int rec = 8;
   uint64_t E = 0, Vtot = 0;
+  uint32_t max_v = 0;
   std::vector<uint64_t> vbase, ebase;  // n+1 each (host)
   eg::DevBuf d_vbase, d_ebase;         // n+1 each (device)
-  eg::DevBuf relabel, t2, out;         // records
+  eg::DevBuf t2, out;                  // records
   eg::DevBuf rowptr;                   // u64[Vtot + n]: graph g's pointers at vbase[g] + g
-  eg::DevBuf hist, scan_tmp, err;
+  eg::DevBuf scan_tmp, err;
   eg::RxWork sortw;
-  uint32_t max_v = 0;
 };
 
 namespace eg {
 namespace {
 
 constexpr int kNT = 256;
+constexpr int kLdsGraphs = 1024;  // bases (graphs + 1) that a block copies to LDS
 
 unsigned grid_for(uint64_t n, uint64_t per = kNT, uint64_t cap = 65536) {
   uint64_t b = ceil_div(n, per);
   return unsigned(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
+// largest g in [0, n) with base[g] <= x (base non-decreasing, base[0] = 0)
 __device__ __forceinline__ int graph_of(const uint64_t* base, int n, uint64_t x) {
-  int lo = 0, hi = n;  // base[lo] <= x < base[hi]
+  int lo = 0, hi = n;
   while (hi - lo > 1) {
     const int mid = (lo + hi) / 2;
     if (base[mid] <= x)
@@ -49,50 +52,54 @@ __device__ __forceinline__ int graph_of(const uint64_t* base, int n, uint64_t x)
   return lo;
 }
 
-// src -> vbase[g] + src; records whose ids fall outside their graph flag err.
+// The n+1 bases in LDS when they fit (a block reads them once), else global.
+__device__ __forceinline__ const uint64_t* stage_bases(const uint64_t* base, int n, uint64_t* lds) {
+  if (n + 1 > kLdsGraphs) return base;
+  for (int i = threadIdx.x; i <= n; i += blockDim.x) lds[i] = base[i];
+  __syncthreads();
+  return lds;
+}
+
+struct KeyLocalSrc {
+  template <class R>
+  __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.src; }
+};
+
+// Run boundaries of the sorted graphs (graph g = records [ebase[g],
+// ebase[g+1])); ids outside their graph flag err and write nothing.  Graph
+// g's pointer array starts at slot vbase[g] + g; entries are global record
+// positions until the localise step.
 template <class R>
-__global__ void k_relabel(const R* __restrict__ in, R* __restrict__ out, uint64_t E, const uint64_t* __restrict__ ebase,
-                          const uint64_t* __restrict__ vbase, int n, uint32_t* __restrict__ err) {
-  for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += uint64_t(gridDim.x) * blockDim.x) {
-    const int g = graph_of(ebase, n, e);
-    R r = in[e];
+__global__ __launch_bounds__(kNT) void k_batch_runs(const R* __restrict__ a, uint64_t E,
+                                                    const uint64_t* __restrict__ ebase_g,
+                                                    const uint64_t* __restrict__ vbase_g, int n,
+                                                    uint64_t* __restrict__ rowptr, uint32_t* __restrict__ err) {
+  __shared__ uint64_t s_e[kLdsGraphs], s_v[kLdsGraphs];
+  const uint64_t* ebase = stage_bases(ebase_g, n, s_e);
+  const uint64_t* vbase = stage_bases(vbase_g, n, s_v);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += uint64_t(gridDim.x) * blockDim.x) {
+    const int g = graph_of(ebase, n, i);
     const uint64_t nv = vbase[g + 1] - vbase[g];
+    const R r = a[i];
     if (r.src >= nv || r.dst >= nv) {
       atomicOr(err, 1u);
-      r.src = 0;
-      r.dst = 0;
+      continue;
     }
-    r.src = uint32_t(vbase[g] + r.src);
-    out[e] = r;
+    const uint64_t slot = vbase[g] + g + r.src;
+    if (i == ebase[g] || a[i - 1].src != r.src) rowptr[slot] = i;
+    if (i + 1 == ebase[g + 1] || a[i + 1].src != r.src) rowptr[slot + 1] = i + 1;
   }
 }
 
-// Run boundaries of the globally sorted list: graph g's pointer array starts
-// at vbase[g] + g and holds vbase[g+1]-vbase[g]+1 entries; entries are global
-// record positions until the localise step.
-template <class R>
-__global__ void k_batch_runs(const R* __restrict__ a, uint64_t E, const uint64_t* __restrict__ vbase, int n,
-                             uint64_t* __restrict__ rowptr) {
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t s = a[i].src;
-    const int g = graph_of(vbase, n, s);
-    const uint64_t slot = uint64_t(s) + g;  // = vbase[g] + g + local id
-    if (i == 0 || a[i - 1].src != s) rowptr[slot] = i;
-    if (i == E - 1 || a[i + 1].src != s) rowptr[slot + 1] = i + 1;
-  }
-}
-
-// Back to local ids: src -= vbase[g]; pointers -= ebase[g].
-template <class R>
-__global__ void k_batch_localize(R* __restrict__ a, uint64_t E, uint64_t* __restrict__ rowptr, uint64_t nptr,
-                                 const uint64_t* __restrict__ vbase, const uint64_t* __restrict__ ebase, int n) {
-  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += stride) {
-    const uint32_t s = a[i].src;
-    a[i].src = uint32_t(s - vbase[graph_of(vbase, n, s)]);
-  }
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nptr; i += stride) {
-    // pointer slot i belongs to graph g with vbase[g] + g <= i <= vbase[g+1] + g
+// Global positions -> local: pointer slot i of graph g (vbase[g] + g <= i
+// <= vbase[g+1] + g) loses ebase[g].
+__global__ __launch_bounds__(kNT) void k_batch_localize(uint64_t* __restrict__ rowptr, uint64_t nptr,
+                                                        const uint64_t* __restrict__ vbase_g,
+                                                        const uint64_t* __restrict__ ebase_g, int n) {
+  __shared__ uint64_t s_e[kLdsGraphs], s_v[kLdsGraphs];
+  const uint64_t* ebase = stage_bases(ebase_g, n, s_e);
+  const uint64_t* vbase = stage_bases(vbase_g, n, s_v);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nptr; i += uint64_t(gridDim.x) * blockDim.x) {
     int lo = 0, hi = n;
     while (hi - lo > 1) {
       const int mid = (lo + hi) / 2;
@@ -105,56 +112,39 @@ __global__ void k_batch_localize(R* __restrict__ a, uint64_t E, uint64_t* __rest
   }
 }
 
-struct KeyGlobalSrc {
-  template <class R>
-  __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.src; }
-};
-
 template <class R>
 int batch_build(eg_batch* b, const R* d_in, hipStream_t s) {
   const uint64_t E = b->E, n = uint64_t(b->n);
   const uint64_t nptr = b->Vtot + n;
-  EG_TRY(b->relabel.resize(std::max<uint64_t>(E, 1) * sizeof(R)));
   EG_TRY(b->t2.resize(std::max<uint64_t>(E, 1) * sizeof(R)));
   EG_TRY(b->out.resize(std::max<uint64_t>(E, 1) * sizeof(R)));
   EG_TRY(b->rowptr.resize(nptr * 8));
   EG_TRY(b->scan_tmp.resize(scan_tmp_elems(nptr) * 8));
-  DevBuf& err = b->err;
-  EG_TRY(err.resize(4));
-  EG_HIP(hipMemsetAsync(err.ptr, 0, 4, s));
+  EG_TRY(b->err.resize(4));
+  uint32_t* err = b->err.as<uint32_t>();
+  EG_HIP(hipMemsetAsync(err, 0, 4, s));
   const uint64_t* vb = b->d_vbase.as<uint64_t>();
   const uint64_t* eb = b->d_ebase.as<uint64_t>();
-  if (E) {
-    hipLaunchKernelGGL((k_relabel<R>), dim3(grid_for(E)), dim3(kNT), 0, s, d_in, b->relabel.as<R>(), E, eb, vb, b->n,
-                       err.as<uint32_t>());
-    EG_HIP(hipGetLastError());
-  }
-  uint32_t h_err = 0;
-  EG_HIP(hipMemcpyAsync(&h_err, err.ptr, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  if (h_err) return fail(EG_ERR_RANGE, "a record's vertex id is outside its graph");
-  const int bits = std::max(1, counting_bits(b->Vtot - 1));
-  EG_TRY(b->sortw.resize_segs(1));
+  const int bits = std::max(1, counting_bits(uint64_t(b->max_v ? b->max_v - 1 : 0)));
+  EG_TRY(b->sortw.resize_segs(b->n));
   EG_HIP(hipMemsetAsync(b->sortw.err.ptr, 0, 4, s));
-  EG_TRY(rx_sort(static_cast<const R*>(b->relabel.as<R>()), b->out.as<R>(), b->t2.as<R>(), std::vector<uint64_t>{0, E},
-                 KeyGlobalSrc{}, bits, b->sortw, s));
+  EG_TRY(rx_sort(d_in, b->out.as<R>(), b->t2.as<R>(), b->ebase, KeyLocalSrc{}, bits, b->sortw, s));
   uint64_t* rp = b->rowptr.as<uint64_t>();
   EG_HIP(hipMemsetAsync(rp, 0, nptr * 8, s));
   if (E) {
-    hipLaunchKernelGGL((k_batch_runs<R>), dim3(grid_for(E)), dim3(kNT), 0, s, (const R*)b->out.as<R>(), E, vb, b->n,
-                       rp);
+    hipLaunchKernelGGL((k_batch_runs<R>), dim3(grid_for(E, kNT, 8192)), dim3(kNT), 0, s, (const R*)b->out.as<R>(), E,
+                       eb, vb, b->n, rp, err);
     EG_HIP(hipGetLastError());
   }
-  // Empty leading rows of graph g read the previous graph's end = ebase[g]:
-  // positions are global and non-decreasing across graphs.
   EG_TRY(inclusive_max_scan_u64(rp, rp, nptr, b->scan_tmp.as<uint64_t>(), s));
-  hipLaunchKernelGGL((k_batch_localize<R>), dim3(grid_for(std::max(E, nptr))), dim3(kNT), 0, s, b->out.as<R>(), E, rp,
-                     nptr, vb, eb, b->n);
+  hipLaunchKernelGGL(k_batch_localize, dim3(grid_for(nptr, kNT, 8192)), dim3(kNT), 0, s, rp, nptr, vb, eb, b->n);
   EG_HIP(hipGetLastError());
-  uint32_t sort_err = 0;
-  EG_HIP(hipMemcpyAsync(&sort_err, b->sortw.err.ptr, 4, hipMemcpyDeviceToHost, s));
+  uint32_t h_err[2] = {0, 0};
+  EG_HIP(hipMemcpyAsync(&h_err[0], err, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipMemcpyAsync(&h_err[1], b->sortw.err.ptr, 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
-  if (sort_err) return fail(EG_ERR_HIP, "radix sort look-back exceeded its spin limit");
+  if (h_err[0]) return fail(EG_ERR_RANGE, "a record's vertex id is outside its graph");
+  if (h_err[1]) return fail(EG_ERR_HIP, "radix sort look-back exceeded its spin limit");
   return EG_OK;
 }
 
@@ -180,6 +170,7 @@ int eg_batch_csr_build(const void* d_edges, int rec_bytes, int n_graphs, const u
   for (int g = 0; g < n_graphs; ++g) {
     b->vbase[g + 1] = b->vbase[g] + h_vertices[g];
     b->ebase[g + 1] = b->ebase[g] + h_edges[g];
+    b->max_v = std::max(b->max_v, h_vertices[g]);
   }
   b->Vtot = b->vbase[n_graphs];
   b->E = b->ebase[n_graphs];
