@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 import subprocess
 
 import numpy as np
@@ -46,14 +47,22 @@ class _Info(C.Structure):
                 ("symmetrized", C.c_int)]
 
 
+# True when the library bound to the HIP runtime torch loaded (torch was
+# imported first).  A process holds one HIP runtime: loaded first, this
+# library binds to ROCm's runtime and torch, imported later, then loads its
+# own copy and finds no GPU; imported first, torch's runtime serves both.
+lib_shares_torch_runtime = False
+
+
 def lib():
     """The loaded native library (raises if it is missing)."""
-    global _lib
+    global _lib, lib_shares_torch_runtime
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"native library missing at {LIB_PATH}: run `make -C endgraph_amd` "
                            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib_shares_torch_runtime = "torch" in sys.modules
     L = C.CDLL(LIB_PATH)
     vp, u32, u64, i32, f32, f64 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float, C.c_double
     pp = C.POINTER(C.c_void_p)
@@ -111,6 +120,9 @@ def lib():
         "eg_cc": ([vp, i32, vp, C.POINTER(u32), vp], i32),
         "eg_sssp_iteration": ([vp, i32, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64), vp], i32),
         "eg_sssp": ([vp, u32, i32, vp, C.POINTER(u32), vp], i32),
+        "eg_ppr_slots": ([vp, vp, vp, vp, vp], i32),
+        "eg_ppr_forward": ([vp, vp, vp, f64, i32, vp, vp], i32),
+        "eg_ppr_adjoint": ([vp, vp, vp, vp, f64, i32, vp, vp, vp], i32),
         "eg_bfs_multi": ([vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(u32), vp], i32),
         "eg_sssp_multi": ([vp, vp, i32, i32, vp, C.POINTER(u32), vp], i32),
     }
@@ -403,6 +415,24 @@ class Graph:
         it = C.c_uint32()
         check(lib().eg_cc(self._h, policy, C.c_void_p(label.ptr), C.byref(it), _stream(stream)))
         return (label.download(np.uint32, self.V) if download else None), it.value
+
+    # ------------------------------------------------ personalised PageRank --
+    def ppr_slots(self, stream=None):
+        """(src, dst, record weight) of every CSC slot, the order of PPR edge weights (device buffers)."""
+        E = self.edges_num
+        src, dst, w = DeviceBuffer(max(E, 1) * 4), DeviceBuffer(max(E, 1) * 4), DeviceBuffer(max(E, 1) * 4)
+        check(lib().eg_ppr_slots(self._h, C.c_void_p(src.ptr), C.c_void_p(dst.ptr), C.c_void_p(w.ptr),
+                                 _stream(stream)))
+        return src, dst, w
+
+    def ppr_forward(self, d_w: int, d_p: int, damping: float, iterations: int, d_x: int, stream=None):
+        check(lib().eg_ppr_forward(self._h, C.c_void_p(d_w), C.c_void_p(d_p), damping, iterations, C.c_void_p(d_x),
+                                   _stream(stream)))
+
+    def ppr_adjoint(self, d_w: int, d_x: int, d_c: int, damping: float, iterations: int, d_grad_p: int,
+                    d_grad_w: int, stream=None):
+        check(lib().eg_ppr_adjoint(self._h, C.c_void_p(d_w), C.c_void_p(d_x), C.c_void_p(d_c), damping, iterations,
+                                   C.c_void_p(d_grad_p or None), C.c_void_p(d_grad_w or None), _stream(stream)))
 
     def bfs_multi(self, roots, policy: int = POLICY_REFERENCE, parent: DeviceBuffer = None,
                   depth: DeviceBuffer = None, download: bool = True, stream=None):

@@ -674,6 +674,7 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   g->eng_ready = false;
   g->pv_ready = false;
   g->tv_ready = false;
+  g->ad_ready = false;
   g->E0 = E0;
   g->E = E;
   g->symmetrized = sym;

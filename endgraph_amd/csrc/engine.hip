@@ -20,6 +20,7 @@
 #include <cmath>
 
 #include "graph_state.hpp"
+#include "rowreduce.hpp"
 
 namespace eg {
 namespace {
@@ -371,19 +372,8 @@ __global__ void k_gather_by(const uint32_t* __restrict__ idx, uint32_t V, const 
     to[v] = from[idx[v]];
 }
 
-// ----------------------------------------------------- dense row reduce --
-// Walks CSC edges 64 per step (a wave takes words [w0, w0+K)), gathers one
-// value per edge with `val(e)`, reduces rows with a segmented wave scan and
-// hands each finished row's partial to `emit(row, value, shared)`; shared is
-// true for the rows cut by the wave's range ends (they need atomics).
-constexpr int kRowWordsPerWave = 8;  // 4 and 16 measured slower on the pull
-
-// first word of the calling wave's range in a one-range-per-wave grid
-__device__ __forceinline__ uint64_t first_word() {
-  return (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave * kRowWordsPerWave;
-}
-
-// Compensated (Neumaier) pair for the FP32 pull sums.
+// Compensated (Neumaier) pair for the FP32 pull sums (row reduction: see
+// rowreduce.hpp).
 struct F2 {
   float s, c;
 };
@@ -409,86 +399,6 @@ __device__ __forceinline__ F2 shfl_t<F2>(F2 v, int lane) {
 }
 
 namespace {
-
-template <class T, class Op>
-__device__ __forceinline__ T seg_scan(T v, bool head, Op op) {
-  const int lane = threadIdx.x & (kWave - 1);
-  bool h = head;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const T ov = shfl_up_t(v, d);
-    const bool oh = __shfl_up(int(h), d, kWave) != 0;
-    if (lane >= d && !h) {
-      v = op(ov, v);
-      h = oh;
-    }
-  }
-  return v;
-}
-
-template <class T, class Op, class Val, class Emit>
-__device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const uint32_t* __restrict__ src,
-                                                const uint64_t* __restrict__ rowflag,
-                                                const uint32_t* __restrict__ wordrank, T identity, Op op, Val val,
-                                                Emit emit) {
-  constexpr int K = kRowWordsPerWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t nwords = ceil_div(E, 64);
-  if (w0 >= nwords) return;
-  const int nwd = w0 + K < nwords ? K : int(nwords - w0);
-  // all row bits, sources and gathered values of the wave's range first, so
-  // every load of the range is in flight before the first use
-  uint64_t f[K];
-  uint32_t base[K], sv[K];
-  T v[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const uint64_t wd = w0 + k;
-    const uint64_t e = wd * 64 + lane;
-    const bool ok = k < nwd;
-    f[k] = ok ? rowflag[wd] : 0ull;
-    base[k] = ok ? wordrank[wd] : 0u;
-    sv[k] = ok && e < E ? src[e] : 0u;
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const uint64_t e = (w0 + k) * 64 + lane;
-    v[k] = k < nwd && e < E ? val(e, sv[k]) : identity;
-  }
-  const uint64_t lt = (1ull << lane) - 1;
-  T carry = identity;
-  bool carry_live = false;     // a row is open from earlier edges
-  bool carry_shared = true;    // ... and it started before this wave's range
-  uint32_t carry_row = 0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (k >= nwd) break;  // wave-uniform
-    const uint64_t e = (w0 + k) * 64 + lane;
-    T x = v[k];
-    const bool head = (f[k] >> lane) & 1ull;
-    if (lane == 0 && !head) x = op(carry, x);
-    if ((f[k] & 1ull) && carry_live && lane == 0) emit(carry_row, carry, carry_shared);
-    const T inc = seg_scan(x, head, op);
-    // tails: lanes whose successor starts a row (lane 63 stays open)
-    const bool tail = lane < 63 ? ((f[k] >> (lane + 1)) & 1ull) : false;
-    const uint64_t heads_le = __popcll(f[k] & (lt | (1ull << lane)));
-    const uint32_t row = base[k] + uint32_t(heads_le) - 1;  // rows started before + own
-    const bool first_seg_open = heads_le == 0;              // segment continues the carried row
-    if (tail && e < E) emit(row, inc, first_seg_open ? carry_shared : false);
-    // new carry = lane 63's segment
-    const T c63 = shfl_t(inc, 63);
-    const uint64_t h63 = __popcll(f[k]);
-    carry = c63;
-    carry_live = true;
-    if (h63 == 0) {
-      carry_row = base[k] - 1;  // no head in this word: the open row continues
-    } else {
-      carry_shared = false;
-      carry_row = base[k] + uint32_t(h63) - 1;
-    }
-  }
-  if (lane == 0 && carry_live) emit(carry_row, carry, true);
-}
 
 // --------------------------------------------------------------- PageRank
 // dense_signal/dense_slot of toolkits/pagerank.cpp:64-75: next[dst] += sum
@@ -896,7 +806,6 @@ int level_end(eg_graph* g, const uint64_t* out, uint64_t* visited, uint32_t* dep
   return EG_OK;
 }
 
-unsigned row_grid(uint64_t E) { return grid_for(ceil_div(ceil_div(E, 64), kRowWordsPerWave), kNT / kWave, 1u << 30); }
 
 // core/graph.hpp:1483-1498: rooted programs are sparse under the reference
 // policy; undirected ones (and every program under auto) switch on the

@@ -73,6 +73,15 @@ struct eg_graph {
   eg::DevBuf tv_w;                   // f32[E] (12-byte records only)
   eg::DevBuf tv_items;               // u64 (v << 32 | chunk): in-edge chunks of at most 512 edges
   uint64_t tv_nitems = 0;
+  // transposed view for the PPR adjoint: CSC slots grouped by source
+  bool ad_ready = false;
+  uint32_t ad_R = 0;
+  eg::DevBuf ad_slot_dst;            // u32[E] destination of every CSC slot
+  eg::DevBuf ad_slot;                // u32[E] slot of transposed position k
+  eg::DevBuf ad_dst;                 // u32[E] destination of that slot (gather index)
+  eg::DevBuf ad_rowflag, ad_wordrank;
+  eg::DevBuf ad_row_src;             // u32[R] source of each transposed row
+  eg::DevBuf e_ppr, e_ppr_adj;       // PPR work vectors
   eg::DevBuf e_multi;                // batched traversal masks, work items, counters
   eg::DevBuf e_multi_roots;          // roots and per-root statistics
   double eng_prepare_ms = 0;

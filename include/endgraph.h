@@ -252,6 +252,24 @@ int eg_sssp_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_
                       uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream);
 int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_iterations, void* stream);
 
+/* Personalized PageRank with per-edge weights and its adjoint (no reference
+ * counterpart).  Edge slots are the engine's CSC order (the sub-partitions'
+ * incoming lists back to back); eg_ppr_slots writes the source, destination
+ * and record weight (1 for 8-byte records) of every slot (each pointer may be
+ * NULL).  d_w holds one FP64 weight per slot.
+ * eg_ppr_forward: x_{k+1} = d * A x_k + (1-d) * p from x_0 = p, `iterations`
+ *   times, A[v][u] = sum of the weights of the slots u -> v; FP64.
+ * eg_ppr_adjoint: for a loss L with dL/dx = d_c at the forward result d_x:
+ *   lambda = c + d * A^T lambda (same iteration count, from lambda = c),
+ *   d_grad_p = (1-d) * lambda, d_grad_w[e] = d * lambda[dst_e] * x[src_e]
+ *   (either output may be NULL).  The first call after a build derives a
+ *   source-grouped copy of the slots.  All three synchronise. */
+int eg_ppr_slots(eg_graph* g, uint32_t* d_src, uint32_t* d_dst, float* d_w, void* stream);
+int eg_ppr_forward(eg_graph* g, const double* d_w, const double* d_p, double damping, int iterations, double* d_x,
+                   void* stream);
+int eg_ppr_adjoint(eg_graph* g, const double* d_w, const double* d_x, const double* d_c, double damping, int iterations,
+                   double* d_grad_p, double* d_grad_w, void* stream);
+
 /* Batched BFS and SSSP from n_roots (1..64) roots at once (no reference
  * counterpart: bfs.cpp / sssp.cpp run one root per process); each level or
  * relaxation round serves all roots, directions are decided on the device.
