@@ -54,6 +54,35 @@ def median_time(fn, runs=3, warm=1):
     return statistics.median(ts)
 
 
+def ppr_extras(eg, iterations=250, d=0.85):
+    """Personalised PageRank forward and adjoint solves on R-MAT scale 22 with
+    fp32 weights U[0.1,1] row-normalised (FP64 weights per edge slot), p
+    Dirichlet(1) on 1024 seeds, dL/dx ~ N(0,1); `iterations` fixed-point steps
+    each (the gradient check against central differences is a test)."""
+    scale, V = 22, 1 << 22
+    E = 16 * V
+    buf = eg.DeviceBuffer(E * 12)
+    eg.gen_rmat(buf.ptr, E, scale, seed=22, rec_bytes=12, w_lo=0.1, w_hi=1.0)
+    g = eg.Graph(V, 1, 1, 1, 1, 12)
+    g.build(buf.ptr, E)
+    s_src, _, s_w = g.ppr_slots()
+    src = s_src.download(np.uint32, E).astype(np.int64)
+    w = s_w.download(np.float32, E).astype(np.float64)
+    w /= np.bincount(src, weights=w, minlength=V)[src]
+    rng = np.random.default_rng(22)
+    p = np.zeros(V)
+    p[rng.choice(V, 1024, replace=False)] = rng.dirichlet(np.ones(1024))
+    c = rng.standard_normal(V)
+    dw, dp, dc = eg.DeviceBuffer.from_array(w), eg.DeviceBuffer.from_array(p), eg.DeviceBuffer.from_array(c)
+    dx, gp, gw = eg.DeviceBuffer(V * 8), eg.DeviceBuffer(V * 8), eg.DeviceBuffer(E * 8)
+    tf = median_time(lambda: g.ppr_forward(dw.ptr, dp.ptr, d, iterations, dx.ptr))
+    ta = median_time(lambda: g.ppr_adjoint(dw.ptr, dx.ptr, dc.ptr, d, iterations, gp.ptr, gw.ptr))
+    log(f"[bench] ppr (R-MAT 22, {iterations} steps): forward {tf * 1e3:.1f} ms, adjoint {ta * 1e3:.1f} ms")
+    return {"workload": "R-MAT scale 22, weights U[0.1,1] row-normalised, p Dirichlet(1) on 1024 seeds",
+            "iterations": iterations, "ms_forward": round(tf * 1e3, 2), "ms_adjoint": round(ta * 1e3, 2),
+            "gteps_forward": round(iterations * E / tf / 1e9, 2), "gteps_adjoint": round(iterations * E / ta / 1e9, 2)}
+
+
 def batching_extras(eg):
     """Batched programs: 64 seeded roots of BFS and SSSP on R-MAT scale 22 with
     fp32 weights U[0,1) (one batched run against 64 single-root runs), and the
@@ -102,6 +131,7 @@ def batching_extras(eg):
                             "speedup": round(ts / tb, 2)}
     log(f"[bench] sssp 64 roots: batched {tb * 1e3:.2f} ms vs single {ts * 1e3:.2f} ms")
     del dist, g, buf
+    out["ppr_adjoint"] = ppr_extras(eg)
     rng = np.random.default_rng(256)
     n = 256
     nv = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n)).astype(np.uint32)
@@ -136,6 +166,9 @@ def main():
     ap.add_argument("--no-engine", action="store_true", help="skip the BFS / PageRank figures")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batching", action="store_true", help="skip the batched BFS/SSSP/CSR figures")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: two builds of the workload, then a 1 GiB device copy (the "
+                         "calibration of FETCH_SIZE / WRITE_SIZE); prints no bench line")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default sort variant")
     ap.add_argument("--cpu-sample-edges", type=int, default=1 << 25)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
@@ -160,6 +193,14 @@ def main():
     log("[bench] edges generated")
 
     g = eg.Graph(V, P, S, eta, m, 8)
+    if args.traffic_probe:
+        for _ in range(2):
+            g.build(edges.ptr, E)
+        a, b = eg.DeviceBuffer(1 << 30), eg.DeviceBuffer(1 << 30)
+        eg.memcpy_d2d(b.ptr, a.ptr, 1 << 30)
+        eg.synchronize()
+        log("[bench] traffic probe done: 2 builds, one 1 GiB device copy")
+        return
     variant = None
     if not args.no_autotune:
         sc, nt, bits, ms = g.autotune_sort(edges.ptr, E)  # untimed: every variant builds twice

@@ -222,6 +222,10 @@ def profile_read(kernel: str):
     return ms.value, n.value, b.value
 
 
+def memcpy_d2d(d_dst: int, d_src: int, nbytes: int, stream=None):
+    check(lib().eg_memcpy_d2d(C.c_void_p(d_dst), C.c_void_p(d_src), nbytes, _stream(stream)))
+
+
 def synchronize(stream=None):
     check(lib().eg_stream_synchronize(_stream(stream)))
 

@@ -1,9 +1,12 @@
-"""Full-size PageRank precision check on the twitter-shaped graph
-(41,652,230 vertices, 1,468,365,182 edges, seed 3): compensated FP32 against
-the FP64 oracle over the whole graph, 20 iterations, d = 0.85.
-
-It needs ~80 GB of host memory and a few minutes of host CPU for the oracle,
-so it runs only when EG_FULL_SCALE=1 (run it with -s: it prints each step).
+"""Full-size checks, run only when EG_FULL_SCALE=1 (run with -s: each step
+prints):
+  * PageRank precision on the twitter-shaped graph (41,652,230 vertices,
+    1,468,365,182 edges, seed 3): compensated FP32 against the FP64 oracle over
+    the whole graph, 20 iterations, d = 0.85 (~80 GB of host memory, a few
+    minutes of host CPU for the oracle);
+  * personalised PageRank gradients on R-MAT scale 22 (weights U[0.1,1]
+    row-normalised, p Dirichlet(1) on 1024 seeds, c ~ N(0,1)) against FP64
+    central differences on 64 sampled edges and 64 sampled vertices.
 """
 import os
 import time
@@ -46,3 +49,54 @@ def test_twitter_pagerank_l1_vs_fp64_oracle(oracle_lib):
     say(f"relative L1 vs FP64 oracle: fp32c {l1_32:.3e}  fp64 {l1_64:.3e}")
     assert l1_64 <= PR_FP64_TOL
     assert l1_32 <= PR_FP32C_TOL
+
+
+def test_rmat22_ppr_gradients_vs_central_differences():
+    import torch
+    import endgraph_amd as eg
+    from endgraph_amd import ppr
+    t0 = time.time()
+    say = lambda m: print(f"[fullscale {time.time() - t0:6.1f}s] {m}", flush=True)
+    rng = np.random.default_rng(22)
+    scale, D, K = 22, 0.85, 250
+    V, E = 1 << scale, 16 << scale
+    buf = eg.DeviceBuffer(E * 12)
+    eg.gen_rmat(buf.ptr, E, scale, seed=22, rec_bytes=12, w_lo=0.1, w_hi=1.0)
+    g = eg.Graph(V, 1, 1, 1, 1, 12)
+    g.build(buf.ptr, E)
+    src, dst, w0 = ppr.slots(g)
+    w = ppr.row_normalized(src, w0, V).requires_grad_(True)
+    p_np = np.zeros(V)
+    p_np[rng.choice(V, 1024, replace=False)] = rng.dirichlet(np.ones(1024))
+    p = torch.tensor(p_np, device="cuda", requires_grad=True)
+    c = torch.tensor(rng.standard_normal(V), device="cuda")
+    (c * ppr.personalized_pagerank(g, p, w, D, K)).sum().backward()
+    say("forward + adjoint done")
+
+    def L(pp, ww):
+        with torch.no_grad():
+            return float((c * ppr.personalized_pagerank(g, pp, ww, D, K)).sum())
+
+    ei = rng.choice(w.numel(), 64, replace=False)
+    vi = rng.choice(V, 64, replace=False)
+    fd_w, fd_p = [], []
+    with torch.no_grad():
+        w_, p_ = w.detach().clone(), p.detach().clone()
+        for e in ei:
+            h = 1e-4 * float(w_[e])
+            a, b = w_.clone(), w_.clone()
+            a[e] += h
+            b[e] -= h
+            fd_w.append((L(p_, a) - L(p_, b)) / (2 * h))
+        say("edge differences done")
+        for v in vi:
+            a, b = p_.clone(), p_.clone()
+            a[v] += 1e-6
+            b[v] -= 1e-6
+            fd_p.append((L(a, w_) - L(b, w_)) / 2e-6)
+    gw = w.grad.cpu().numpy()[ei]
+    gp = p.grad.cpu().numpy()[vi]
+    err_w = np.abs(np.array(fd_w) - gw).max() / np.abs(gw).max()
+    err_p = np.abs(np.array(fd_p) - gp).max() / np.abs(gp).max()
+    say(f"max rel error vs central differences: edges {err_w:.2e}, vertices {err_p:.2e}")
+    assert err_w < 1e-6 and err_p < 1e-6
