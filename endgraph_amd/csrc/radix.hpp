@@ -101,21 +101,26 @@ __device__ __forceinline__ RxTile rx_tile(const uint64_t* seg, const uint32_t* t
   return x;
 }
 
+// s_waitcnt immediate of gfx9-family ISAs: lgkmcnt(0), vmcnt and expcnt at
+// their maxima (no wait on vector memory)
+constexpr unsigned kLgkmZero = 0xC07F;
+
 // Stable in-tile ranking shared by both schemes.  On return: item/dig/rank per
 // record of this thread, wcnt[w][d] = records of digit d in waves < w, *tot
 // (threads d < nb) = tile count of digit d.  The 16 loads are issued before
 // any of them is used (invalid lanes re-read the tile's first record), so a
-// wave has all its loads in flight at once.
+// wave has all its loads in flight at once.  peer_tab: (NT/64) << BM u64 of
+// LDS, free for the duration of the call.
+
 template <int NT, int BM, class R, class KeyFn>
 __device__ __forceinline__ void rx_rank(const R* __restrict__ in, const RxTile& x, KeyFn key, int shift, int bits,
                                         R (&item)[kRxIPT], uint32_t (&dig)[kRxIPT], uint32_t (&rank)[kRxIPT],
-                                        uint32_t (*wcnt)[1 << BM], uint32_t* tot) {
+                                        uint32_t (*wcnt)[1 << BM], uint32_t* tot, uint64_t* peer_tab) {
   constexpr int W = NT / kWave;
   const int nb = 1 << bits;
   const uint32_t mask = uint32_t(nb - 1);
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
-  const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
   const uint32_t first = uint32_t(w) * (kRxIPT * kWave) + lane;
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
@@ -124,20 +129,34 @@ __device__ __forceinline__ void rx_rank(const R* __restrict__ in, const RxTile& 
   }
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) dig[k] = uint32_t(key(item[k]) >> shift) & mask;
+  // Peers (lanes of this wave with the same digit) through a per-wave LDS
+  // table of lane masks: every lane ORs its bit into its digit's entry, reads
+  // the entry back and clears it (one LDS round trip per access instead of a
+  // ballot per digit bit).
+  uint64_t* tab = peer_tab + w * (1 << BM);
+  for (int i = lane; i < nb; i += kWave) __hip_atomic_store(tab + i, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  __builtin_amdgcn_s_waitcnt(kLgkmZero);
+  __asm__ volatile("" ::: "memory");
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
     const bool valid = first + uint32_t(k) * kWave < x.count;
     const uint32_t d = dig[k];
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int bb = 0; bb < BM; ++bb) {
-      if (bb < bits) {  // wave-uniform
-        const bool set = (d >> bb) & 1u;
-        const uint64_t bal = __ballot(set);
-        peers &= set ? bal : ~bal;
-      }
-    }
-    const uint32_t below = uint32_t(__popcll(peers & lt_mask));
+    // The compiler sees one lane: to it the three accesses of record k and
+    // those of record k+1 touch unrelated addresses, while across lanes they
+    // alias.  Each access completes (s_waitcnt lgkmcnt(0)) before the next is
+    // issued, so the read sees every lane's OR and the clear follows every
+    // lane's read; the empty asm statements keep the compiler from moving
+    // memory accesses across those points.
+    uint64_t peers = 0;
+    if (valid) __hip_atomic_fetch_or(tab + d, uint64_t(1) << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __builtin_amdgcn_s_waitcnt(kLgkmZero);
+    __asm__ volatile("" ::: "memory");
+    if (valid) peers = __hip_atomic_load(tab + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __builtin_amdgcn_s_waitcnt(kLgkmZero);
+    __asm__ volatile("" ::: "memory");
+    if (valid) __hip_atomic_store(tab + d, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __asm__ volatile("" ::: "memory");
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(peers >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(peers), 0u));
     const uint32_t total = uint32_t(__popcll(peers));
     const uint32_t prior = wcnt[w][d];
     rank[k] = prior + below;
@@ -239,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_rx_down(const R* __restrict__ in, R* __r
   __shared__ uint32_t goff[1 << BM];
   __shared__ uint32_t dstart[1 << BM];
   __shared__ uint32_t wt[W];
-  __shared__ R stage[TILE];
+  __shared__ __align__(16) R stage[TILE];
   const int nb = 1 << bits;
   for (int i = threadIdx.x; i < W * nb; i += NT) wcnt[i / nb][i % nb] = 0;
   const RxTile x = rx_tile<TILE>(seg, ts, nseg, blockIdx.x);
@@ -248,7 +267,9 @@ __global__ __launch_bounds__(NT) void k_rx_down(const R* __restrict__ in, R* __r
   R item[kRxIPT];
   uint32_t dig[kRxIPT], rank[kRxIPT];
   uint32_t tot;
-  rx_rank<NT, BM>(in, x, key, shift, bits, item, dig, rank, wcnt, &tot);
+  // the peer table of the ranking lives in the staging buffer, which is
+  // first written after the ranking's last barrier
+  rx_rank<NT, BM>(in, x, key, shift, bits, item, dig, rank, wcnt, &tot, reinterpret_cast<uint64_t*>(stage));
   rx_dstart<NT>(nb, tot, wt, dstart);
   rx_scatter<NT, BM>(out, x, key, shift, bits, item, dig, rank, wcnt, goff, dstart, stage);
 }
@@ -299,7 +320,7 @@ __global__ __launch_bounds__(NT) void k_rx_onepass(const R* __restrict__ in, R* 
   __shared__ uint32_t dstart[1 << BM];
   __shared__ uint32_t wt[W];
   __shared__ uint32_t s_tile;
-  __shared__ R stage[TILE];
+  __shared__ __align__(16) R stage[TILE];
   const int nb = 1 << bits;
   if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
   for (int i = threadIdx.x; i < W * nb; i += NT) wcnt[i / nb][i % nb] = 0;
@@ -310,7 +331,7 @@ __global__ __launch_bounds__(NT) void k_rx_onepass(const R* __restrict__ in, R* 
   R item[kRxIPT];
   uint32_t dig[kRxIPT], rank[kRxIPT];
   uint32_t tot;
-  rx_rank<NT, BM>(in, x, key, shift, bits, item, dig, rank, wcnt, &tot);
+  rx_rank<NT, BM>(in, x, key, shift, bits, item, dig, rank, wcnt, &tot, reinterpret_cast<uint64_t*>(stage));
   // A status word is its own payload (epoch | flag | count): relaxed
   // device-scope atomics suffice; release/acquire would write back or
   // invalidate the L2 on every publish and poll.
@@ -397,6 +418,7 @@ int rx_sort_nt(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, Ke
                RxWork& w, hipStream_t s) {
   static_assert((1 << BM) <= NT, "one digit per thread in the count/scan steps");
   static_assert(NT * kRxIPT * sizeof(R) + (NT / kWave) * (4 << BM) + (8 << BM) <= 160 * 1024, "LDS of one block");
+  static_assert((NT / kWave) * (8 << BM) <= NT * kRxIPT * sizeof(R), "peer tables fit the staging buffer");
   constexpr int TILE = NT * kRxIPT;
   const int nseg = int(seg.size()) - 1;
   std::vector<uint32_t> ts(nseg + 1, 0);
