@@ -88,6 +88,10 @@ def lib():
         "eg_graph_csc": ([vp, i32, i32, pp, C.POINTER(u64), pp, C.POINTER(u64)], i32),
         "eg_graph_csr": ([vp, i32, i32, pp, C.POINTER(u64), pp, pp], i32),
         "eg_gen_er_batch": ([vp, i32, vp, vp, u64, i32, vp], i32),
+        "eg_edge_file_records": ([C.c_char_p, i32, C.POINTER(u64)], i32),
+        "eg_load_edge_file": ([C.c_char_p, i32, vp, u64, C.POINTER(u64), vp], i32),
+        "eg_dump_vertex_array": ([vp, u64, u32, C.c_char_p, vp], i32),
+        "eg_restore_vertex_array": ([vp, u64, u32, C.c_char_p, vp], i32),
         "eg_batch_csr_build": ([vp, i32, i32, vp, vp, pp, vp], i32),
         "eg_batch_csr_rebuild": ([vp, vp, vp], i32),
         "eg_batch_csr_get": ([vp, i32, pp, C.POINTER(u64), pp, C.POINTER(u32)], i32),
@@ -220,6 +224,24 @@ def profile_read(kernel: str):
     ms, n, b = C.c_double(), C.c_uint64(), C.c_uint64()
     check(lib().eg_profile_read(kernel.encode(), C.byref(ms), C.byref(n), C.byref(b)))
     return ms.value, n.value, b.value
+
+
+def load_edge_file(path: str, rec_bytes: int = 8, stream=None):
+    """Read a binary edge-list file into a new DeviceBuffer; returns (buffer, records)."""
+    n = C.c_uint64()
+    check(lib().eg_edge_file_records(str(path).encode(), rec_bytes, C.byref(n)))
+    buf = DeviceBuffer(max(n.value * rec_bytes, 1))
+    check(lib().eg_load_edge_file(str(path).encode(), rec_bytes, C.c_void_p(buf.ptr), n.value, C.byref(n),
+                                  _stream(stream)))
+    return buf, n.value
+
+
+def dump_vertex_array(d_array: int, elem_bytes: int, V: int, path: str, stream=None):
+    check(lib().eg_dump_vertex_array(C.c_void_p(d_array), elem_bytes, V, str(path).encode(), _stream(stream)))
+
+
+def restore_vertex_array(d_array: int, elem_bytes: int, V: int, path: str, stream=None):
+    check(lib().eg_restore_vertex_array(C.c_void_p(d_array), elem_bytes, V, str(path).encode(), _stream(stream)))
 
 
 def memcpy_d2d(d_dst: int, d_src: int, nbytes: int, stream=None):

@@ -219,6 +219,16 @@ class Graph {
     check(eg_memcpy_d2h(h.data(), array, uint64_t(vertices_num) * sizeof(T), nullptr));
     return h;
   }
+  // dump_vertex_array / restore_vertex_array (core/graph.hpp:324-375): the
+  // reference's raw V * sizeof(T) file, all partitions at once
+  template <typename T>
+  void dump_vertex_array(const T* array, const std::string& path) {
+    check(eg_dump_vertex_array(array, sizeof(T), vertices_num, path.c_str(), nullptr));
+  }
+  template <typename T>
+  void restore_vertex_array(T* array, const std::string& path) {
+    check(eg_restore_vertex_array(array, sizeof(T), vertices_num, path.c_str(), nullptr));
+  }
 
   // ---- process_vertices (core/graph.hpp:1377-1441) ----
   template <typename R>
@@ -290,21 +300,13 @@ class Graph {
   double* delta_ = nullptr;
 
   void load(const std::string& path, UINT vertices, bool symmetric) {
-    struct stat st;
-    if (stat(path.c_str(), &st) != 0) throw Error(EG_ERR_ARG, "cannot stat " + path);
-    const size_t rec = sizeof(EdgeType<EdgeData>);
-    if (size_t(st.st_size) % rec != 0) throw Error(EG_ERR_ARG, path + ": size is not a multiple of the record size");
-    const ULLONG n = ULLONG(st.st_size) / rec;
-    std::vector<char> host(size_t(st.st_size));
-    FILE* f = std::fopen(path.c_str(), "rb");
-    if (!f) throw Error(EG_ERR_ARG, "cannot open " + path);
-    const size_t got = host.empty() ? 0 : std::fread(host.data(), 1, host.size(), f);
-    std::fclose(f);
-    if (got != host.size()) throw Error(EG_ERR_ARG, "short read of " + path);
+    const int rec = int(sizeof(EdgeType<EdgeData>));
+    uint64_t n = 0;
+    check(eg_edge_file_records(path.c_str(), rec, &n));
     void* d = nullptr;
-    check(eg_device_alloc(&d, host.size() ? host.size() : 1));
+    check(eg_device_alloc(&d, n ? n * uint64_t(rec) : 1));
     try {
-      check(eg_memcpy_h2d(d, host.data(), host.size(), nullptr));
+      check(eg_load_edge_file(path.c_str(), rec, d, n, &n, nullptr));
       build(d, n, vertices, symmetric);
     } catch (...) {
       eg_device_free(d);

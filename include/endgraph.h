@@ -150,6 +150,22 @@ int eg_graph_csc(const eg_graph* g, int p, int s, const void** d_list, uint64_t*
 int eg_graph_csr(const eg_graph* g, int p, int s, const void** d_list, uint64_t* n_edges, const uint64_t** d_index,
                  const uint64_t** d_bitmap);
 
+/* ---- files ------------------------------------------------------------
+ * Binary edge lists (README.md:40-41: packed EdgeType records in file order)
+ * and vertex arrays (V * elem_bytes raw bytes), as the reference reads and
+ * writes them.  eg_edge_file_records: records in the file (its size must be
+ * a whole number of records).  eg_load_edge_file replaces
+ * preload_graph_file's read (core/graph.hpp:1011-1055): 64 MiB chunks are
+ * read into pinned host buffers and copied to d_edges (capacity_records
+ * records) while the next chunk is read.  eg_dump_vertex_array /
+ * eg_restore_vertex_array replace dump_vertex_array / restore_vertex_array
+ * (core/graph.hpp:324-375) for the whole array.  All synchronise. */
+int eg_edge_file_records(const char* path, int rec_bytes, uint64_t* h_records);
+int eg_load_edge_file(const char* path, int rec_bytes, void* d_edges, uint64_t capacity_records, uint64_t* h_records,
+                      void* stream);
+int eg_dump_vertex_array(const void* d_array, uint64_t elem_bytes, uint32_t V, const char* path, void* stream);
+int eg_restore_vertex_array(void* d_array, uint64_t elem_bytes, uint32_t V, const char* path, void* stream);
+
 /* ---- batched CSR construction (no reference counterpart: one process of
  *      the reference builds one graph) -------------------------------------
  * n_graphs independent graphs stored back to back in d_edges (graph g: local

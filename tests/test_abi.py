@@ -56,3 +56,15 @@ def test_error_path_without_gpu():
     assert rc == -1
     rc = eg.lib().eg_graph_build(None, None, 0, 0, None)
     assert rc == -1
+
+
+def test_edge_file_records_host_only(tmp_path):
+    """eg_edge_file_records only stats the file (no device needed)."""
+    import ctypes as C
+    import endgraph_amd as eg
+    p = tmp_path / "e.bin"
+    p.write_bytes(bytes(8 * 5))
+    n = C.c_uint64()
+    assert eg.lib().eg_edge_file_records(str(p).encode(), 8, C.byref(n)) == 0 and n.value == 5
+    assert eg.lib().eg_edge_file_records(str(p).encode(), 12, C.byref(n)) != 0  # 40 B is not whole 12 B records
+    assert eg.lib().eg_edge_file_records(str(tmp_path / "none").encode(), 8, C.byref(n)) != 0
