@@ -181,14 +181,24 @@ template <int NT, int BM, class R, class KeyFn>
 __device__ __forceinline__ void rx_scatter(R* __restrict__ out, const RxTile& x, KeyFn key, int shift, int bits,
                                            const R (&item)[kRxIPT], const uint32_t (&dig)[kRxIPT],
                                            const uint32_t (&rank)[kRxIPT], uint32_t (*wcnt)[1 << BM],
-                                           const uint32_t* goff, const uint32_t* dstart, R* stage) {
+                                           uint32_t* goff, const uint32_t* dstart, R* stage) {
   const uint32_t mask = uint32_t((1 << bits) - 1);
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
   const uint32_t first = uint32_t(w) * (kRxIPT * kWave) + lane;
+  // fold the tile's digit starts into the per-wave bases and the global
+  // offsets, so each record needs one table read per step
+  const int nb = 1 << bits;
+  for (int d = threadIdx.x; d < nb; d += NT) {
+    const uint32_t ds = dstart[d];
+#pragma unroll
+    for (int ww = 0; ww < NT / kWave; ++ww) wcnt[ww][d] += ds;
+    goff[d] -= ds;  // no wrap: the global prefix of d counts this tile's smaller digits
+  }
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k)
-    if (first + uint32_t(k) * kWave < x.count) stage[dstart[dig[k]] + wcnt[w][dig[k]] + rank[k]] = item[k];
+    if (first + uint32_t(k) * kWave < x.count) stage[wcnt[w][dig[k]] + rank[k]] = item[k];
   __syncthreads();
   R r[kRxIPT];
   uint32_t d[kRxIPT];
@@ -203,7 +213,7 @@ __device__ __forceinline__ void rx_scatter(R* __restrict__ out, const RxTile& x,
   for (int k = 0; k < kRxIPT; ++k) {
     const uint32_t lp = uint32_t(k) * NT + threadIdx.x;
     if (lp < x.count) {
-      const uint64_t pos = uint64_t(goff[d[k]]) + (lp - dstart[d[k]]);
+      const uint64_t pos = uint64_t(goff[d[k]]) + lp;
       if (pos < x.end) out[pos] = r[k];  // always true unless a look-back gave up
     }
   }
