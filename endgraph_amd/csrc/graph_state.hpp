@@ -6,6 +6,34 @@
 #include "common.hpp"
 #include "radix.hpp"
 
+// A captured HIP graph of the batched programs' steps, its stream and the
+// events that order it after and before the caller's stream.
+struct MultiGraph {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<char> key;  // kernel arguments, grids and kernels it was captured with
+  int init() {
+    if (stream) return EG_OK;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_out, hipEventDisableTiming) != hipSuccess)
+      return eg::fail(EG_ERR_HIP, "stream or event creation for the step graph failed");
+    return EG_OK;
+  }
+  void reset_exec() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+  }
+  ~MultiGraph() {
+    reset_exec();
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
 struct eg_graph {
   // configuration (Graph(int eta), core/graph.hpp:202; P ranks and S sockets)
   uint32_t V = 0;
@@ -83,6 +111,7 @@ struct eg_graph {
   eg::DevBuf ad_row_src;             // u32[R] source of each transposed row
   eg::DevBuf e_ppr, e_ppr_adj;       // PPR work vectors
   eg::DevBuf e_multi;                // batched traversal masks, work items, counters
+  MultiGraph multi_graph;            // captured steps of the batched programs
   eg::DevBuf e_multi_roots;          // roots and per-root statistics
   double eng_prepare_ms = 0;
 

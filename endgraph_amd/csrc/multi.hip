@@ -187,6 +187,7 @@ struct MultiState {
   uint64_t* items[2];       // push work items (v << 32 | chunk), ping-pong by step parity
   MCount* cnt;              // ring of 3 by step: read at s, written at s-1, cleared at s-2
   uint32_t* steps;          // steps that had a nonempty frontier
+  uint32_t* base;           // first step of the queued batch (step = *base + j)
 };
 
 uint64_t item_cap(const eg_graph* g) { return uint64_t(g->V) + g->E / kItemEdges + 1; }
@@ -204,6 +205,7 @@ int multi_state(eg_graph* g, MultiState* st) {
   st->items[1] = st->items[0] + ic;
   st->cnt = reinterpret_cast<MCount*>(st->items[1] + ic);
   st->steps = reinterpret_cast<uint32_t*>(st->cnt + 3);
+  st->base = st->steps + 1;
   return EG_OK;
 }
 
@@ -328,7 +330,8 @@ struct BfsArgs {
 
 // Sparse step (bfs.cpp:52-58 per root): unvisited out-neighbours of the
 // frontier take the smallest frontier id as parent.
-__global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t step) {
+__global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
   if (!c.nv || !sparse_level(a.policy, c.ne, a.E)) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -360,7 +363,8 @@ __global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t step) {
 // source of a root in the chunk is the chunk's smallest, write_min over the
 // chunks gives the smallest of the row.  Only roots that still have a
 // frontier are looked for, and a chunk stops once all of them are found.
-__global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t step) {
+__global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
   if (!c.nv || sparse_level(a.policy, c.ne, a.E)) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -395,7 +399,8 @@ __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t step) {
 
 // Level end (bfs.cpp:70-90 per root): visited |= next, depth = level + 1,
 // next becomes the frontier; counters and work items of the next step.
-__global__ __launch_bounds__(kNT) void k_mbfs_end(BfsArgs a, uint32_t step) {
+__global__ __launch_bounds__(kNT) void k_mbfs_end(BfsArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
   const bool live = a.st.cnt[step % 3].nv != 0;
   step_bookkeeping(a.st, step, live);
   if (!live) return;  // block-uniform
@@ -430,7 +435,8 @@ struct SsspArgs {
 
 // Sparse round (sssp.cpp:44-58 per root): relax dist[u] + w into the
 // out-neighbours; write_min on the bits of non-negative floats.
-__global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t step) {
+__global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
   if (!c.nv || !sparse_level(a.policy, c.ne, a.E)) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -463,7 +469,8 @@ __global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t step) {
 // dist[u] + w; lane r is root r, so each in-edge reads its source's 64
 // distances as one row.  A wave takes an in-edge chunk of at most 512 edges
 // (hub rows spread over many waves) and lowers dist[v] with write_min.
-__global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t step) {
+__global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
   if (!c.nv || sparse_level(a.policy, c.ne, a.E)) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -513,7 +520,8 @@ __global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t step) {
 }
 
 // Round end: the lowered vertices are the next active set.
-__global__ __launch_bounds__(kNT) void k_msssp_end(SsspArgs a, uint32_t step) {
+__global__ __launch_bounds__(kNT) void k_msssp_end(SsspArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
   const bool live = a.st.cnt[step % 3].nv != 0;
   step_bookkeeping(a.st, step, live);
   if (!live) return;  // block-uniform
@@ -555,7 +563,7 @@ int start_multi(eg_graph* g, const uint32_t* h_roots, int n, MultiState* st, uin
   const uint64_t V = g->V;
   EG_TRY(fill_t<uint64_t>(st->vis, 3 * V, 0ull, s));  // vis, fr, nx
   EG_TRY(fill_t<unsigned long long>(reinterpret_cast<unsigned long long*>(st->cnt), 3 * sizeof(MCount) / 8, 0ull, s));
-  EG_TRY(fill_t<uint32_t>(st->steps, 1, 0u, s));
+  EG_TRY(fill_t<uint32_t>(st->steps, 2, 0u, s));  // steps, base
   if (parent) EG_TRY(fill_t<uint32_t>(parent, V * 64, g->V, s));
   if (depth) EG_TRY(fill_t<uint32_t>(depth, V * 64, kUnreached, s));
   if (dist) EG_TRY(fill_t<float>(dist, V * 64, kInfDist, s));
@@ -568,33 +576,79 @@ int start_multi(eg_graph* g, const uint32_t* h_roots, int n, MultiState* st, uin
   return EG_OK;
 }
 
-// Runs steps until a step has an empty frontier: kBatch steps are queued
-// between host checks; a queued step after the last live one returns at
-// once (its counters are zero).
+__global__ void k_multi_advance(uint32_t* base, uint32_t n) { *base += n; }
+
+// Runs steps until a step has an empty frontier.  kBatch steps (three
+// launches each, then one that advances the batch base) are captured once
+// into a HIP graph on the graph handle's own stream and replayed between
+// host checks; a step after the last live one returns at once (its counters
+// are zero).  The captured graph is kept for the next call with the same
+// arguments (same buffers).
 constexpr uint32_t kBatch = 4;
+
+struct StepGrids {
+  unsigned push, pull, end;
+};
+
+template <class Args>
+int capture_steps(MultiGraph& mg, const Args& a, const StepGrids& gr, void (*push)(Args, uint32_t),
+                  void (*pull)(Args, uint32_t), void (*end)(Args, uint32_t)) {
+  mg.reset_exec();
+  EG_HIP(hipStreamBeginCapture(mg.stream, hipStreamCaptureModeThreadLocal));
+  for (uint32_t j = 0; j < kBatch; ++j) {
+    hipLaunchKernelGGL(push, dim3(gr.push), dim3(kNT), 0, mg.stream, a, j);
+    hipLaunchKernelGGL(pull, dim3(gr.pull), dim3(kNT), 0, mg.stream, a, j);
+    hipLaunchKernelGGL(end, dim3(gr.end), dim3(kNT), 0, mg.stream, a, j);
+  }
+  hipLaunchKernelGGL(k_multi_advance, dim3(1), dim3(1), 0, mg.stream, a.st.base, kBatch);
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(mg.stream, &graph);
+  if (ce != hipSuccess) return fail(EG_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+  const hipError_t ie = hipGraphInstantiate(&mg.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) {
+    mg.exec = nullptr;
+    return fail(EG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+  }
+  mg.key.assign(reinterpret_cast<const char*>(&a), reinterpret_cast<const char*>(&a) + sizeof(Args));
+  mg.key.insert(mg.key.end(), reinterpret_cast<const char*>(&gr), reinterpret_cast<const char*>(&gr) + sizeof(gr));
+  const void* fns[3] = {reinterpret_cast<const void*>(push), reinterpret_cast<const void*>(pull),
+                        reinterpret_cast<const void*>(end)};
+  mg.key.insert(mg.key.end(), reinterpret_cast<const char*>(fns), reinterpret_cast<const char*>(fns) + sizeof(fns));
+  return EG_OK;
+}
 
 template <class Args>
 int run_steps(eg_graph* g, const Args& a, void (*push)(Args, uint32_t), void (*pull)(Args, uint32_t),
               void (*end)(Args, uint32_t), uint32_t* h_steps, hipStream_t s) {
-  const unsigned wave_grid = grid_for(g->tv_nitems, kNT / kWave, 65536);
-  const unsigned push_grid = 8192;
-  const unsigned end_grid = grid_for(ceil_div(g->V, kWave), kNT / kWave, 1024);
+  MultiGraph& mg = g->multi_graph;
+  EG_TRY(mg.init());
+  const StepGrids gr{8192, grid_for(g->tv_nitems, kNT / kWave, 65536),
+                     grid_for(ceil_div(g->V, kWave), kNT / kWave, 1024)};
+  std::vector<char> key(reinterpret_cast<const char*>(&a), reinterpret_cast<const char*>(&a) + sizeof(Args));
+  key.insert(key.end(), reinterpret_cast<const char*>(&gr), reinterpret_cast<const char*>(&gr) + sizeof(gr));
+  const void* fns[3] = {reinterpret_cast<const void*>(push), reinterpret_cast<const void*>(pull),
+                        reinterpret_cast<const void*>(end)};
+  key.insert(key.end(), reinterpret_cast<const char*>(fns), reinterpret_cast<const char*>(fns) + sizeof(fns));
+  if (!mg.exec || mg.key != key) EG_TRY(capture_steps(mg, a, gr, push, pull, end));
+  // the caller's queued work (state set-up) before the graph, the graph
+  // before the caller's later work
+  EG_HIP(hipEventRecord(mg.ev_in, s));
+  EG_HIP(hipStreamWaitEvent(mg.stream, mg.ev_in, 0));
   const uint32_t max_steps = g->V + 1;  // a frontier is empty after at most V steps
   uint32_t step = 0;
   for (;;) {
-    for (uint32_t j = 0; j < kBatch && step < max_steps; ++j, ++step) {
-      hipLaunchKernelGGL(push, dim3(push_grid), dim3(kNT), 0, s, a, step);
-      hipLaunchKernelGGL(pull, dim3(wave_grid), dim3(kNT), 0, s, a, step);
-      hipLaunchKernelGGL(end, dim3(end_grid), dim3(kNT), 0, s, a, step);
-      EG_HIP(hipGetLastError());
-    }
+    EG_HIP(hipGraphLaunch(mg.exec, mg.stream));
+    step += kBatch;
     unsigned long long next_nv = 0;
-    EG_HIP(hipMemcpyAsync(&next_nv, &a.st.cnt[step % 3].nv, 8, hipMemcpyDeviceToHost, s));
-    EG_HIP(hipStreamSynchronize(s));
+    EG_HIP(hipMemcpyAsync(&next_nv, &a.st.cnt[step % 3].nv, 8, hipMemcpyDeviceToHost, mg.stream));
+    EG_HIP(hipStreamSynchronize(mg.stream));
     if (!next_nv || step >= max_steps) break;
   }
-  EG_HIP(hipMemcpyAsync(h_steps, a.st.steps, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
+  EG_HIP(hipMemcpyAsync(h_steps, a.st.steps, 4, hipMemcpyDeviceToHost, mg.stream));
+  EG_HIP(hipStreamSynchronize(mg.stream));
+  EG_HIP(hipEventRecord(mg.ev_out, mg.stream));
+  EG_HIP(hipStreamWaitEvent(s, mg.ev_out, 0));
   return EG_OK;
 }
 
@@ -615,7 +669,7 @@ int eg_bfs_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, 
   EG_TRY(ensure_tview(g, s));
   MultiState st;
   EG_TRY(start_multi(g, h_roots, n_roots, &st, d_parent, d_depth, nullptr, s));
-  BfsArgs a;
+  BfsArgs a{};  // zeroed padding: the bytes key the captured step graph
   a.V = g->V;
   a.E = g->E;
   a.policy = policy;
@@ -666,7 +720,7 @@ int eg_sssp_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy,
   EG_TRY(ensure_tview(g, s));
   MultiState st;
   EG_TRY(start_multi(g, h_roots, n_roots, &st, nullptr, nullptr, d_dist, s));
-  SsspArgs a;
+  SsspArgs a{};  // zeroed padding: the bytes key the captured step graph
   a.V = g->V;
   a.E = g->E;
   a.policy = policy;

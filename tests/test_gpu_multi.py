@@ -98,3 +98,31 @@ def test_multi_errors(eg):
         g.bfs_multi(np.array([V], np.uint32))
     with pytest.raises(eg.EndGraphError):
         g.sssp_multi(np.array([1], np.uint32))  # 8-byte records carry no weights
+
+
+def test_multi_step_graph_replays(eg):
+    """The steps run as a captured HIP graph that is kept between calls with
+    the same buffers: replays for new root sets, with single-root programs
+    (eager launches on the caller's stream) and the other batched program
+    (a recapture) in between, all equal the single-root results."""
+    edges, V = _rmat(eg, 12, seed=23)
+    g, _ = _graph(eg, edges, V)
+    parent, depth = eg.DeviceBuffer(V * 64 * 4), eg.DeviceBuffer(V * 64 * 4)
+    dist = eg.DeviceBuffer(V * 64 * 4)
+    rng = np.random.default_rng(23)
+    for rnd in range(2):
+        for i in range(3):
+            print(f"step graph replay round={rnd} bfs call={i}", flush=True)
+            roots = rng.integers(0, V, 64 if i != 2 else 17).astype(np.uint32)
+            par, dep, lev, edg, _ = g.bfs_multi(roots, 1, parent, depth)
+            for r, root in enumerate(roots):
+                p1, d1, l1, e1 = g.bfs(int(root), 1)
+                assert np.array_equal(par[:, r], p1) and np.array_equal(dep[:, r], d1), f"call {i} root {root}"
+                assert lev[r] == l1 and edg[r] == e1
+        for i in range(3):
+            print(f"step graph replay round={rnd} sssp call={i}", flush=True)
+            roots = rng.integers(0, V, 64).astype(np.uint32)
+            d, _ = g.sssp_multi(roots, 1, dist)
+            for r, root in enumerate(roots):
+                d1, _ = g.sssp(int(root), 1)
+                assert np.array_equal(d[:, r].view(np.uint32), d1.view(np.uint32)), f"call {i} root {root}"
