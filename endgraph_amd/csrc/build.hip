@@ -441,27 +441,130 @@ __global__ __launch_bounds__(kNT) void k_cindex_sentinel(const R* __restrict__ a
   }
 }
 
-// Dense CSR index (core/graph.hpp:1312-1353) in global positions: a run of
-// source u in sub-partition q writes index[q][u] = first and
-// index[q][u+1] = last+1; gaps are filled by the max-scan afterwards.
+// Dense CSR index (core/graph.hpp:1312-1353) and its bitmap, every entry
+// defined: index[q][u] = local position of the first record of sub-partition
+// q whose source is >= u (u in [0, V]), so index[q][u]..index[q][u+1] is u's
+// run and empty u get an empty range.  The head of the run of source u, whose
+// previous record in q has source u' (none: u' = -1), writes entries
+// u'+1..u; the last record of q writes u+1..V.  Gaps longer than kShortGap
+// are cut into pieces of at most kGapPiece entries that k_csr_long_gaps
+// writes, one wave per piece.  Bit u of the bitmap is set by u's run head;
+// the lanes of a round holding the same bitmap word combine their bits
+// first (one atomic per word and round).
+constexpr uint32_t kShortGap = 64;
+constexpr uint32_t kGapPiece = 1u << 16;
+
+struct GapPiece {
+  uint64_t start;  // first index entry
+  uint32_t count;
+  uint32_t value;
+};
+
+__device__ __forceinline__ void put_gap(uint64_t* __restrict__ index, uint64_t start, uint64_t cnt, uint32_t value,
+                                        GapPiece* __restrict__ gaps, uint32_t* __restrict__ ngaps, uint32_t max_gaps) {
+  if (cnt <= kShortGap) {
+    for (uint64_t c = 0; c < cnt; ++c) index[start + c] = value;
+    return;
+  }
+  const uint32_t pieces = uint32_t(ceil_div(cnt, uint64_t(kGapPiece)));
+  const uint32_t slot = atomicAdd(ngaps, pieces);
+  if (uint64_t(slot) + pieces <= max_gaps) {
+    for (uint32_t k = 0; k < pieces; ++k) {
+      const uint64_t off = uint64_t(k) * kGapPiece;
+      gaps[slot + k] = GapPiece{start + off, uint32_t(cnt - off < kGapPiece ? cnt - off : kGapPiece), value};
+    }
+    return;
+  }
+  // list full: empty the slots taken and write the gap here
+  for (uint64_t k = slot; k < max_gaps && k < uint64_t(slot) + pieces; ++k) gaps[k] = GapPiece{0, 0, 0};
+  for (uint64_t c = 0; c < cnt; ++c) index[start + c] = value;
+}
+
 template <class R>
-__global__ __launch_bounds__(kNT) void k_csr_runs(const R* __restrict__ a, uint64_t n, KeyQ<false> key, uint32_t V,
-                                                  uint64_t* __restrict__ index) {
+__global__ __launch_bounds__(kNT) void k_csr_gaps(const R* __restrict__ a, uint64_t n, KeyQ<false> key, uint32_t V,
+                                                  const eg_graph::Meta* __restrict__ meta, uint64_t* __restrict__ index,
+                                                  unsigned long long* __restrict__ bitmap,
+                                                  GapPiece* __restrict__ gaps, uint32_t* __restrict__ ngaps,
+                                                  uint32_t max_gaps) {
+  // per wave: bits of the 64 bitmap words from the chunk's first word on
+  __shared__ unsigned long long bm[kNT / kWave][kWave];
   const int lane = threadIdx.x & (kWave - 1);
+  unsigned long long* my = bm[threadIdx.x / kWave];
   const uint64_t vmask = (1ull << key.vbits) - 1;
+  const uint64_t row = uint64_t(V) + 1;
+  const uint64_t words = (V >> 6) + 1;
+  __hip_atomic_store(my + lane, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  __asm__ volatile("" ::: "memory");
   EG_FOR_CHUNKS(n) {
     const uint64_t c0 = chunk * kChunk;
     ChunkKeys c;
     chunk_keys(a, n, c0, key, c);
+    const uint64_t k0 = shfl_t(c.k[0], 0);
+    const uint64_t wbase = (k0 >> key.vbits) * words + ((k0 & vmask) >> 6);
 #pragma unroll
     for (int j = 0; j < kCR; ++j) {
       const uint64_t i = c0 + uint64_t(j) * kWave + lane;
-      const uint64_t q = c.k[j] >> key.vbits;
-      const uint32_t u = uint32_t(c.k[j] & vmask);
-      if (chunk_head(c, j, i, n)) index[q * (V + 1) + u] = i;
-      if (chunk_tail(c, j, i, n)) index[q * (V + 1) + u + 1] = i + 1;
+      const uint64_t k = c.k[j];
+      const uint32_t q = uint32_t(k >> key.vbits);
+      const uint32_t u = uint32_t(k & vmask);
+      const uint64_t up = shfl_up_t(k, 1);
+      const uint64_t last_prev = j > 0 ? shfl_t(c.k[j > 0 ? j - 1 : 0], kWave - 1) : c.prev;
+      const uint64_t pk = lane > 0 ? up : last_prev;
+      const uint64_t dn = __shfl_down(k, 1, kWave);
+      const uint64_t first_next = j < kCR - 1 ? shfl_t(c.k[j < kCR - 1 ? j + 1 : j], 0) : c.next;
+      const uint64_t nk = lane < kWave - 1 ? dn : first_next;
+      const bool valid = i < n;
+      if (valid && pk != k) {  // run head
+        const uint64_t qbase = meta->q_off_csr[q];
+        const uint64_t lo = (pk != kNoKey && (pk >> key.vbits) == q) ? (pk & vmask) + 1 : 0;
+        put_gap(index, q * row + lo, u - lo + 1, uint32_t(i - qbase), gaps, ngaps, max_gaps);
+        const uint64_t wd = q * words + (u >> 6);
+        const unsigned long long bit = 1ull << (u & 63);
+        if (wd - wbase < kWave)
+          __hip_atomic_fetch_or(my + (wd - wbase), bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        else
+          atomicOr(&bitmap[wd], bit);
+      }
+      if (valid && (i == n - 1 || (nk >> key.vbits) != q))  // last record of q
+        put_gap(index, q * row + u + 1, V - u, uint32_t(i + 1 - meta->q_off_csr[q]), gaps, ngaps, max_gaps);
     }
+    // Flush the chunk's words.  Every OR of the wave has completed
+    // (lgkmcnt(0)) before the read, and the read before the clear; the empty
+    // asm statements keep the compiler from moving LDS accesses across.
+    __builtin_amdgcn_s_waitcnt(kLgkmZero);
+    __asm__ volatile("" ::: "memory");
+    const unsigned long long b = __hip_atomic_load(my + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __builtin_amdgcn_s_waitcnt(kLgkmZero);
+    __asm__ volatile("" ::: "memory");
+    if (b) {
+      atomicOr(&bitmap[wbase + lane], b);
+      __hip_atomic_store(my + lane, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+    __asm__ volatile("" ::: "memory");
   }
+}
+
+// Pieces of long gaps: one wave per piece.
+__global__ __launch_bounds__(kNT) void k_csr_long_gaps(uint64_t* __restrict__ index, const GapPiece* __restrict__ gaps,
+                                                       const uint32_t* __restrict__ ngaps, uint32_t max_gaps) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t n = *ngaps < max_gaps ? *ngaps : max_gaps;
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t p = gw; p < n; p += nw) {
+    const GapPiece g = gaps[p];
+    for (uint32_t e = lane; e < g.count; e += kWave) index[g.start + e] = g.value;
+  }
+}
+
+// Rows of empty sub-partitions (no records, so no run writes them): all 0.
+__global__ __launch_bounds__(kNT) void k_csr_empty_rows(uint64_t* __restrict__ index, uint32_t V,
+                                                        const eg_graph::Meta* __restrict__ meta) {
+  const uint32_t q = blockIdx.y;
+  if (meta->q_off_csr[q] != meta->q_off_csr[q + 1]) return;
+  const uint64_t row = uint64_t(V) + 1;
+  for (uint64_t u = uint64_t(blockIdx.x) * kNT + threadIdx.x; u < row; u += uint64_t(gridDim.x) * kNT)
+    index[q * row + u] = 0;
 }
 
 template <class R, bool kSrc>
@@ -499,31 +602,6 @@ __global__ __launch_bounds__(kNT) void k_run_degree(const R* __restrict__ a, uin
   }
 }
 
-// Sub-partition-local positions, and the bitmap: u has edges in q iff
-// index[q][u+1] > index[q][u].  A wave takes one bitmap word (64 ids) per
-// step; the ballot of its lanes is the word.
-__global__ __launch_bounds__(kNT) void k_csr_localize(const uint64_t* __restrict__ gpos, uint64_t* __restrict__ index,
-                                                      uint64_t* __restrict__ bitmap, uint32_t V, int Q,
-                                                      const eg_graph::Meta* __restrict__ meta) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t words = (V >> 6) + 1;
-  const uint64_t total = uint64_t(Q) * words;
-  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  for (uint64_t t = gw; t < total; t += nw) {
-    const uint64_t q = t / words, wd = t % words;
-    const uint64_t u = wd * 64 + lane;
-    const uint64_t* row = gpos + q * (uint64_t(V) + 1);
-    bool present = false;
-    if (u <= V) {
-      const uint64_t v = row[u];
-      if (u < V) present = row[u + 1] > v;
-      index[q * (uint64_t(V) + 1) + u] = v - meta->q_off_csr[q];
-    }
-    const uint64_t m = __ballot(present);
-    if (lane == 0) bitmap[q * words + wd] = m;
-  }
-}
 
 // --- P > 1: regroup the rank slices by sub-partition ---------------------
 // A rank slice sorted by its owner id is already grouped by sub-partition
@@ -573,56 +651,6 @@ __global__ void k_block_table(const uint64_t* __restrict__ bpos, int P, int Q, u
   tab_dst[b] = d;
 }
 
-// A wave copies kSpan consecutive destination records: one binary search for
-// the span, then per 512-record chunk a fast path when the chunk lies inside
-// one block (the usual case: blocks hold millions of records).
-constexpr int kSpanChunks = 8;
-
-template <class R>
-__global__ __launch_bounds__(kNT) void k_block_copy(const R* __restrict__ in, R* __restrict__ out, uint64_t n,
-                                                    const uint64_t* __restrict__ tab_dst,
-                                                    const uint64_t* __restrict__ tab_src, int nblk) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t span = uint64_t(kSpanChunks) * kChunk;
-  const uint64_t nspan = ceil_div(n, span);
-  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  for (uint64_t sp = gw; sp < nspan; sp += nw) {
-    const uint64_t s0 = sp * span;
-    int b = 0, hi = nblk;  // last block starting at or before s0
-    while (hi - b > 1) {
-      const int mid = (b + hi) >> 1;
-      if (tab_dst[mid] <= s0)
-        b = mid;
-      else
-        hi = mid;
-    }
-    for (int ch = 0; ch < kSpanChunks; ++ch) {
-      const uint64_t c0 = s0 + uint64_t(ch) * kChunk;
-      if (c0 >= n) break;
-      while (b + 1 < nblk && tab_dst[b + 1] <= c0) ++b;  // wave-uniform
-      const uint64_t cend = c0 + kChunk < n ? c0 + kChunk : n;
-      const bool one_block = b + 1 >= nblk || tab_dst[b + 1] >= cend;
-      uint64_t src[kCR];
-#pragma unroll
-      for (int j = 0; j < kCR; ++j) {
-        const uint64_t i = c0 + uint64_t(j) * kWave + lane;
-        int bb = b;
-        if (!one_block)
-          while (bb + 1 < nblk && tab_dst[bb + 1] <= i) ++bb;
-        src[j] = i < n ? tab_src[bb] + (i - tab_dst[bb]) : tab_src[b];
-      }
-      R r[kCR];
-#pragma unroll
-      for (int j = 0; j < kCR; ++j) r[j] = in[src[j]];
-#pragma unroll
-      for (int j = 0; j < kCR; ++j) {
-        const uint64_t i = c0 + uint64_t(j) * kWave + lane;
-        if (i < n) out[i] = r[j];
-      }
-    }
-  }
-}
 
 unsigned grid_for(uint64_t n, uint64_t per_block = kNT, uint64_t cap = 65536) {
   return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, per_block), cap)));
@@ -701,7 +729,7 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   EG_TRY(g->t2.resize(Emax * rb));
   if (P > 1 || Q > 1) EG_TRY(g->t3.resize(Emax * rb));
   const uint64_t hist_n = 1;
-  const uint64_t scan_n = std::max<uint64_t>({scan_tmp_elems(hist_n), scan_tmp_elems(idx_n), scan_tmp_elems(pages + 1),
+  const uint64_t scan_n = std::max<uint64_t>({scan_tmp_elems(hist_n), scan_tmp_elems(pages + 1),
                                               scan_tmp_elems(ceil_div(E, kChunk) + 1)});
   EG_TRY(g->scan_tmp.resize(scan_n * 8));
   EG_TRY(g->tile_cnt.resize((ceil_div(E, kChunk) + 2) * 4));
@@ -797,16 +825,13 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
       EG_HIP(hipGetLastError());
       hipLaunchKernelGGL(k_block_table, dim3(1), dim3(kWave), 0, s, (const uint64_t*)bpos, P, Q, tab_dst, tab_src);
       EG_HIP(hipGetLastError());
-      hipLaunchKernelGGL((k_block_copy<R>), dim3(grid_for(ceil_div(E, uint64_t(kSpanChunks) * kChunk), kNT / kWave,
-                                                           65536)),
-                         dim3(kNT), 0, s, (const R*)in, X, E, (const uint64_t*)tab_dst, (const uint64_t*)tab_src,
-                         P * Q);
-      EG_HIP(hipGetLastError());
-      // the slice array is free now: it is the ping-pong of the segmented sort
+      // the first pass of the segmented sort reads the (q, r) blocks in place
       if (pull)
-        EG_TRY(rx_sort((const R*)X, g->csc_list.as<R>(), in, seg_csc, KeyDst{}, g->vbits, g->sortw, s));
+        EG_TRY(rx_sort_blocks((const R*)in, tab_dst, tab_src, P * Q, g->csc_list.as<R>(), X, seg_csc, KeyDst{},
+                              g->vbits, g->sortw, s));
       else
-        EG_TRY(rx_sort((const R*)X, g->csr_list.as<R>(), in, seg_csr, KeySrc{}, g->vbits, g->sortw, s));
+        EG_TRY(rx_sort_blocks((const R*)in, tab_dst, tab_src, P * Q, g->csr_list.as<R>(), X, seg_csr, KeySrc{},
+                              g->vbits, g->sortw, s));
     }
   } else if (Q == 1) {
     g->csr_list.swap(g->t1);  // A is the CSR, B the CSC
@@ -841,24 +866,36 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
     EG_HIP(hipGetLastError());
   }
 
-  // --- dense CSR index + bitmap: run ends as global positions into gpos,
-  // max-scan fills the gaps, localize writes the index and the bitmap ---
+  // --- dense CSR index + bitmap: runs write their gaps, long gaps go to a
+  // list written by a second launch, empty sub-partitions by a third ---
   {
     uint64_t* index = g->csr_index.as<uint64_t>();
     uint64_t* bitmap = g->csr_bitmap.as<uint64_t>();
-    EG_TRY(g->idx_tmp.resize(idx_n * 8));
-    uint64_t* gpos = g->idx_tmp.as<uint64_t>();
-    EG_TRY(fill<uint64_t>(gpos, idx_n, uint64_t(0), s));
+    // a gap of L > kShortGap entries takes ceil(L / kGapPiece) <= L / 65 + 1
+    // pieces and the gaps of a row add up to at most V + 1 entries, so the
+    // list cannot fill up (the fallback in put_gap is never taken)
+    const uint64_t max_pieces = 2 * idx_n / (kShortGap + 1) + 1;
+    if (max_pieces >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "CSR index of %llu entries is too large",
+                                                       (unsigned long long)idx_n);
+    const uint32_t max_gaps = uint32_t(max_pieces);
+    EG_TRY(g->idx_tmp.resize(uint64_t(max_gaps) * sizeof(GapPiece) + 64));
+    GapPiece* gaps = g->idx_tmp.as<GapPiece>();
+    uint32_t* ngaps = reinterpret_cast<uint32_t*>(gaps + max_gaps);
+    EG_TRY(fill<uint64_t>(bitmap, uint64_t(Q) * words, uint64_t(0), s));
+    EG_TRY(fill<uint32_t>(ngaps, 1, 0u, s));
     if (E) {
       prof_begin(s);
-      hipLaunchKernelGGL((k_csr_runs<R>), dim3(chunk_grid(E)), dim3(kNT), 0, s, g->csr_list.as<R>(), E,
-                         KeyQ<false>{qmap, g->vbits}, V, gpos);
+      hipLaunchKernelGGL((k_csr_gaps<R>), dim3(chunk_grid(E)), dim3(kNT), 0, s, g->csr_list.as<R>(), E,
+                         KeyQ<false>{qmap, g->vbits}, V, (const eg_graph::Meta*)meta, index,
+                         reinterpret_cast<unsigned long long*>(bitmap), gaps, ngaps, max_gaps);
       prof_end(PK_CSR_INDEX, E * rb, s);
       EG_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_csr_long_gaps, dim3(1024), dim3(kNT), 0, s, index, (const GapPiece*)gaps,
+                         (const uint32_t*)ngaps, max_gaps);
+      EG_HIP(hipGetLastError());
     }
-    EG_TRY(inclusive_max_scan_u64(gpos, gpos, idx_n, g->scan_tmp.as<uint64_t>(), s));
-    hipLaunchKernelGGL(k_csr_localize, dim3(grid_for(uint64_t(Q) * words, kNT / kWave, 65536)), dim3(kNT), 0, s,
-                       (const uint64_t*)gpos, index, bitmap, V, Q, meta);
+    hipLaunchKernelGGL(k_csr_empty_rows, dim3(64, unsigned(Q)), dim3(kNT), 0, s, index, V,
+                       (const eg_graph::Meta*)meta);
     EG_HIP(hipGetLastError());
   }
 

@@ -59,7 +59,7 @@ struct eg_graph {
   eg::DevBuf csr_index;              // u64[Q][V+1] (sub-partition-local positions)
   eg::DevBuf csr_bitmap;             // u64[Q][(V>>6)+1]
   eg::DevBuf blk;                    // block tables of the P > 1 regrouping
-  eg::DevBuf idx_tmp;                // u64[Q][V+1] global positions before localizing
+  eg::DevBuf idx_tmp;                // long-gap pieces of the dense CSR index build
   eg::DevBuf sym;                    // symmetrised logical list
   eg::DevBuf t1, t2, t3;             // first-sort results and sort ping-pong
   eg::DevBuf hist, scan_tmp, tile_cnt, err;

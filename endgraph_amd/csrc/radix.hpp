@@ -56,6 +56,41 @@ inline RxPlan rx_plan(int key_bits, int max_bits) {
   return p;
 }
 
+// Input of a sort pass.  Plain: record i of the sort order is p[i].  Block
+// view (first pass of a sort that regroups contiguous blocks on the fly):
+// record i is p[i + shift[t]] for tile t holding i, or, in a tile that
+// straddles a block boundary (shift[t] == kRxStraddle), p[bsrc[b] + i - bdst[b]]
+// for the last block b with bdst[b] <= i.
+template <class R>
+struct RxIn {
+  const R* p;
+  const int64_t* shift;  // per tile, or null for plain input
+  const uint64_t* bdst;  // nblk + 1 block starts in sort order
+  const uint64_t* bsrc;  // nblk block starts in p
+  int nblk;
+};
+
+constexpr int64_t kRxStraddle = INT64_MIN;
+
+template <class R>
+__device__ __forceinline__ int64_t rx_tile_shift(const RxIn<R>& in, uint32_t t) {
+  return in.shift ? in.shift[t] : 0;
+}
+
+template <class R>
+__device__ __forceinline__ R rx_load(const RxIn<R>& in, int64_t sh, uint64_t i) {
+  if (sh != kRxStraddle) return in.p[uint64_t(int64_t(i) + sh)];
+  int lo = 0, hi = in.nblk;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (in.bdst[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return in.p[in.bsrc[lo] + (i - in.bdst[lo])];
+}
+
 struct RxPassArgs {
   int npass;
   int shift[kRxMaxPasses];
@@ -101,6 +136,18 @@ __device__ __forceinline__ RxTile rx_tile(const uint64_t* seg, const uint32_t* t
   return x;
 }
 
+// Tile of block b in the three-launch kernels.  Blocks go to the 8 XCDs in
+// turn (block b to XCD b % 8); numbering tiles so that each XCD takes one
+// contiguous range puts neighbouring tiles in the same L2, where the partial
+// lines they share (table entries of adjacent tiles, the ends of a digit's
+// runs) are combined before they are written back.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t rx_block_tile(uint32_t b, uint32_t tiles) {
+  const uint32_t q = tiles / kXcds, r = tiles % kXcds, x = b % kXcds;
+  return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 // s_waitcnt immediate of gfx9-family ISAs: lgkmcnt(0), vmcnt and expcnt at
 // their maxima (no wait on vector memory)
 constexpr unsigned kLgkmZero = 0xC07F;
@@ -113,7 +160,7 @@ constexpr unsigned kLgkmZero = 0xC07F;
 // LDS, free for the duration of the call.
 
 template <int NT, int BM, class R, class KeyFn>
-__device__ __forceinline__ void rx_rank(const R* __restrict__ in, const RxTile& x, KeyFn key, int shift, int bits,
+__device__ __forceinline__ void rx_rank(const RxIn<R>& in, int64_t sh, const RxTile& x, KeyFn key, int shift, int bits,
                                         R (&item)[kRxIPT], uint32_t (&dig)[kRxIPT], uint32_t (&rank)[kRxIPT],
                                         uint32_t (*wcnt)[1 << BM], uint32_t* tot, uint64_t* peer_tab) {
   constexpr int W = NT / kWave;
@@ -125,7 +172,7 @@ __device__ __forceinline__ void rx_rank(const R* __restrict__ in, const RxTile& 
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
     const uint32_t li = first + uint32_t(k) * kWave;
-    item[k] = in[x.base + (li < x.count ? li : 0u)];
+    item[k] = rx_load(in, sh, x.base + (li < x.count ? li : 0u));
   }
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) dig[k] = uint32_t(key(item[k]) >> shift) & mask;
@@ -232,7 +279,7 @@ __device__ __forceinline__ void rx_dstart(int nb, uint32_t tot, uint32_t* wt, ui
 // ---------------------------------------------------------- three-launch --
 // table index of (tile t of segment s, digit d): ts[s]*nb + d*nt + lt
 template <int NT, int BM, class R, class KeyFn>
-__global__ __launch_bounds__(NT) void k_rx_up(const R* __restrict__ in, const uint64_t* __restrict__ seg,
+__global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __restrict__ seg,
                                               const uint32_t* __restrict__ ts, int nseg, KeyFn key, int shift,
                                               int bits, uint32_t* __restrict__ hist) {
   constexpr int TILE = NT * kRxIPT;
@@ -240,13 +287,15 @@ __global__ __launch_bounds__(NT) void k_rx_up(const R* __restrict__ in, const ui
   const int nb = 1 << bits;
   for (int i = threadIdx.x; i < nb; i += NT) h[i] = 0;
   __syncthreads();
-  const RxTile x = rx_tile<TILE>(seg, ts, nseg, blockIdx.x);
+  const uint32_t t = rx_block_tile(blockIdx.x, gridDim.x);
+  const RxTile x = rx_tile<TILE>(seg, ts, nseg, t);
+  const int64_t sh = rx_tile_shift(in, t);
   const uint32_t mask = uint32_t(nb - 1);
   R r[kRxIPT];
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
     const uint32_t li = uint32_t(k) * NT + threadIdx.x;
-    r[k] = in[x.base + (li < x.count ? li : 0u)];
+    r[k] = rx_load(in, sh, x.base + (li < x.count ? li : 0u));
   }
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
@@ -258,7 +307,7 @@ __global__ __launch_bounds__(NT) void k_rx_up(const R* __restrict__ in, const ui
 }
 
 template <int NT, int BM, class R, class KeyFn>
-__global__ __launch_bounds__(NT) void k_rx_down(const R* __restrict__ in, R* __restrict__ out,
+__global__ __launch_bounds__(NT) void k_rx_down(RxIn<R> in, R* __restrict__ out,
                                                 const uint64_t* __restrict__ seg, const uint32_t* __restrict__ ts,
                                                 int nseg, KeyFn key, int shift, int bits,
                                                 const uint32_t* __restrict__ offs) {
@@ -271,7 +320,8 @@ __global__ __launch_bounds__(NT) void k_rx_down(const R* __restrict__ in, R* __r
   __shared__ __align__(16) R stage[TILE];
   const int nb = 1 << bits;
   for (int i = threadIdx.x; i < W * nb; i += NT) wcnt[i / nb][i % nb] = 0;
-  const RxTile x = rx_tile<TILE>(seg, ts, nseg, blockIdx.x);
+  const uint32_t t = rx_block_tile(blockIdx.x, gridDim.x);
+  const RxTile x = rx_tile<TILE>(seg, ts, nseg, t);
   for (int d = threadIdx.x; d < nb; d += NT) goff[d] = offs[uint64_t(ts[x.s]) * nb + uint64_t(d) * x.nt + x.lt];
   __syncthreads();
   R item[kRxIPT];
@@ -279,14 +329,15 @@ __global__ __launch_bounds__(NT) void k_rx_down(const R* __restrict__ in, R* __r
   uint32_t tot;
   // the peer table of the ranking lives in the staging buffer, which is
   // first written after the ranking's last barrier
-  rx_rank<NT, BM>(in, x, key, shift, bits, item, dig, rank, wcnt, &tot, reinterpret_cast<uint64_t*>(stage));
+  rx_rank<NT, BM>(in, rx_tile_shift(in, t), x, key, shift, bits, item, dig, rank, wcnt, &tot,
+                  reinterpret_cast<uint64_t*>(stage));
   rx_dstart<NT>(nb, tot, wt, dstart);
   rx_scatter<NT, BM>(out, x, key, shift, bits, item, dig, rank, wcnt, goff, dstart, stage);
 }
 
 // -------------------------------------------------------------- one-pass --
 template <int NT, int BM, class R, class KeyFn>
-__global__ __launch_bounds__(NT) void k_rx_hist_all(const R* __restrict__ in, const uint64_t* __restrict__ seg,
+__global__ __launch_bounds__(NT) void k_rx_hist_all(RxIn<R> in, const uint64_t* __restrict__ seg,
                                                     const uint32_t* __restrict__ ts, int nseg, KeyFn key,
                                                     RxPassArgs pa, uint32_t* __restrict__ ghist) {
   constexpr int TILE = NT * kRxIPT;
@@ -294,11 +345,12 @@ __global__ __launch_bounds__(NT) void k_rx_hist_all(const R* __restrict__ in, co
   for (int i = threadIdx.x; i < kRxMaxPasses * (1 << BM); i += NT) (&h[0][0])[i] = 0;
   __syncthreads();
   const RxTile x = rx_tile<TILE>(seg, ts, nseg, blockIdx.x);
+  const int64_t sh = rx_tile_shift(in, blockIdx.x);
   R r[kRxIPT];
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
     const uint32_t li = uint32_t(k) * NT + threadIdx.x;
-    r[k] = in[x.base + (li < x.count ? li : 0u)];
+    r[k] = rx_load(in, sh, x.base + (li < x.count ? li : 0u));
   }
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
@@ -317,7 +369,7 @@ __global__ __launch_bounds__(NT) void k_rx_hist_all(const R* __restrict__ in, co
 }
 
 template <int NT, int BM, class R, class KeyFn>
-__global__ __launch_bounds__(NT) void k_rx_onepass(const R* __restrict__ in, R* __restrict__ out,
+__global__ __launch_bounds__(NT) void k_rx_onepass(RxIn<R> in, R* __restrict__ out,
                                                    const uint64_t* __restrict__ seg, const uint32_t* __restrict__ ts,
                                                    int nseg, KeyFn key, int shift, int bits,
                                                    const uint32_t* __restrict__ dbase, uint64_t* __restrict__ status,
@@ -341,7 +393,8 @@ __global__ __launch_bounds__(NT) void k_rx_onepass(const R* __restrict__ in, R* 
   R item[kRxIPT];
   uint32_t dig[kRxIPT], rank[kRxIPT];
   uint32_t tot;
-  rx_rank<NT, BM>(in, x, key, shift, bits, item, dig, rank, wcnt, &tot, reinterpret_cast<uint64_t*>(stage));
+  rx_rank<NT, BM>(in, rx_tile_shift(in, t), x, key, shift, bits, item, dig, rank, wcnt, &tot,
+                  reinterpret_cast<uint64_t*>(stage));
   // A status word is its own payload (epoch | flag | count): relaxed
   // device-scope atomics suffice; release/acquire would write back or
   // invalidate the L2 on every publish and poll.
@@ -380,7 +433,7 @@ __global__ __launch_bounds__(NT) void k_rx_onepass(const R* __restrict__ in, R* 
 
 namespace {  // internal linkage: this header is included by several sources
 
-__global__ void k_rx_base(uint32_t* __restrict__ ghist, const uint64_t* __restrict__ seg, int nseg, int npass) {
+__global__ __attribute__((unused)) void k_rx_base(uint32_t* __restrict__ ghist, const uint64_t* __restrict__ seg, int nseg, int npass) {
   const int s = blockIdx.x;
   if (s >= nseg || int(threadIdx.x) >= npass) return;
   uint32_t* g = ghist + (uint64_t(s) * kRxMaxPasses + threadIdx.x) * (1 << kRxMaxBits);
@@ -392,7 +445,26 @@ __global__ void k_rx_base(uint32_t* __restrict__ ghist, const uint64_t* __restri
   }
 }
 
-__global__ void k_rx_zero(uint32_t* __restrict__ p, uint64_t n) {
+// per-tile shift of a block view (kRxStraddle where a tile spans blocks)
+template <int TILE>
+__global__ void k_rx_tile_shift(const uint64_t* __restrict__ seg, const uint32_t* __restrict__ ts, int nseg,
+                                uint32_t tiles, const uint64_t* __restrict__ bdst, const uint64_t* __restrict__ bsrc,
+                                int nblk, int64_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tiles) return;
+  const RxTile x = rx_tile<TILE>(seg, ts, nseg, t);
+  int lo = 0, hi = nblk;  // last block starting at or before the tile
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bdst[mid] <= x.base)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  out[t] = bdst[lo + 1] >= x.base + x.count ? int64_t(bsrc[lo]) - int64_t(bdst[lo]) : kRxStraddle;
+}
+
+__global__ __attribute__((unused)) void k_rx_zero(uint32_t* __restrict__ p, uint64_t n) {
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
     p[i] = 0;
 }
@@ -413,7 +485,7 @@ SortVariant& sort_variant(int rec_bytes);
 
 // Device workspace (grow-only, owned by the caller).
 struct RxWork {
-  DevBuf table, ghist, status, ticket, segs, err, scan;
+  DevBuf table, ghist, status, ticket, segs, err, scan, tshift;
   uint32_t epoch = 0;
   int resize_segs(int nseg) {
     EG_TRY(segs.resize((uint64_t(nseg) + 1) * 12 + 16));
@@ -424,7 +496,7 @@ struct RxWork {
 };
 
 template <int NT, int BM, class R, class KeyFn>
-int rx_sort_nt(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, int scheme,
+int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, int scheme,
                RxWork& w, hipStream_t s) {
   static_assert((1 << BM) <= NT, "one digit per thread in the count/scan steps");
   static_assert(NT * kRxIPT * sizeof(R) + (NT / kWave) * (4 << BM) + (8 << BM) <= 160 * 1024, "LDS of one block");
@@ -442,7 +514,14 @@ int rx_sort_nt(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, Ke
   EG_HIP(hipMemcpyAsync(d_ts, ts.data(), (nseg + 1) * 4, hipMemcpyHostToDevice, s));
   const uint64_t n = seg[nseg] - seg[0];
   const RxPlan plan = rx_plan(key_bits, BM);
-  const R* src = in;
+  if (in.nblk > 0) {
+    EG_TRY(w.tshift.resize(uint64_t(tiles) * 8));
+    hipLaunchKernelGGL((k_rx_tile_shift<TILE>), dim3(ceil_div(tiles, 256u)), dim3(256), 0, s, (const uint64_t*)d_seg,
+                       (const uint32_t*)d_ts, nseg, tiles, in.bdst, in.bsrc, in.nblk, w.tshift.as<int64_t>());
+    EG_HIP(hipGetLastError());
+    in.shift = w.tshift.as<int64_t>();
+  }
+  RxIn<R> src = in;
   if (scheme == kThreeLaunch) {
     const uint64_t tn = uint64_t(tiles) << kRxMaxBits;
     EG_TRY(w.table.resize(tn * 4));
@@ -462,7 +541,7 @@ int rx_sort_nt(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, Ke
                          (const uint32_t*)w.table.as<uint32_t>());
       prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
       EG_HIP(hipGetLastError());
-      src = dst;
+      src = RxIn<R>{dst, nullptr, nullptr, nullptr, 0};
     }
     return EG_OK;
   }
@@ -501,7 +580,7 @@ int rx_sort_nt(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, Ke
                        w.status.as<uint64_t>(), w.ticket.as<uint32_t>() + p, epoch, w.err.as<uint32_t>());
     prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
     EG_HIP(hipGetLastError());
-    src = dst;
+    src = RxIn<R>{dst, nullptr, nullptr, nullptr, 0};
   }
   return EG_OK;
 }
@@ -510,7 +589,7 @@ int rx_sort_nt(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, Ke
 // positions of `out` (tmp = ping-pong; in, out, tmp distinct; in unchanged).
 // The caller zeroes w.err before and checks it after synchronising.
 template <class R, class KeyFn>
-int rx_sort(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w,
+int rx_sort(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w,
             hipStream_t s) {
   const int nseg = int(seg.size()) - 1;
   if (nseg < 1 || seg.back() == seg.front()) return EG_OK;
@@ -523,6 +602,25 @@ int rx_sort(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn
   }
   if (v.bits >= 9) return rx_sort_nt<512, 9>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
   return rx_sort_nt<512, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
+}
+
+// plain input
+template <class R, class KeyFn>
+int rx_sort(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w,
+            hipStream_t s) {
+  return rx_sort(RxIn<R>{in, nullptr, nullptr, nullptr, 0}, out, tmp, seg, key, key_bits, w, s);
+}
+
+// Sort of a block view: the input in sort order is nblk contiguous blocks,
+// block b = in[bsrc[b], bsrc[b] + bdst[b+1] - bdst[b]) placed at bdst[b]
+// (device tables; bdst[0] == 0, bdst[nblk] == the records).  The first pass
+// reads the blocks in place, so no regrouping copy is made; tmp must not
+// overlap in.
+template <class R, class KeyFn>
+int rx_sort_blocks(const R* in, const uint64_t* bdst, const uint64_t* bsrc, int nblk, R* out, R* tmp,
+                   const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w, hipStream_t s) {
+  if (nblk < 1) return fail(EG_ERR_ARG, "internal: a block view needs at least one block");
+  return rx_sort(RxIn<R>{in, nullptr, bdst, bsrc, nblk}, out, tmp, seg, key, key_bits, w, s);
 }
 
 }  // namespace eg
