@@ -278,32 +278,58 @@ __device__ __forceinline__ void rx_dstart(int nb, uint32_t tot, uint32_t* wt, ui
 
 // ---------------------------------------------------------- three-launch --
 // table index of (tile t of segment s, digit d): ts[s]*nb + d*nt + lt
+// Histogram order is free, so a full tile of 8-byte records at an even
+// record address is read as 16-byte pairs; each wave counts into its own
+// LDS histogram (fewer lanes of a skewed digit on one counter), and the
+// wave histograms are added at the end.
 template <int NT, int BM, class R, class KeyFn>
 __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __restrict__ seg,
                                               const uint32_t* __restrict__ ts, int nseg, KeyFn key, int shift,
                                               int bits, uint32_t* __restrict__ hist) {
   constexpr int TILE = NT * kRxIPT;
-  __shared__ uint32_t h[1 << BM];
+  constexpr int W = NT / kWave;
+  __shared__ uint32_t h[W][1 << BM];
   const int nb = 1 << bits;
-  for (int i = threadIdx.x; i < nb; i += NT) h[i] = 0;
+  for (int i = threadIdx.x; i < W * nb; i += NT) h[i / nb][i % nb] = 0;
   __syncthreads();
   const uint32_t t = rx_block_tile(blockIdx.x, gridDim.x);
   const RxTile x = rx_tile<TILE>(seg, ts, nseg, t);
   const int64_t sh = rx_tile_shift(in, t);
   const uint32_t mask = uint32_t(nb - 1);
+  uint32_t* hw = h[threadIdx.x / kWave];
   R r[kRxIPT];
+  bool pairs = false;
+  if constexpr (sizeof(R) == 8) {
+    pairs = x.count == uint32_t(TILE) && sh != kRxStraddle && ((int64_t(x.base) + sh) & 1) == 0;
+    if (pairs) {
+      const uint4* p = reinterpret_cast<const uint4*>(in.p + uint64_t(int64_t(x.base) + sh));
 #pragma unroll
-  for (int k = 0; k < kRxIPT; ++k) {
-    const uint32_t li = uint32_t(k) * NT + threadIdx.x;
-    r[k] = rx_load(in, sh, x.base + (li < x.count ? li : 0u));
+      for (int k = 0; k < kRxIPT / 2; ++k) {
+        const uint4 v = p[uint32_t(k) * NT + threadIdx.x];
+        __builtin_memcpy(&r[2 * k], &v, 16);
+      }
+    }
+  }
+  if (!pairs) {
+#pragma unroll
+    for (int k = 0; k < kRxIPT; ++k) {
+      const uint32_t li = uint32_t(k) * NT + threadIdx.x;
+      r[k] = rx_load(in, sh, x.base + (li < x.count ? li : 0u));
+    }
   }
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
     const uint32_t d = uint32_t(key(r[k]) >> shift) & mask;
-    if (uint32_t(k) * NT + threadIdx.x < x.count) atomicAdd(&h[d], 1u);
+    // full tiles count every record; otherwise record k of this thread is li
+    if (pairs || uint32_t(k) * NT + threadIdx.x < x.count) atomicAdd(&hw[d], 1u);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < nb; d += NT) hist[uint64_t(ts[x.s]) * nb + uint64_t(d) * x.nt + x.lt] = h[d];
+  for (int d = threadIdx.x; d < nb; d += NT) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int ww = 0; ww < W; ++ww) c += h[ww][d];
+    hist[uint64_t(ts[x.s]) * nb + uint64_t(d) * x.nt + x.lt] = c;
+  }
 }
 
 template <int NT, int BM, class R, class KeyFn>
