@@ -227,7 +227,7 @@ def main():
     variant = None
     if not args.no_autotune:
         sc, nt, bits, ms = g.autotune_sort(edges.ptr, E)  # untimed: every variant builds twice
-        variant = {"scheme": ["three-launch", "one-pass"][sc], "threads": nt, "tile_records": nt * 16,
+        variant = {"scheme": ["three-launch", "one-pass", "three-launch-xcd"][sc], "threads": nt, "tile_records": nt * 16,
                    "digit_bits": bits, "autotune_best_ms": round(ms, 2)}
         log(f"[bench] autotuned sort: {variant}")
     for i in range(args.warmup):
@@ -262,6 +262,23 @@ def main():
                 "frac": round(dk["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": int(dk["bytes_per_launch"]), "avg_launch_ms": round(dk["avg_ms"], 4),
                 "launches": dk["launches"]}
+    # what a plain streaming copy of the same bytes reaches on this device
+    # (read + write, 16-byte vectors): the practical ceiling of a pass that
+    # reads and writes every record once
+    half = int(dk["bytes_per_launch"]) // 2
+    ca, cb = eg.DeviceBuffer(half), eg.DeviceBuffer(half)
+    eg.memcpy_d2d(cb.ptr, ca.ptr, half)
+    eg.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        eg.memcpy_d2d(cb.ptr, ca.ptr, half)
+    eg.synchronize()
+    copy_gbs = 2 * half * 10 / (time.perf_counter() - t0) / 1e9
+    del ca, cb
+    roofline["copy_gbs"] = round(copy_gbs, 1)
+    roofline["frac_of_copy"] = round(dk["gbs"] / copy_gbs, 4)
+    log(f"[bench] {dom}: {dk['gbs']:.0f} GB/s = {dk['gbs'] / HBM_PEAK_GBS:.3f} of peak, "
+        f"{dk['gbs'] / copy_gbs:.3f} of a streaming copy ({copy_gbs:.0f} GB/s)")
 
     extra = {"kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in d.items()}
                          for k, d in kernels.items()}}

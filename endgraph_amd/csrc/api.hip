@@ -115,9 +115,9 @@ __global__ __launch_bounds__(kCopyNT) void k_copy16(const uint4* __restrict__ in
 bool prof_on() { return g_prof; }
 
 // Sort variant per record size (8 or 12 bytes): the autotuner's choice on
-// RMAT-24 P=8 (three launches, 512 threads, 8-bit digits), replaced by
-// eg_set_sort_variant / eg_autotune_sort.
-static SortVariant g_variant8{kThreeLaunch, 512, 8}, g_variant12{kThreeLaunch, 512, 8};
+// RMAT-24 P=8 (three launches in XCD order, 512 threads, 8-bit digits),
+// replaced by eg_set_sort_variant / eg_autotune_sort.
+static SortVariant g_variant8{kThreeLaunchXcd, 512, 8}, g_variant12{kThreeLaunchXcd, 512, 8};
 SortVariant& sort_variant(int rec_bytes) { return rec_bytes <= 8 ? g_variant8 : g_variant12; }
 
 void prof_begin(hipStream_t s) {
@@ -223,7 +223,8 @@ int eg_profile_read(const char* kernel, double* total_ms, uint64_t* launches, ui
 
 int eg_set_sort_variant(int rec_bytes, int scheme, int threads, int digit_bits) {
   if (rec_bytes != 8 && rec_bytes != 12) return fail(EG_ERR_ARG, "rec_bytes must be 8 or 12");
-  if (scheme != kThreeLaunch && scheme != kOnePass) return fail(EG_ERR_ARG, "scheme must be 0 or 1");
+  if (scheme != kThreeLaunch && scheme != kOnePass && scheme != kThreeLaunchXcd)
+    return fail(EG_ERR_ARG, "scheme must be 0, 1 or 2");
   if (threads != 256 && threads != 512 && threads != 1024) return fail(EG_ERR_ARG, "threads must be 256, 512 or 1024");
   if (rec_bytes == 12 && threads == 1024) return fail(EG_ERR_ARG, "12-byte records need threads <= 512 (LDS)");
   if (digit_bits != 8 && digit_bits != 9) return fail(EG_ERR_ARG, "digit_bits must be 8 or 9");

@@ -967,11 +967,12 @@ int eg_autotune_sort(eg_graph* g, const void* d_edges, uint64_t E0, int symmetri
   struct Cand {
     int scheme, nt, bits;
   };
-  std::vector<Cand> cands = {{kThreeLaunch, 256, 8}, {kThreeLaunch, 512, 8}, {kThreeLaunch, 512, 9},
-                             {kOnePass, 512, 8},     {kOnePass, 512, 9}};
-  if (g->rec == 8) {
-    cands.push_back({kThreeLaunch, 1024, 8});
-    cands.push_back({kOnePass, 1024, 8});
+  std::vector<Cand> cands;
+  for (int sc : {kThreeLaunch, kThreeLaunchXcd, kOnePass}) {
+    cands.push_back({sc, 512, 8});
+    cands.push_back({sc, 512, 9});
+    if (sc != kOnePass) cands.push_back({sc, 256, 8});
+    if (g->rec == 8) cands.push_back({sc, 1024, 8});
   }
   const SortVariant keep = sort_variant(g->rec);
   double best = 1e300;

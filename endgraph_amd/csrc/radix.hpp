@@ -36,7 +36,7 @@ constexpr int kRxMaxPasses = 8;
 constexpr uint32_t kRxSpinLimit = 1u << 22;
 constexpr uint64_t kFlagAgg = 1, kFlagInc = 2;
 
-enum SortScheme { kThreeLaunch = 0, kOnePass = 1 };
+enum SortScheme { kThreeLaunch = 0, kOnePass = 1, kThreeLaunchXcd = 2 };
 
 struct RxPlan {
   int passes = 0;
@@ -136,11 +136,12 @@ __device__ __forceinline__ RxTile rx_tile(const uint64_t* seg, const uint32_t* t
   return x;
 }
 
-// Tile of block b in the three-launch kernels.  Blocks go to the 8 XCDs in
-// turn (block b to XCD b % 8); numbering tiles so that each XCD takes one
-// contiguous range puts neighbouring tiles in the same L2, where the partial
-// lines they share (table entries of adjacent tiles, the ends of a digit's
-// runs) are combined before they are written back.
+// Tile of block b in the three-launch kernels with XCD order (scheme 2).
+// Blocks go to the 8 XCDs in turn (block b to XCD b % 8); numbering tiles so
+// that each XCD takes one contiguous range puts neighbouring tiles in the
+// same L2, where the partial lines they share (table entries of adjacent
+// tiles, the ends of a digit's runs) are combined before they are written
+// back.  Scheme 0 keeps tile = block.
 constexpr uint32_t kXcds = 8;
 
 __device__ __forceinline__ uint32_t rx_block_tile(uint32_t b, uint32_t tiles) {
@@ -285,14 +286,14 @@ __device__ __forceinline__ void rx_dstart(int nb, uint32_t tot, uint32_t* wt, ui
 template <int NT, int BM, class R, class KeyFn>
 __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __restrict__ seg,
                                               const uint32_t* __restrict__ ts, int nseg, KeyFn key, int shift,
-                                              int bits, uint32_t* __restrict__ hist) {
+                                              int bits, uint32_t* __restrict__ hist, int xcd_order) {
   constexpr int TILE = NT * kRxIPT;
   constexpr int W = NT / kWave;
   __shared__ uint32_t h[W][1 << BM];
   const int nb = 1 << bits;
   for (int i = threadIdx.x; i < W * nb; i += NT) h[i / nb][i % nb] = 0;
   __syncthreads();
-  const uint32_t t = rx_block_tile(blockIdx.x, gridDim.x);
+  const uint32_t t = xcd_order ? rx_block_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const RxTile x = rx_tile<TILE>(seg, ts, nseg, t);
   const int64_t sh = rx_tile_shift(in, t);
   const uint32_t mask = uint32_t(nb - 1);
@@ -336,7 +337,7 @@ template <int NT, int BM, class R, class KeyFn>
 __global__ __launch_bounds__(NT) void k_rx_down(RxIn<R> in, R* __restrict__ out,
                                                 const uint64_t* __restrict__ seg, const uint32_t* __restrict__ ts,
                                                 int nseg, KeyFn key, int shift, int bits,
-                                                const uint32_t* __restrict__ offs) {
+                                                const uint32_t* __restrict__ offs, int xcd_order) {
   constexpr int TILE = NT * kRxIPT;
   constexpr int W = NT / kWave;
   __shared__ uint32_t wcnt[W][1 << BM];
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(NT) void k_rx_down(RxIn<R> in, R* __restrict__ out,
   __shared__ __align__(16) R stage[TILE];
   const int nb = 1 << bits;
   for (int i = threadIdx.x; i < W * nb; i += NT) wcnt[i / nb][i % nb] = 0;
-  const uint32_t t = rx_block_tile(blockIdx.x, gridDim.x);
+  const uint32_t t = xcd_order ? rx_block_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const RxTile x = rx_tile<TILE>(seg, ts, nseg, t);
   for (int d = threadIdx.x; d < nb; d += NT) goff[d] = offs[uint64_t(ts[x.s]) * nb + uint64_t(d) * x.nt + x.lt];
   __syncthreads();
@@ -498,7 +499,8 @@ __global__ __attribute__((unused)) void k_rx_zero(uint32_t* __restrict__ p, uint
 }  // namespace
 
 // ------------------------------------------------------------------ host --
-// Sort variant: scheme x tile threads.  LDS per block ~ NT*16*rec bytes, so
+// Sort variant: scheme (three-launch in block order, one-pass, three-launch
+// in XCD order) x tile threads.  LDS per block ~ NT*16*rec bytes, so
 // 12-byte records stop at 512 threads (96 KB), 8-byte ones at 1024 (128 KB).
 struct SortVariant {
   int scheme = kThreeLaunch;
@@ -548,7 +550,8 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
     in.shift = w.tshift.as<int64_t>();
   }
   RxIn<R> src = in;
-  if (scheme == kThreeLaunch) {
+  if (scheme == kThreeLaunch || scheme == kThreeLaunchXcd) {
+    const int xcd = scheme == kThreeLaunchXcd;
     const uint64_t tn = uint64_t(tiles) << kRxMaxBits;
     EG_TRY(w.table.resize(tn * 4));
     if (w.scan.bytes < scan_tmp_elems(tn) * 4) EG_TRY(w.scan.resize(scan_tmp_elems(tn) * 4));
@@ -556,7 +559,7 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
       R* dst = ((plan.passes - 1 - p) % 2 == 0) ? out : tmp;
       prof_begin(s);
       hipLaunchKernelGGL((k_rx_up<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, (const uint64_t*)d_seg,
-                         (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p], w.table.as<uint32_t>());
+                         (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p], w.table.as<uint32_t>(), xcd);
       prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
       EG_HIP(hipGetLastError());
       EG_TRY(exclusive_scan_u32(w.table.as<uint32_t>(), w.table.as<uint32_t>(), uint64_t(tiles) << plan.bits[p],
@@ -564,7 +567,7 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
       prof_begin(s);
       hipLaunchKernelGGL((k_rx_down<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, (const uint64_t*)d_seg,
                          (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p],
-                         (const uint32_t*)w.table.as<uint32_t>());
+                         (const uint32_t*)w.table.as<uint32_t>(), xcd);
       prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
       EG_HIP(hipGetLastError());
       src = RxIn<R>{dst, nullptr, nullptr, nullptr, 0};

@@ -68,9 +68,10 @@ int eg_profile_reset(void);
 int eg_profile_read(const char* kernel, double* total_ms, uint64_t* launches, uint64_t* bytes);
 
 /* ---- radix-sort variant (no reference counterpart) ------------------------
- * Every stable sort of the build is a segmented LSD radix sort with two pass
- * schemes (0: upsweep/scan/downsweep, 1: one pass per digit with decoupled
- * look-back), a tile of threads x 16 records (threads 256, 512 or 1024;
+ * Every stable sort of the build is a segmented LSD radix sort with three
+ * pass schemes (0: upsweep/scan/downsweep with tiles in block order, 1: one
+ * pass per digit with decoupled look-back, 2: as 0 with each XCD taking one
+ * contiguous range of tiles), a tile of threads x 16 records (threads 256, 512 or 1024;
  * 1024 only for 8-byte records) and digits of up to 8 or 9 bits (9 only with
  * 512 threads).  Process-wide per record size; chosen by eg_autotune_sort or
  * set here.  Results never depend on the variant. */

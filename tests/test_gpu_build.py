@@ -95,7 +95,7 @@ def test_rebuild_reuses_handle(eg, oracle_lib):
 VARIANTS = [(256, 8), (512, 8), (512, 9), (1024, 8)]
 
 
-@pytest.mark.parametrize("scheme", [0, 1])
+@pytest.mark.parametrize("scheme", [0, 1, 2])
 @pytest.mark.parametrize("threads,bits", VARIANTS)
 @pytest.mark.parametrize("rec", [8, 12])
 def test_build_every_sort_variant(eg, oracle_lib, scheme, threads, bits, rec):
@@ -125,7 +125,7 @@ def test_autotune_picks_a_variant_and_stays_exact(eg, oracle_lib):
     keep = eg.get_sort_variant(8)
     try:
         scheme, threads, bits, ms = g.autotune_sort(buf.ptr, edges.shape[0])
-        assert scheme in (0, 1) and threads in (256, 512, 1024) and bits in (8, 9) and ms > 0
+        assert scheme in (0, 1, 2) and threads in (256, 512, 1024) and bits in (8, 9) and ms > 0
         og = oracle_lib.OracleGraph(edges, V, P=3, S=1, eta=1, m=1, threads=8)
         assert_build_equal(g, og)
     finally:

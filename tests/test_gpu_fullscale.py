@@ -1,8 +1,10 @@
 """Full-size checks, run only when EG_FULL_SCALE=1 (run with -s: each step
 prints):
-  * PageRank precision on the twitter-shaped graph (41,652,230 vertices,
-    1,468,365,182 edges, seed 3): compensated FP32 against the FP64 oracle over
-    the whole graph, 20 iterations, d = 0.85 (~80 GB of host memory, a few
+  * the twitter-shaped graph (41,652,230 vertices, 1,468,365,182 edges,
+    seed 3) at P = 1: the GPU build bit-exact against the oracle's build
+    (degrees, CSC and CSR lists, compressed CSC index, CSR index and bitmap),
+    and PageRank precision: compensated FP32 against the FP64 oracle over the
+    whole graph, 20 iterations, d = 0.85 (~110 GB of host memory, a few
     minutes of host CPU for the oracle);
   * personalised PageRank gradients on R-MAT scale 22 (weights U[0.1,1]
     row-normalised, p Dirichlet(1) on 1024 seeds, c ~ N(0,1)) against FP64
@@ -13,6 +15,8 @@ import time
 
 import numpy as np
 import pytest
+
+from parity import assert_build_equal
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(os.environ.get("EG_FULL_SCALE") != "1", reason="set EG_FULL_SCALE=1")]
@@ -35,12 +39,15 @@ def test_twitter_pagerank_l1_vs_fp64_oracle(oracle_lib):
     x32, _ = g.pagerank(20, eg.PR_FP32C)
     say("GPU PageRank FP64 and FP32C done")
     edges = buf.download(eg.REC8, E)
-    del g, buf
+    del buf
     say("edge list on the host")
     threads = min(16, os.cpu_count() or 1)
     og = oracle_lib.OracleGraph(edges, V, P=1, S=1, eta=1, m=20, threads=threads)
     del edges
     say(f"oracle build done ({og.build_seconds:.1f} s on {threads} threads)")
+    assert_build_equal(g, og)
+    del g
+    say("GPU build bit-exact against the oracle build")
     ref, _ = og.pagerank(20)
     say("oracle PageRank done")
     n1 = np.abs(ref).sum()
