@@ -286,6 +286,7 @@ def main():
     if not args.no_engine:
         prep_ms = g.prepare_engine()
         extra["engine_prepare_ms"] = round(prep_ms, 2)
+        extra["traversal_view_prepare_ms"] = round(g.prepare_traversal_view(), 2)
         deg = g.out_degree()
         root = int(np.argmax(deg))
         parent = eg.DeviceBuffer(V * 4)

@@ -117,6 +117,7 @@ def lib():
         "eg_pr_apply": ([vp, vp, vp, vp, i32, vp, vp], i32),
         "eg_pagerank": ([vp, i32, i32, vp, vp, vp], i32),
         "eg_pagerank_view_prepare": ([vp, C.POINTER(f64), vp], i32),
+        "eg_traversal_view_prepare": ([vp, C.POINTER(f64), vp], i32),
         "eg_bfs_level": ([vp, i32, u32, vp, vp, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64),
                           C.POINTER(i32), vp], i32),
         "eg_bfs": ([vp, u32, i32, vp, vp, C.POINTER(u32), C.POINTER(u64), vp], i32),
@@ -413,6 +414,12 @@ class Graph:
         """Derive the degree-ordered PageRank view now; returns its wall ms."""
         ms = C.c_double(0)
         check(lib().eg_pagerank_view_prepare(self._h, C.byref(ms), _stream(stream)))
+        return ms.value
+
+    def prepare_traversal_view(self, stream=None) -> float:
+        """Derive the traversal view (in-edges by destination, sources ascending) now; returns its wall ms."""
+        ms = C.c_double(0)
+        check(lib().eg_traversal_view_prepare(self._h, C.byref(ms), _stream(stream)))
         return ms.value
 
     def pagerank(self, iterations: int, precision: int = PR_FP64, out: DeviceBuffer = None, stream=None):

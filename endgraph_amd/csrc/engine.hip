@@ -657,25 +657,72 @@ __global__ void k_front_deg(const uint32_t* __restrict__ flist, uint32_t fn, con
     fdeg[i] = i < fn ? out_ptr[flist[i] + 1] - out_ptr[flist[i]] : 0u;
 }
 
-// Dense signal of toolkits/bfs.cpp:60-68 with R12: for an unvisited row, the
-// smallest in-neighbour in the frontier (all in-edges are scanned).
-__global__ __launch_bounds__(kNT) void k_bfs_pull(uint64_t R, const uint32_t* __restrict__ row_start,
-                                                   const uint32_t* __restrict__ row_dst,
-                                                   const uint32_t* __restrict__ src,
-                                                   const uint64_t* __restrict__ frontier,
-                                                   const uint64_t* __restrict__ visited, uint32_t V,
-                                                   uint32_t* __restrict__ parent, uint64_t* __restrict__ active_out) {
-  for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < R; r += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t v = row_dst[r];
-    if (bm_get(visited, v)) continue;
-    uint32_t best = V;
-    for (uint32_t k = row_start[r], e = row_start[r + 1]; k < e; ++k) {
-      const uint32_t u = src[k];
-      if (u < best && bm_get(frontier, u)) best = u;
+// Dense signal of toolkits/bfs.cpp:60-68 with R12 on the traversal view
+// (in-edges by destination, sources ascending): the first frontier source
+// of an unvisited vertex is its smallest, so a scan stops there.  Rows of at
+// most kTvSmallRow in-edges take one lane each (4 sources in flight per
+// step); larger rows are cut into work items of kTvPullEdges edges, one wave
+// each, and write_min over the items gives the row's smallest.
+__global__ __launch_bounds__(kNT) void k_bfs_pull_rows(uint32_t V, const uint32_t* __restrict__ in_ptr,
+                                                       const uint32_t* __restrict__ in_src,
+                                                       const uint64_t* __restrict__ frontier,
+                                                       const uint64_t* __restrict__ visited,
+                                                       uint32_t* __restrict__ parent,
+                                                       uint64_t* __restrict__ active_out) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x) {
+    if (bm_get(visited, uint32_t(v))) continue;
+    const uint32_t b = in_ptr[v], e = in_ptr[v + 1];
+    if (e - b > kTvSmallRow) continue;  // a wave item's row
+    for (uint32_t k = b; k < e; k += 4) {
+      uint32_t u[4];
+      bool f[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) u[j] = k + j < e ? in_src[k + j] : 0u;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) f[j] = k + j < e && bm_get(frontier, u[j]);
+      int hit = -1;
+#pragma unroll
+      for (int j = 3; j >= 0; --j)
+        if (f[j]) hit = j;
+      if (hit >= 0) {
+        parent[v] = u[hit];  // the row's only writer in this step
+        bm_set(active_out, uint32_t(v));
+        break;
+      }
     }
-    if (best < V) {
-      atomicMin(&parent[v], best);
-      bm_set(active_out, v);
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_bfs_pull_items(const uint64_t* __restrict__ items, uint64_t n_items,
+                                                        const uint32_t* __restrict__ in_ptr,
+                                                        const uint32_t* __restrict__ in_src,
+                                                        const uint64_t* __restrict__ frontier,
+                                                        const uint64_t* __restrict__ visited,
+                                                        uint32_t* __restrict__ parent,
+                                                        uint64_t* __restrict__ active_out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t i = gw; i < n_items; i += nw) {
+    const uint64_t it = items[i];
+    const uint32_t v = uint32_t(it >> 32);
+    if (bm_get(visited, v)) continue;  // wave-uniform
+    const uint32_t e0 = in_ptr[v] + uint32_t(it) * kTvPullEdges;
+    const uint32_t e1 = min(e0 + kTvPullEdges, in_ptr[v + 1]);
+    // an earlier item of the row already holds a smaller parent
+    if (parent[v] < in_src[e0]) continue;
+    for (uint32_t e = e0; e < e1; e += kWave) {
+      const bool ok = e + lane < e1;
+      const uint32_t u = ok ? in_src[e + lane] : 0u;
+      const uint64_t m = __ballot(ok && bm_get(frontier, u));
+      if (m) {
+        const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
+        if (lane == 0) {
+          atomicMin(&parent[v], p);
+          bm_set(active_out, v);
+        }
+        break;
+      }
     }
   }
 }
@@ -1032,6 +1079,13 @@ int eg_pagerank_view_prepare(eg_graph* g, double* h_ms, void* stream) {
   return EG_OK;
 }
 
+int eg_traversal_view_prepare(eg_graph* g, double* h_ms, void* stream) {
+  if (!g) return fail(EG_ERR_ARG, "graph is NULL");
+  EG_TRY(ensure_tview(g, static_cast<hipStream_t>(stream)));
+  if (h_ms) *h_ms = g->tv_prepare_ms;
+  return EG_OK;
+}
+
 // ---- BFS (toolkits/bfs.cpp) ----
 int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontier, uint64_t* d_next,
                  uint64_t* d_visited, uint32_t* d_parent, uint32_t* d_depth, uint64_t frontier_edges,
@@ -1049,12 +1103,18 @@ int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_fron
     EG_TRY(expand(g, sc, fn, total,
                   BfsPushOp{g->e_out_dst.as<uint32_t>(), (const uint64_t*)d_visited, d_parent, d_next}, s));
     prof_end(PK_BFS, uint64_t(total) * 4 + uint64_t(fn) * 8, s);
-  } else if (g->R) {
-    hipLaunchKernelGGL(k_bfs_pull, dim3(grid_for(g->R, kNT, 65536)), dim3(kNT), 0, s, g->R,
-                       (const uint32_t*)g->e_row_start.as<uint32_t>(), (const uint32_t*)g->e_row_dst.as<uint32_t>(),
-                       (const uint32_t*)g->e_csc_src.as<uint32_t>(), d_frontier, (const uint64_t*)d_visited, g->V,
-                       d_parent, d_next);
-    prof_end(PK_BFS, g->R * 12, s);
+  } else if (g->E) {
+    EG_TRY(ensure_tview(g, s));
+    const uint32_t* in_ptr = g->tv_ptr.as<uint32_t>();
+    const uint32_t* in_src = g->tv_src.as<uint32_t>();
+    hipLaunchKernelGGL(k_bfs_pull_rows, dim3(grid_for(g->V, kNT, 65536)), dim3(kNT), 0, s, g->V, in_ptr, in_src,
+                       d_frontier, (const uint64_t*)d_visited, d_parent, d_next);
+    EG_HIP(hipGetLastError());
+    if (g->tv_nbig)
+      hipLaunchKernelGGL(k_bfs_pull_items, dim3(grid_for(g->tv_nbig, kNT / kWave, 65536)), dim3(kNT), 0, s,
+                         (const uint64_t*)g->tv_big.as<uint64_t>(), g->tv_nbig, in_ptr, in_src, d_frontier,
+                         (const uint64_t*)d_visited, d_parent, d_next);
+    prof_end(PK_BFS, uint64_t(g->V) * 12, s);
   }
   EG_HIP(hipGetLastError());
   if (h_was_sparse) *h_was_sparse = sparse ? 1 : 0;

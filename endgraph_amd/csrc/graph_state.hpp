@@ -101,6 +101,9 @@ struct eg_graph {
   eg::DevBuf tv_w;                   // f32[E] (12-byte records only)
   eg::DevBuf tv_items;               // u64 (v << 32 | chunk): in-edge chunks of at most 512 edges
   uint64_t tv_nitems = 0;
+  eg::DevBuf tv_big;                 // the same, of the vertices with more than kTvSmallRow in-edges
+  uint64_t tv_nbig = 0;
+  double tv_prepare_ms = 0;
   // transposed view for the PPR adjoint: CSC slots grouped by source
   bool ad_ready = false;
   uint32_t ad_R = 0;
@@ -140,4 +143,11 @@ struct eg_graph {
 namespace eg {
 // Derives the engine's view after a build (first engine call; engine.hip).
 int engine_ensure(eg_graph* g, hipStream_t s);
+// Rows of in-edges by destination with sources ascending and the pull work
+// items over them (first traversal that needs it; multi.hip).
+int ensure_tview(eg_graph* g, hipStream_t s);
+// In-edges per pull work item, and the in-degree up to which a single-root
+// BFS pull scans a row in one lane instead.
+constexpr uint32_t kTvPullEdges = 512;
+constexpr uint32_t kTvSmallRow = 32;
 }  // namespace eg

@@ -24,6 +24,7 @@
 // level that reaches v), depths are levels, SSSP distances are the unique
 // fixpoint of dist[v] = min(dist[v], dist[u] + w) with fp32 adds.
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "graph_state.hpp"
@@ -33,7 +34,7 @@ namespace {
 
 constexpr int kNT = 256;
 constexpr uint32_t kItemEdges = 4096;  // out-edges per push work item
-constexpr uint32_t kPullEdges = 512;   // in-edges per pull work item (SSSP)
+constexpr uint32_t kPullEdges = kTvPullEdges;  // in-edges per pull work item
 constexpr uint32_t kUnreached = 0xffffffffu;
 constexpr float kInfDist = 1e9f;       // toolkits/sssp.cpp INF
 
@@ -107,10 +108,12 @@ __global__ void k_copy_u32(const uint32_t* __restrict__ a, uint32_t* __restrict_
     b[i] = a[i];
 }
 
-// Pull work items: in-edge chunks of at most kPullEdges edges per vertex.
-__global__ void k_tv_item_count(const uint32_t* __restrict__ in_deg, uint32_t V, uint32_t* __restrict__ cnt) {
+// Pull work items: in-edge chunks of at most kPullEdges edges per vertex
+// (only of the vertices with more than min_deg in-edges).
+__global__ void k_tv_item_count(const uint32_t* __restrict__ in_deg, uint32_t V, uint32_t min_deg,
+                                uint32_t* __restrict__ cnt) {
   for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
-    cnt[v] = uint32_t(ceil_div(in_deg[v], kPullEdges));
+    cnt[v] = in_deg[v] > min_deg ? uint32_t(ceil_div(in_deg[v], kPullEdges)) : 0u;
 }
 
 __global__ void k_tv_items(const uint32_t* __restrict__ off, uint32_t V, uint64_t* __restrict__ items) {
@@ -132,22 +135,24 @@ int tview_impl(eg_graph* g, hipStream_t s) {
   EG_HIP(hipGetLastError());
   EG_TRY(fill_t<uint32_t>(ptr + V, 1, 0u, s));
   EG_TRY(exclusive_scan_u32(ptr, ptr, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
-  {
+  // items of every row (batched programs) and of the rows above
+  // kTvSmallRow in-edges (single-root BFS pull; smaller rows go one per lane)
+  for (int list = 0; list < 2; ++list) {
     DevBuf off;
     EG_TRY(off.resize((uint64_t(V) + 1) * 4));
     uint32_t* o = off.as<uint32_t>();
     hipLaunchKernelGGL(k_tv_item_count, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)g->in_deg.as<uint32_t>(),
-                       V, o);
+                       V, list == 0 ? 0u : kTvSmallRow, o);
     EG_HIP(hipGetLastError());
     EG_TRY(fill_t<uint32_t>(o + V, 1, 0u, s));
     EG_TRY(exclusive_scan_u32(o, o, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
     uint32_t n = 0;
     EG_HIP(hipMemcpyAsync(&n, o + V, 4, hipMemcpyDeviceToHost, s));
     EG_HIP(hipStreamSynchronize(s));
-    g->tv_nitems = n;
-    EG_TRY(g->tv_items.resize(std::max<uint64_t>(n, 1) * 8));
-    hipLaunchKernelGGL(k_tv_items, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)o, V,
-                       g->tv_items.as<uint64_t>());
+    DevBuf& items = list == 0 ? g->tv_items : g->tv_big;
+    (list == 0 ? g->tv_nitems : g->tv_nbig) = n;
+    EG_TRY(items.resize(std::max<uint64_t>(n, 1) * 8));
+    hipLaunchKernelGGL(k_tv_items, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)o, V, items.as<uint64_t>());
     EG_HIP(hipGetLastError());
     EG_HIP(hipStreamSynchronize(s));  // `off` is freed on scope exit
   }
@@ -173,13 +178,19 @@ int tview_impl(eg_graph* g, hipStream_t s) {
   return EG_OK;
 }
 
+}  // namespace
+
 int ensure_tview(eg_graph* g, hipStream_t s) {
   EG_TRY(engine_ensure(g, s));
   if (g->tv_ready) return EG_OK;
+  const auto t0 = std::chrono::steady_clock::now();
   EG_TRY(g->rec == 8 ? tview_impl<Rec8>(g, s) : tview_impl<Rec12>(g, s));
+  g->tv_prepare_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   g->tv_ready = true;
   return EG_OK;
 }
+
+namespace {
 
 // ------------------------------------------------------------ shared state --
 struct MultiState {

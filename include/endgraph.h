@@ -250,12 +250,17 @@ int eg_pagerank_view_prepare(eg_graph* g, double* h_ms, void* stream);
  * d_visited |= d_next, d_depth (optional) = level+1 for them;
  * frontier_edges = sum of out-degrees of the frontier (mode switch input);
  * returns the new vertex count and their out-edge sum.  eg_bfs runs from
- * root; h_edges gets the number of out-edges of reached vertices. */
+ * root; h_edges gets the number of out-edges of reached vertices.  Dense
+ * (pull) steps and the batched programs read the traversal view: in-edges
+ * by destination with sources ascending, derived on first use after a
+ * build; eg_traversal_view_prepare derives it now (h_ms gets its wall time)
+ * so that its one-time cost is kept out of a timed run. */
 int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontier, uint64_t* d_next,
                  uint64_t* d_visited, uint32_t* d_parent, uint32_t* d_depth, uint64_t frontier_edges,
                  uint64_t* h_new_vertices, uint64_t* h_new_edges, int* h_was_sparse, void* stream);
 int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t* d_depth, uint32_t* h_levels,
            uint64_t* h_edges, void* stream);
+int eg_traversal_view_prepare(eg_graph* g, double* h_ms, void* stream);
 
 /* CC, toolkits/cc.cpp:22-112 (graph built with symmetrize=1): label =
  * smallest id of the component.  One iteration or the whole loop. */
