@@ -509,30 +509,56 @@ __global__ void k_bm_or(uint64_t* __restrict__ dst, const uint64_t* __restrict__
 
 // Level bookkeeping: visited |= out; depth of new vertices; counts of new
 // vertices and of their out-edges (for the next mode decision).
+// A wave loads 64 words at once (one per lane: the visited update and the
+// count are per lane), then walks the non-empty ones: lane b holds vertex
+// 64*word + b, so the depth stores and degree loads of a word are one
+// coalesced access each, and empty words cost nothing more.
 __global__ __launch_bounds__(kNT) void k_level_end(const uint64_t* __restrict__ out, uint64_t* __restrict__ visited,
                                                     uint64_t words, uint32_t* __restrict__ depth, uint32_t level,
                                                     const uint32_t* __restrict__ deg,
                                                     unsigned long long* __restrict__ counters) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
   unsigned long long nv = 0, ne = 0;
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += uint64_t(gridDim.x) * blockDim.x) {
-    uint64_t w = out[i];
-    if (!w) continue;
-    if (visited) visited[i] |= w;
-    nv += __popcll(w);
+  for (uint64_t base = gw * kWave; base < words; base += nw * kWave) {
+    const uint64_t wi = base + lane;
+    const uint64_t mine = wi < words ? out[wi] : 0ull;
+    if (mine) {
+      if (visited) visited[wi] |= mine;
+      nv += __popcll(mine);
+    }
     if (!depth && !deg) continue;
-    while (w) {
-      const int b = __ffsll((unsigned long long)w) - 1;
-      const uint32_t v = uint32_t(i * 64 + b);
-      if (depth) depth[v] = level;
-      if (deg) ne += deg[v];
-      w &= w - 1;
+    uint64_t nz = __ballot(mine != 0);
+    while (nz) {  // wave-uniform
+      const int j = __ffsll((unsigned long long)nz) - 1;
+      nz &= nz - 1;
+      const uint64_t w = shfl_t(mine, j);
+      if ((w >> lane) & 1ull) {
+        const uint64_t v = (base + j) * 64 + lane;
+        if (depth) depth[v] = level;
+        if (deg) ne += deg[v];
+      }
     }
   }
+  // one atomic pair per block (every wave adding its own serialised the
+  // counters' cache line)
+  __shared__ unsigned long long part[2][kNT / kWave];
   nv = wave_reduce(nv, OpSum{});
   ne = wave_reduce(ne, OpSum{});
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    if (nv) atomicAdd(&counters[0], nv);
-    if (ne) atomicAdd(&counters[1], ne);
+  if (lane == 0) {
+    part[0][threadIdx.x / kWave] = nv;
+    part[1][threadIdx.x / kWave] = ne;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, b = 0;
+    for (int k = 0; k < kNT / kWave; ++k) {
+      a += part[0][k];
+      b += part[1][k];
+    }
+    if (a) atomicAdd(&counters[0], a);
+    if (b) atomicAdd(&counters[1], b);
   }
 }
 
@@ -662,7 +688,9 @@ __global__ void k_front_deg(const uint32_t* __restrict__ flist, uint32_t fn, con
 // of an unvisited vertex is its smallest, so a scan stops there.  Rows of at
 // most kTvSmallRow in-edges take one lane each (4 sources in flight per
 // step); larger rows are cut into work items of kTvPullEdges edges, one wave
-// each, and write_min over the items gives the row's smallest.
+// each, and write_min over the items gives the row's smallest.  A row's
+// first item runs in an earlier launch than its others, which skip when the
+// first found a source (it is smaller than all of theirs).
 __global__ __launch_bounds__(kNT) void k_bfs_pull_rows(uint32_t V, const uint32_t* __restrict__ in_ptr,
                                                        const uint32_t* __restrict__ in_src,
                                                        const uint64_t* __restrict__ frontier,
@@ -805,7 +833,12 @@ int get_scratch(eg_graph* g, int iters, Scratch* sc) {
 
 // Ascending list of the set bits of bm and the prefix of their out-degrees;
 // returns the count and the total degree on the host.
-int make_frontier(eg_graph* g, const uint64_t* bm, Scratch& sc, uint32_t* h_n, uint32_t* h_total, hipStream_t s) {
+// Frontier list and degree prefix of the set bits of bm.  known_n >= 0 is the
+// caller's exact count of set bits (from the previous level end): then
+// nothing is read back and *h_total is left unchanged (the caller knows it
+// too); otherwise both counts are read back.
+int make_frontier(eg_graph* g, const uint64_t* bm, Scratch& sc, uint32_t* h_n, uint32_t* h_total, hipStream_t s,
+                  int64_t known_n = -1) {
   const uint64_t words = words_of(g->V);
   hipLaunchKernelGGL(k_bm_count, dim3(grid_for(words, kNT, 65536)), dim3(kNT), 0, s, bm, words, sc.wcnt);
   EG_HIP(hipGetLastError());
@@ -814,15 +847,21 @@ int make_frontier(eg_graph* g, const uint64_t* bm, Scratch& sc, uint32_t* h_n, u
   hipLaunchKernelGGL(k_bm_list, dim3(grid_for(words, kNT, 65536)), dim3(kNT), 0, s, bm, words,
                      (const uint32_t*)sc.wcnt, sc.flist);
   EG_HIP(hipGetLastError());
-  EG_HIP(hipMemcpyAsync(h_n, sc.wcnt + words, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
+  if (known_n >= 0) {
+    *h_n = uint32_t(known_n);
+  } else {
+    EG_HIP(hipMemcpyAsync(h_n, sc.wcnt + words, 4, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+  }
   const uint32_t fn = *h_n;
   hipLaunchKernelGGL(k_front_deg, dim3(grid_for(uint64_t(fn) + 1, kNT, 65536)), dim3(kNT), 0, s,
                      (const uint32_t*)sc.flist, fn, (const uint32_t*)g->e_out_ptr.as<uint32_t>(), sc.foff);
   EG_HIP(hipGetLastError());
   EG_TRY(exclusive_scan_u32(sc.foff, sc.foff, uint64_t(fn) + 1, g->scan_tmp.as<uint32_t>(), s));
-  EG_HIP(hipMemcpyAsync(h_total, sc.foff + fn, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
+  if (known_n < 0) {
+    EG_HIP(hipMemcpyAsync(h_total, sc.foff + fn, 4, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+  }
   return EG_OK;
 }
 
@@ -842,7 +881,7 @@ int level_end(eg_graph* g, const uint64_t* out, uint64_t* visited, uint32_t* dep
               uint64_t* h_nv, uint64_t* h_ne, hipStream_t s) {
   const uint64_t words = words_of(g->V);
   EG_TRY(fill<unsigned long long>(sc.counters, 2, 0ull, s));
-  hipLaunchKernelGGL(k_level_end, dim3(grid_for(words, kNT, 8192)), dim3(kNT), 0, s, out, visited, words, depth, level,
+  hipLaunchKernelGGL(k_level_end, dim3(grid_for(words, kNT, 1024)), dim3(kNT), 0, s, out, visited, words, depth, level,
                      (const uint32_t*)g->out_deg.as<uint32_t>(), sc.counters);
   EG_HIP(hipGetLastError());
   unsigned long long h[2];
@@ -1086,11 +1125,16 @@ int eg_traversal_view_prepare(eg_graph* g, double* h_ms, void* stream) {
   return EG_OK;
 }
 
+}  // extern "C"
+
+namespace eg {
 // ---- BFS (toolkits/bfs.cpp) ----
-int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontier, uint64_t* d_next,
-                 uint64_t* d_visited, uint32_t* d_parent, uint32_t* d_depth, uint64_t frontier_edges,
-                 uint64_t* h_new_vertices, uint64_t* h_new_edges, int* h_was_sparse, void* stream) {
-  hipStream_t s = static_cast<hipStream_t>(stream);
+// One level.  frontier_n >= 0: the exact frontier size, and frontier_edges
+// its exact out-degree sum (both from the previous level end, as in eg_bfs's
+// loop), so the push step reads nothing back.
+int bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontier, uint64_t* d_next,
+              uint64_t* d_visited, uint32_t* d_parent, uint32_t* d_depth, uint64_t frontier_edges, int64_t frontier_n,
+              uint64_t* h_new_vertices, uint64_t* h_new_edges, int* h_was_sparse, hipStream_t s) {
   EG_TRY(ensure_engine(g, s));
   Scratch sc;
   EG_TRY(get_scratch(g, 0, &sc));
@@ -1098,8 +1142,8 @@ int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_fron
   EG_TRY(fill<uint64_t>(d_next, words_of(g->V), 0ull, s));
   prof_begin(s);
   if (sparse) {
-    uint32_t fn = 0, total = 0;
-    EG_TRY(make_frontier(g, d_frontier, sc, &fn, &total, s));
+    uint32_t fn = 0, total = uint32_t(frontier_edges);
+    EG_TRY(make_frontier(g, d_frontier, sc, &fn, &total, s, frontier_n));
     EG_TRY(expand(g, sc, fn, total,
                   BfsPushOp{g->e_out_dst.as<uint32_t>(), (const uint64_t*)d_visited, d_parent, d_next}, s));
     prof_end(PK_BFS, uint64_t(total) * 4 + uint64_t(fn) * 8, s);
@@ -1110,15 +1154,31 @@ int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_fron
     hipLaunchKernelGGL(k_bfs_pull_rows, dim3(grid_for(g->V, kNT, 65536)), dim3(kNT), 0, s, g->V, in_ptr, in_src,
                        d_frontier, (const uint64_t*)d_visited, d_parent, d_next);
     EG_HIP(hipGetLastError());
+    // first chunks of the large rows, then the others, which an earlier
+    // chunk's hit (always smaller: sources ascend) lets skip
     if (g->tv_nbig)
       hipLaunchKernelGGL(k_bfs_pull_items, dim3(grid_for(g->tv_nbig, kNT / kWave, 65536)), dim3(kNT), 0, s,
                          (const uint64_t*)g->tv_big.as<uint64_t>(), g->tv_nbig, in_ptr, in_src, d_frontier,
+                         (const uint64_t*)d_visited, d_parent, d_next);
+    if (g->tv_nbig_rest)
+      hipLaunchKernelGGL(k_bfs_pull_items, dim3(grid_for(g->tv_nbig_rest, kNT / kWave, 65536)), dim3(kNT), 0, s,
+                         (const uint64_t*)g->tv_big_rest.as<uint64_t>(), g->tv_nbig_rest, in_ptr, in_src, d_frontier,
                          (const uint64_t*)d_visited, d_parent, d_next);
     prof_end(PK_BFS, uint64_t(g->V) * 12, s);
   }
   EG_HIP(hipGetLastError());
   if (h_was_sparse) *h_was_sparse = sparse ? 1 : 0;
   return level_end(g, d_next, d_visited, d_depth, level + 1, sc, h_new_vertices, h_new_edges, s);
+}
+}  // namespace eg
+
+extern "C" {
+
+int eg_bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontier, uint64_t* d_next,
+                 uint64_t* d_visited, uint32_t* d_parent, uint32_t* d_depth, uint64_t frontier_edges,
+                 uint64_t* h_new_vertices, uint64_t* h_new_edges, int* h_was_sparse, void* stream) {
+  return bfs_level(g, policy, level, d_frontier, d_next, d_visited, d_parent, d_depth, frontier_edges, -1,
+                   h_new_vertices, h_new_edges, h_was_sparse, static_cast<hipStream_t>(stream));
 }
 
 int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t* d_depth, uint32_t* h_levels,
@@ -1147,7 +1207,7 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
   uint64_t nv = 1, ne = deg_root, traversed = deg_root;
   uint32_t level = 0;
   while (nv > 0) {
-    EG_TRY(eg_bfs_level(g, policy, level, fr, nx, vis, d_parent, d_depth, ne, &nv, &ne, nullptr, stream));
+    EG_TRY(bfs_level(g, policy, level, fr, nx, vis, d_parent, d_depth, ne, int64_t(nv), &nv, &ne, nullptr, s));
     traversed += ne;
     ++level;
     std::swap(fr, nx);

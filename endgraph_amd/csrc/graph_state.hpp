@@ -101,8 +101,10 @@ struct eg_graph {
   eg::DevBuf tv_w;                   // f32[E] (12-byte records only)
   eg::DevBuf tv_items;               // u64 (v << 32 | chunk): in-edge chunks of at most 512 edges
   uint64_t tv_nitems = 0;
-  eg::DevBuf tv_big;                 // the same, of the vertices with more than kTvSmallRow in-edges
+  eg::DevBuf tv_big;                 // first chunks of the vertices with more than kTvSmallRow in-edges
   uint64_t tv_nbig = 0;
+  eg::DevBuf tv_big_rest;            // their other chunks
+  uint64_t tv_nbig_rest = 0;
   double tv_prepare_ms = 0;
   // transposed view for the PPR adjoint: CSC slots grouped by source
   bool ad_ready = false;

@@ -108,17 +108,21 @@ __global__ void k_copy_u32(const uint32_t* __restrict__ a, uint32_t* __restrict_
     b[i] = a[i];
 }
 
-// Pull work items: in-edge chunks of at most kPullEdges edges per vertex
-// (only of the vertices with more than min_deg in-edges).
-__global__ void k_tv_item_count(const uint32_t* __restrict__ in_deg, uint32_t V, uint32_t min_deg,
-                                uint32_t* __restrict__ cnt) {
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
-    cnt[v] = in_deg[v] > min_deg ? uint32_t(ceil_div(in_deg[v], kPullEdges)) : 0u;
+// Pull work items: in-edge chunks [c0, c1) of at most kPullEdges edges per
+// vertex, of the vertices with more than min_deg in-edges; chunks c0.. up
+// to the last (c1 = 0) or to c1.
+__global__ void k_tv_item_count(const uint32_t* __restrict__ in_deg, uint32_t V, uint32_t min_deg, uint32_t c0,
+                                uint32_t c1, uint32_t* __restrict__ cnt) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x) {
+    uint32_t n = in_deg[v] > min_deg ? uint32_t(ceil_div(in_deg[v], kPullEdges)) : 0u;
+    if (c1 && n > c1) n = c1;
+    cnt[v] = n > c0 ? n - c0 : 0u;
+  }
 }
 
-__global__ void k_tv_items(const uint32_t* __restrict__ off, uint32_t V, uint64_t* __restrict__ items) {
+__global__ void k_tv_items(const uint32_t* __restrict__ off, uint32_t V, uint32_t c0, uint64_t* __restrict__ items) {
   for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
-    for (uint32_t j = 0; j < off[v + 1] - off[v]; ++j) items[off[v] + j] = (v << 32) | j;
+    for (uint32_t j = 0; j < off[v + 1] - off[v]; ++j) items[off[v] + j] = (v << 32) | (c0 + j);
 }
 
 template <class R>
@@ -135,24 +139,33 @@ int tview_impl(eg_graph* g, hipStream_t s) {
   EG_HIP(hipGetLastError());
   EG_TRY(fill_t<uint32_t>(ptr + V, 1, 0u, s));
   EG_TRY(exclusive_scan_u32(ptr, ptr, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
-  // items of every row (batched programs) and of the rows above
-  // kTvSmallRow in-edges (single-root BFS pull; smaller rows go one per lane)
-  for (int list = 0; list < 2; ++list) {
+  // items of every row (batched programs); of the rows above kTvSmallRow
+  // in-edges (single-root BFS pull; smaller rows go one per lane): their
+  // first chunks, then the others
+  struct List {
+    uint32_t min_deg, c0, c1;
+    DevBuf* items;
+    uint64_t* n;
+  };
+  const List lists[3] = {{0u, 0u, 0u, &g->tv_items, &g->tv_nitems},
+                         {kTvSmallRow, 0u, 1u, &g->tv_big, &g->tv_nbig},
+                         {kTvSmallRow, 1u, 0u, &g->tv_big_rest, &g->tv_nbig_rest}};
+  for (const List& l : lists) {
     DevBuf off;
     EG_TRY(off.resize((uint64_t(V) + 1) * 4));
     uint32_t* o = off.as<uint32_t>();
     hipLaunchKernelGGL(k_tv_item_count, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)g->in_deg.as<uint32_t>(),
-                       V, list == 0 ? 0u : kTvSmallRow, o);
+                       V, l.min_deg, l.c0, l.c1, o);
     EG_HIP(hipGetLastError());
     EG_TRY(fill_t<uint32_t>(o + V, 1, 0u, s));
     EG_TRY(exclusive_scan_u32(o, o, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
     uint32_t n = 0;
     EG_HIP(hipMemcpyAsync(&n, o + V, 4, hipMemcpyDeviceToHost, s));
     EG_HIP(hipStreamSynchronize(s));
-    DevBuf& items = list == 0 ? g->tv_items : g->tv_big;
-    (list == 0 ? g->tv_nitems : g->tv_nbig) = n;
-    EG_TRY(items.resize(std::max<uint64_t>(n, 1) * 8));
-    hipLaunchKernelGGL(k_tv_items, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)o, V, items.as<uint64_t>());
+    *l.n = n;
+    EG_TRY(l.items->resize(std::max<uint64_t>(n, 1) * 8));
+    hipLaunchKernelGGL(k_tv_items, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)o, V, l.c0,
+                       static_cast<uint64_t*>(l.items->ptr));
     EG_HIP(hipGetLastError());
     EG_HIP(hipStreamSynchronize(s));  // `off` is freed on scope exit
   }
