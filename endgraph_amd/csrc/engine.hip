@@ -387,18 +387,6 @@ struct OpF2 {
   __device__ __forceinline__ F2 operator()(F2 a, F2 b) const { return f2_add(a, b); }
 };
 
-}  // namespace
-
-template <>
-__device__ __forceinline__ F2 shfl_up_t<F2>(F2 v, unsigned d) {
-  return F2{__shfl_up(v.s, d, kWave), __shfl_up(v.c, d, kWave)};
-}
-template <>
-__device__ __forceinline__ F2 shfl_t<F2>(F2 v, int lane) {
-  return F2{__shfl(v.s, lane, kWave), __shfl(v.c, lane, kWave)};
-}
-
-namespace {
 
 // --------------------------------------------------------------- PageRank
 // dense_signal/dense_slot of toolkits/pagerank.cpp:64-75: next[dst] += sum

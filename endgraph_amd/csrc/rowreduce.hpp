@@ -21,19 +21,66 @@ __device__ __forceinline__ uint64_t first_word() {
   return (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave * kRowWordsPerWave;
 }
 
-template <class T, class Op>
-__device__ __forceinline__ T seg_scan(T v, bool head, Op op) {
-  const int lane = threadIdx.x & (kWave - 1);
-  bool h = head;
+// Lane value moved by a DPP control (gfx9 wave64: row_shr:n = 0x110 + n
+// within rows of 16 lanes, row_bcast:15 = 0x142 and row_bcast:31 = 0x143
+// across rows).  Lanes whose source is outside the row, or whose row is not
+// in row_mask, get `old`.  T is 4 or 8 bytes (moved as 32-bit halves).
+template <int kCtrl, int kRowMask, class T>
+__device__ __forceinline__ T dpp_move(T old, T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte lane values");
+  int o[sizeof(T) / 4], x[sizeof(T) / 4], r[sizeof(T) / 4];
+  __builtin_memcpy(o, &old, sizeof(T));
+  __builtin_memcpy(x, &v, sizeof(T));
 #pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const T ov = shfl_up_t(v, d);
-    const bool oh = __shfl_up(int(h), d, kWave) != 0;
-    if (lane >= d && !h) {
-      v = op(ov, v);
-      h = oh;
-    }
-  }
+  for (int i = 0; i < int(sizeof(T) / 4); ++i) r[i] = __builtin_amdgcn_update_dpp(o[i], x[i], kCtrl, kRowMask, 0xF, false);
+  T out;
+  __builtin_memcpy(&out, r, sizeof(T));
+  return out;
+}
+
+// Value of one lane (wave-uniform lane index) in every lane, through the
+// scalar unit.
+template <class T>
+__device__ __forceinline__ T lane_value(T v, int lane) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit multiple");
+  int x[sizeof(T) / 4];
+  __builtin_memcpy(x, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 4); ++i) x[i] = __builtin_amdgcn_readlane(x[i], lane);
+  T out;
+  __builtin_memcpy(&out, x, sizeof(T));
+  return out;
+}
+
+// Inclusive segmented scan over the wave: start = the lane of the segment's
+// head at or below this lane (-1: the segment began before lane 0).  Shifts
+// by 1, 2, 4, 8 inside rows of 16 lanes, then the last lane of row 0 / 2 is
+// broadcast to rows 1 / 3 and the last lane of row 1 to rows 2 and 3; a lane
+// takes a moved value only when its source lies in its own segment.  DPP
+// moves stay in the VALU (no LDS round trip), and the takes are selects, so
+// the scan is branch-free.
+template <class T, class Op>
+__device__ __forceinline__ T seg_scan(T v, int start, T identity, Op op) {
+  const int lane = threadIdx.x & (kWave - 1);
+  T t, r;
+  t = dpp_move<0x111, 0xF>(identity, v);
+  r = op(t, v);
+  v = lane - 1 >= start ? r : v;
+  t = dpp_move<0x112, 0xF>(identity, v);
+  r = op(t, v);
+  v = lane - 2 >= start ? r : v;
+  t = dpp_move<0x114, 0xF>(identity, v);
+  r = op(t, v);
+  v = lane - 4 >= start ? r : v;
+  t = dpp_move<0x118, 0xF>(identity, v);
+  r = op(t, v);
+  v = lane - 8 >= start ? r : v;
+  t = dpp_move<0x142, 0xA>(identity, v);
+  r = op(t, v);
+  v = (lane & 16) && start < (lane & ~15) ? r : v;
+  t = dpp_move<0x143, 0xC>(identity, v);
+  r = op(t, v);
+  v = (lane & 32) && start < 32 ? r : v;
   return v;
 }
 
@@ -42,60 +89,109 @@ __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const u
                                                 const uint64_t* __restrict__ rowflag,
                                                 const uint32_t* __restrict__ wordrank, T identity, Op op, Val val,
                                                 Emit emit) {
+  // The wave's K words are taken in rounds of 4 words (256 edges); in a
+  // round lane l holds the 4 consecutive edges 4l..4l+3 (one 16-byte load
+  // of sources), reduces them serially, hands the open end of its edges to
+  // one segmented scan across the lanes, and emits the rows that end in it.
   constexpr int K = kRowWordsPerWave;
+  constexpr int IPL = 4;                 // edges per lane per round
+  constexpr int RW = kWave * IPL / 64;   // words per round
+  constexpr int RR = K / RW;             // rounds
+  static_assert(K % RW == 0, "whole rounds");
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t nwords = ceil_div(E, 64);
   if (w0 >= nwords) return;
   const int nwd = w0 + K < nwords ? K : int(nwords - w0);
-  // all row bits, sources and gathered values of the wave's range first, so
-  // every load of the range is in flight before the first use
-  uint64_t f[K];
-  uint32_t base[K], sv[K];
-  T v[K];
+  const int lw = lane / (64 / IPL);       // word of this lane's edges within a round
+  const int pos0 = (lane % (64 / IPL)) * IPL;  // bit of its first edge in that word
+  const bool vec = (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+  // every load of the range first, so all are in flight before the first use
+  uint64_t f[RR];
+  uint32_t base[RR], sv[RR][IPL];
+  T v[RR][IPL];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
+  for (int r = 0; r < RR; ++r) {
+    const int k = r * RW + lw;
     const uint64_t wd = w0 + k;
-    const uint64_t e = wd * 64 + lane;
     const bool ok = k < nwd;
-    f[k] = ok ? rowflag[wd] : 0ull;
-    base[k] = ok ? wordrank[wd] : 0u;
-    sv[k] = ok && e < E ? src[e] : 0u;
+    f[r] = ok ? rowflag[wd] : 0ull;
+    base[r] = ok ? wordrank[wd] : 0u;
+    const uint64_t e0 = wd * 64 + pos0;
+    if (ok && vec && e0 + IPL <= E) {
+      const uint4 q = *reinterpret_cast<const uint4*>(src + e0);
+      sv[r][0] = q.x;
+      sv[r][1] = q.y;
+      sv[r][2] = q.z;
+      sv[r][3] = q.w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < IPL; ++i) sv[r][i] = ok && e0 + i < E ? src[e0 + i] : 0u;
+    }
   }
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const uint64_t e = (w0 + k) * 64 + lane;
-    v[k] = k < nwd && e < E ? val(e, sv[k]) : identity;
+  for (int r = 0; r < RR; ++r) {
+    const uint64_t e0 = (w0 + r * RW + lw) * 64 + pos0;
+    const bool ok = r * RW + lw < nwd;
+#pragma unroll
+    for (int i = 0; i < IPL; ++i) v[r][i] = ok && e0 + i < E ? val(e0 + i, sv[r][i]) : identity;
   }
   const uint64_t lt = (1ull << lane) - 1;
   T carry = identity;
-  bool carry_live = false;     // a row is open from earlier edges
-  bool carry_shared = true;    // ... and it started before this wave's range
-  uint32_t carry_row = 0;
+  bool carry_live = false;   // a row is open from earlier edges of the range
+  bool carry_shared = true;  // ... and it started before the range
+  // the row open at the range start (rows started before word w0, minus one;
+  // word 0 starts with a head, so this is never used as -1)
+  uint32_t carry_row = uint32_t(__builtin_amdgcn_readlane(int(base[0]), 0)) - 1u;
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (k >= nwd) break;  // wave-uniform
-    const uint64_t e = (w0 + k) * 64 + lane;
-    T x = v[k];
-    const bool head = (f[k] >> lane) & 1ull;
-    if (lane == 0 && !head) x = op(carry, x);
-    if ((f[k] & 1ull) && carry_live && lane == 0) emit(carry_row, carry, carry_shared);
-    const T inc = seg_scan(x, head, op);
-    // tails: lanes whose successor starts a row (lane 63 stays open)
-    const bool tail = lane < 63 ? ((f[k] >> (lane + 1)) & 1ull) : false;
-    const uint64_t heads_le = __popcll(f[k] & (lt | (1ull << lane)));
-    const uint32_t row = base[k] + uint32_t(heads_le) - 1;  // rows started before + own
-    const bool first_seg_open = heads_le == 0;              // segment continues the carried row
-    if (tail && e < E) emit(row, inc, first_seg_open ? carry_shared : false);
-    // new carry = lane 63's segment
-    const T c63 = shfl_t(inc, 63);
-    const uint64_t h63 = __popcll(f[k]);
-    carry = c63;
+  for (int r = 0; r < RR; ++r) {
+    if (r * RW >= nwd) break;  // wave-uniform
+    const uint32_t bits = uint32_t(f[r] >> pos0) & ((1u << IPL) - 1);
+    const uint32_t below = uint32_t(__popcll(f[r] & ((1ull << pos0) - 1)));  // heads before the lane in its word
+    // serial pass: pre = edges before the lane's first head (they close the
+    // open row), rows between two heads of the lane are emitted here, post
+    // = edges from the last head on (the whole lane when it has no head)
+    T pre = identity, acc = identity;
+    uint32_t nh = 0;
+#pragma unroll
+    for (int i = 0; i < IPL; ++i) {
+      if ((bits >> i) & 1u) {
+        if (nh == 0)
+          pre = acc;
+        else
+          emit(base[r] + below + nh - 1, acc, false);
+        acc = identity;
+        ++nh;
+      }
+      acc = op(acc, v[r][i]);
+    }
+    const bool has = nh != 0;
+    if (lane == 0) {  // the open row goes on in lane 0
+      if (has)
+        pre = op(carry, pre);
+      else
+        acc = op(carry, acc);
+    }
+    const uint64_t H = __ballot(has);
+    const uint64_t upto = lane == 63 ? H : (H & ((2ull << lane) - 1));
+    const int start = upto ? 63 - __clzll((long long)upto) : -1;
+    const T S = seg_scan(acc, start, identity, op);
+    // the open row at the end of the previous lane (lane 0: folded above)
+    const T prev = dpp_move<0x138, 0xF>(identity, S);  // wave_shr:1
+    if (has) {
+      const T close = lane == 0 ? pre : op(prev, pre);
+      const bool first_in_round = (H & lt) == 0;
+      const bool open_before = lane > 0 || carry_live || (bits & 1u) == 0;
+      if (open_before) emit(base[r] + below - 1, close, first_in_round ? carry_shared : false);
+    }
+    // the row open after this round: lane 63's scan, started at the round's
+    // last head (or carried on)
+    carry = lane_value(S, 63);
     carry_live = true;
-    if (h63 == 0) {
-      carry_row = base[k] - 1;  // no head in this word: the open row continues
-    } else {
+    if (H) {
+      const int L = 63 - __clzll((long long)H);
+      const uint32_t my_last = base[r] + below + nh - 1;
+      carry_row = uint32_t(__builtin_amdgcn_readlane(int(my_last), L));
       carry_shared = false;
-      carry_row = base[k] + uint32_t(h63) - 1;
     }
   }
   if (lane == 0 && carry_live) emit(carry_row, carry, true);
