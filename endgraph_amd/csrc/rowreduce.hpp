@@ -14,11 +14,12 @@
 namespace eg {
 namespace {
 
-constexpr int kRowWordsPerWave = 8;  // 4 and 16 measured slower on the pull
+constexpr int kRowWordsPerWave = 16;  // 1024 edges per wave: 8 and 32 measured slower / no better
 
 // first word of the calling wave's range in a one-range-per-wave grid
+template <int K = kRowWordsPerWave>
 __device__ __forceinline__ uint64_t first_word() {
-  return (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave * kRowWordsPerWave;
+  return (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave * K;
 }
 
 // Lane value moved by a DPP control (gfx9 wave64: row_shr:n = 0x110 + n
@@ -84,7 +85,7 @@ __device__ __forceinline__ T seg_scan(T v, int start, T identity, Op op) {
   return v;
 }
 
-template <class T, class Op, class Val, class Emit>
+template <class T, int K = kRowWordsPerWave, class Op, class Val, class Emit>
 __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const uint32_t* __restrict__ src,
                                                 const uint64_t* __restrict__ rowflag,
                                                 const uint32_t* __restrict__ wordrank, T identity, Op op, Val val,
@@ -93,7 +94,6 @@ __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const u
   // round lane l holds the 4 consecutive edges 4l..4l+3 (one 16-byte load
   // of sources), reduces them serially, hands the open end of its edges to
   // one segmented scan across the lanes, and emits the rows that end in it.
-  constexpr int K = kRowWordsPerWave;
   constexpr int IPL = 4;                 // edges per lane per round
   constexpr int RW = kWave * IPL / 64;   // words per round
   constexpr int RR = K / RW;             // rounds
@@ -198,8 +198,9 @@ __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const u
 }
 
 // grid of 256-thread blocks with one K-word range per wave
+template <int K = kRowWordsPerWave>
 inline unsigned row_grid(uint64_t E) {
-  const uint64_t waves = ceil_div(ceil_div(E, 64), kRowWordsPerWave);
+  const uint64_t waves = ceil_div(ceil_div(E, 64), K);
   const uint64_t b = ceil_div(waves, 4);
   return unsigned(b < 1 ? 1 : (b > (1u << 30) ? (1u << 30) : b));
 }
