@@ -719,25 +719,39 @@ __global__ __launch_bounds__(kNT) void k_bfs_pull_items(const uint64_t* __restri
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  for (uint64_t i = gw; i < n_items; i += nw) {
-    const uint64_t it = items[i];
-    const uint32_t v = uint32_t(it >> 32);
-    if (bm_get(visited, v)) continue;  // wave-uniform
-    const uint32_t e0 = in_ptr[v] + uint32_t(it) * kTvPullEdges;
-    const uint32_t e1 = min(e0 + kTvPullEdges, in_ptr[v + 1]);
-    // an earlier item of the row already holds a smaller parent
-    if (parent[v] < in_src[e0]) continue;
-    for (uint32_t e = e0; e < e1; e += kWave) {
-      const bool ok = e + lane < e1;
-      const uint32_t u = ok ? in_src[e + lane] : 0u;
-      const uint64_t m = __ballot(ok && bm_get(frontier, u));
-      if (m) {
-        const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
-        if (lane == 0) {
-          atomicMin(&parent[v], p);
-          bm_set(active_out, v);
+  // 64 items per step: every lane checks one (visited, or an earlier item
+  // of the row already holds a smaller parent), then the wave scans the
+  // items that are left
+  for (uint64_t i0 = gw * kWave; i0 < n_items; i0 += nw * kWave) {
+    bool live = false;
+    uint32_t my_v = 0, my_e0 = 0, my_e1 = 0;
+    if (i0 + lane < n_items) {
+      const uint64_t it = items[i0 + lane];
+      my_v = uint32_t(it >> 32);
+      if (!bm_get(visited, my_v)) {
+        my_e0 = in_ptr[my_v] + uint32_t(it) * kTvPullEdges;
+        my_e1 = min(my_e0 + kTvPullEdges, in_ptr[my_v + 1]);
+        live = !(parent[my_v] < in_src[my_e0]);
+      }
+    }
+    uint64_t pending = __ballot(live);
+    while (pending) {  // wave-uniform
+      const int jj = __ffsll((unsigned long long)pending) - 1;
+      pending &= pending - 1;
+      const uint32_t v = __shfl(my_v, jj, kWave);
+      const uint32_t e0 = __shfl(my_e0, jj, kWave), e1 = __shfl(my_e1, jj, kWave);
+      for (uint32_t e = e0; e < e1; e += kWave) {
+        const bool ok = e + lane < e1;
+        const uint32_t u = ok ? in_src[e + lane] : 0u;
+        const uint64_t m = __ballot(ok && bm_get(frontier, u));
+        if (m) {
+          const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
+          if (lane == 0) {
+            atomicMin(&parent[v], p);
+            bm_set(active_out, v);
+          }
+          break;
         }
-        break;
       }
     }
   }
@@ -1145,11 +1159,11 @@ int bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontie
     // first chunks of the large rows, then the others, which an earlier
     // chunk's hit (always smaller: sources ascend) lets skip
     if (g->tv_nbig)
-      hipLaunchKernelGGL(k_bfs_pull_items, dim3(grid_for(g->tv_nbig, kNT / kWave, 65536)), dim3(kNT), 0, s,
+      hipLaunchKernelGGL(k_bfs_pull_items, dim3(grid_for(g->tv_nbig, kNT, 65536)), dim3(kNT), 0, s,
                          (const uint64_t*)g->tv_big.as<uint64_t>(), g->tv_nbig, in_ptr, in_src, d_frontier,
                          (const uint64_t*)d_visited, d_parent, d_next);
     if (g->tv_nbig_rest)
-      hipLaunchKernelGGL(k_bfs_pull_items, dim3(grid_for(g->tv_nbig_rest, kNT / kWave, 65536)), dim3(kNT), 0, s,
+      hipLaunchKernelGGL(k_bfs_pull_items, dim3(grid_for(g->tv_nbig_rest, kNT, 65536)), dim3(kNT), 0, s,
                          (const uint64_t*)g->tv_big_rest.as<uint64_t>(), g->tv_nbig_rest, in_ptr, in_src, d_frontier,
                          (const uint64_t*)d_visited, d_parent, d_next);
     prof_end(PK_BFS, uint64_t(g->V) * 12, s);

@@ -341,6 +341,7 @@ __device__ __forceinline__ void step_bookkeeping(const MultiState& st, uint32_t 
 
 // ------------------------------------------------------------------ BFS --
 struct BfsArgs {
+  static constexpr uint64_t kPullItemsPerWave = 64;  // the pull filters 64 items per step
   uint32_t V;
   uint64_t E;
   int policy;
@@ -395,29 +396,38 @@ __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t j) {
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
   const uint64_t live_roots = a.lanes & c.roots;
-  for (uint64_t i = gw; i < a.n_pull_items; i += nw) {
-    const uint64_t it = a.pull_items[i];
-    const uint64_t v = it >> 32;
-    const uint64_t need = live_roots & ~a.st.vis[v];
-    if (!need) continue;
-    const uint32_t e0 = a.in_ptr[v] + uint32_t(it) * kPullEdges;
-    const uint32_t b1 = min(e0 + kPullEdges, a.in_ptr[v + 1]);
-    uint64_t found = 0;
-    for (uint32_t e = e0; e < b1 && found != need; e += kWave) {
-      const bool ok = e + lane < b1;
-      const uint32_t u = ok ? a.in_src[e + lane] : 0u;
-      const uint64_t f = ok ? (a.st.fr[u] & need) : 0ull;
-      uint64_t todo = wave_or(f) & ~found;
-      while (todo) {
-        const int r = __ffsll((unsigned long long)todo) - 1;
-        const uint64_t m = __ballot((f >> r) & 1ull);
-        const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
-        if (lane == r) atomicMin(&a.parent[v * 64 + r], p);
-        found |= 1ull << r;
-        todo &= todo - 1;
+  // 64 items per step: every lane checks one (all their loads in flight at
+  // once), then the wave works through the items that still need a root
+  for (uint64_t i0 = gw * kWave; i0 < a.n_pull_items; i0 += nw * kWave) {
+    const bool mine = i0 + lane < a.n_pull_items;
+    const uint64_t my_it = mine ? a.pull_items[i0 + lane] : 0ull;
+    const uint64_t my_need = mine ? live_roots & ~a.st.vis[my_it >> 32] : 0ull;
+    uint64_t pending = __ballot(my_need != 0);
+    while (pending) {  // wave-uniform
+      const int jj = __ffsll((unsigned long long)pending) - 1;
+      pending &= pending - 1;
+      const uint64_t it = shfl64(my_it, jj);
+      const uint64_t v = it >> 32;
+      const uint64_t need = shfl64(my_need, jj);
+      const uint32_t e0 = a.in_ptr[v] + uint32_t(it) * kPullEdges;
+      const uint32_t b1 = min(e0 + kPullEdges, a.in_ptr[v + 1]);
+      uint64_t found = 0;
+      for (uint32_t e = e0; e < b1 && found != need; e += kWave) {
+        const bool ok = e + lane < b1;
+        const uint32_t u = ok ? a.in_src[e + lane] : 0u;
+        const uint64_t f = ok ? (a.st.fr[u] & need) : 0ull;
+        uint64_t todo = wave_or(f) & ~found;
+        while (todo) {
+          const int r = __ffsll((unsigned long long)todo) - 1;
+          const uint64_t m = __ballot((f >> r) & 1ull);
+          const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
+          if (lane == r) atomicMin(&a.parent[v * 64 + r], p);
+          found |= 1ull << r;
+          todo &= todo - 1;
+        }
       }
+      if (found && lane == 0) atomicOr((unsigned long long*)&a.st.nx[v], found);
     }
-    if (found && lane == 0) atomicOr((unsigned long long*)&a.st.nx[v], found);
   }
 }
 
@@ -445,6 +455,7 @@ __global__ __launch_bounds__(kNT) void k_mbfs_end(BfsArgs a, uint32_t j) {
 
 // ----------------------------------------------------------------- SSSP --
 struct SsspArgs {
+  static constexpr uint64_t kPullItemsPerWave = 1;
   uint32_t V;
   uint64_t E;
   int policy;
@@ -647,7 +658,7 @@ int run_steps(eg_graph* g, const Args& a, void (*push)(Args, uint32_t), void (*p
               void (*end)(Args, uint32_t), uint32_t* h_steps, hipStream_t s) {
   MultiGraph& mg = g->multi_graph;
   EG_TRY(mg.init());
-  const StepGrids gr{8192, grid_for(g->tv_nitems, kNT / kWave, 65536),
+  const StepGrids gr{8192, grid_for(g->tv_nitems, Args::kPullItemsPerWave * (kNT / kWave), 65536),
                      grid_for(ceil_div(g->V, kWave), kNT / kWave, 1024)};
   std::vector<char> key(reinterpret_cast<const char*>(&a), reinterpret_cast<const char*>(&a) + sizeof(Args));
   key.insert(key.end(), reinterpret_cast<const char*>(&gr), reinterpret_cast<const char*>(&gr) + sizeof(gr));
