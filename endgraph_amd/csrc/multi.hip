@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "graph_state.hpp"
+#include "rowreduce.hpp"
 
 namespace eg {
 namespace {
@@ -63,6 +64,17 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t v) {
     const uint32_t hi = __shfl_xor(uint32_t(v >> 32), d, kWave), lo = __shfl_xor(uint32_t(v), d, kWave);
     v |= (uint64_t(hi) << 32) | lo;
   }
+  return v;
+}
+
+// Inclusive OR over lanes 0..l (DPP row shifts, then row broadcasts).
+__device__ __forceinline__ uint64_t or_scan(uint64_t v) {
+  v |= dpp_move<0x111, 0xF>(uint64_t(0), v);
+  v |= dpp_move<0x112, 0xF>(uint64_t(0), v);
+  v |= dpp_move<0x114, 0xF>(uint64_t(0), v);
+  v |= dpp_move<0x118, 0xF>(uint64_t(0), v);
+  v |= dpp_move<0x142, 0xA>(uint64_t(0), v);
+  v |= dpp_move<0x143, 0xC>(uint64_t(0), v);
   return v;
 }
 
@@ -396,6 +408,8 @@ __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t j) {
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
   const uint64_t live_roots = a.lanes & c.roots;
+  __shared__ uint32_t hand[kNT / kWave][kWave];
+  uint32_t* pslot = hand[threadIdx.x / kWave];
   // 64 items per step: every lane checks one (all their loads in flight at
   // once), then the wave works through the items that still need a root
   for (uint64_t i0 = gw * kWave; i0 < a.n_pull_items; i0 += nw * kWave) {
@@ -415,16 +429,19 @@ __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t j) {
       for (uint32_t e = e0; e < b1 && found != need; e += kWave) {
         const bool ok = e + lane < b1;
         const uint32_t u = ok ? a.in_src[e + lane] : 0u;
-        const uint64_t f = ok ? (a.st.fr[u] & need) : 0ull;
-        uint64_t todo = wave_or(f) & ~found;
-        while (todo) {
-          const int r = __ffsll((unsigned long long)todo) - 1;
-          const uint64_t m = __ballot((f >> r) & 1ull);
-          const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
-          if (lane == r) atomicMin(&a.parent[v * 64 + r], p);
-          found |= 1ull << r;
-          todo &= todo - 1;
-        }
+        const uint64_t f = ok ? (a.st.fr[u] & need & ~found) : 0ull;
+        // root r's parent is the source of the first lane with bit r (the
+        // smallest: sources ascend); that lane hands it to lane r in LDS
+        const uint64_t incl = or_scan(f);
+        const uint64_t below = dpp_move<0x138, 0xF>(uint64_t(0), incl);  // wave_shr:1: lanes below
+        const uint64_t newly = lane_value(incl, 63);
+        if (!newly) continue;  // wave-uniform
+        for (uint64_t m = f & ~below; m; m &= m - 1) pslot[__ffsll((unsigned long long)m) - 1] = u;
+        __builtin_amdgcn_s_waitcnt(kLgkmZero);  // every hand-over written before lane r reads it
+        __asm__ volatile("" ::: "memory");
+        if ((newly >> lane) & 1ull) atomicMin(&a.parent[v * 64 + lane], pslot[lane]);
+        __asm__ volatile("" ::: "memory");
+        found |= newly;
       }
       if (found && lane == 0) atomicOr((unsigned long long*)&a.st.nx[v], found);
     }
