@@ -74,7 +74,8 @@ def file_load_extras(eg, edges, E):
         os.unlink(path)
     log(f"[bench] edge file load: {E * 8 / t / 1e9:.2f} GB/s ({t * 1e3:.1f} ms for {E * 8 / 2**30:.1f} GiB)")
     return {"bytes": E * 8, "ms": round(t * 1e3, 2), "gbs": round(E * 8 / t / 1e9, 2),
-            "note": "64 MiB pread chunks into two pinned buffers, H2D overlapped; file in the page cache"}
+            "note": "4 reader threads, 32 MiB pread chunks into per-thread pinned buffers kept between loads, "
+                    "H2D on per-thread streams overlapped with the reads; file in the page cache"}
 
 
 def ppr_extras(eg, iterations=250, d=0.85):

@@ -2,16 +2,23 @@
 // build (preload_graph_file, core/graph.hpp:1011-1084; format README.md:40-41)
 // and the vertex-array dump / restore after it (core/graph.hpp:324-375).
 //
-// Edge files are read in 64 MiB chunks (pread) into two pinned host buffers;
-// the H2D copy of one chunk runs while the next chunk is read, so the upload
-// hides behind the file read.  Vertex arrays use the same double buffering
-// in the other direction.  The file formats are the reference's: packed
-// EdgeType records in file order; a vertex array is V * sizeof(T) raw bytes.
+// Files are read by kIoThreads host threads, each taking every kIoThreads-th
+// 32 MiB chunk: pread into one of its two pinned buffers, then an H2D copy
+// on its own stream, so reads from the page cache run in parallel and every
+// copy overlaps the next read.  The pinned buffers are kept for the
+// process's later loads (kIoThreads x 2 x 32 MiB).  Vertex arrays are written
+// back with double buffering in the other direction.  The file formats are
+// the reference's: packed EdgeType records in file order; a vertex array is
+// V * sizeof(T) raw bytes.
 #include <fcntl.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "common.hpp"
 
@@ -70,20 +77,83 @@ int full_pwrite(int fd, const char* src, uint64_t n, uint64_t off) {
   return EG_OK;
 }
 
-// host file [0, n) -> device, chunk k's copy overlapping chunk k+1's read
+constexpr int kIoThreads = 4;
+constexpr uint64_t kUpChunk = 32ull << 20;
+
+// Staging of the parallel reader: per thread two pinned buffers, two events
+// and a stream, created on first use and kept (one load at a time).
+struct UploadPool {
+  std::mutex busy;
+  bool ready = false;
+  void* buf[kIoThreads][2] = {};
+  hipEvent_t ev[kIoThreads][2] = {};
+  hipStream_t st[kIoThreads] = {};
+  int init() {
+    if (ready) return EG_OK;
+    for (int t = 0; t < kIoThreads; ++t) {
+      if (hipStreamCreateWithFlags(&st[t], hipStreamNonBlocking) != hipSuccess)
+        return fail(EG_ERR_HIP, "stream creation for the file reader failed");
+      for (int b = 0; b < 2; ++b) {
+        if (hipHostMalloc(&buf[t][b], kUpChunk, hipHostMallocDefault) != hipSuccess) {
+          buf[t][b] = nullptr;
+          return fail(EG_ERR_NOMEM, "pinned host allocation of %llu bytes failed", (unsigned long long)kUpChunk);
+        }
+        EG_HIP(hipEventCreateWithFlags(&ev[t][b], hipEventDisableTiming));
+      }
+    }
+    ready = true;
+    return EG_OK;
+  }
+};
+
+UploadPool& upload_pool() {
+  static UploadPool* p = new UploadPool;  // kept to process end
+  return *p;
+}
+
+// host file [0, n) -> device with kIoThreads readers
 int upload_file(int fd, uint64_t n, char* d, hipStream_t s) {
   if (!n) return EG_OK;
-  Pinned pin;
-  EG_TRY(pin.init(std::min(n, kIoChunk)));
-  for (uint64_t off = 0, k = 0; off < n; off += kIoChunk, ++k) {
-    const int b = int(k & 1);
-    const uint64_t len = std::min(kIoChunk, n - off);
-    EG_HIP(hipEventSynchronize(pin.ev[b]));  // the copy that last used buffer b is done
-    EG_TRY(full_pread(fd, static_cast<char*>(pin.p[b]), len, off));
-    EG_HIP(hipMemcpyAsync(d + off, pin.p[b], len, hipMemcpyHostToDevice, s));
-    EG_HIP(hipEventRecord(pin.ev[b], s));
-  }
-  EG_HIP(hipStreamSynchronize(s));
+  UploadPool& pool = upload_pool();
+  std::lock_guard<std::mutex> lock(pool.busy);
+  EG_TRY(pool.init());
+  // the caller's earlier work on s (it may still use d) before the copies
+  hipEvent_t after_caller = nullptr;
+  EG_HIP(hipEventCreateWithFlags(&after_caller, hipEventDisableTiming));
+  EG_HIP(hipEventRecord(after_caller, s));
+  for (int t = 0; t < kIoThreads; ++t) EG_HIP(hipStreamWaitEvent(pool.st[t], after_caller, 0));
+  (void)hipEventDestroy(after_caller);
+  const uint64_t chunks = ceil_div(n, kUpChunk);
+  int status[kIoThreads];
+  std::string msg[kIoThreads];
+  auto reader = [&](int t) {
+    status[t] = EG_OK;
+    for (uint64_t k = uint64_t(t), j = 0; k < chunks; k += kIoThreads, ++j) {
+      const int b = int(j & 1);
+      const uint64_t off = k * kUpChunk, len = std::min(kUpChunk, n - off);
+      hipError_t e = hipEventSynchronize(pool.ev[t][b]);  // the copy that last used buffer b is done
+      if (e == hipSuccess && full_pread(fd, static_cast<char*>(pool.buf[t][b]), len, off) != EG_OK) {
+        status[t] = EG_ERR_ARG;
+        msg[t] = eg_last_error();
+        return;
+      }
+      if (e == hipSuccess) e = hipMemcpyAsync(d + off, pool.buf[t][b], len, hipMemcpyHostToDevice, pool.st[t]);
+      if (e == hipSuccess) e = hipEventRecord(pool.ev[t][b], pool.st[t]);
+      if (e != hipSuccess) {
+        status[t] = EG_ERR_HIP;
+        msg[t] = hipGetErrorString(e);
+        return;
+      }
+    }
+  };
+  const int nt = int(std::min<uint64_t>(kIoThreads, chunks));
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(reader, t);
+  reader(0);
+  for (auto& x : th) x.join();
+  for (int t = 0; t < nt; ++t) EG_HIP(hipStreamSynchronize(pool.st[t]));
+  for (int t = 0; t < nt; ++t)
+    if (status[t] != EG_OK) return fail(status[t], "file reader: %s", msg[t].c_str());
   return EG_OK;
 }
 
