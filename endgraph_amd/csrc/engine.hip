@@ -783,14 +783,16 @@ __global__ __launch_bounds__(kNT) void k_sssp_pull(uint64_t E, const uint32_t* _
                                                     const uint32_t* __restrict__ row_dst, float* __restrict__ dist,
                                                     uint64_t* __restrict__ active_out) {
   row_reduce_wave<float>(
-      first_word(), E, src, rowflag, wordrank, 1e9f, OpMin{}, [&](uint64_t e, uint32_t s) { return __fadd_rn(dist[s], w[e]); },
+      first_word(), E, src, rowflag, wordrank, 1e9f, OpMin{},
+      [&](uint64_t, uint32_t s, float we) { return __fadd_rn(dist[s], we); },
       [&](uint32_t row, float m, bool) {
         const uint32_t v = row_dst[row];
         if (m < 1e9f && m < dist[v]) {
           const int old = atomicMin(reinterpret_cast<int*>(&dist[v]), __float_as_int(m));
           if (m < __int_as_float(old)) bm_set(active_out, v);
         }
-      });
+      },
+      w);
 }
 
 }  // namespace

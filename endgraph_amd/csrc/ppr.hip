@@ -184,14 +184,15 @@ __global__ __launch_bounds__(kNT) void k_wpull(uint64_t E, const uint32_t* __res
                                                double* __restrict__ y, int all_atomic) {
   row_reduce_wave<double, kPprWords>(
       first_word<kPprWords>(), E, idx, rowflag, wordrank, 0.0, OpSum{},
-      [&](uint64_t e, uint32_t s) { return __dmul_rn(w[e], x[s]); },
+      [&](uint64_t, uint32_t s, double we) { return __dmul_rn(we, x[s]); },
       [&](uint32_t row, double v, bool shared) {
         double* p = &y[row_target[row]];
         if (shared || all_atomic)
           atomicAdd(p, v);
         else
           *p = v;
-      });
+      },
+      w);
 }
 
 // out = a * y + b * base

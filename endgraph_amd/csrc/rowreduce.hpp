@@ -9,6 +9,8 @@
 // ends, which need atomics.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace eg {
@@ -85,11 +87,16 @@ __device__ __forceinline__ T seg_scan(T v, int start, T identity, Op op) {
   return v;
 }
 
-template <class T, int K = kRowWordsPerWave, class Op, class Val, class Emit>
+// No second per-edge stream: val(e, source).
+struct NoEdgeValue {};
+
+// ev (optional): a second per-edge stream (weights), loaded like the sources
+// (16-byte vectors when aligned) and handed to val(e, source, ev[e]).
+template <class T, int K = kRowWordsPerWave, class EV = NoEdgeValue, class Op, class Val, class Emit>
 __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const uint32_t* __restrict__ src,
                                                 const uint64_t* __restrict__ rowflag,
                                                 const uint32_t* __restrict__ wordrank, T identity, Op op, Val val,
-                                                Emit emit) {
+                                                Emit emit, const EV* __restrict__ ev = nullptr) {
   // The wave's K words are taken in rounds of 4 words (256 edges); in a
   // round lane l holds the 4 consecutive edges 4l..4l+3 (one 16-byte load
   // of sources), reduces them serially, hands the open end of its edges to
@@ -128,12 +135,38 @@ __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const u
       for (int i = 0; i < IPL; ++i) sv[r][i] = ok && e0 + i < E ? src[e0 + i] : 0u;
     }
   }
+  constexpr bool kEV = !std::is_same<EV, NoEdgeValue>::value;
+  constexpr int kEvSize = kEV ? int(sizeof(EV)) : 4;
+  EV evv[kEV ? RR : 1][kEV ? IPL : 1];
+  if constexpr (kEV) {
+    static_assert((kEvSize * IPL) % 16 == 0, "a lane's edge values are whole 16-byte vectors");
+    const bool evec = (reinterpret_cast<uintptr_t>(ev) & 15) == 0;
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+      const uint64_t e0 = (w0 + r * RW + lw) * 64 + pos0;
+      const bool ok = r * RW + lw < nwd;
+      if (ok && evec && e0 + IPL <= E) {
+        uint4 q[kEvSize * IPL / 16];
+#pragma unroll
+        for (int h = 0; h < kEvSize * IPL / 16; ++h) q[h] = reinterpret_cast<const uint4*>(ev + e0)[h];
+        __builtin_memcpy(evv[r], q, sizeof(q));
+      } else {
+#pragma unroll
+        for (int i = 0; i < IPL; ++i) evv[r][i] = ok && e0 + i < E ? ev[e0 + i] : EV{};
+      }
+    }
+  }
 #pragma unroll
   for (int r = 0; r < RR; ++r) {
     const uint64_t e0 = (w0 + r * RW + lw) * 64 + pos0;
     const bool ok = r * RW + lw < nwd;
 #pragma unroll
-    for (int i = 0; i < IPL; ++i) v[r][i] = ok && e0 + i < E ? val(e0 + i, sv[r][i]) : identity;
+    for (int i = 0; i < IPL; ++i) {
+      if constexpr (kEV)
+        v[r][i] = ok && e0 + i < E ? val(e0 + i, sv[r][i], evv[r][i]) : identity;
+      else
+        v[r][i] = ok && e0 + i < E ? val(e0 + i, sv[r][i]) : identity;
+    }
   }
   const uint64_t lt = (1ull << lane) - 1;
   T carry = identity;
