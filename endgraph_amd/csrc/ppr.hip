@@ -170,9 +170,6 @@ int ensure_adjoint(eg_graph* g, hipStream_t s) {
   return EG_OK;
 }
 
-// 512 edges per wave (two gathers per edge: weight and value)
-constexpr int kPprWords = 8;
-
 // --------------------------------------------------------------- kernels --
 // y[row target] (+)= sum over the row's slots of w * x[gather index]; rows
 // of one target split over sub-partitions or wave ranges meet in atomics.
@@ -182,8 +179,8 @@ __global__ __launch_bounds__(kNT) void k_wpull(uint64_t E, const uint32_t* __res
                                                const uint32_t* __restrict__ row_target,
                                                const double* __restrict__ w, const double* __restrict__ x,
                                                double* __restrict__ y, int all_atomic) {
-  row_reduce_wave<double, kPprWords>(
-      first_word<kPprWords>(), E, idx, rowflag, wordrank, 0.0, OpSum{},
+  row_reduce_wave<double>(
+      first_word(), E, idx, rowflag, wordrank, 0.0, OpSum{},
       [&](uint64_t, uint32_t s, double we) { return __dmul_rn(we, x[s]); },
       [&](uint32_t row, double v, bool shared) {
         double* p = &y[row_target[row]];
@@ -238,11 +235,11 @@ int fixed_point(eg_graph* g, bool transposed, const double* w_rows, const double
     hipLaunchKernelGGL((k_fill_v<double>), dim3(vg), dim3(kNT), 0, s, y, uint64_t(V), 0.0);
     if (E) {
       if (!transposed)
-        hipLaunchKernelGGL(k_wpull, dim3(row_grid<kPprWords>(E)), dim3(kNT), 0, s, E, g->e_csc_src.as<uint32_t>(),
+        hipLaunchKernelGGL(k_wpull, dim3(row_grid(E)), dim3(kNT), 0, s, E, g->e_csc_src.as<uint32_t>(),
                            g->e_rowflag.as<uint64_t>(), g->e_wordrank.as<uint32_t>(), g->e_row_dst.as<uint32_t>(),
                            w_rows, (const double*)cur, y, g->Q > 1 ? 1 : 0);
       else
-        hipLaunchKernelGGL(k_wpull, dim3(row_grid<kPprWords>(E)), dim3(kNT), 0, s, E, g->ad_dst.as<uint32_t>(),
+        hipLaunchKernelGGL(k_wpull, dim3(row_grid(E)), dim3(kNT), 0, s, E, g->ad_dst.as<uint32_t>(),
                            g->ad_rowflag.as<uint64_t>(), g->ad_wordrank.as<uint32_t>(),
                            g->ad_row_src.as<uint32_t>(), w_rows, (const double*)cur, y, 0);
     }
