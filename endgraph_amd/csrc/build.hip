@@ -260,14 +260,20 @@ __global__ void k_q_offsets(eg_graph::Meta* meta, int Q) {
 template <bool kSrc>
 struct KeyChecked {
   uint32_t V;
-  uint32_t* err;
+  uint32_t* err;  // null: clamp only (the ids were checked in an earlier launch)
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const {
-    if (r.src >= V || r.dst >= V) atomicOr(err, 1u);
+    if (err && (r.src >= V || r.dst >= V)) atomicOr(err, 1u);
     const uint32_t k = kSrc ? r.src : r.dst;
     return k < V ? k : V - 1;
   }
 };
+// the range check runs once per record: in the first upsweep of the first
+// rank sort only
+template <bool kSrc>
+KeyChecked<kSrc> rx_quiet(const KeyChecked<kSrc>& k) {
+  return KeyChecked<kSrc>{k.V, nullptr};
+}
 // Stable pass on the sub-partition of the owner only.
 template <bool kPull>
 struct KeyQOnly {
@@ -761,7 +767,7 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   R* B = g->t2.as<R>();
   R* T = P > 1 ? g->t3.as<R>() : g->csc_list.as<R>();  // ping-pong of the first sorts
   EG_TRY(rank_sorts(g, L, A, T, E0, P, mult, KeyChecked<true>{V, err}, g->vbits, s));
-  EG_TRY(rank_sorts(g, L, B, T, E0, P, mult, KeyChecked<false>{V, err}, g->vbits, s));
+  EG_TRY(rank_sorts(g, L, B, T, E0, P, mult, KeyChecked<false>{V, nullptr}, g->vbits, s));  // checked above
   uint32_t h_err = 0;
   EG_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));

@@ -56,6 +56,13 @@ inline RxPlan rx_plan(int key_bits, int max_bits) {
   return p;
 }
 
+// Key of every launch after a sort's first upsweep: a key functor with a
+// one-time side effect (a range check) overloads this for its own type.
+template <class KeyFn>
+KeyFn rx_quiet(const KeyFn& k) {
+  return k;
+}
+
 // Input of a sort pass.  Plain: record i of the sort order is p[i].  Block
 // view (first pass of a sort that regroups contiguous blocks on the fly):
 // record i is p[i + shift[t]] for tile t holding i, or, in a tile that
@@ -542,6 +549,7 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
   EG_HIP(hipMemcpyAsync(d_ts, ts.data(), (nseg + 1) * 4, hipMemcpyHostToDevice, s));
   const uint64_t n = seg[nseg] - seg[0];
   const RxPlan plan = rx_plan(key_bits, BM);
+  const KeyFn quiet = rx_quiet(key);
   if (in.nblk > 0) {
     EG_TRY(w.tshift.resize(uint64_t(tiles) * 8));
     hipLaunchKernelGGL((k_rx_tile_shift<TILE>), dim3(ceil_div(tiles, 256u)), dim3(256), 0, s, (const uint64_t*)d_seg,
@@ -559,14 +567,15 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
       R* dst = ((plan.passes - 1 - p) % 2 == 0) ? out : tmp;
       prof_begin(s);
       hipLaunchKernelGGL((k_rx_up<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, (const uint64_t*)d_seg,
-                         (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p], w.table.as<uint32_t>(), xcd);
+                         (const uint32_t*)d_ts, nseg, p == 0 ? key : quiet, plan.shift[p], plan.bits[p],
+                         w.table.as<uint32_t>(), xcd);
       prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
       EG_HIP(hipGetLastError());
       EG_TRY(exclusive_scan_u32(w.table.as<uint32_t>(), w.table.as<uint32_t>(), uint64_t(tiles) << plan.bits[p],
                                 w.scan.as<uint32_t>(), s));
       prof_begin(s);
       hipLaunchKernelGGL((k_rx_down<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, (const uint64_t*)d_seg,
-                         (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p],
+                         (const uint32_t*)d_ts, nseg, quiet, plan.shift[p], plan.bits[p],
                          (const uint32_t*)w.table.as<uint32_t>(), xcd);
       prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
       EG_HIP(hipGetLastError());
@@ -604,7 +613,7 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
     const uint32_t epoch = ++w.epoch;
     prof_begin(s);
     hipLaunchKernelGGL((k_rx_onepass<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, (const uint64_t*)d_seg,
-                       (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p],
+                       (const uint32_t*)d_ts, nseg, quiet, plan.shift[p], plan.bits[p],
                        (const uint32_t*)w.ghist.as<uint32_t>() + uint64_t(p) * (1 << kRxMaxBits),
                        w.status.as<uint64_t>(), w.ticket.as<uint32_t>() + p, epoch, w.err.as<uint32_t>());
     prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
