@@ -88,17 +88,23 @@ struct UploadPool {
   void* buf[kIoThreads][2] = {};
   hipEvent_t ev[kIoThreads][2] = {};
   hipStream_t st[kIoThreads] = {};
+  // creates what is still missing (a failed attempt keeps what it made)
   int init() {
     if (ready) return EG_OK;
     for (int t = 0; t < kIoThreads; ++t) {
-      if (hipStreamCreateWithFlags(&st[t], hipStreamNonBlocking) != hipSuccess)
+      if (!st[t] && hipStreamCreateWithFlags(&st[t], hipStreamNonBlocking) != hipSuccess) {
+        st[t] = nullptr;
         return fail(EG_ERR_HIP, "stream creation for the file reader failed");
+      }
       for (int b = 0; b < 2; ++b) {
-        if (hipHostMalloc(&buf[t][b], kUpChunk, hipHostMallocDefault) != hipSuccess) {
+        if (!buf[t][b] && hipHostMalloc(&buf[t][b], kUpChunk, hipHostMallocDefault) != hipSuccess) {
           buf[t][b] = nullptr;
           return fail(EG_ERR_NOMEM, "pinned host allocation of %llu bytes failed", (unsigned long long)kUpChunk);
         }
-        EG_HIP(hipEventCreateWithFlags(&ev[t][b], hipEventDisableTiming));
+        if (!ev[t][b] && hipEventCreateWithFlags(&ev[t][b], hipEventDisableTiming) != hipSuccess) {
+          ev[t][b] = nullptr;
+          return fail(EG_ERR_HIP, "event creation for the file reader failed");
+        }
       }
     }
     ready = true;
