@@ -15,9 +15,11 @@ def eg():
     return endgraph_amd
 
 
-@pytest.mark.parametrize("weighted,E", [(False, 0), (False, 1), (False, 12_345_678), (True, 9_000_001)])
+@pytest.mark.parametrize("weighted,E", [(False, 0), (False, 1), (False, 12_345_678), (True, 9_000_001),
+                                        (False, 50_000_003)])
 def test_edge_file_round_trip(eg, tmp_path, weighted, E):
-    """Sizes below, at and across several 64 MiB chunks (odd chunk tails)."""
+    """Sizes below one 32 MiB chunk, across the four reader threads' first
+    chunks (odd tails), and over 12 chunks (every thread reuses its buffers)."""
     rng = np.random.default_rng(E)
     edges = H.random_edges(rng, 1 << 20, E, weighted=weighted, skew=False) if E else np.zeros(0, H.REC8)
     path = tmp_path / "g.bin"
@@ -66,3 +68,19 @@ def test_vertex_array_dump_restore(eg, tmp_path, dtype):
     assert back.download(dtype, V).tobytes() == x.tobytes()
     with pytest.raises(eg.EndGraphError):
         eg.restore_vertex_array(back.ptr, x.itemsize, V + 1, path)
+
+
+@pytest.mark.parametrize("nbytes,src_off,dst_off", [(0, 0, 0), (1, 0, 0), (16, 0, 0), (1 << 20, 0, 0),
+                                                    ((1 << 24) + 13, 0, 0), (4099, 4, 8), (1000, 1, 3)])
+def test_memcpy_d2d(eg, nbytes, src_off, dst_off):
+    """Device copies: the 16-byte streaming kernel when both ends are aligned
+    (with a byte tail), the runtime copy otherwise."""
+    print(f"memcpy_d2d {nbytes} bytes, offsets {src_off}/{dst_off}", flush=True)
+    rng = np.random.default_rng(nbytes)
+    x = rng.integers(0, 256, nbytes + src_off, dtype=np.uint8)
+    a = eg.DeviceBuffer.from_array(x) if x.size else eg.DeviceBuffer(1)
+    b = eg.DeviceBuffer(nbytes + dst_off + 1)
+    eg.memcpy_d2d(b.ptr + dst_off, a.ptr + src_off, nbytes)
+    eg.synchronize()
+    got = b.download(np.uint8, nbytes + dst_off)[dst_off:]
+    assert got.tobytes() == x[src_off:].tobytes()
