@@ -21,6 +21,7 @@
 
 #include "graph_state.hpp"
 #include "rowreduce.hpp"
+#include "stepgraph.hpp"
 
 namespace eg {
 namespace {
@@ -679,12 +680,11 @@ __global__ void k_front_deg(const uint32_t* __restrict__ flist, uint32_t fn, con
 // each, and write_min over the items gives the row's smallest.  A row's
 // first item runs in an earlier launch than its others, which skip when the
 // first found a source (it is smaller than all of theirs).
-__global__ __launch_bounds__(kNT) void k_bfs_pull_rows(uint32_t V, const uint32_t* __restrict__ in_ptr,
-                                                       const uint32_t* __restrict__ in_src,
-                                                       const uint64_t* __restrict__ frontier,
-                                                       const uint64_t* __restrict__ visited,
-                                                       uint32_t* __restrict__ parent,
-                                                       uint64_t* __restrict__ active_out) {
+__device__ __forceinline__ void bfs_pull_rows(uint32_t V, const uint32_t* __restrict__ in_ptr,
+                                              const uint32_t* __restrict__ in_src,
+                                              const uint64_t* __restrict__ frontier,
+                                              const uint64_t* __restrict__ visited, uint32_t* __restrict__ parent,
+                                              uint64_t* __restrict__ active_out) {
   for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x) {
     if (bm_get(visited, uint32_t(v))) continue;
     const uint32_t b = in_ptr[v], e = in_ptr[v + 1];
@@ -709,13 +709,12 @@ __global__ __launch_bounds__(kNT) void k_bfs_pull_rows(uint32_t V, const uint32_
   }
 }
 
-__global__ __launch_bounds__(kNT) void k_bfs_pull_items(const uint64_t* __restrict__ items, uint64_t n_items,
-                                                        const uint32_t* __restrict__ in_ptr,
-                                                        const uint32_t* __restrict__ in_src,
-                                                        const uint64_t* __restrict__ frontier,
-                                                        const uint64_t* __restrict__ visited,
-                                                        uint32_t* __restrict__ parent,
-                                                        uint64_t* __restrict__ active_out) {
+__device__ __forceinline__ void bfs_pull_items(const uint64_t* __restrict__ items, uint64_t n_items,
+                                               const uint32_t* __restrict__ in_ptr,
+                                               const uint32_t* __restrict__ in_src,
+                                               const uint64_t* __restrict__ frontier,
+                                               const uint64_t* __restrict__ visited, uint32_t* __restrict__ parent,
+                                               uint64_t* __restrict__ active_out) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
@@ -754,6 +753,261 @@ __global__ __launch_bounds__(kNT) void k_bfs_pull_items(const uint64_t* __restri
         }
       }
     }
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_bfs_pull_rows(uint32_t V, const uint32_t* __restrict__ in_ptr,
+                                                       const uint32_t* __restrict__ in_src,
+                                                       const uint64_t* __restrict__ frontier,
+                                                       const uint64_t* __restrict__ visited,
+                                                       uint32_t* __restrict__ parent,
+                                                       uint64_t* __restrict__ active_out) {
+  bfs_pull_rows(V, in_ptr, in_src, frontier, visited, parent, active_out);
+}
+
+__global__ __launch_bounds__(kNT) void k_bfs_pull_items(const uint64_t* __restrict__ items, uint64_t n_items,
+                                                        const uint32_t* __restrict__ in_ptr,
+                                                        const uint32_t* __restrict__ in_src,
+                                                        const uint64_t* __restrict__ frontier,
+                                                        const uint64_t* __restrict__ visited,
+                                                        uint32_t* __restrict__ parent,
+                                                        uint64_t* __restrict__ active_out) {
+  bfs_pull_items(items, n_items, in_ptr, in_src, frontier, visited, parent, active_out);
+}
+
+// ------------------------------------------- single-root BFS, device-driven
+// eg_bfs's levels as steps of a captured graph (no host round trip per
+// level): the level end counts the next frontier and its out-edges (which
+// pick the next step's direction on the device) and cuts the out-edges of
+// its hubs (out-degree above kSbfsHub) into push work items of
+// kSbfsItemEdges edges; a push step walks the frontier bitmap for the other
+// vertices and the items for the hubs.  The frontier and next bitmaps
+// alternate by step parity, and the level end clears the bitmap the next
+// step writes into.
+constexpr uint32_t kSbfsItemEdges = 1024;  // out-edges per push work item of a hub
+constexpr uint32_t kSbfsHub = 4096;        // out-degree above which a vertex is cut into items
+constexpr uint32_t kSbfsLaneEdges = 16;    // out-degree up to which a vertex runs in one lane
+
+struct SCount {
+  unsigned long long nv, ne;  // frontier vertices and their out-edges
+  unsigned long long items;   // push work items listed for the step
+};
+
+struct SbfsState {
+  uint64_t* bm[2];     // frontier of step s is bm[s & 1], its next bm[(s + 1) & 1]
+  uint64_t* vis;
+  uint64_t* items[2];  // push items of step s: items[s & 1]
+  SCount* cnt;         // ring of 3 by step
+  uint32_t* steps;     // steps that had a frontier
+  uint32_t* base;      // first step of the queued batch
+  unsigned long long* total;  // out-edges of every reached vertex
+};
+
+struct SbfsArgs {
+  uint32_t V;
+  uint64_t E;
+  int policy;
+  const uint32_t *out_ptr, *out_dst, *deg, *in_ptr, *in_src;
+  const uint64_t *big, *big_rest;
+  uint64_t n_big, n_big_rest;
+  SbfsState st;
+  uint32_t *parent, *depth;
+};
+
+__device__ __forceinline__ bool sbfs_sparse(int policy, unsigned long long ne, uint64_t E) {
+  if (policy == EG_POLICY_PULL) return false;
+  if (policy == EG_POLICY_AUTO) return ne < E / 20;
+  return true;  // reference (rooted programs push, R11) and push
+}
+
+// Sparse step (bfs.cpp:52-58 with R12): frontier vertices of out-degree up
+// to kSbfsLaneEdges run in their lane, up to kSbfsHub with the whole wave;
+// hubs' items are strided over the waves.
+__global__ __launch_bounds__(kNT) void k_sbfs_push(SbfsArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
+  const SCount c = a.st.cnt[step % 3];
+  if (!c.nv || !sbfs_sparse(a.policy, c.ne, a.E)) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t* fr = a.st.bm[step & 1];
+  uint64_t* nx = a.st.bm[(step + 1) & 1];
+  auto relax = [&](uint32_t u, uint32_t e) {
+    const uint32_t v = a.out_dst[e];
+    if (!bm_get(a.st.vis, v) && a.parent[v] > u) {
+      atomicMin(&a.parent[v], u);
+      if (!bm_get(nx, v)) bm_set(nx, v);
+    }
+  };
+  // the frontier bitmap, 64 words per wave step: each lane first runs the
+  // small vertices of its own word, then the wave walks the words that hold
+  // larger ones
+  const uint64_t words = (uint64_t(a.V) >> 6) + 1;
+  for (uint64_t b0 = gw * kWave; b0 < words; b0 += nw * kWave) {
+    const uint64_t wi = b0 + lane;
+    const uint64_t mine = wi < words ? fr[wi] : 0ull;
+    bool has_mid = false;
+    for (uint64_t m = mine; m; m &= m - 1) {
+      const uint32_t u = uint32_t(wi * 64 + __ffsll((unsigned long long)m) - 1);
+      const uint32_t e0 = a.out_ptr[u], e1 = a.out_ptr[u + 1];
+      if (e1 - e0 <= kSbfsLaneEdges)
+        for (uint32_t e = e0; e < e1; ++e) relax(u, e);
+      else if (e1 - e0 <= kSbfsHub)
+        has_mid = true;
+    }
+    uint64_t wm = __ballot(has_mid);
+    while (wm) {  // wave-uniform
+      const int jj = __ffsll((unsigned long long)wm) - 1;
+      wm &= wm - 1;
+      const bool set = (shfl_t(mine, jj) >> lane) & 1ull;
+      const uint32_t u = uint32_t((b0 + jj) * 64 + lane);
+      const uint32_t e0 = set ? a.out_ptr[u] : 0u, e1 = set ? a.out_ptr[u + 1] : 0u;
+      uint64_t mid = __ballot(e1 - e0 > kSbfsLaneEdges && e1 - e0 <= kSbfsHub);
+      while (mid) {  // wave-uniform
+        const int l = __ffsll((unsigned long long)mid) - 1;
+        mid &= mid - 1;
+        const uint32_t uu = __shfl(u, l, kWave), b1 = __shfl(e0, l, kWave), b2 = __shfl(e1, l, kWave);
+        for (uint32_t e = b1 + lane; e < b2; e += kWave) relax(uu, e);
+      }
+    }
+  }
+  // hub items, strided over the waves
+  const uint64_t* items = a.st.items[step & 1];
+  for (uint64_t i = gw; i < c.items; i += nw) {
+    const uint64_t it = items[i];
+    const uint32_t u = uint32_t(it >> 32);
+    const uint32_t e0 = a.out_ptr[u] + uint32_t(it) * kSbfsItemEdges;
+    const uint32_t e1 = min(e0 + kSbfsItemEdges, a.out_ptr[u + 1]);
+    for (uint32_t e = e0 + lane; e < e1; e += kWave) relax(u, e);
+  }
+}
+
+// Dense step: the pulls above, on this step's bitmaps.
+__global__ __launch_bounds__(kNT) void k_sbfs_pull_rows(SbfsArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
+  const SCount c = a.st.cnt[step % 3];
+  if (!c.nv || sbfs_sparse(a.policy, c.ne, a.E)) return;
+  bfs_pull_rows(a.V, a.in_ptr, a.in_src, a.st.bm[step & 1], a.st.vis, a.parent, a.st.bm[(step + 1) & 1]);
+}
+
+template <int kRest>
+__global__ __launch_bounds__(kNT) void k_sbfs_pull_items(SbfsArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
+  const SCount c = a.st.cnt[step % 3];
+  if (!c.nv || sbfs_sparse(a.policy, c.ne, a.E)) return;
+  bfs_pull_items(kRest ? a.big_rest : a.big, kRest ? a.n_big_rest : a.n_big, a.in_ptr, a.in_src,
+                 a.st.bm[step & 1], a.st.vis, a.parent, a.st.bm[(step + 1) & 1]);
+}
+
+// Level end (bfs.cpp:70-90): visited |= next, depth = level + 1, counters
+// and push items of the next step; clears this step's frontier bitmap (the
+// next step's next).  A wave loads 64 words at once and walks the
+// non-empty ones with a lane per vertex.
+__global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
+  const bool live = a.st.cnt[step % 3].nv != 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st.cnt[(step + 2) % 3] = SCount{0, 0, 0};  // last read at step - 1
+    if (live) *a.st.steps = step + 1;
+  }
+  if (!live) return;  // block-uniform
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t words = (uint64_t(a.V) >> 6) + 1;
+  uint64_t* fr = a.st.bm[step & 1];
+  const uint64_t* nx = a.st.bm[(step + 1) & 1];
+  uint64_t* items = a.st.items[(step + 1) & 1];
+  SCount* nc = &a.st.cnt[(step + 1) % 3];
+  // pass 1: depths, visited, counts, and this lane's push items
+  unsigned long long nv = 0, ne = 0;
+  uint32_t ksum = 0;
+  for (uint64_t b0 = gw * kWave; b0 < words; b0 += nw * kWave) {
+    const uint64_t wi = b0 + lane;
+    const uint64_t mine = wi < words ? nx[wi] : 0ull;
+    if (wi < words) fr[wi] = 0;
+    if (mine) {
+      a.st.vis[wi] |= mine;
+      nv += __popcll(mine);
+    }
+    uint64_t nz = __ballot(mine != 0);
+    while (nz) {  // wave-uniform
+      const int jj = __ffsll((unsigned long long)nz) - 1;
+      nz &= nz - 1;
+      if ((shfl_t(mine, jj) >> lane) & 1ull) {
+        const uint32_t v = uint32_t((b0 + jj) * 64 + lane);
+        a.depth[v] = step + 1;
+        const uint32_t d = a.deg[v];
+        ne += d;
+        if (d > kSbfsHub) ksum += uint32_t(ceil_div(d, kSbfsItemEdges));
+      }
+    }
+  }
+  // one reservation per wave; pass 2 (only waves that met a hub) writes
+  // the hubs' items (v << 32 | chunk)
+  const uint32_t inc = wave_inclusive_scan(ksum, OpSum{});
+  const uint32_t tot = __shfl(inc, kWave - 1, kWave);
+  unsigned long long at0 = 0;
+  if (lane == 0 && tot) at0 = atomicAdd(&nc->items, (unsigned long long)tot);
+  uint64_t at = __shfl(at0, 0, kWave) + inc - ksum;
+  if (tot) {
+    for (uint64_t b0 = gw * kWave; b0 < words; b0 += nw * kWave) {
+      const uint64_t wi = b0 + lane;
+      const uint64_t mine = wi < words ? nx[wi] : 0ull;
+      uint64_t nz = __ballot(mine != 0);
+      while (nz) {  // wave-uniform
+        const int jj = __ffsll((unsigned long long)nz) - 1;
+        nz &= nz - 1;
+        if ((shfl_t(mine, jj) >> lane) & 1ull) {
+          const uint32_t v = uint32_t((b0 + jj) * 64 + lane);
+          const uint32_t d = a.deg[v];
+          const uint32_t n = d > kSbfsHub ? uint32_t(ceil_div(d, kSbfsItemEdges)) : 0u;
+          for (uint32_t k = 0; k < n; ++k) items[at + k] = (uint64_t(v) << 32) | k;
+          at += n;
+        }
+      }
+    }
+  }
+  __shared__ unsigned long long part[2][kNT / kWave];
+  nv = wave_reduce(nv, OpSum{});
+  ne = wave_reduce(ne, OpSum{});
+  if (lane == 0) {
+    part[0][threadIdx.x / kWave] = nv;
+    part[1][threadIdx.x / kWave] = ne;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long x = 0, y = 0;
+    for (int k = 0; k < kNT / kWave; ++k) {
+      x += part[0][k];
+      y += part[1][k];
+    }
+    if (x) atomicAdd(&nc->nv, x);
+    if (y) {
+      atomicAdd(&nc->ne, y);
+      atomicAdd(a.st.total, y);
+    }
+  }
+}
+
+// The root's state: its bit in frontier and visited, parent and depth, the
+// first counters and push items (the bitmaps, parent and depth arrays are
+// filled before).
+__global__ void k_sbfs_init(SbfsArgs a, uint32_t root) {
+  const uint32_t d = a.out_ptr[root + 1] - a.out_ptr[root];
+  const uint32_t n = d > kSbfsHub ? uint32_t(ceil_div(d, kSbfsItemEdges)) : 0u;
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) a.st.items[0][k] = (uint64_t(root) << 32) | k;
+  if (threadIdx.x == 0) {
+    a.st.bm[0][root >> 6] = 1ull << (root & 63);
+    a.st.vis[root >> 6] = 1ull << (root & 63);
+    a.parent[root] = root;
+    a.depth[root] = 0;
+    a.st.cnt[0] = SCount{1, d, n};
+    a.st.cnt[1] = SCount{0, 0, 0};
+    a.st.cnt[2] = SCount{0, 0, 0};
+    *a.st.steps = 0;
+    *a.st.base = 0;
+    *a.st.total = d;
   }
 }
 
@@ -1191,33 +1445,64 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
   EG_TRY(ensure_engine(g, s));
   if (root >= g->V) return fail(EG_ERR_ARG, "root %u >= vertices_num", root);
   if (!d_parent) return fail(EG_ERR_ARG, "d_parent is NULL");
+  if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
   const uint32_t V = g->V;
-  uint64_t* fr = g->e_bm[0].as<uint64_t>();
-  uint64_t* nx = g->e_bm[1].as<uint64_t>();
-  uint64_t* vis = g->e_bm[2].as<uint64_t>();
+  const bool pulls = policy == EG_POLICY_AUTO || policy == EG_POLICY_PULL;
+  if (pulls) EG_TRY(ensure_tview(g, s));
+  // state: frontier / next bitmaps, visited, push items (two lists), the
+  // counter ring and scalars, and depths when the caller keeps none
+  const uint64_t words = words_of(V);
+  const uint64_t item_cap = uint64_t(V) + g->E / kSbfsItemEdges + 1;
+  const uint64_t bytes = 3 * words * 8 + 2 * item_cap * 8 + 3 * sizeof(SCount) + 64 + (d_depth ? 0 : uint64_t(V) * 4);
+  EG_TRY(g->e_sbfs.resize(bytes));
+  SbfsArgs a{};  // zeroed padding: the bytes key the captured step graph
+  SbfsState& st = a.st;
+  st.bm[0] = g->e_sbfs.as<uint64_t>();
+  st.bm[1] = st.bm[0] + words;
+  st.vis = st.bm[1] + words;
+  st.items[0] = st.vis + words;
+  st.items[1] = st.items[0] + item_cap;
+  st.cnt = reinterpret_cast<SCount*>(st.items[1] + item_cap);
+  st.total = reinterpret_cast<unsigned long long*>(st.cnt + 3);
+  st.steps = reinterpret_cast<uint32_t*>(st.total + 1);
+  st.base = st.steps + 1;
+  uint32_t* depth = d_depth ? d_depth : reinterpret_cast<uint32_t*>(st.total + 8);
+  a.V = V;
+  a.E = g->E;
+  a.policy = policy;
+  a.out_ptr = g->e_out_ptr.as<uint32_t>();
+  a.out_dst = g->e_out_dst.as<uint32_t>();
+  a.deg = g->out_deg.as<uint32_t>();
+  if (pulls) {
+    a.in_ptr = g->tv_ptr.as<uint32_t>();
+    a.in_src = g->tv_src.as<uint32_t>();
+    a.big = g->tv_big.as<uint64_t>();
+    a.big_rest = g->tv_big_rest.as<uint64_t>();
+    a.n_big = g->tv_nbig;
+    a.n_big_rest = g->tv_nbig_rest;
+  }
+  a.parent = d_parent;
+  a.depth = depth;
+  EG_TRY(fill<uint64_t>(st.bm[0], 3 * words, 0ull, s));
   EG_TRY(fill<uint32_t>(d_parent, V, V, s));
-  EG_TRY(fill<uint32_t>(d_parent + root, 1, root, s));
-  if (d_depth) {
-    EG_TRY(fill<uint32_t>(d_depth, V, 0xffffffffu, s));
-    EG_TRY(fill<uint32_t>(d_depth + root, 1, 0u, s));
-  }
-  EG_TRY(fill<uint64_t>(fr, words_of(V), 0ull, s));
-  EG_TRY(fill<uint64_t>(fr + (root >> 6), 1, 1ull << (root & 63), s));
-  EG_TRY(fill<uint64_t>(vis, words_of(V), 0ull, s));
-  EG_TRY(fill<uint64_t>(vis + (root >> 6), 1, 1ull << (root & 63), s));
-  uint32_t deg_root = 0;
-  EG_HIP(hipMemcpyAsync(&deg_root, g->out_deg.as<uint32_t>() + root, 4, hipMemcpyDeviceToHost, s));
+  EG_TRY(fill<uint32_t>(depth, V, 0xffffffffu, s));
+  hipLaunchKernelGGL(k_sbfs_init, dim3(1), dim3(kNT), 0, s, a, root);
+  EG_HIP(hipGetLastError());
+  const StepLaunch<SbfsArgs> ls[5] = {{k_sbfs_push, 2048},
+                                      {k_sbfs_pull_rows, grid_for(V, kNT, 16384)},
+                                      {k_sbfs_pull_items<0>, grid_for(a.n_big, kNT, 65536)},
+                                      {k_sbfs_pull_items<1>, grid_for(a.n_big_rest, kNT, 65536)},
+                                      {k_sbfs_end, grid_for(words, kNT, 1024)}};
+  const unsigned long long* const nv[3] = {&st.cnt[0].nv, &st.cnt[1].nv, &st.cnt[2].nv};
+  uint32_t steps = 0;
+  prof_begin(s);
+  EG_TRY(run_step_graph(g->multi_graph, a, ls, st.base, nv, st.steps, V + 1, &steps, s));
+  prof_end(PK_BFS, 0, s);
+  unsigned long long total = 0;
+  EG_HIP(hipMemcpyAsync(&total, st.total, 8, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
-  uint64_t nv = 1, ne = deg_root, traversed = deg_root;
-  uint32_t level = 0;
-  while (nv > 0) {
-    EG_TRY(bfs_level(g, policy, level, fr, nx, vis, d_parent, d_depth, ne, int64_t(nv), &nv, &ne, nullptr, s));
-    traversed += ne;
-    ++level;
-    std::swap(fr, nx);
-  }
-  if (h_levels) *h_levels = level;
-  if (h_edges) *h_edges = traversed;
+  if (h_levels) *h_levels = steps;
+  if (h_edges) *h_edges = total;
   return EG_OK;
 }
 

@@ -115,8 +115,9 @@ struct eg_graph {
   eg::DevBuf ad_rowflag, ad_wordrank;
   eg::DevBuf ad_row_src;             // u32[R] source of each transposed row
   eg::DevBuf e_ppr, e_ppr_adj;       // PPR work vectors
+  eg::DevBuf e_sbfs;                 // single-root BFS state (bitmaps, push items, counters)
   eg::DevBuf e_multi;                // batched traversal masks, work items, counters
-  MultiGraph multi_graph;            // captured steps of the batched programs
+  MultiGraph multi_graph;            // captured steps of the device-driven programs
   eg::DevBuf e_multi_roots;          // roots and per-root statistics
   double eng_prepare_ms = 0;
 

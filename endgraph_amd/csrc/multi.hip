@@ -29,6 +29,7 @@
 
 #include "graph_state.hpp"
 #include "rowreduce.hpp"
+#include "stepgraph.hpp"
 
 namespace eg {
 namespace {
@@ -628,80 +629,16 @@ int start_multi(eg_graph* g, const uint32_t* h_roots, int n, MultiState* st, uin
   return EG_OK;
 }
 
-__global__ void k_multi_advance(uint32_t* base, uint32_t n) { *base += n; }
-
-// Runs steps until a step has an empty frontier.  kBatch steps (three
-// launches each, then one that advances the batch base) are captured once
-// into a HIP graph on the graph handle's own stream and replayed between
-// host checks; a step after the last live one returns at once (its counters
-// are zero).  The captured graph is kept for the next call with the same
-// arguments (same buffers).
-constexpr uint32_t kBatch = 4;
-
-struct StepGrids {
-  unsigned push, pull, end;
-};
-
-template <class Args>
-int capture_steps(MultiGraph& mg, const Args& a, const StepGrids& gr, void (*push)(Args, uint32_t),
-                  void (*pull)(Args, uint32_t), void (*end)(Args, uint32_t)) {
-  mg.reset_exec();
-  EG_HIP(hipStreamBeginCapture(mg.stream, hipStreamCaptureModeThreadLocal));
-  for (uint32_t j = 0; j < kBatch; ++j) {
-    hipLaunchKernelGGL(push, dim3(gr.push), dim3(kNT), 0, mg.stream, a, j);
-    hipLaunchKernelGGL(pull, dim3(gr.pull), dim3(kNT), 0, mg.stream, a, j);
-    hipLaunchKernelGGL(end, dim3(gr.end), dim3(kNT), 0, mg.stream, a, j);
-  }
-  hipLaunchKernelGGL(k_multi_advance, dim3(1), dim3(1), 0, mg.stream, a.st.base, kBatch);
-  hipGraph_t graph = nullptr;
-  const hipError_t ce = hipStreamEndCapture(mg.stream, &graph);
-  if (ce != hipSuccess) return fail(EG_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
-  const hipError_t ie = hipGraphInstantiate(&mg.exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (ie != hipSuccess) {
-    mg.exec = nullptr;
-    return fail(EG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
-  }
-  mg.key.assign(reinterpret_cast<const char*>(&a), reinterpret_cast<const char*>(&a) + sizeof(Args));
-  mg.key.insert(mg.key.end(), reinterpret_cast<const char*>(&gr), reinterpret_cast<const char*>(&gr) + sizeof(gr));
-  const void* fns[3] = {reinterpret_cast<const void*>(push), reinterpret_cast<const void*>(pull),
-                        reinterpret_cast<const void*>(end)};
-  mg.key.insert(mg.key.end(), reinterpret_cast<const char*>(fns), reinterpret_cast<const char*>(fns) + sizeof(fns));
-  return EG_OK;
-}
-
+// Runs steps until a step has an empty frontier (push, pull, level end per
+// step, as a captured step graph).
 template <class Args>
 int run_steps(eg_graph* g, const Args& a, void (*push)(Args, uint32_t), void (*pull)(Args, uint32_t),
               void (*end)(Args, uint32_t), uint32_t* h_steps, hipStream_t s) {
-  MultiGraph& mg = g->multi_graph;
-  EG_TRY(mg.init());
-  const StepGrids gr{8192, grid_for(g->tv_nitems, Args::kPullItemsPerWave * (kNT / kWave), 65536),
-                     grid_for(ceil_div(g->V, kWave), kNT / kWave, 1024)};
-  std::vector<char> key(reinterpret_cast<const char*>(&a), reinterpret_cast<const char*>(&a) + sizeof(Args));
-  key.insert(key.end(), reinterpret_cast<const char*>(&gr), reinterpret_cast<const char*>(&gr) + sizeof(gr));
-  const void* fns[3] = {reinterpret_cast<const void*>(push), reinterpret_cast<const void*>(pull),
-                        reinterpret_cast<const void*>(end)};
-  key.insert(key.end(), reinterpret_cast<const char*>(fns), reinterpret_cast<const char*>(fns) + sizeof(fns));
-  if (!mg.exec || mg.key != key) EG_TRY(capture_steps(mg, a, gr, push, pull, end));
-  // the caller's queued work (state set-up) before the graph, the graph
-  // before the caller's later work
-  EG_HIP(hipEventRecord(mg.ev_in, s));
-  EG_HIP(hipStreamWaitEvent(mg.stream, mg.ev_in, 0));
-  const uint32_t max_steps = g->V + 1;  // a frontier is empty after at most V steps
-  uint32_t step = 0;
-  for (;;) {
-    EG_HIP(hipGraphLaunch(mg.exec, mg.stream));
-    step += kBatch;
-    unsigned long long next_nv = 0;
-    EG_HIP(hipMemcpyAsync(&next_nv, &a.st.cnt[step % 3].nv, 8, hipMemcpyDeviceToHost, mg.stream));
-    EG_HIP(hipStreamSynchronize(mg.stream));
-    if (!next_nv || step >= max_steps) break;
-  }
-  EG_HIP(hipMemcpyAsync(h_steps, a.st.steps, 4, hipMemcpyDeviceToHost, mg.stream));
-  EG_HIP(hipStreamSynchronize(mg.stream));
-  EG_HIP(hipEventRecord(mg.ev_out, mg.stream));
-  EG_HIP(hipStreamWaitEvent(s, mg.ev_out, 0));
-  return EG_OK;
+  const StepLaunch<Args> ls[3] = {{push, 8192},
+                                  {pull, grid_for(g->tv_nitems, Args::kPullItemsPerWave * (kNT / kWave), 65536)},
+                                  {end, grid_for(ceil_div(g->V, kWave), kNT / kWave, 1024)}};
+  const unsigned long long* const nv[3] = {&a.st.cnt[0].nv, &a.st.cnt[1].nv, &a.st.cnt[2].nv};
+  return run_step_graph(g->multi_graph, a, ls, a.st.base, nv, a.st.steps, g->V + 1, h_steps, s);
 }
 
 }  // namespace
