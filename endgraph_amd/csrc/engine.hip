@@ -832,11 +832,12 @@ __global__ __launch_bounds__(kNT) void k_sbfs_push(SbfsArgs a, uint32_t j) {
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
   const uint64_t* fr = a.st.bm[step & 1];
   uint64_t* nx = a.st.bm[(step + 1) & 1];
+  // parent is V until a vertex is reached: the one atomic that sees V
+  // sets its next bit (no read of the next bitmap per edge)
   auto relax = [&](uint32_t u, uint32_t e) {
     const uint32_t v = a.out_dst[e];
     if (!bm_get(a.st.vis, v) && a.parent[v] > u) {
-      atomicMin(&a.parent[v], u);
-      if (!bm_get(nx, v)) bm_set(nx, v);
+      if (atomicMin(&a.parent[v], u) == a.V) bm_set(nx, v);
     }
   };
   // the frontier bitmap, 64 words per wave step: each lane first runs the
@@ -901,14 +902,22 @@ __global__ __launch_bounds__(kNT) void k_sbfs_pull_items(SbfsArgs a, uint32_t j)
 
 // Level end (bfs.cpp:70-90): visited |= next, depth = level + 1, counters
 // and push items of the next step; clears this step's frontier bitmap (the
-// next step's next).  A wave loads 64 words at once and walks the
-// non-empty ones with a lane per vertex.
+// next step's next).  A wave takes kEndWords bitmap words at a time with a
+// lane per vertex of each word; the depth stores and degree loads of all
+// its words are issued before the first degree is used, and the degrees
+// stay in registers for the hubs' items.
+constexpr int kEndWords = 16;
+
 __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
-  const bool live = a.st.cnt[step % 3].nv != 0;
+  const SCount c = a.st.cnt[step % 3];
+  const bool live = c.nv != 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.st.cnt[(step + 2) % 3] = SCount{0, 0, 0};  // last read at step - 1
-    if (live) *a.st.steps = step + 1;
+    if (live) {
+      *a.st.steps = step + 1;
+      *a.st.total += c.ne;  // this step's frontier edges (its only writer)
+    }
   }
   if (!live) return;  // block-uniform
   const int lane = threadIdx.x & (kWave - 1);
@@ -919,50 +928,46 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
   const uint64_t* nx = a.st.bm[(step + 1) & 1];
   uint64_t* items = a.st.items[(step + 1) & 1];
   SCount* nc = &a.st.cnt[(step + 1) % 3];
-  // pass 1: depths, visited, counts, and this lane's push items
   unsigned long long nv = 0, ne = 0;
-  uint32_t ksum = 0;
-  for (uint64_t b0 = gw * kWave; b0 < words; b0 += nw * kWave) {
-    const uint64_t wi = b0 + lane;
-    const uint64_t mine = wi < words ? nx[wi] : 0ull;
-    if (wi < words) fr[wi] = 0;
-    if (mine) {
-      a.st.vis[wi] |= mine;
-      nv += __popcll(mine);
+  for (uint64_t b0 = gw * kEndWords; b0 < words; b0 += nw * kEndWords) {
+    // lanes below kEndWords move the words: visited |= next, frontier = 0
+    uint64_t mine = 0;
+    if (lane < kEndWords && b0 + lane < words) {
+      mine = nx[b0 + lane];
+      fr[b0 + lane] = 0;
+      if (mine) a.st.vis[b0 + lane] |= mine;
     }
-    uint64_t nz = __ballot(mine != 0);
-    while (nz) {  // wave-uniform
-      const int jj = __ffsll((unsigned long long)nz) - 1;
-      nz &= nz - 1;
-      if ((shfl_t(mine, jj) >> lane) & 1ull) {
-        const uint32_t v = uint32_t((b0 + jj) * 64 + lane);
+    nv += __popcll(mine);
+    uint32_t d[kEndWords];
+#pragma unroll
+    for (int k = 0; k < kEndWords; ++k) {
+      const uint64_t wk = lane_value(mine, k);
+      d[k] = 0;
+      if ((wk >> lane) & 1ull) {
+        const uint32_t v = uint32_t((b0 + k) * 64 + lane);
         a.depth[v] = step + 1;
-        const uint32_t d = a.deg[v];
-        ne += d;
-        if (d > kSbfsHub) ksum += uint32_t(ceil_div(d, kSbfsItemEdges));
+        d[k] = a.deg[v];
       }
     }
-  }
-  // one reservation per wave; pass 2 (only waves that met a hub) writes
-  // the hubs' items (v << 32 | chunk)
-  const uint32_t inc = wave_inclusive_scan(ksum, OpSum{});
-  const uint32_t tot = __shfl(inc, kWave - 1, kWave);
-  unsigned long long at0 = 0;
-  if (lane == 0 && tot) at0 = atomicAdd(&nc->items, (unsigned long long)tot);
-  uint64_t at = __shfl(at0, 0, kWave) + inc - ksum;
-  if (tot) {
-    for (uint64_t b0 = gw * kWave; b0 < words; b0 += nw * kWave) {
-      const uint64_t wi = b0 + lane;
-      const uint64_t mine = wi < words ? nx[wi] : 0ull;
-      uint64_t nz = __ballot(mine != 0);
-      while (nz) {  // wave-uniform
-        const int jj = __ffsll((unsigned long long)nz) - 1;
-        nz &= nz - 1;
-        if ((shfl_t(mine, jj) >> lane) & 1ull) {
-          const uint32_t v = uint32_t((b0 + jj) * 64 + lane);
-          const uint32_t d = a.deg[v];
-          const uint32_t n = d > kSbfsHub ? uint32_t(ceil_div(d, kSbfsItemEdges)) : 0u;
-          for (uint32_t k = 0; k < n; ++k) items[at + k] = (uint64_t(v) << 32) | k;
+    uint32_t n_items = 0;  // this lane's hub items in the chunk
+#pragma unroll
+    for (int k = 0; k < kEndWords; ++k) {
+      ne += d[k];
+      n_items += d[k] > kSbfsHub ? uint32_t(ceil_div(d[k], kSbfsItemEdges)) : 0u;
+    }
+    // hubs are rare: one reservation per chunk that holds any
+    const uint32_t inc = wave_inclusive_scan(n_items, OpSum{});
+    const uint32_t tot = lane_value(inc, kWave - 1);
+    if (tot) {  // wave-uniform
+      unsigned long long at0 = 0;
+      if (lane == 0) at0 = atomicAdd(&nc->items, (unsigned long long)tot);
+      uint64_t at = __shfl(at0, 0, kWave) + inc - n_items;
+#pragma unroll
+      for (int k = 0; k < kEndWords; ++k) {
+        if (d[k] > kSbfsHub) {
+          const uint32_t v = uint32_t((b0 + k) * 64 + lane);
+          const uint32_t n = uint32_t(ceil_div(d[k], kSbfsItemEdges));
+          for (uint32_t q = 0; q < n; ++q) items[at + q] = (uint64_t(v) << 32) | q;
           at += n;
         }
       }
@@ -983,10 +988,7 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
       y += part[1][k];
     }
     if (x) atomicAdd(&nc->nv, x);
-    if (y) {
-      atomicAdd(&nc->ne, y);
-      atomicAdd(a.st.total, y);
-    }
+    if (y) atomicAdd(&nc->ne, y);
   }
 }
 
@@ -1007,7 +1009,7 @@ __global__ void k_sbfs_init(SbfsArgs a, uint32_t root) {
     a.st.cnt[2] = SCount{0, 0, 0};
     *a.st.steps = 0;
     *a.st.base = 0;
-    *a.st.total = d;
+    *a.st.total = 0;
   }
 }
 
@@ -1492,7 +1494,7 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
                                       {k_sbfs_pull_rows, grid_for(V, kNT, 16384)},
                                       {k_sbfs_pull_items<0>, grid_for(a.n_big, kNT, 65536)},
                                       {k_sbfs_pull_items<1>, grid_for(a.n_big_rest, kNT, 65536)},
-                                      {k_sbfs_end, grid_for(words, kNT, 1024)}};
+                                      {k_sbfs_end, grid_for(ceil_div(words, uint64_t(kEndWords)), kNT / kWave, 2048)}};
   const unsigned long long* const nv[3] = {&st.cnt[0].nv, &st.cnt[1].nv, &st.cnt[2].nv};
   uint32_t steps = 0;
   prof_begin(s);
