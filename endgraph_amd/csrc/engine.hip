@@ -152,6 +152,28 @@ __global__ void k_soa_csr(const R* __restrict__ lst, uint64_t E, uint32_t* __res
   }
 }
 
+// Single-root BFS push: vertices of out-degree above kSbfsHub ("hubs") are
+// cut into work items of kSbfsItemEdges out-edges, listed once per graph.
+constexpr uint32_t kSbfsItemEdges = 1024;  // out-edges per push work item of a hub
+constexpr uint32_t kSbfsHub = 4096;        // out-degree above which a vertex is cut into items
+
+__global__ void k_hub_item_count(const uint32_t* __restrict__ deg, uint32_t V, uint32_t* __restrict__ cnt) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
+    cnt[v] = deg[v] > kSbfsHub ? uint32_t(ceil_div(deg[v], kSbfsItemEdges)) : 0u;
+}
+
+__global__ void k_hub_items(const uint32_t* __restrict__ off, uint32_t V, uint64_t* __restrict__ items) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
+    for (uint32_t j = 0; j < off[v + 1] - off[v]; ++j) items[off[v] + j] = (v << 32) | j;
+}
+
+// out-degree in one byte, saturated at 255 (the exact value is read only
+// for the saturated ones)
+__global__ void k_deg8(const uint32_t* __restrict__ deg, uint32_t V, uint8_t* __restrict__ deg8) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
+    deg8[v] = uint8_t(deg[v] < 255u ? deg[v] : 255u);
+}
+
 template <class R>
 int prepare_impl(eg_graph* g, hipStream_t s) {
   const uint64_t E = g->E;
@@ -226,7 +248,29 @@ int prepare_impl(eg_graph* g, hipStream_t s) {
       EG_HIP(hipGetLastError());
     }
   }
-  EG_HIP(hipStreamSynchronize(s));
+  EG_TRY(g->e_deg8.resize(V ? V : 1));
+  hipLaunchKernelGGL(k_deg8, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s, (const uint32_t*)g->out_deg.as<uint32_t>(),
+                     V, g->e_deg8.as<uint8_t>());
+  EG_HIP(hipGetLastError());
+  {  // the hubs' push items: a count per vertex, its exclusive prefix, the items
+    DevBuf off;
+    EG_TRY(off.resize((uint64_t(V) + 1) * 4));
+    uint32_t* o = off.as<uint32_t>();
+    hipLaunchKernelGGL(k_hub_item_count, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
+                       (const uint32_t*)g->out_deg.as<uint32_t>(), V, o);
+    EG_HIP(hipGetLastError());
+    EG_TRY(fill<uint32_t>(o + V, 1, 0u, s));
+    EG_TRY(exclusive_scan_u32(o, o, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
+    uint32_t n = 0;
+    EG_HIP(hipMemcpyAsync(&n, o + V, 4, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    EG_TRY(g->e_hub_items.resize(std::max<uint64_t>(n, 1) * 8));
+    g->e_nhub_items = n;
+    hipLaunchKernelGGL(k_hub_items, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s, (const uint32_t*)o, V,
+                       g->e_hub_items.as<uint64_t>());
+    EG_HIP(hipGetLastError());
+    EG_HIP(hipStreamSynchronize(s));  // `off` is freed on scope exit
+  }
   return EG_OK;
 }
 
@@ -685,27 +729,40 @@ __device__ __forceinline__ void bfs_pull_rows(uint32_t V, const uint32_t* __rest
                                               const uint64_t* __restrict__ frontier,
                                               const uint64_t* __restrict__ visited, uint32_t* __restrict__ parent,
                                               uint64_t* __restrict__ active_out) {
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x) {
-    if (bm_get(visited, uint32_t(v))) continue;
-    const uint32_t b = in_ptr[v], e = in_ptr[v + 1];
-    if (e - b > kTvSmallRow) continue;  // a wave item's row
-    for (uint32_t k = b; k < e; k += 4) {
-      uint32_t u[4];
-      bool f[4];
+  // a wave's lanes are the 64 vertices of one bitmap word, so the word's
+  // found rows are one plain store (the item launches, the word's other
+  // writers, run after this one)
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~(kWave - 1)); v0 < V; v0 += stride) {
+    const uint64_t vis = visited[v0 >> 6];
+    if (vis == ~0ull) continue;  // wave-uniform
+    const uint64_t v = v0 + lane;
+    bool found = false;
+    if (v < V && !((vis >> lane) & 1ull)) {
+      const uint32_t b = in_ptr[v], e = in_ptr[v + 1];
+      if (e - b <= kTvSmallRow) {  // else a wave item's row
+        for (uint32_t k = b; k < e; k += 4) {
+          uint32_t u[4];
+          bool f[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) u[j] = k + j < e ? in_src[k + j] : 0u;
+          for (int j = 0; j < 4; ++j) u[j] = k + j < e ? in_src[k + j] : 0u;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) f[j] = k + j < e && bm_get(frontier, u[j]);
-      int hit = -1;
+          for (int j = 0; j < 4; ++j) f[j] = k + j < e && bm_get(frontier, u[j]);
+          int hit = -1;
 #pragma unroll
-      for (int j = 3; j >= 0; --j)
-        if (f[j]) hit = j;
-      if (hit >= 0) {
-        parent[v] = u[hit];  // the row's only writer in this step
-        bm_set(active_out, uint32_t(v));
-        break;
+          for (int j = 3; j >= 0; --j)
+            if (f[j]) hit = j;
+          if (hit >= 0) {
+            parent[v] = u[hit];  // the row's only writer in this step
+            found = true;
+            break;
+          }
+        }
       }
     }
+    const uint64_t m = __ballot(found);
+    if (lane == 0 && m) active_out[v0 >> 6] = m;
   }
 }
 
@@ -778,25 +835,32 @@ __global__ __launch_bounds__(kNT) void k_bfs_pull_items(const uint64_t* __restri
 // ------------------------------------------- single-root BFS, device-driven
 // eg_bfs's levels as steps of a captured graph (no host round trip per
 // level): the level end counts the next frontier and its out-edges (which
-// pick the next step's direction on the device) and cuts the out-edges of
-// its hubs (out-degree above kSbfsHub) into push work items of
-// kSbfsItemEdges edges; a push step walks the frontier bitmap for the other
-// vertices and the items for the hubs.  The frontier and next bitmaps
+// pick the next step's direction on the device); a push step walks the
+// frontier bitmap for the vertices up to kSbfsHub out-edges and the graph's
+// list of hub items for the others.  The frontier and next bitmaps
 // alternate by step parity, and the level end clears the bitmap the next
 // step writes into.
-constexpr uint32_t kSbfsItemEdges = 1024;  // out-edges per push work item of a hub
-constexpr uint32_t kSbfsHub = 4096;        // out-degree above which a vertex is cut into items
 constexpr uint32_t kSbfsLaneEdges = 16;    // out-degree up to which a vertex runs in one lane
 
+// Counters of one step's frontier.  The frontier's out-edges are added by
+// every block of the level end, so they are spread over kNeSlots cache
+// lines (atomics on one address serialise); the vertex count is only ever
+// tested for zero, so it is a flag set by plain stores.
+constexpr int kNeSlots = 16;
+
 struct SCount {
-  unsigned long long nv, ne;  // frontier vertices and their out-edges
-  unsigned long long items;   // push work items listed for the step
+  unsigned long long nv;  // nonzero iff the frontier has a vertex
+  unsigned long long pad[7];
+  unsigned long long ne[kNeSlots][8];  // out-edges: the sum of ne[i][0]
+};
+
+struct SCountView {
+  unsigned long long nv, ne;
 };
 
 struct SbfsState {
   uint64_t* bm[2];     // frontier of step s is bm[s & 1], its next bm[(s + 1) & 1]
   uint64_t* vis;
-  uint64_t* items[2];  // push items of step s: items[s & 1]
   SCount* cnt;         // ring of 3 by step
   uint32_t* steps;     // steps that had a frontier
   uint32_t* base;      // first step of the queued batch
@@ -808,11 +872,22 @@ struct SbfsArgs {
   uint64_t E;
   int policy;
   const uint32_t *out_ptr, *out_dst, *deg, *in_ptr, *in_src;
+  const uint8_t* deg8;
   const uint64_t *big, *big_rest;
   uint64_t n_big, n_big_rest;
+  const uint64_t* hub_items;
+  uint64_t n_hub_items;
   SbfsState st;
   uint32_t *parent, *depth;
 };
+
+__device__ __forceinline__ SCountView count_of(const SbfsArgs& a, uint32_t step) {
+  const SCount& c = a.st.cnt[step % 3];
+  SCountView r{c.nv, 0};
+#pragma unroll
+  for (int i = 0; i < kNeSlots; ++i) r.ne += c.ne[i][0];
+  return r;
+}
 
 __device__ __forceinline__ bool sbfs_sparse(int policy, unsigned long long ne, uint64_t E) {
   if (policy == EG_POLICY_PULL) return false;
@@ -820,26 +895,66 @@ __device__ __forceinline__ bool sbfs_sparse(int policy, unsigned long long ne, u
   return true;  // reference (rooted programs push, R11) and push
 }
 
-// Sparse step (bfs.cpp:52-58 with R12): frontier vertices of out-degree up
-// to kSbfsLaneEdges run in their lane, up to kSbfsHub with the whole wave;
-// hubs' items are strided over the waves.
-__global__ __launch_bounds__(kNT) void k_sbfs_push(SbfsArgs a, uint32_t j) {
-  const uint32_t step = *a.st.base + j;
-  const SCount c = a.st.cnt[step % 3];
-  if (!c.nv || !sbfs_sparse(a.policy, c.ne, a.E)) return;
+// A push step whose frontier has at least V/4 out-edges leaves the next
+// bitmap to the level end, which reads it off parent (V until reached) with
+// one coalesced pass: cheaper than an atomic per reached vertex.
+__device__ __forceinline__ bool sbfs_derive(unsigned long long ne, uint32_t V) { return ne >= V / 4; }
+
+constexpr int kRelaxBatch = 16;  // out-edges per lane whose loads are all in flight at once
+
+// Push relaxation (bfs.cpp:52-58 with R12) of the out-edges
+// e = first + k * stride, k < n <= kRelaxBatch, of frontier vertex u: the
+// targets, their visited words and their parents are each loaded for the
+// whole batch before the first is used, then the atomics of the batch go
+// out together.  parent is V until a vertex is reached, so the one atomic
+// that sees V sets the vertex's next bit (kDerive: the level end does).
+static_assert(kSbfsLaneEdges <= uint32_t(kRelaxBatch), "a lane's vertex is one batch");
+
+template <bool kDerive>
+__device__ __forceinline__ void sbfs_relax(const SbfsArgs& a, uint64_t* __restrict__ nx, uint32_t u, uint32_t first,
+                                           uint32_t stride, uint32_t n) {
+  uint32_t v[kRelaxBatch], p[kRelaxBatch], old[kRelaxBatch];
+  uint64_t w[kRelaxBatch];
+#pragma unroll
+  for (int k = 0; k < kRelaxBatch; ++k) v[k] = uint32_t(k) < n ? a.out_dst[first + uint32_t(k) * stride] : 0u;
+#pragma unroll
+  for (int k = 0; k < kRelaxBatch; ++k) w[k] = uint32_t(k) < n ? a.st.vis[v[k] >> 6] : ~0ull;
+#pragma unroll
+  for (int k = 0; k < kRelaxBatch; ++k) p[k] = ((w[k] >> (v[k] & 63)) & 1ull) ? 0u : a.parent[v[k]];
+  if constexpr (kDerive) {
+#pragma unroll
+    for (int k = 0; k < kRelaxBatch; ++k)
+      if (p[k] > u) atomicMin(&a.parent[v[k]], u);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRelaxBatch; ++k) old[k] = p[k] > u ? atomicMin(&a.parent[v[k]], u) : 0u;
+#pragma unroll
+    for (int k = 0; k < kRelaxBatch; ++k)
+      if (old[k] == a.V) bm_set(nx, v[k]);
+  }
+}
+
+// the edges [e0, e1) of u over the wave, kRelaxBatch per lane at a time
+template <bool kDerive>
+__device__ __forceinline__ void sbfs_relax_wave(const SbfsArgs& a, uint64_t* __restrict__ nx, uint32_t u, uint32_t e0,
+                                                uint32_t e1, int lane) {
+  for (uint32_t b = e0; b < e1; b += kWave * kRelaxBatch) {
+    const uint32_t f = b + uint32_t(lane);
+    const uint32_t n = f < e1 ? ceil_div(e1 - f, uint32_t(kWave)) : 0u;
+    sbfs_relax<kDerive>(a, nx, u, f, kWave, n < uint32_t(kRelaxBatch) ? n : uint32_t(kRelaxBatch));
+  }
+}
+
+// Sparse step: frontier vertices of out-degree up to kSbfsLaneEdges run in
+// their lane, up to kSbfsHub with the whole wave; hubs' items are strided
+// over the waves.
+template <bool kDerive>
+__device__ __forceinline__ void sbfs_push(const SbfsArgs& a, uint32_t step) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
   const uint64_t* fr = a.st.bm[step & 1];
   uint64_t* nx = a.st.bm[(step + 1) & 1];
-  // parent is V until a vertex is reached: the one atomic that sees V
-  // sets its next bit (no read of the next bitmap per edge)
-  auto relax = [&](uint32_t u, uint32_t e) {
-    const uint32_t v = a.out_dst[e];
-    if (!bm_get(a.st.vis, v) && a.parent[v] > u) {
-      if (atomicMin(&a.parent[v], u) == a.V) bm_set(nx, v);
-    }
-  };
   // the frontier bitmap, 64 words per wave step: each lane first runs the
   // small vertices of its own word, then the wave walks the words that hold
   // larger ones
@@ -852,7 +967,7 @@ __global__ __launch_bounds__(kNT) void k_sbfs_push(SbfsArgs a, uint32_t j) {
       const uint32_t u = uint32_t(wi * 64 + __ffsll((unsigned long long)m) - 1);
       const uint32_t e0 = a.out_ptr[u], e1 = a.out_ptr[u + 1];
       if (e1 - e0 <= kSbfsLaneEdges)
-        for (uint32_t e = e0; e < e1; ++e) relax(u, e);
+        sbfs_relax<kDerive>(a, nx, u, e0, 1, e1 - e0);
       else if (e1 - e0 <= kSbfsHub)
         has_mid = true;
     }
@@ -867,26 +982,48 @@ __global__ __launch_bounds__(kNT) void k_sbfs_push(SbfsArgs a, uint32_t j) {
       while (mid) {  // wave-uniform
         const int l = __ffsll((unsigned long long)mid) - 1;
         mid &= mid - 1;
-        const uint32_t uu = __shfl(u, l, kWave), b1 = __shfl(e0, l, kWave), b2 = __shfl(e1, l, kWave);
-        for (uint32_t e = b1 + lane; e < b2; e += kWave) relax(uu, e);
+        sbfs_relax_wave<kDerive>(a, nx, __shfl(u, l, kWave), __shfl(e0, l, kWave), __shfl(e1, l, kWave), lane);
       }
     }
   }
-  // hub items, strided over the waves
-  const uint64_t* items = a.st.items[step & 1];
-  for (uint64_t i = gw; i < c.items; i += nw) {
-    const uint64_t it = items[i];
-    const uint32_t u = uint32_t(it >> 32);
-    const uint32_t e0 = a.out_ptr[u] + uint32_t(it) * kSbfsItemEdges;
-    const uint32_t e1 = min(e0 + kSbfsItemEdges, a.out_ptr[u + 1]);
-    for (uint32_t e = e0 + lane; e < e1; e += kWave) relax(u, e);
+  // hubs: the graph's list of their items, 64 per wave step with item
+  // (r + lane) * nw + gw in lane `lane` of wave gw (a hub's consecutive
+  // items go to different waves); the lanes whose hub is in the frontier
+  // are run by the wave one after another
+  for (uint64_t r = 0; r * nw < a.n_hub_items; r += kWave) {
+    const uint64_t i = (r + lane) * nw + gw;
+    uint64_t it = 0;
+    bool live = false;
+    if (i < a.n_hub_items) {
+      it = a.hub_items[i];
+      live = bm_get(fr, uint32_t(it >> 32));
+    }
+    uint64_t lm = __ballot(live);
+    while (lm) {  // wave-uniform
+      const int l = __ffsll((unsigned long long)lm) - 1;
+      lm &= lm - 1;
+      const uint64_t t = shfl_t(it, l);
+      const uint32_t u = uint32_t(t >> 32);
+      const uint32_t e0 = a.out_ptr[u] + uint32_t(t) * kSbfsItemEdges;
+      sbfs_relax_wave<kDerive>(a, nx, u, e0, min(e0 + kSbfsItemEdges, a.out_ptr[u + 1]), lane);
+    }
   }
+}
+
+__global__ __launch_bounds__(kNT) void k_sbfs_push(SbfsArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
+  const SCountView c = count_of(a, step);
+  if (!c.nv || !sbfs_sparse(a.policy, c.ne, a.E)) return;
+  if (sbfs_derive(c.ne, a.V))
+    sbfs_push<true>(a, step);
+  else
+    sbfs_push<false>(a, step);
 }
 
 // Dense step: the pulls above, on this step's bitmaps.
 __global__ __launch_bounds__(kNT) void k_sbfs_pull_rows(SbfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
-  const SCount c = a.st.cnt[step % 3];
+  const SCountView c = count_of(a, step);
   if (!c.nv || sbfs_sparse(a.policy, c.ne, a.E)) return;
   bfs_pull_rows(a.V, a.in_ptr, a.in_src, a.st.bm[step & 1], a.st.vis, a.parent, a.st.bm[(step + 1) & 1]);
 }
@@ -894,15 +1031,15 @@ __global__ __launch_bounds__(kNT) void k_sbfs_pull_rows(SbfsArgs a, uint32_t j) 
 template <int kRest>
 __global__ __launch_bounds__(kNT) void k_sbfs_pull_items(SbfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
-  const SCount c = a.st.cnt[step % 3];
+  const SCountView c = count_of(a, step);
   if (!c.nv || sbfs_sparse(a.policy, c.ne, a.E)) return;
   bfs_pull_items(kRest ? a.big_rest : a.big, kRest ? a.n_big_rest : a.n_big, a.in_ptr, a.in_src,
                  a.st.bm[step & 1], a.st.vis, a.parent, a.st.bm[(step + 1) & 1]);
 }
 
 // Level end (bfs.cpp:70-90): visited |= next, depth = level + 1, counters
-// and push items of the next step; clears this step's frontier bitmap (the
-// next step's next).  A wave takes kEndWords bitmap words at a time with a
+// of the next step; clears this step's frontier bitmap (the next step's
+// next).  A wave takes kEndWords bitmap words at a time with a
 // lane per vertex of each word; the depth stores and degree loads of all
 // its words are issued before the first degree is used, and the degrees
 // stay in registers for the hubs' items.
@@ -910,10 +1047,14 @@ constexpr int kEndWords = 16;
 
 __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
-  const SCount c = a.st.cnt[step % 3];
+  const SCountView c = count_of(a, step);
   const bool live = c.nv != 0;
+  if (blockIdx.x == 0) {
+    // the slot after next, last read at step - 1
+    unsigned long long* z = reinterpret_cast<unsigned long long*>(&a.st.cnt[(step + 2) % 3]);
+    for (uint32_t i = threadIdx.x; i < sizeof(SCount) / 8; i += blockDim.x) z[i] = 0;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.st.cnt[(step + 2) % 3] = SCount{0, 0, 0};  // last read at step - 1
     if (live) {
       *a.st.steps = step + 1;
       *a.st.total += c.ne;  // this step's frontier edges (its only writer)
@@ -925,14 +1066,35 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
   const uint64_t words = (uint64_t(a.V) >> 6) + 1;
   uint64_t* fr = a.st.bm[step & 1];
-  const uint64_t* nx = a.st.bm[(step + 1) & 1];
-  uint64_t* items = a.st.items[(step + 1) & 1];
+  uint64_t* nx = a.st.bm[(step + 1) & 1];
   SCount* nc = &a.st.cnt[(step + 1) % 3];
+  const bool derive = sbfs_sparse(a.policy, c.ne, a.E) && sbfs_derive(c.ne, a.V);
   unsigned long long nv = 0, ne = 0;
   for (uint64_t b0 = gw * kEndWords; b0 < words; b0 += nw * kEndWords) {
     // lanes below kEndWords move the words: visited |= next, frontier = 0
+    const bool wl = lane < kEndWords && b0 + lane < words;
     uint64_t mine = 0;
-    if (lane < kEndWords && b0 + lane < words) {
+    if (derive) {  // uniform: next = reached (parent set) and not visited
+      const uint64_t vw = wl ? a.st.vis[b0 + lane] : ~0ull;
+      uint32_t pv[kEndWords];
+#pragma unroll
+      for (int k = 0; k < kEndWords; ++k) {
+        const uint64_t v = (b0 + k) * 64 + lane;
+        pv[k] = v < a.V ? a.parent[v] : a.V;
+      }
+#pragma unroll
+      for (int k = 0; k < kEndWords; ++k) {
+        const uint64_t m = __ballot(pv[k] != a.V && !((lane_value(vw, k) >> lane) & 1ull));
+        if (lane == k) mine = m;
+      }
+      if (wl) {
+        fr[b0 + lane] = 0;
+        if (mine) {
+          nx[b0 + lane] = mine;
+          a.st.vis[b0 + lane] = vw | mine;
+        }
+      }
+    } else if (wl) {
       mine = nx[b0 + lane];
       fr[b0 + lane] = 0;
       if (mine) a.st.vis[b0 + lane] |= mine;
@@ -946,32 +1108,14 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
       if ((wk >> lane) & 1ull) {
         const uint32_t v = uint32_t((b0 + k) * 64 + lane);
         a.depth[v] = step + 1;
-        d[k] = a.deg[v];
+        d[k] = a.deg8[v];
       }
     }
-    uint32_t n_items = 0;  // this lane's hub items in the chunk
 #pragma unroll
-    for (int k = 0; k < kEndWords; ++k) {
-      ne += d[k];
-      n_items += d[k] > kSbfsHub ? uint32_t(ceil_div(d[k], kSbfsItemEdges)) : 0u;
-    }
-    // hubs are rare: one reservation per chunk that holds any
-    const uint32_t inc = wave_inclusive_scan(n_items, OpSum{});
-    const uint32_t tot = lane_value(inc, kWave - 1);
-    if (tot) {  // wave-uniform
-      unsigned long long at0 = 0;
-      if (lane == 0) at0 = atomicAdd(&nc->items, (unsigned long long)tot);
-      uint64_t at = __shfl(at0, 0, kWave) + inc - n_items;
+    for (int k = 0; k < kEndWords; ++k)
+      if (d[k] == 255u) d[k] = a.deg[uint32_t((b0 + k) * 64 + lane)];  // saturated: the exact degree
 #pragma unroll
-      for (int k = 0; k < kEndWords; ++k) {
-        if (d[k] > kSbfsHub) {
-          const uint32_t v = uint32_t((b0 + k) * 64 + lane);
-          const uint32_t n = uint32_t(ceil_div(d[k], kSbfsItemEdges));
-          for (uint32_t q = 0; q < n; ++q) items[at + q] = (uint64_t(v) << 32) | q;
-          at += n;
-        }
-      }
-    }
+    for (int k = 0; k < kEndWords; ++k) ne += d[k];
   }
   __shared__ unsigned long long part[2][kNT / kWave];
   nv = wave_reduce(nv, OpSum{});
@@ -987,8 +1131,8 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
       x += part[0][k];
       y += part[1][k];
     }
-    if (x) atomicAdd(&nc->nv, x);
-    if (y) atomicAdd(&nc->ne, y);
+    if (x) nc->nv = 1;
+    if (y) atomicAdd(&nc->ne[blockIdx.x % kNeSlots][0], y);
   }
 }
 
@@ -997,16 +1141,16 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
 // filled before).
 __global__ void k_sbfs_init(SbfsArgs a, uint32_t root) {
   const uint32_t d = a.out_ptr[root + 1] - a.out_ptr[root];
-  const uint32_t n = d > kSbfsHub ? uint32_t(ceil_div(d, kSbfsItemEdges)) : 0u;
-  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) a.st.items[0][k] = (uint64_t(root) << 32) | k;
+  unsigned long long* z = reinterpret_cast<unsigned long long*>(a.st.cnt);
+  for (uint32_t i = threadIdx.x; i < 3 * sizeof(SCount) / 8; i += blockDim.x) z[i] = 0;
+  __syncthreads();
   if (threadIdx.x == 0) {
     a.st.bm[0][root >> 6] = 1ull << (root & 63);
     a.st.vis[root >> 6] = 1ull << (root & 63);
     a.parent[root] = root;
     a.depth[root] = 0;
-    a.st.cnt[0] = SCount{1, d, n};
-    a.st.cnt[1] = SCount{0, 0, 0};
-    a.st.cnt[2] = SCount{0, 0, 0};
+    a.st.cnt[0].nv = 1;
+    a.st.cnt[0].ne[0][0] = d;
     *a.st.steps = 0;
     *a.st.base = 0;
     *a.st.total = 0;
@@ -1451,20 +1595,17 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
   const uint32_t V = g->V;
   const bool pulls = policy == EG_POLICY_AUTO || policy == EG_POLICY_PULL;
   if (pulls) EG_TRY(ensure_tview(g, s));
-  // state: frontier / next bitmaps, visited, push items (two lists), the
-  // counter ring and scalars, and depths when the caller keeps none
+  // state: frontier / next bitmaps, visited, the counter ring and scalars,
+  // and depths when the caller keeps none
   const uint64_t words = words_of(V);
-  const uint64_t item_cap = uint64_t(V) + g->E / kSbfsItemEdges + 1;
-  const uint64_t bytes = 3 * words * 8 + 2 * item_cap * 8 + 3 * sizeof(SCount) + 64 + (d_depth ? 0 : uint64_t(V) * 4);
+  const uint64_t bytes = 3 * words * 8 + 64 + 3 * sizeof(SCount) + 64 + (d_depth ? 0 : uint64_t(V) * 4);
   EG_TRY(g->e_sbfs.resize(bytes));
   SbfsArgs a{};  // zeroed padding: the bytes key the captured step graph
   SbfsState& st = a.st;
   st.bm[0] = g->e_sbfs.as<uint64_t>();
   st.bm[1] = st.bm[0] + words;
   st.vis = st.bm[1] + words;
-  st.items[0] = st.vis + words;
-  st.items[1] = st.items[0] + item_cap;
-  st.cnt = reinterpret_cast<SCount*>(st.items[1] + item_cap);
+  st.cnt = reinterpret_cast<SCount*>((reinterpret_cast<uintptr_t>(st.vis + words) + 63) & ~uintptr_t(63));
   st.total = reinterpret_cast<unsigned long long*>(st.cnt + 3);
   st.steps = reinterpret_cast<uint32_t*>(st.total + 1);
   st.base = st.steps + 1;
@@ -1475,6 +1616,9 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
   a.out_ptr = g->e_out_ptr.as<uint32_t>();
   a.out_dst = g->e_out_dst.as<uint32_t>();
   a.deg = g->out_deg.as<uint32_t>();
+  a.deg8 = g->e_deg8.as<uint8_t>();
+  a.hub_items = g->e_hub_items.as<uint64_t>();
+  a.n_hub_items = g->e_nhub_items;
   if (pulls) {
     a.in_ptr = g->tv_ptr.as<uint32_t>();
     a.in_src = g->tv_src.as<uint32_t>();

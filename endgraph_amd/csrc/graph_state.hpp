@@ -78,6 +78,9 @@ struct eg_graph {
   eg::DevBuf e_out_ptr;              // u32[V+1] merged CSR over all sub-partitions
   eg::DevBuf e_out_dst;              // u32[E]
   eg::DevBuf e_out_w;                // f32[E]
+  eg::DevBuf e_deg8;                 // u8[V]    out-degree saturated at 255 (BFS level ends)
+  eg::DevBuf e_hub_items;            // u64 (v << 32 | chunk): out-edge chunks of the BFS push hubs
+  uint64_t e_nhub_items = 0;
   eg::DevBuf e_scratch;              // frontier list, counters
   eg::DevBuf e_bm[3];                // three bitmaps of (V>>6)+1 words
   eg::DevBuf e_pr;                   // PageRank work vectors
