@@ -108,6 +108,10 @@ struct OpMin {
   template <class T>
   __device__ __forceinline__ T operator()(T a, T b) const { return b < a ? b : a; }
 };
+struct OpOr {
+  template <class T>
+  __device__ __forceinline__ T operator()(T a, T b) const { return a | b; }
+};
 
 // Shuffles through one name so that record types can add specialisations.
 template <class T>

@@ -742,13 +742,15 @@ __device__ __forceinline__ void bfs_pull_rows(uint32_t V, const uint32_t* __rest
     if (v < V && !((vis >> lane) & 1ull)) {
       const uint32_t b = in_ptr[v], e = in_ptr[v + 1];
       if (e - b <= kTvSmallRow) {  // else a wave item's row
-        for (uint32_t k = b; k < e; k += 4) {
-          uint32_t u[4];
+        // 16-byte aligned groups of 4 sources, one vector load each (the
+        // view holds 4 words of padding past its end); sources outside the
+        // row are masked
+        for (uint32_t k = b & ~3u; k < e; k += 4) {
+          const uint4 q = *reinterpret_cast<const uint4*>(in_src + k);
+          const uint32_t u[4] = {q.x, q.y, q.z, q.w};
           bool f[4];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) u[j] = k + j < e ? in_src[k + j] : 0u;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) f[j] = k + j < e && bm_get(frontier, u[j]);
+          for (int j = 0; j < 4; ++j) f[j] = k + j >= b && k + j < e && bm_get(frontier, u[j]);
           int hit = -1;
 #pragma unroll
           for (int j = 3; j >= 0; --j)
@@ -766,6 +768,11 @@ __device__ __forceinline__ void bfs_pull_rows(uint32_t V, const uint32_t* __rest
   }
 }
 
+// kFirst: the items are the first chunks of their rows, one per row, so a
+// found parent is a plain store; other chunks of a row race and take the
+// smallest.  The found rows' next bits are set with one atomic per bitmap
+// word among the wave's items.
+template <bool kFirst>
 __device__ __forceinline__ void bfs_pull_items(const uint64_t* __restrict__ items, uint64_t n_items,
                                                const uint32_t* __restrict__ in_ptr,
                                                const uint32_t* __restrict__ in_src,
@@ -790,11 +797,11 @@ __device__ __forceinline__ void bfs_pull_items(const uint64_t* __restrict__ item
         live = !(parent[my_v] < in_src[my_e0]);
       }
     }
+    uint32_t my_p = ~0u;  // parent found for this lane's item
     uint64_t pending = __ballot(live);
     while (pending) {  // wave-uniform
       const int jj = __ffsll((unsigned long long)pending) - 1;
       pending &= pending - 1;
-      const uint32_t v = __shfl(my_v, jj, kWave);
       const uint32_t e0 = __shfl(my_e0, jj, kWave), e1 = __shfl(my_e1, jj, kWave);
       for (uint32_t e = e0; e < e1; e += kWave) {
         const bool ok = e + lane < e1;
@@ -802,13 +809,25 @@ __device__ __forceinline__ void bfs_pull_items(const uint64_t* __restrict__ item
         const uint64_t m = __ballot(ok && bm_get(frontier, u));
         if (m) {
           const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
-          if (lane == 0) {
-            atomicMin(&parent[v], p);
-            bm_set(active_out, v);
-          }
+          if (lane == jj) my_p = p;
           break;
         }
       }
+    }
+    const bool found = my_p != ~0u;
+    if (found) {
+      if constexpr (kFirst)
+        parent[my_v] = my_p;
+      else
+        atomicMin(&parent[my_v], my_p);
+    }
+    uint64_t fm = __ballot(found);
+    while (fm) {  // wave-uniform: one word of found rows at a time
+      const uint32_t w = __shfl(my_v, __ffsll((unsigned long long)fm) - 1, kWave) >> 6;
+      const bool mine = found && (my_v >> 6) == w;
+      const uint64_t bits = wave_reduce(mine ? 1ull << (my_v & 63) : 0ull, OpOr{});
+      if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&active_out[w]), (unsigned long long)bits);
+      fm &= ~__ballot(mine);
     }
   }
 }
@@ -822,6 +841,7 @@ __global__ __launch_bounds__(kNT) void k_bfs_pull_rows(uint32_t V, const uint32_
   bfs_pull_rows(V, in_ptr, in_src, frontier, visited, parent, active_out);
 }
 
+template <bool kFirst>
 __global__ __launch_bounds__(kNT) void k_bfs_pull_items(const uint64_t* __restrict__ items, uint64_t n_items,
                                                         const uint32_t* __restrict__ in_ptr,
                                                         const uint32_t* __restrict__ in_src,
@@ -829,7 +849,7 @@ __global__ __launch_bounds__(kNT) void k_bfs_pull_items(const uint64_t* __restri
                                                         const uint64_t* __restrict__ visited,
                                                         uint32_t* __restrict__ parent,
                                                         uint64_t* __restrict__ active_out) {
-  bfs_pull_items(items, n_items, in_ptr, in_src, frontier, visited, parent, active_out);
+  bfs_pull_items<kFirst>(items, n_items, in_ptr, in_src, frontier, visited, parent, active_out);
 }
 
 // ------------------------------------------- single-root BFS, device-driven
@@ -1033,7 +1053,7 @@ __global__ __launch_bounds__(kNT) void k_sbfs_pull_items(SbfsArgs a, uint32_t j)
   const uint32_t step = *a.st.base + j;
   const SCountView c = count_of(a, step);
   if (!c.nv || sbfs_sparse(a.policy, c.ne, a.E)) return;
-  bfs_pull_items(kRest ? a.big_rest : a.big, kRest ? a.n_big_rest : a.n_big, a.in_ptr, a.in_src,
+  bfs_pull_items<!kRest>(kRest ? a.big_rest : a.big, kRest ? a.n_big_rest : a.n_big, a.in_ptr, a.in_src,
                  a.st.bm[step & 1], a.st.vis, a.parent, a.st.bm[(step + 1) & 1]);
 }
 
@@ -1070,11 +1090,34 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
   SCount* nc = &a.st.cnt[(step + 1) % 3];
   const bool derive = sbfs_sparse(a.policy, c.ne, a.E) && sbfs_derive(c.ne, a.V);
   unsigned long long nv = 0, ne = 0;
-  for (uint64_t b0 = gw * kEndWords; b0 < words; b0 += nw * kEndWords) {
-    // lanes below kEndWords move the words: visited |= next, frontier = 0
-    const bool wl = lane < kEndWords && b0 + lane < words;
-    uint64_t mine = 0;
-    if (derive) {  // uniform: next = reached (parent set) and not visited
+  // the vertices of up to kEndWords words b0 + idx[k] (idx[k] < 0: none),
+  // whose bits lane idx[k] holds in `mine`: depth stores and degree loads
+  // all issued before the first degree is used
+  auto vertices = [&](uint64_t b0, const int (&idx)[kEndWords], uint64_t mine) {
+    uint32_t d[kEndWords];
+#pragma unroll
+    for (int k = 0; k < kEndWords; ++k) {
+      const uint64_t wk = idx[k] >= 0 ? lane_value(mine, idx[k]) : 0ull;
+      d[k] = 0;
+      if ((wk >> lane) & 1ull) {
+        const uint32_t v = uint32_t((b0 + uint64_t(idx[k])) * 64 + lane);
+        a.depth[v] = step + 1;
+        d[k] = a.deg8[v];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kEndWords; ++k)
+      if (d[k] == 255u) d[k] = a.deg[uint32_t((b0 + uint64_t(idx[k])) * 64 + lane)];  // saturated: the exact degree
+#pragma unroll
+    for (int k = 0; k < kEndWords; ++k) ne += d[k];
+  };
+  if (derive) {
+    // next = reached (parent set) and not visited, kEndWords words at a
+    // time (a big push level: most words have vertices); lanes below
+    // kEndWords move the words
+    for (uint64_t b0 = gw * kEndWords; b0 < words; b0 += nw * kEndWords) {
+      const bool wl = lane < kEndWords && b0 + lane < words;
+      uint64_t mine = 0;
       const uint64_t vw = wl ? a.st.vis[b0 + lane] : ~0ull;
       uint32_t pv[kEndWords];
 #pragma unroll
@@ -1094,28 +1137,35 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
           a.st.vis[b0 + lane] = vw | mine;
         }
       }
-    } else if (wl) {
-      mine = nx[b0 + lane];
-      fr[b0 + lane] = 0;
-      if (mine) a.st.vis[b0 + lane] |= mine;
-    }
-    nv += __popcll(mine);
-    uint32_t d[kEndWords];
+      nv += __popcll(mine);
+      int idx[kEndWords];
 #pragma unroll
-    for (int k = 0; k < kEndWords; ++k) {
-      const uint64_t wk = lane_value(mine, k);
-      d[k] = 0;
-      if ((wk >> lane) & 1ull) {
-        const uint32_t v = uint32_t((b0 + k) * 64 + lane);
-        a.depth[v] = step + 1;
-        d[k] = a.deg8[v];
+      for (int k = 0; k < kEndWords; ++k) idx[k] = k;
+      vertices(b0, idx, mine);
+    }
+  } else {
+    // 64 words per wave step, a lane each (visited |= next, frontier = 0),
+    // then the non-empty ones kEndWords at a time
+    for (uint64_t b0 = gw * kWave; b0 < words; b0 += nw * kWave) {
+      const uint64_t wi = b0 + lane;
+      uint64_t mine = 0;
+      if (wi < words) {
+        mine = nx[wi];
+        fr[wi] = 0;
+        if (mine) a.st.vis[wi] |= mine;
+      }
+      nv += __popcll(mine);
+      uint64_t nz = __ballot(mine != 0);
+      while (nz) {  // wave-uniform
+        int idx[kEndWords];
+#pragma unroll
+        for (int k = 0; k < kEndWords; ++k) {
+          idx[k] = nz ? __ffsll((unsigned long long)nz) - 1 : -1;
+          nz &= nz - 1;
+        }
+        vertices(b0, idx, mine);
       }
     }
-#pragma unroll
-    for (int k = 0; k < kEndWords; ++k)
-      if (d[k] == 255u) d[k] = a.deg[uint32_t((b0 + k) * 64 + lane)];  // saturated: the exact degree
-#pragma unroll
-    for (int k = 0; k < kEndWords; ++k) ne += d[k];
   }
   __shared__ unsigned long long part[2][kNT / kWave];
   nv = wave_reduce(nv, OpSum{});
@@ -1136,21 +1186,30 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
   }
 }
 
-// The root's state: its bit in frontier and visited, parent and depth, the
-// first counters and push items (the bitmaps, parent and depth arrays are
-// filled before).
+// The start state in one launch: parent V and depth ~0 but the root's,
+// the frontier, next and visited bitmaps empty but the root's bit in the
+// frontier and visited, the counter ring and scalars (the frontier is the
+// root and its out-edges).
 __global__ void k_sbfs_init(SbfsArgs a, uint32_t root) {
-  const uint32_t d = a.out_ptr[root + 1] - a.out_ptr[root];
+  const uint64_t words = (uint64_t(a.V) >> 6) + 1;
+  const uint64_t n = a.V > 3 * words ? a.V : 3 * words;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    if (i < a.V) {
+      a.parent[i] = i == root ? root : a.V;
+      a.depth[i] = i == root ? 0u : 0xffffffffu;
+    }
+    if (i < 3 * words) {  // bm[0], bm[1], vis are contiguous
+      const bool rw = i == (root >> 6) || i == 2 * words + (root >> 6);
+      a.st.bm[0][i] = rw ? 1ull << (root & 63) : 0ull;
+    }
+  }
+  if (blockIdx.x != 0) return;
   unsigned long long* z = reinterpret_cast<unsigned long long*>(a.st.cnt);
   for (uint32_t i = threadIdx.x; i < 3 * sizeof(SCount) / 8; i += blockDim.x) z[i] = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
-    a.st.bm[0][root >> 6] = 1ull << (root & 63);
-    a.st.vis[root >> 6] = 1ull << (root & 63);
-    a.parent[root] = root;
-    a.depth[root] = 0;
     a.st.cnt[0].nv = 1;
-    a.st.cnt[0].ne[0][0] = d;
+    a.st.cnt[0].ne[0][0] = a.out_ptr[root + 1] - a.out_ptr[root];
     *a.st.steps = 0;
     *a.st.base = 0;
     *a.st.total = 0;
@@ -1561,11 +1620,11 @@ int bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontie
     // first chunks of the large rows, then the others, which an earlier
     // chunk's hit (always smaller: sources ascend) lets skip
     if (g->tv_nbig)
-      hipLaunchKernelGGL(k_bfs_pull_items, dim3(grid_for(g->tv_nbig, kNT, 65536)), dim3(kNT), 0, s,
+      hipLaunchKernelGGL(k_bfs_pull_items<true>, dim3(grid_for(g->tv_nbig, kNT, 65536)), dim3(kNT), 0, s,
                          (const uint64_t*)g->tv_big.as<uint64_t>(), g->tv_nbig, in_ptr, in_src, d_frontier,
                          (const uint64_t*)d_visited, d_parent, d_next);
     if (g->tv_nbig_rest)
-      hipLaunchKernelGGL(k_bfs_pull_items, dim3(grid_for(g->tv_nbig_rest, kNT, 65536)), dim3(kNT), 0, s,
+      hipLaunchKernelGGL(k_bfs_pull_items<false>, dim3(grid_for(g->tv_nbig_rest, kNT, 65536)), dim3(kNT), 0, s,
                          (const uint64_t*)g->tv_big_rest.as<uint64_t>(), g->tv_nbig_rest, in_ptr, in_src, d_frontier,
                          (const uint64_t*)d_visited, d_parent, d_next);
     prof_end(PK_BFS, uint64_t(g->V) * 12, s);
@@ -1629,10 +1688,8 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
   }
   a.parent = d_parent;
   a.depth = depth;
-  EG_TRY(fill<uint64_t>(st.bm[0], 3 * words, 0ull, s));
-  EG_TRY(fill<uint32_t>(d_parent, V, V, s));
-  EG_TRY(fill<uint32_t>(depth, V, 0xffffffffu, s));
-  hipLaunchKernelGGL(k_sbfs_init, dim3(1), dim3(kNT), 0, s, a, root);
+  hipLaunchKernelGGL(k_sbfs_init, dim3(grid_for(std::max<uint64_t>(V, 3 * words), kNT, 65536)), dim3(kNT), 0, s, a,
+                     root);
   EG_HIP(hipGetLastError());
   const StepLaunch<SbfsArgs> ls[5] = {{k_sbfs_push, 2048},
                                       {k_sbfs_pull_rows, grid_for(V, kNT, 16384)},
@@ -1642,11 +1699,9 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
   const unsigned long long* const nv[3] = {&st.cnt[0].nv, &st.cnt[1].nv, &st.cnt[2].nv};
   uint32_t steps = 0;
   prof_begin(s);
-  EG_TRY(run_step_graph(g->multi_graph, a, ls, st.base, nv, st.steps, V + 1, &steps, s));
-  prof_end(PK_BFS, 0, s);
   unsigned long long total = 0;
-  EG_HIP(hipMemcpyAsync(&total, st.total, 8, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
+  EG_TRY(run_step_graph(g->multi_graph, a, ls, st.base, nv, st.steps, V + 1, &steps, s, st.total, &total));
+  prof_end(PK_BFS, 0, s);
   if (h_levels) *h_levels = steps;
   if (h_edges) *h_edges = total;
   return EG_OK;

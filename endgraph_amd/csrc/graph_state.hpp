@@ -8,17 +8,31 @@
 
 // A captured HIP graph of the batched programs' steps, its stream and the
 // events that order it after and before the caller's stream.
+// What the last node of a replayed step graph leaves for the host, in
+// pinned host memory (no copy call per replay).
+struct StepMail {
+  unsigned long long next_nv;  // frontier flag of the step after the replay
+  unsigned long long extra;    // a program counter the caller names (or 0)
+  uint32_t steps;              // steps that had a frontier
+};
+
 struct MultiGraph {
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<char> key;  // kernel arguments, grids and kernels it was captured with
+  StepMail* mail = nullptr;  // pinned host memory, written by the graph
+  std::vector<char> key;     // kernel arguments, grids and kernels it was captured with
   int init() {
-    if (stream) return EG_OK;
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_in, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_out, hipEventDisableTiming) != hipSuccess)
+    if (stream && mail) return EG_OK;
+    if ((!stream && hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) ||
+        (!ev_in && hipEventCreateWithFlags(&ev_in, hipEventDisableTiming) != hipSuccess) ||
+        (!ev_out && hipEventCreateWithFlags(&ev_out, hipEventDisableTiming) != hipSuccess))
       return eg::fail(EG_ERR_HIP, "stream or event creation for the step graph failed");
+    if (!mail && hipHostMalloc(reinterpret_cast<void**>(&mail), sizeof(StepMail), hipHostMallocDefault) != hipSuccess) {
+      mail = nullptr;
+      (void)hipGetLastError();
+      return eg::fail(EG_ERR_NOMEM, "pinned host memory for the step graph failed");
+    }
     return EG_OK;
   }
   void reset_exec() {
@@ -31,6 +45,7 @@ struct MultiGraph {
     if (ev_in) (void)hipEventDestroy(ev_in);
     if (ev_out) (void)hipEventDestroy(ev_out);
     if (stream) (void)hipStreamDestroy(stream);
+    if (mail) (void)hipHostFree(mail);
   }
 };
 
@@ -100,7 +115,7 @@ struct eg_graph {
   // sub-partitions with each destination's in-edges sorted by source
   bool tv_ready = false;
   eg::DevBuf tv_ptr;                 // u32[V+1] row pointer (prefix of in-degree)
-  eg::DevBuf tv_src;                 // u32[E]
+  eg::DevBuf tv_src;                 // u32[E + 4] (4 words of padding)
   eg::DevBuf tv_w;                   // f32[E] (12-byte records only)
   eg::DevBuf tv_items;               // u64 (v << 32 | chunk): in-edge chunks of at most 512 edges
   uint64_t tv_nitems = 0;

@@ -143,7 +143,7 @@ int tview_impl(eg_graph* g, hipStream_t s) {
   const uint64_t E = g->E;
   const uint32_t V = g->V;
   EG_TRY(g->tv_ptr.resize((uint64_t(V) + 1) * 4));
-  EG_TRY(g->tv_src.resize(std::max<uint64_t>(E, 1) * 4));
+  EG_TRY(g->tv_src.resize((std::max<uint64_t>(E, 1) + 4) * 4));  // 4 words of padding for 16-byte group loads
   if (sizeof(R) == 12) EG_TRY(g->tv_w.resize(std::max<uint64_t>(E, 1) * 4));
   uint32_t* ptr = g->tv_ptr.as<uint32_t>();
   // rows sorted by destination: row v starts at the prefix of the in-degree
