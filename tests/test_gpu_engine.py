@@ -47,6 +47,42 @@ def test_bfs_bit_exact(eg, oracle_lib, scale, P, S, policy):
     assert levels == o_levels
 
 
+def _hub_chain_edges(eg):
+    """A 300-vertex chain from the root (one push level per link), then a
+    hub of 19,000 out-edges (19 push work items of 1024 edges) whose level
+    pushes at least V/4 out-edges, vertices of out-degree 255, 256, 4096
+    and 4097 (the byte degree's saturation and the hub threshold) behind
+    it, and 50,000 random edges among the hub's targets."""
+    V = 20000
+    rng = np.random.default_rng(11)
+    src, dst = [], []
+    chain = np.arange(100, 400)
+    src += list(chain[:-1]); dst += list(chain[1:])
+    src.append(399); dst.append(5)
+    src += [5] * (V - 1000); dst += list(range(1000, V))
+    for v, d in ((6, 255), (7, 256), (8, 4096), (9, 4097)):
+        src.append(5); dst.append(v)
+        src += [v] * d; dst += list(rng.choice(np.arange(400, V), d, replace=False))
+    src += list(rng.integers(1000, V, 50000)); dst += list(rng.integers(400, V, 50000))
+    e = np.zeros(len(src), eg.REC8)
+    e["src"], e["dst"] = src, dst
+    return rng.permutation(e), V
+
+
+@pytest.mark.parametrize("P", [1, 4])
+@pytest.mark.parametrize("policy", [0, 1, 2, 3])
+def test_bfs_hub_and_deep_levels(eg, oracle_lib, P, policy):
+    print(f"bfs hub/chain P={P} policy={policy}", flush=True)
+    edges, V = _hub_chain_edges(eg)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, P, 1)
+    assert int(og.out_degree()[5]) == V - 1000 + 4
+    parent, depth, levels, _ = g.bfs(100, policy)
+    o_parent, o_depth, o_levels = og.bfs(100, 0)
+    assert levels == o_levels and levels > 300
+    assert np.array_equal(depth, o_depth)
+    assert np.array_equal(parent, o_parent)
+
+
 def test_bfs_pull_only_matches(eg, oracle_lib):
     edges, V = _rmat(eg, 13, seed=4)
     g, og, _ = _pair(eg, oracle_lib, edges, V, 2, 1)
