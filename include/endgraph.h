@@ -250,7 +250,10 @@ int eg_pagerank_view_prepare(eg_graph* g, double* h_ms, void* stream);
  * d_visited |= d_next, d_depth (optional) = level+1 for them;
  * frontier_edges = sum of out-degrees of the frontier (mode switch input);
  * returns the new vertex count and their out-edge sum.  eg_bfs runs from
- * root; h_edges gets the number of out-edges of reached vertices.  Dense
+ * root; h_edges gets the number of out-edges of reached vertices.  Its
+ * levels run on the device as a captured step graph (direction chosen per
+ * level on the device, the host checking a pinned mailbox once per 4
+ * levels); it returns when the run is done.  Dense
  * (pull) steps and the batched programs read the traversal view: in-edges
  * by destination with sources ascending, derived on first use after a
  * build; eg_traversal_view_prepare derives it now (h_ms gets its wall time)
