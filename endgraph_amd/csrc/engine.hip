@@ -799,19 +799,49 @@ __device__ __forceinline__ void bfs_pull_items(const uint64_t* __restrict__ item
     }
     uint32_t my_p = ~0u;  // parent found for this lane's item
     uint64_t pending = __ballot(live);
+    // four items at a time, one per group of 16 lanes; a group scans its
+    // item in 16-byte aligned groups of 4 sources per lane (64 per round;
+    // the view holds 4 words of padding past its end), and its first hit
+    // in address order is the smallest source
+    const int grp = lane >> 4;
+    const uint64_t gmask = 0xffffull << (grp * 16);
     while (pending) {  // wave-uniform
-      const int jj = __ffsll((unsigned long long)pending) - 1;
-      pending &= pending - 1;
-      const uint32_t e0 = __shfl(my_e0, jj, kWave), e1 = __shfl(my_e1, jj, kWave);
-      for (uint32_t e = e0; e < e1; e += kWave) {
-        const bool ok = e + lane < e1;
-        const uint32_t u = ok ? in_src[e + lane] : 0u;
-        const uint64_t m = __ballot(ok && bm_get(frontier, u));
-        if (m) {
-          const uint32_t p = __shfl(u, __ffsll((unsigned long long)m) - 1, kWave);
-          if (lane == jj) my_p = p;
-          break;
+      int jj[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        jj[g] = pending ? __ffsll((unsigned long long)pending) - 1 : -1;
+        pending &= pending - 1;
+      }
+      const int mj = grp == 0 ? jj[0] : grp == 1 ? jj[1] : grp == 2 ? jj[2] : jj[3];
+      const uint32_t e0 = __shfl(my_e0, mj < 0 ? 0 : mj, kWave), e1 = __shfl(my_e1, mj < 0 ? 0 : mj, kWave);
+      bool done = mj < 0;
+      uint32_t found = ~0u;  // the group's parent
+      for (uint64_t k = (e0 & ~3u) + uint64_t(lane & 15) * 4; __ballot(!done); k += 64) {
+        int hit = -1;
+        uint32_t u[4] = {0u, 0u, 0u, 0u};
+        if (!done && k < e1) {
+          const uint4 q = *reinterpret_cast<const uint4*>(in_src + k);
+          u[0] = q.x;
+          u[1] = q.y;
+          u[2] = q.z;
+          u[3] = q.w;
+#pragma unroll
+          for (int j = 3; j >= 0; --j)
+            if (k + j >= e0 && k + j < e1 && bm_get(frontier, u[j])) hit = j;
         }
+        const uint32_t cand = hit == 0 ? u[0] : hit == 1 ? u[1] : hit == 2 ? u[2] : u[3];
+        const uint64_t hm = __ballot(hit >= 0) & gmask;
+        if (hm) {
+          const uint32_t p = __shfl(cand, __ffsll((unsigned long long)hm) - 1, kWave);
+          if (!done) found = p;
+          done = true;
+        }
+        if (k - uint64_t(lane & 15) * 4 + 64 >= e1) done = true;  // the group's last round
+      }
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const uint32_t pg = __shfl(found, g * 16, kWave);
+        if (jj[g] >= 0 && lane == jj[g]) my_p = pg;
       }
     }
     const bool found = my_p != ~0u;
