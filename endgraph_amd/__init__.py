@@ -160,10 +160,13 @@ class DeviceBuffer:
         p = C.c_void_p()
         check(lib().eg_device_alloc(C.byref(p), self.nbytes))
         self.ptr = p.value
+        # kept on the object: module globals may be gone when it is collected
+        # at interpreter exit
+        self._free_fn, self._vp = lib().eg_device_free, C.c_void_p
 
     def free(self):
         if getattr(self, "ptr", None):
-            lib().eg_device_free(C.c_void_p(self.ptr))
+            self._free_fn(self._vp(self.ptr))
             self.ptr = None
 
     def __del__(self):
@@ -284,10 +287,11 @@ class BatchCSR:
                                        e.ctypes.data_as(C.c_void_p), C.byref(h), _stream(stream)))
         self._h, self.n, self.rec = h, int(v.size), rec_bytes
         self.rec_dtype = REC8 if rec_bytes == 8 else REC12
+        self._destroy_fn = lib().eg_batch_destroy  # usable at interpreter exit
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
-            lib().eg_batch_destroy(self._h)
+            self._destroy_fn(self._h)
             self._h = None
 
     def __del__(self):
@@ -313,12 +317,13 @@ class Graph:
         h = C.c_void_p()
         check(lib().eg_graph_create(vertices_num, partitions, sockets, eta, m, record_bytes, C.byref(h)))
         self._h = h
+        self._destroy_fn = lib().eg_graph_destroy  # usable at interpreter exit
         self.V, self.P, self.S, self.rec = int(vertices_num), int(partitions), int(sockets), int(record_bytes)
         self.rec_dtype = REC8 if self.rec == 8 else REC12
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
-            lib().eg_graph_destroy(self._h)
+            self._destroy_fn(self._h)
             self._h = None
 
     def __del__(self):

@@ -931,7 +931,8 @@ struct SbfsArgs {
   uint32_t *parent, *depth;
 };
 
-__device__ __forceinline__ SCountView count_of(const SbfsArgs& a, uint32_t step) {
+template <class A>
+__device__ __forceinline__ SCountView count_of(const A& a, uint32_t step) {
   const SCount& c = a.st.cnt[step % 3];
   SCountView r{c.nv, 0};
 #pragma unroll
@@ -1243,6 +1244,243 @@ __global__ void k_sbfs_init(SbfsArgs a, uint32_t root) {
     *a.st.steps = 0;
     *a.st.base = 0;
     *a.st.total = 0;
+  }
+}
+
+// ------------------------------------------ single-root SSSP, device-driven
+// eg_sssp's iterations (sssp.cpp:38-85) as steps of a captured graph, like
+// eg_bfs: a push step walks the active bitmap (vertices up to kSbfsHub
+// out-edges) and the hub items, a dense step is the pull above, and the
+// step end counts the next active set and its out-edges (the mode switch
+// input) and clears the bitmap the next step writes into.
+// Mode switch of the auto policy for SSSP: dense once the active
+// vertices' out-edges reach E/80.  A push step's improving relaxations are
+// atomics and most of its edges improve, while the dense pass streams;
+// 64 R-MAT 22 runs: E/2 594 ms, E/4 344, E/8 244, E/20 198, E/40 189,
+// E/80 187, E/160 190.
+constexpr uint64_t kDssspDenseDiv = 80;
+
+__device__ __forceinline__ bool dsssp_sparse(int policy, unsigned long long ne, uint64_t E) {
+  return policy == EG_POLICY_AUTO ? ne < E / kDssspDenseDiv : sbfs_sparse(policy, ne, E);
+}
+
+struct DssspArgs {
+  uint32_t V;
+  uint64_t E;
+  int policy;
+  const uint32_t *out_ptr, *out_dst, *deg;
+  const float* out_w;
+  const uint8_t* deg8;
+  const uint64_t* hub_items;
+  uint64_t n_hub_items;
+  const uint32_t *csc_src, *wordrank, *row_dst;
+  const float* csc_w;
+  const uint64_t* rowflag;
+  SbfsState st;  // bm[2], cnt, steps, base (vis and total unused)
+  float* dist;
+};
+
+// Sparse slot (sssp.cpp:44-58) for the out-edges e = first + k * stride,
+// k < n <= kRelaxBatch, of active vertex u at distance du: targets, weights
+// and target distances loaded for the batch before the first use.
+__device__ __forceinline__ void dsssp_relax(const DssspArgs& a, uint64_t* __restrict__ nx, float du, uint32_t first,
+                                            uint32_t stride, uint32_t n) {
+  uint32_t v[kRelaxBatch];
+  float w[kRelaxBatch], dv[kRelaxBatch];
+  int old[kRelaxBatch];
+#pragma unroll
+  for (int k = 0; k < kRelaxBatch; ++k) {
+    const bool ok = uint32_t(k) < n;
+    v[k] = ok ? a.out_dst[first + uint32_t(k) * stride] : 0u;
+    w[k] = ok ? a.out_w[first + uint32_t(k) * stride] : 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < kRelaxBatch; ++k) dv[k] = uint32_t(k) < n ? a.dist[v[k]] : 0.0f;
+#pragma unroll
+  for (int k = 0; k < kRelaxBatch; ++k) {
+    const float nd = __fadd_rn(du, w[k]);
+    old[k] = uint32_t(k) < n && nd < dv[k] ? atomicMin(reinterpret_cast<int*>(&a.dist[v[k]]), __float_as_int(nd))
+                                           : __float_as_int(0.0f);
+  }
+#pragma unroll
+  for (int k = 0; k < kRelaxBatch; ++k)
+    if (uint32_t(k) < n && __fadd_rn(du, w[k]) < __int_as_float(old[k])) bm_set(nx, v[k]);
+}
+
+__device__ __forceinline__ void dsssp_relax_wave(const DssspArgs& a, uint64_t* __restrict__ nx, float du, uint32_t e0,
+                                                 uint32_t e1, int lane) {
+  for (uint32_t b = e0; b < e1; b += kWave * kRelaxBatch) {
+    const uint32_t f = b + uint32_t(lane);
+    const uint32_t n = f < e1 ? ceil_div(e1 - f, uint32_t(kWave)) : 0u;
+    dsssp_relax(a, nx, du, f, kWave, n < uint32_t(kRelaxBatch) ? n : uint32_t(kRelaxBatch));
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_dsssp_push(DssspArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
+  const SCountView c = count_of(a, step);
+  if (!c.nv || !dsssp_sparse(a.policy, c.ne, a.E)) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t* fr = a.st.bm[step & 1];
+  uint64_t* nx = a.st.bm[(step + 1) & 1];
+  const uint64_t words = (uint64_t(a.V) >> 6) + 1;
+  for (uint64_t b0 = gw * kWave; b0 < words; b0 += nw * kWave) {
+    const uint64_t wi = b0 + lane;
+    const uint64_t mine = wi < words ? fr[wi] : 0ull;
+    bool has_mid = false;
+    for (uint64_t m = mine; m; m &= m - 1) {
+      const uint32_t u = uint32_t(wi * 64 + __ffsll((unsigned long long)m) - 1);
+      const uint32_t e0 = a.out_ptr[u], e1 = a.out_ptr[u + 1];
+      if (e1 - e0 <= kSbfsLaneEdges)
+        dsssp_relax(a, nx, a.dist[u], e0, 1, e1 - e0);
+      else if (e1 - e0 <= kSbfsHub)
+        has_mid = true;
+    }
+    uint64_t wm = __ballot(has_mid);
+    while (wm) {  // wave-uniform
+      const int jj = __ffsll((unsigned long long)wm) - 1;
+      wm &= wm - 1;
+      const bool set = (shfl_t(mine, jj) >> lane) & 1ull;
+      const uint32_t u = uint32_t((b0 + jj) * 64 + lane);
+      const uint32_t e0 = set ? a.out_ptr[u] : 0u, e1 = set ? a.out_ptr[u + 1] : 0u;
+      const bool is_mid = e1 - e0 > kSbfsLaneEdges && e1 - e0 <= kSbfsHub;
+      const float du = is_mid ? a.dist[u] : 0.0f;
+      uint64_t mid = __ballot(is_mid);
+      while (mid) {  // wave-uniform
+        const int l = __ffsll((unsigned long long)mid) - 1;
+        mid &= mid - 1;
+        dsssp_relax_wave(a, nx, __shfl(du, l, kWave), __shfl(e0, l, kWave), __shfl(e1, l, kWave), lane);
+      }
+    }
+  }
+  for (uint64_t r = 0; r * nw < a.n_hub_items; r += kWave) {  // hub items, as in k_sbfs_push
+    const uint64_t i = (r + lane) * nw + gw;
+    uint64_t it = 0;
+    bool live = false;
+    if (i < a.n_hub_items) {
+      it = a.hub_items[i];
+      live = bm_get(fr, uint32_t(it >> 32));
+    }
+    uint64_t lm = __ballot(live);
+    while (lm) {  // wave-uniform
+      const int l = __ffsll((unsigned long long)lm) - 1;
+      lm &= lm - 1;
+      const uint64_t t = shfl_t(it, l);
+      const uint32_t u = uint32_t(t >> 32);
+      const uint32_t e0 = a.out_ptr[u] + uint32_t(t) * kSbfsItemEdges;
+      dsssp_relax_wave(a, nx, a.dist[u], e0, min(e0 + kSbfsItemEdges, a.out_ptr[u + 1]), lane);
+    }
+  }
+}
+
+// Dense step: k_sssp_pull's reduction on this step's bitmaps.
+__global__ __launch_bounds__(kNT) void k_dsssp_pull(DssspArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
+  const SCountView c = count_of(a, step);
+  if (!c.nv || dsssp_sparse(a.policy, c.ne, a.E)) return;
+  uint64_t* nx = a.st.bm[(step + 1) & 1];
+  float* dist = a.dist;
+  row_reduce_wave<float>(
+      first_word(), a.E, a.csc_src, a.rowflag, a.wordrank, 1e9f, OpMin{},
+      [&](uint64_t, uint32_t s, float we) { return __fadd_rn(dist[s], we); },
+      [&](uint32_t row, float m, bool) {
+        const uint32_t v = a.row_dst[row];
+        if (m < 1e9f && m < dist[v]) {
+          const int old = atomicMin(reinterpret_cast<int*>(&dist[v]), __float_as_int(m));
+          if (m < __int_as_float(old)) bm_set(nx, v);
+        }
+      },
+      a.csc_w);
+}
+
+// Step end: counters of the next active set, this step's bitmap cleared
+// (the next step's next); 64 words per wave step, the non-empty ones
+// kEndWords at a time.
+__global__ __launch_bounds__(kNT) void k_dsssp_end(DssspArgs a, uint32_t j) {
+  const uint32_t step = *a.st.base + j;
+  const bool live = count_of(a, step).nv != 0;
+  if (blockIdx.x == 0) {
+    unsigned long long* z = reinterpret_cast<unsigned long long*>(&a.st.cnt[(step + 2) % 3]);
+    for (uint32_t i = threadIdx.x; i < sizeof(SCount) / 8; i += blockDim.x) z[i] = 0;
+    if (threadIdx.x == 0 && live) *a.st.steps = step + 1;
+  }
+  if (!live) return;  // block-uniform
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t words = (uint64_t(a.V) >> 6) + 1;
+  uint64_t* fr = a.st.bm[step & 1];
+  const uint64_t* nx = a.st.bm[(step + 1) & 1];
+  SCount* nc = &a.st.cnt[(step + 1) % 3];
+  unsigned long long nv = 0, ne = 0;
+  for (uint64_t b0 = gw * kWave; b0 < words; b0 += nw * kWave) {
+    const uint64_t wi = b0 + lane;
+    uint64_t mine = 0;
+    if (wi < words) {
+      mine = nx[wi];
+      fr[wi] = 0;
+    }
+    nv += __popcll(mine);
+    uint64_t nz = __ballot(mine != 0);
+    while (nz) {  // wave-uniform
+      int idx[kEndWords];
+#pragma unroll
+      for (int k = 0; k < kEndWords; ++k) {
+        idx[k] = nz ? __ffsll((unsigned long long)nz) - 1 : -1;
+        nz &= nz - 1;
+      }
+      uint32_t d[kEndWords];
+#pragma unroll
+      for (int k = 0; k < kEndWords; ++k) {
+        const uint64_t wk = idx[k] >= 0 ? lane_value(mine, idx[k]) : 0ull;
+        d[k] = (wk >> lane) & 1ull ? a.deg8[uint32_t((b0 + uint64_t(idx[k])) * 64 + lane)] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < kEndWords; ++k)
+        if (d[k] == 255u) d[k] = a.deg[uint32_t((b0 + uint64_t(idx[k])) * 64 + lane)];
+#pragma unroll
+      for (int k = 0; k < kEndWords; ++k) ne += d[k];
+    }
+  }
+  __shared__ unsigned long long part[2][kNT / kWave];
+  nv = wave_reduce(nv, OpSum{});
+  ne = wave_reduce(ne, OpSum{});
+  if (lane == 0) {
+    part[0][threadIdx.x / kWave] = nv;
+    part[1][threadIdx.x / kWave] = ne;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long x = 0, y = 0;
+    for (int k = 0; k < kNT / kWave; ++k) {
+      x += part[0][k];
+      y += part[1][k];
+    }
+    if (x) nc->nv = 1;
+    if (y) atomicAdd(&nc->ne[blockIdx.x % kNeSlots][0], y);
+  }
+}
+
+// Start state in one launch: distances 1e9 but the root's 0, the root the
+// only active vertex, the counter ring and scalars.
+__global__ void k_dsssp_init(DssspArgs a, uint32_t root) {
+  const uint64_t words = (uint64_t(a.V) >> 6) + 1;
+  const uint64_t n = a.V > 2 * words ? a.V : 2 * words;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    if (i < a.V) a.dist[i] = i == root ? 0.0f : 1e9f;
+    if (i < 2 * words) a.st.bm[0][i] = i == (root >> 6) ? 1ull << (root & 63) : 0ull;  // bm[0], bm[1] contiguous
+  }
+  if (blockIdx.x != 0) return;
+  unsigned long long* z = reinterpret_cast<unsigned long long*>(a.st.cnt);
+  for (uint32_t i = threadIdx.x; i < 3 * sizeof(SCount) / 8; i += blockDim.x) z[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a.st.cnt[0].nv = 1;
+    a.st.cnt[0].ne[0][0] = a.out_ptr[root + 1] - a.out_ptr[root];
+    *a.st.steps = 0;
+    *a.st.base = 0;
   }
 }
 
@@ -1808,23 +2046,43 @@ int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_i
   EG_TRY(ensure_engine(g, s));
   if (g->rec != 12) return fail(EG_ERR_STATE, "SSSP needs a graph of EdgeType<float> (12-byte) records");
   if (root >= g->V || !d_dist) return fail(EG_ERR_ARG, "bad root or NULL d_dist");
-  uint64_t* a = g->e_bm[0].as<uint64_t>();
-  uint64_t* b = g->e_bm[1].as<uint64_t>();
-  EG_TRY(fill<float>(d_dist, g->V, 1e9f, s));
-  EG_TRY(fill<float>(d_dist + root, 1, 0.0f, s));
-  EG_TRY(fill<uint64_t>(a, words_of(g->V), 0ull, s));
-  EG_TRY(fill<uint64_t>(a + (root >> 6), 1, 1ull << (root & 63), s));
-  uint32_t deg_root = 0;
-  EG_HIP(hipMemcpyAsync(&deg_root, g->out_deg.as<uint32_t>() + root, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  uint64_t nv = 1, ne = deg_root;
-  uint32_t it = 0;
-  while (nv > 0) {
-    EG_TRY(eg_sssp_iteration(g, policy, a, b, d_dist, ne, &nv, &ne, stream));
-    std::swap(a, b);
-    ++it;
-  }
-  if (h_iterations) *h_iterations = it;
+  if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
+  const uint32_t V = g->V;
+  const uint64_t words = words_of(V);
+  EG_TRY(g->e_dsssp.resize(2 * words * 8 + 64 + 3 * sizeof(SCount) + 64));
+  DssspArgs a{};  // zeroed padding: the bytes key the captured step graph
+  SbfsState& st = a.st;
+  st.bm[0] = g->e_dsssp.as<uint64_t>();
+  st.bm[1] = st.bm[0] + words;
+  st.cnt = reinterpret_cast<SCount*>((reinterpret_cast<uintptr_t>(st.bm[1] + words) + 63) & ~uintptr_t(63));
+  st.steps = reinterpret_cast<uint32_t*>(st.cnt + 3);
+  st.base = st.steps + 1;
+  a.V = V;
+  a.E = g->E;
+  a.policy = policy;
+  a.out_ptr = g->e_out_ptr.as<uint32_t>();
+  a.out_dst = g->e_out_dst.as<uint32_t>();
+  a.out_w = g->e_out_w.as<float>();
+  a.deg = g->out_deg.as<uint32_t>();
+  a.deg8 = g->e_deg8.as<uint8_t>();
+  a.hub_items = g->e_hub_items.as<uint64_t>();
+  a.n_hub_items = g->e_nhub_items;
+  a.csc_src = g->e_csc_src.as<uint32_t>();
+  a.csc_w = g->e_csc_w.as<float>();
+  a.rowflag = g->e_rowflag.as<uint64_t>();
+  a.wordrank = g->e_wordrank.as<uint32_t>();
+  a.row_dst = g->e_row_dst.as<uint32_t>();
+  a.dist = d_dist;
+  hipLaunchKernelGGL(k_dsssp_init, dim3(grid_for(std::max<uint64_t>(V, 2 * words), kNT, 65536)), dim3(kNT), 0, s, a,
+                     root);
+  EG_HIP(hipGetLastError());
+  const StepLaunch<DssspArgs> ls[3] = {{k_dsssp_push, 2048},
+                                       {k_dsssp_pull, row_grid(g->E)},
+                                       {k_dsssp_end, grid_for(ceil_div(words, uint64_t(kEndWords)), kNT / kWave, 2048)}};
+  const unsigned long long* const nv[3] = {&st.cnt[0].nv, &st.cnt[1].nv, &st.cnt[2].nv};
+  uint32_t steps = 0;
+  EG_TRY(run_step_graph(g->sssp_graph, a, ls, st.base, nv, st.steps, V + 1, &steps, s));
+  if (h_iterations) *h_iterations = steps;
   return EG_OK;
 }
 

@@ -136,6 +136,8 @@ struct eg_graph {
   eg::DevBuf e_sbfs;                 // single-root BFS state (bitmaps, push items, counters)
   eg::DevBuf e_multi;                // batched traversal masks, work items, counters
   MultiGraph multi_graph;            // captured steps of the device-driven programs
+  MultiGraph sssp_graph;             // captured steps of the single-root SSSP
+  eg::DevBuf e_dsssp;                // single-root SSSP state (bitmaps, counters)
   eg::DevBuf e_multi_roots;          // roots and per-root statistics
   double eng_prepare_ms = 0;
 

@@ -83,6 +83,20 @@ def test_bfs_hub_and_deep_levels(eg, oracle_lib, P, policy):
     assert np.array_equal(parent, o_parent)
 
 
+@pytest.mark.parametrize("policy", [0, 1, 2, 3])
+def test_sssp_hub_and_deep_levels(eg, oracle_lib, policy):
+    print(f"sssp hub/chain policy={policy}", flush=True)
+    e8, V = _hub_chain_edges(eg)
+    edges = np.zeros(e8.shape[0], eg.REC12)
+    edges["src"], edges["dst"] = e8["src"], e8["dst"]
+    edges["w"] = np.random.default_rng(12).random(e8.shape[0], dtype=np.float32)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, 2, 1)
+    dist, iters = g.sssp(100, policy)
+    o_dist, _ = og.sssp(100)
+    assert iters > 300
+    assert np.array_equal(dist.view(np.uint32), o_dist.view(np.uint32))
+
+
 def test_bfs_pull_only_matches(eg, oracle_lib):
     edges, V = _rmat(eg, 13, seed=4)
     g, og, _ = _pair(eg, oracle_lib, edges, V, 2, 1)
