@@ -79,12 +79,19 @@ __device__ __forceinline__ uint64_t or_scan(uint64_t v) {
   return v;
 }
 
-__device__ __forceinline__ bool sparse_level(int policy, unsigned long long active_edges, uint64_t E) {
+__device__ __forceinline__ bool sparse_level(int policy, unsigned long long active_edges, uint64_t E,
+                                             uint64_t dense_div = 20) {
   if (policy == EG_POLICY_PUSH) return true;
   if (policy == EG_POLICY_PULL) return false;
   if (policy == EG_POLICY_REFERENCE) return true;  // rooted programs
-  return active_edges < E / 20;
+  return active_edges < E / dense_div;
 }
+
+// auto policy of the batched SSSP: dense once the active out-edges reach
+// E / kMssspDenseDiv.  Its pull skips the sources no root has active and
+// gathers 64-root distance rows, so pushes pay until half of E (64 R-MAT 22
+// roots: E/20 47.7 ms, E/5 46.1, E/3 45.4, E/2 44.4, E/1 106, push only 109).
+constexpr uint64_t kMssspDenseDiv = 2;
 
 template <class T>
 __global__ void k_fill_t(T* __restrict__ p, uint64_t n, T v) {
@@ -491,7 +498,7 @@ struct SsspArgs {
 __global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
-  if (!c.nv || !sparse_level(a.policy, c.ne, a.E)) return;
+  if (!c.nv || !sparse_level(a.policy, c.ne, a.E, kMssspDenseDiv)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
@@ -525,7 +532,7 @@ __global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t j) {
 __global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
-  if (!c.nv || sparse_level(a.policy, c.ne, a.E)) return;
+  if (!c.nv || sparse_level(a.policy, c.ne, a.E, kMssspDenseDiv)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
