@@ -940,9 +940,13 @@ __device__ __forceinline__ SCountView count_of(const A& a, uint32_t step) {
   return r;
 }
 
+// auto policy: dense once the frontier's out-edges reach E / 20 (twitter-
+// shaped BFS: E/10 4.73 ms, E/20 3.97, E/40 6.38; RMAT-24 within 1%)
+constexpr uint64_t kSbfsDenseDiv = 20;
+
 __device__ __forceinline__ bool sbfs_sparse(int policy, unsigned long long ne, uint64_t E) {
   if (policy == EG_POLICY_PULL) return false;
-  if (policy == EG_POLICY_AUTO) return ne < E / 20;
+  if (policy == EG_POLICY_AUTO) return ne < E / kSbfsDenseDiv;
   return true;  // reference (rooted programs push, R11) and push
 }
 

@@ -92,6 +92,9 @@ __device__ __forceinline__ bool sparse_level(int policy, unsigned long long acti
 // gathers 64-root distance rows, so pushes pay until half of E (64 R-MAT 22
 // roots: E/20 47.7 ms, E/5 46.1, E/3 45.4, E/2 44.4, E/1 106, push only 109).
 constexpr uint64_t kMssspDenseDiv = 2;
+// auto policy of the batched BFS: dense once the frontier's out-edges reach
+// E / kMbfsDenseDiv (64 R-MAT 22 roots: E/5..E/20 within 1%, E/40 7% slower)
+constexpr uint64_t kMbfsDenseDiv = 20;
 
 template <class T>
 __global__ void k_fill_t(T* __restrict__ p, uint64_t n, T v) {
@@ -378,7 +381,7 @@ struct BfsArgs {
 __global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
-  if (!c.nv || !sparse_level(a.policy, c.ne, a.E)) return;
+  if (!c.nv || !sparse_level(a.policy, c.ne, a.E, kMbfsDenseDiv)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
@@ -411,7 +414,7 @@ __global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t j) {
 __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
-  if (!c.nv || sparse_level(a.policy, c.ne, a.E)) return;
+  if (!c.nv || sparse_level(a.policy, c.ne, a.E, kMbfsDenseDiv)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
