@@ -97,6 +97,60 @@ def test_sssp_hub_and_deep_levels(eg, oracle_lib, policy):
     assert np.array_equal(dist.view(np.uint32), o_dist.view(np.uint32))
 
 
+def _oracle8(oracle_lib, edges12, V, P, S):
+    """The oracle's unweighted graph of the same (src, dst) list: its BFS,
+    like the reference's, runs on EdgeType<Empty>."""
+    e8 = np.zeros(edges12.shape[0], np.dtype([("src", "<u4"), ("dst", "<u4")]))
+    e8["src"], e8["dst"] = edges12["src"], edges12["dst"]
+    return oracle_lib.OracleGraph(e8, V, P=P, S=S, eta=1, m=1, symmetrize=False, threads=8)
+
+
+def test_traversals_edge_cases(eg, oracle_lib):
+    """Device-driven BFS and SSSP on degenerate graphs: fewer edges than
+    ranks with isolated and last-id roots (V not a multiple of 64), V = 1
+    with self loops only, self loops spread over pages; every policy."""
+    rng = np.random.default_rng(6)
+    cases = []
+    e = np.zeros(3, eg.REC12); e["src"] = [0, 1, 2]; e["dst"] = [2, 2, 0]; e["w"] = [0.5, 0.25, 1.0]
+    cases.append((e, 5000, 4, 2, [0, 1, 2, 4999]))
+    e = np.zeros(7, eg.REC12); e["w"] = 0.5
+    cases.append((e, 1, 1, 1, [0]))
+    e = np.zeros(400, eg.REC12); e["src"] = rng.integers(0, 50, 400); e["dst"] = e["src"]; e["w"] = 0.125
+    cases.append((e, 9 * 4096 + 5, 3, 3, [int(e["src"][0]), 9 * 4096 + 4]))
+    for edges, V, P, S, roots in cases:
+        g, og, _ = _pair(eg, oracle_lib, edges, V, P, S)
+        og8 = _oracle8(oracle_lib, edges, V, P, S)
+        for root in roots:
+            o_parent, o_depth, o_levels = og8.bfs(root, 0)
+            o_dist, _ = og.sssp(root)
+            for policy in (0, 1, 2, 3):
+                print(f"edge case V={V} E={edges.shape[0]} root={root} policy={policy}", flush=True)
+                parent, depth, levels, _ = g.bfs(root, policy)
+                assert levels == o_levels
+                assert np.array_equal(depth, o_depth) and np.array_equal(parent, o_parent)
+                dist, _ = g.sssp(root, policy)
+                assert np.array_equal(dist.view(np.uint32), o_dist.view(np.uint32))
+
+
+def test_programs_interleaved_on_one_graph(eg, oracle_lib):
+    """BFS, SSSP and the batched programs called in turn on one graph (each
+    keeps or re-captures its step graph) stay equal to the oracle."""
+    edges, V = _rmat(eg, 13, rec=12, seed=9)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, 2, 1)
+    og8 = _oracle8(oracle_lib, edges, V, 2, 1)
+    roots = [int(np.argmax(og.out_degree())), 7, 123, 4000]
+    for i, root in enumerate(roots):
+        print(f"interleaved root={root}", flush=True)
+        parent, depth, _, _ = g.bfs(root, 1)
+        o_parent, o_depth, _ = og8.bfs(root, 0)
+        assert np.array_equal(parent, o_parent) and np.array_equal(depth, o_depth)
+        dist, _ = g.sssp(root, 1)
+        o_dist, _ = og.sssp(root)
+        assert np.array_equal(dist.view(np.uint32), o_dist.view(np.uint32))
+        mpar, mdep, _, _, _ = g.bfs_multi(np.array(roots[: i + 1], np.uint32), 1)
+        assert np.array_equal(mpar[:, i], o_parent) and np.array_equal(mdep[:, i], o_depth)
+
+
 def test_bfs_pull_only_matches(eg, oracle_lib):
     edges, V = _rmat(eg, 13, seed=4)
     g, og, _ = _pair(eg, oracle_lib, edges, V, 2, 1)
