@@ -724,6 +724,8 @@ __global__ void k_front_deg(const uint32_t* __restrict__ flist, uint32_t fn, con
 // each, and write_min over the items gives the row's smallest.  A row's
 // first item runs in an earlier launch than its others, which skip when the
 // first found a source (it is smaller than all of theirs).
+constexpr int kPullRowWords = 4;  // bitmap words a wave takes per step, their loads batched
+
 __device__ __forceinline__ void bfs_pull_rows(uint32_t V, const uint32_t* __restrict__ in_ptr,
                                               const uint32_t* __restrict__ in_src,
                                               const uint64_t* __restrict__ frontier,
@@ -731,40 +733,68 @@ __device__ __forceinline__ void bfs_pull_rows(uint32_t V, const uint32_t* __rest
                                               uint64_t* __restrict__ active_out) {
   // a wave's lanes are the 64 vertices of one bitmap word, so the word's
   // found rows are one plain store (the item launches, the word's other
-  // writers, run after this one)
+  // writers, run after this one); a wave takes kPullRowWords words per step
+  // and issues each stage's loads for all of them before the next stage
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~(kWave - 1)); v0 < V; v0 += stride) {
-    const uint64_t vis = visited[v0 >> 6];
-    if (vis == ~0ull) continue;  // wave-uniform
-    const uint64_t v = v0 + lane;
-    bool found = false;
-    if (v < V && !((vis >> lane) & 1ull)) {
-      const uint32_t b = in_ptr[v], e = in_ptr[v + 1];
-      if (e - b <= kTvSmallRow) {  // else a wave item's row
+  constexpr int R = kPullRowWords;
+  for (uint64_t v0 = uint64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~(kWave - 1)); v0 < V; v0 += stride * R) {
+    uint64_t vis[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) vis[r] = v0 + r * stride < V ? visited[(v0 + r * stride) >> 6] : ~0ull;
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) any |= vis[r] != ~0ull;
+    if (!any) continue;  // wave-uniform
+    uint32_t b[R], e[R];
+    bool act[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t v = v0 + r * stride + lane;
+      act[r] = v < V && !((vis[r] >> lane) & 1ull);
+      b[r] = act[r] ? in_ptr[v] : 0u;
+      e[r] = act[r] ? in_ptr[v + 1] : 0u;
+    }
+    // each row's first source alone first (its smallest: most rows of a
+    // sparse level's pull end there, on one probe instead of four)
+    uint32_t u0[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u0[r] = b[r] < e[r] && e[r] - b[r] <= kTvSmallRow ? in_src[b[r]] : 0u;
+    bool found[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) found[r] = b[r] < e[r] && e[r] - b[r] <= kTvSmallRow && bm_get(frontier, u0[r]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t v = v0 + r * stride + lane;
+      if (found[r]) {
+        parent[v] = u0[r];  // the row's only writer in this step
+      } else if (e[r] - b[r] > 1 && e[r] - b[r] <= kTvSmallRow) {  // larger rows are wave items
         // 16-byte aligned groups of 4 sources, one vector load each (the
         // view holds 4 words of padding past its end); sources outside the
         // row are masked
-        for (uint32_t k = b & ~3u; k < e; k += 4) {
+        for (uint32_t k = (b[r] + 1) & ~3u; k < e[r]; k += 4) {
           const uint4 q = *reinterpret_cast<const uint4*>(in_src + k);
           const uint32_t u[4] = {q.x, q.y, q.z, q.w};
           bool f[4];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) f[j] = k + j >= b && k + j < e && bm_get(frontier, u[j]);
+          for (int jj = 0; jj < 4; ++jj) f[jj] = k + jj > b[r] && k + jj < e[r] && bm_get(frontier, u[jj]);
           int hit = -1;
 #pragma unroll
-          for (int j = 3; j >= 0; --j)
-            if (f[j]) hit = j;
+          for (int jj = 3; jj >= 0; --jj)
+            if (f[jj]) hit = jj;
           if (hit >= 0) {
-            parent[v] = u[hit];  // the row's only writer in this step
-            found = true;
+            parent[v] = u[hit];
+            found[r] = true;
             break;
           }
         }
       }
     }
-    const uint64_t m = __ballot(found);
-    if (lane == 0 && m) active_out[v0 >> 6] = m;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t m = __ballot(found[r]);
+      if (lane == 0 && m) active_out[(v0 + r * stride) >> 6] = m;
+    }
   }
 }
 
@@ -798,6 +828,15 @@ __device__ __forceinline__ void bfs_pull_items(const uint64_t* __restrict__ item
       }
     }
     uint32_t my_p = ~0u;  // parent found for this lane's item
+    // every lane probes its item's first source (the item's smallest)
+    // before the wave scans the items that are left
+    if (live) {
+      const uint32_t u0 = in_src[my_e0];
+      if (bm_get(frontier, u0)) {
+        my_p = u0;
+        live = false;
+      }
+    }
     uint64_t pending = __ballot(live);
     // four items at a time, one per group of 16 lanes; a group scans its
     // item in 16-byte aligned groups of 4 sources per lane (64 per round;
