@@ -35,6 +35,7 @@ namespace eg {
 namespace {
 
 constexpr int kNT = 256;
+constexpr int kEndStrides = 4;         // grid strides a level-end wave takes per step
 constexpr uint32_t kItemEdges = 4096;  // out-edges per push work item
 constexpr uint32_t kPullEdges = kTvPullEdges;  // in-edges per pull work item
 constexpr uint32_t kUnreached = 0xffffffffu;
@@ -292,42 +293,68 @@ template <class Visit>
 __device__ __forceinline__ void level_end(uint32_t V, const uint32_t* __restrict__ deg, uint64_t* __restrict__ nx,
                                           uint64_t* __restrict__ fr, MCount* nc, uint64_t* __restrict__ items,
                                           Visit visit) {
+  // A wave takes kEndStrides grid strides of 64 vertices per step, with the
+  // mask loads of all of them, then the degree loads, in flight together;
+  // the block reserves its work items with one atomic (a block scan).
+  constexpr int R = kEndStrides;
   __shared__ unsigned long long s_nv[kNT / kWave], s_ne[kNT / kWave], s_roots[kNT / kWave];
+  __shared__ uint32_t s_wt[kNT / kWave];
+  __shared__ uint32_t s_base;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t stride = nw * kWave;
   unsigned long long nv = 0, ne = 0, roots = 0;
   uint32_t ksum = 0;
-  for (uint64_t v0 = gw * kWave; v0 < V; v0 += nw * kWave) {
-    const uint64_t v = v0 + lane;
-    const uint64_t nb = v < V ? nx[v] : 0ull;
-    if (nb) {
-      const uint32_t d = deg[v];
-      roots |= nb;
-      ++nv;
-      ne += d;
-      ksum += uint32_t(ceil_div(d, kItemEdges));
+  for (uint64_t v0 = gw * kWave; v0 < V; v0 += stride * R) {
+    uint64_t nb[R];
+    uint32_t d[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t v = v0 + r * stride + lane;
+      nb[r] = v < V ? nx[v] : 0ull;
     }
-  }
-  const uint32_t inc = wave_inclusive_scan(ksum, OpSum{});
-  const uint32_t tot = __shfl(inc, kWave - 1, kWave);
-  uint32_t base = 0;
-  if (lane == 0 && tot) base = atomicAdd(&nc->items, tot);
-  uint32_t at = __shfl(base, 0, kWave) + inc - ksum;
-  for (uint64_t v0 = gw * kWave; v0 < V; v0 += nw * kWave) {
-    const uint64_t v = v0 + lane;
-    uint64_t nb = 0;
-    if (v < V) {
-      nb = nx[v];
-      if (nb) {
-        nx[v] = 0;
-        const uint32_t k = uint32_t(ceil_div(deg[v], kItemEdges));
-        for (uint32_t j = 0; j < k; ++j) items[at + j] = (v << 32) | j;
-        at += k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r] = nb[r] ? deg[v0 + r * stride + lane] : 0u;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (nb[r]) {
+        roots |= nb[r];
+        ++nv;
+        ne += d[r];
+        ksum += uint32_t(ceil_div(d[r], kItemEdges));
       }
-      if (fr[v] != nb) fr[v] = nb;
+  }
+  uint32_t tot = 0;
+  const uint32_t excl = block_exclusive_scan<kNT>(ksum, 0u, OpSum{}, s_wt, &tot);
+  if (threadIdx.x == 0) s_base = tot ? atomicAdd(&nc->items, tot) : 0u;
+  __syncthreads();
+  uint32_t at = s_base + excl;
+  for (uint64_t v0 = gw * kWave; v0 < V; v0 += stride * R) {
+    uint64_t nb[R], f[R];
+    uint32_t d[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t v = v0 + r * stride + lane;
+      nb[r] = v < V ? nx[v] : 0ull;
+      f[r] = v < V ? fr[v] : 0ull;
     }
-    visit(v0, v, nb);
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r] = nb[r] ? deg[v0 + r * stride + lane] : 0u;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t v = v0 + r * stride + lane;
+      if (v < V) {
+        if (nb[r]) {
+          nx[v] = 0;
+          const uint32_t k = uint32_t(ceil_div(d[r], kItemEdges));
+          for (uint32_t q = 0; q < k; ++q) items[at + q] = (v << 32) | q;
+          at += k;
+        }
+        if (f[r] != nb[r]) fr[v] = nb[r];
+      }
+      if (v0 + r * stride < V) visit(v0 + r * stride, v, nb[r]);  // wave-uniform
+    }
   }
   nv = wave_reduce(nv, OpSum{});
   ne = wave_reduce(ne, OpSum{});
