@@ -272,7 +272,12 @@ int eg_cc_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t*
 int eg_cc(eg_graph* g, int policy, uint32_t* d_label, uint32_t* h_iterations, void* stream);
 
 /* SSSP, toolkits/sssp.cpp:24-108 (12-byte records): float distances, 1e9
- * for unreached, relax = dist[src] + w in one round-to-nearest fp32 add. */
+ * for unreached, relax = dist[src] + w in one round-to-nearest fp32 add.
+ * eg_sssp_iteration runs one process_edges step from the host; eg_sssp runs
+ * the whole program on the device as a captured step graph (mode chosen
+ * per step on the device; the auto policy goes dense at E/80 active
+ * out-edges) and returns when it is done.  Distances do not depend on the
+ * policy or the step order (the least fixpoint of the relaxation). */
 int eg_sssp_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t* d_next, float* d_dist,
                       uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream);
 int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_iterations, void* stream);
