@@ -97,11 +97,12 @@ class OracleGraph:
         _check(lib().orc_build(edges.ctypes.data, edges.shape[0], self.rec, self.V, self.P, self.S,
                                int(eta), int(m), int(bool(symmetrize)), int(threads), C.byref(h)), "build")
         self._h = h
+        self._free_fn = lib().orc_free  # usable at interpreter exit, when module globals are gone
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().orc_free(h)
+            self._free_fn(h)
             self._h = None
 
     @property
