@@ -461,8 +461,13 @@ __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t j) {
       const uint64_t it = shfl64(my_it, jj);
       const uint64_t v = it >> 32;
       const uint64_t need = shfl64(my_need, jj);
-      const uint32_t e0 = a.in_ptr[v] + uint32_t(it) * kPullEdges;
-      const uint32_t b1 = min(e0 + kPullEdges, a.in_ptr[v + 1]);
+      const uint32_t r0 = a.in_ptr[v], r1 = a.in_ptr[v + 1];
+      const uint32_t e0 = r0 + uint32_t(it) * kPullEdges;
+      const uint32_t b1 = min(e0 + kPullEdges, r1);
+      // an item that is its row's only one (at most kPullEdges in-edges) is
+      // the only writer of the row's parents and next mask in this step:
+      // plain stores instead of atomics
+      const bool whole = e0 == r0 && b1 == r1;
       uint64_t found = 0;
       for (uint32_t e = e0; e < b1 && found != need; e += kWave) {
         const bool ok = e + lane < b1;
@@ -477,11 +482,21 @@ __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t j) {
         for (uint64_t m = f & ~below; m; m &= m - 1) pslot[__ffsll((unsigned long long)m) - 1] = u;
         __builtin_amdgcn_s_waitcnt(kLgkmZero);  // every hand-over written before lane r reads it
         __asm__ volatile("" ::: "memory");
-        if ((newly >> lane) & 1ull) atomicMin(&a.parent[v * 64 + lane], pslot[lane]);
+        if ((newly >> lane) & 1ull) {
+          if (whole)
+            a.parent[v * 64 + lane] = pslot[lane];
+          else
+            atomicMin(&a.parent[v * 64 + lane], pslot[lane]);
+        }
         __asm__ volatile("" ::: "memory");
         found |= newly;
       }
-      if (found && lane == 0) atomicOr((unsigned long long*)&a.st.nx[v], found);
+      if (found && lane == 0) {
+        if (whole)
+          a.st.nx[v] = found;
+        else
+          atomicOr((unsigned long long*)&a.st.nx[v], found);
+      }
     }
   }
 }
@@ -569,8 +584,10 @@ __global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t j) {
   for (uint64_t i = gw; i < a.n_pull_items; i += nw) {
     const uint64_t it = a.pull_items[i];
     const uint64_t v = it >> 32;
-    const uint32_t e0 = a.in_ptr[v] + uint32_t(it) * kPullEdges;
-    const uint32_t b1 = min(e0 + kPullEdges, a.in_ptr[v + 1]);
+    const uint32_t r0 = a.in_ptr[v], r1 = a.in_ptr[v + 1];
+    const uint32_t e0 = r0 + uint32_t(it) * kPullEdges;
+    const uint32_t b1 = min(e0 + kPullEdges, r1);
+    const bool whole = e0 == r0 && b1 == r1;  // the row's only item: its only writer in this step
     float best = kInfDist;
     for (uint32_t e = e0; e < b1; e += kWave) {
       const bool ok = e + lane < b1;
@@ -603,9 +620,19 @@ __global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t j) {
     }
     float* p = &a.dist[v * 64 + lane];
     bool lower = ((a.lanes >> lane) & 1ull) && best < *p;
-    if (lower) lower = best < __int_as_float(atomicMin(reinterpret_cast<int*>(p), __float_as_int(best)));
+    if (lower) {
+      if (whole)
+        *p = best;
+      else
+        lower = best < __int_as_float(atomicMin(reinterpret_cast<int*>(p), __float_as_int(best)));
+    }
     const uint64_t m = __ballot(lower);
-    if (m && lane == 0) atomicOr((unsigned long long*)&a.st.nx[v], m);
+    if (m && lane == 0) {
+      if (whole)
+        a.st.nx[v] = m;
+      else
+        atomicOr((unsigned long long*)&a.st.nx[v], m);
+    }
   }
 }
 
