@@ -651,20 +651,42 @@ __global__ __launch_bounds__(kNT) void k_msssp_end(SsspArgs a, uint32_t j) {
 __global__ __launch_bounds__(kNT) void k_bfs_multi_stats(const uint32_t* __restrict__ depth, uint32_t V,
                                                          const uint32_t* __restrict__ deg, uint32_t* __restrict__ maxd,
                                                          unsigned long long* __restrict__ edges) {
+  // a wave reads 8 vertices' depth rows per step (all in flight), the block
+  // combines its waves in LDS, then one atomic pair per lane per block
+  constexpr int U = 8;
+  __shared__ uint32_t s_md[kNT / kWave][kWave];
+  __shared__ unsigned long long s_ed[kNT / kWave][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
   uint32_t md = 0;
   unsigned long long ed = 0;
-  for (uint64_t v = gw; v < V; v += nw) {
-    const uint32_t d = depth[v * 64 + lane];
-    if (d != kUnreached) {
-      md = max(md, d);
-      ed += deg[v];
+  for (uint64_t v0 = gw * U; v0 < V; v0 += nw * U) {
+    uint32_t d[U], dg[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint64_t v = v0 + k;
+      d[k] = v < V ? depth[v * 64 + lane] : kUnreached;
+      dg[k] = v < V ? deg[v] : 0u;
     }
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (d[k] != kUnreached) {
+        md = max(md, d[k]);
+        ed += dg[k];
+      }
   }
-  atomicMax(&maxd[lane], md);
-  atomicAdd(&edges[lane], ed);
+  s_md[threadIdx.x / kWave][lane] = md;
+  s_ed[threadIdx.x / kWave][lane] = ed;
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    for (int w = 1; w < kNT / kWave; ++w) {
+      md = max(md, s_md[w][lane]);
+      ed += s_ed[w][lane];
+    }
+    if (md) atomicMax(&maxd[lane], md);
+    if (ed) atomicAdd(&edges[lane], ed);
+  }
 }
 
 int check_roots(const eg_graph* g, const uint32_t* h_roots, int n) {
@@ -746,8 +768,8 @@ int eg_bfs_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, 
   unsigned long long* edges = reinterpret_cast<unsigned long long*>(g->e_multi_roots.as<char>() + 512);
   EG_TRY(fill_t<uint32_t>(maxd, 64, 0u, s));
   EG_TRY(fill_t<unsigned long long>(edges, 64, 0ull, s));
-  hipLaunchKernelGGL(k_bfs_multi_stats, dim3(grid_for(g->V, kNT / kWave, 2048)), dim3(kNT), 0, s,
-                     (const uint32_t*)d_depth, g->V, (const uint32_t*)g->out_deg.as<uint32_t>(), maxd, edges);
+  hipLaunchKernelGGL(k_bfs_multi_stats, dim3(grid_for(ceil_div(uint64_t(g->V), uint64_t(8)), kNT / kWave, 512)),
+                     dim3(kNT), 0, s, (const uint32_t*)d_depth, g->V, (const uint32_t*)g->out_deg.as<uint32_t>(), maxd, edges);
   EG_HIP(hipGetLastError());
   uint32_t h_maxd[64];
   unsigned long long h_ed[64];
