@@ -1133,10 +1133,11 @@ __global__ __launch_bounds__(kNT) void k_sbfs_pull_items(SbfsArgs a, uint32_t j)
 
 // Level end (bfs.cpp:70-90): visited |= next, depth = level + 1, counters
 // of the next step; clears this step's frontier bitmap (the next step's
-// next).  A wave takes kEndWords bitmap words at a time with a
-// lane per vertex of each word; the depth stores and degree loads of all
-// its words are issued before the first degree is used, and the degrees
-// stay in registers for the hubs' items.
+// next).  The next bitmap's non-empty words are taken kEndWords at a time
+// with a lane per vertex of each word; the depth stores and byte-degree
+// loads of all of them are issued before the first degree is used.  After
+// a push step that left the next bitmap to it, the level end reads it off
+// parent (V until reached), kEndWords words at a time.
 constexpr int kEndWords = 16;
 
 __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
