@@ -2,10 +2,12 @@
 // one step select the next step's direction and tell when to stop) run as a
 // captured HIP graph: kStepBatch steps, each a fixed list of launches
 // reading the step number as *base + j, then one launch that advances the
-// base.  The graph is captured once on the graph handle's own stream and
-// replayed between host checks of the counters; a step after the last live
-// one returns at once.  The captured graph is kept for the next call with
-// the same arguments (same buffers) and launch list.
+// base and copies the next step's frontier flag, the step count and an
+// optional counter into pinned host memory.  The graph is captured once on
+// the graph handle's own stream and replayed until that flag is zero (the
+// host reads it after each replay, with no copy call); a step after the last
+// live one returns at once.  The captured graph is kept for the next call
+// with the same arguments (same buffers) and launch list.
 #pragma once
 
 #include <vector>
