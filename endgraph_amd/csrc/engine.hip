@@ -1322,7 +1322,11 @@ struct DssspArgs {
   const uint64_t* rowflag;
   SbfsState st;  // bm[2], cnt, steps, base (vis and total unused)
   float* dist;
+  uint64_t* sum[2];  // active summary of step s in sum[s & 1]: bit w set iff word w of its bitmap is not empty
+  uint32_t nsum;     // summary words, ((V >> 6) >> 6) + 1
 };
+
+constexpr uint32_t kSumMax = 4096;  // summary words a dense step stages in LDS (V up to 2^24)
 
 // Sparse slot (sssp.cpp:44-58) for the out-edges e = first + k * stride,
 // k < n <= kRelaxBatch, of active vertex u at distance du: targets, weights
@@ -1419,16 +1423,31 @@ __global__ __launch_bounds__(kNT) void k_dsssp_push(DssspArgs a, uint32_t j) {
   }
 }
 
-// Dense step: k_sssp_pull's reduction on this step's bitmaps.
+// Dense step: k_sssp_pull's reduction on this step's bitmaps, over the
+// in-edges whose source lies in a word of the active bitmap that is not
+// empty (its summary staged in LDS, when it fits).  A source outside the
+// active set has not changed since its out-edges were last relaxed, so
+// dist[dst] <= dist[src] + w already holds for it and skipping it changes
+// no distance; the reference relaxes every in-edge (sssp.cpp:65 has its
+// active test commented out).
 __global__ __launch_bounds__(kNT) void k_dsssp_pull(DssspArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const SCountView c = count_of(a, step);
-  if (!c.nv || dsssp_sparse(a.policy, c.ne, a.E)) return;
+  if (!c.nv || dsssp_sparse(a.policy, c.ne, a.E)) return;  // block-uniform
+  __shared__ uint64_t s_sum[kSumMax];
+  const bool use_sum = a.nsum <= kSumMax;
+  if (use_sum) {
+    const uint64_t* sum = a.sum[step & 1];
+    for (uint32_t i = threadIdx.x; i < a.nsum; i += blockDim.x) s_sum[i] = sum[i];
+    __syncthreads();
+  }
   uint64_t* nx = a.st.bm[(step + 1) & 1];
   float* dist = a.dist;
   row_reduce_wave<float>(
       first_word(), a.E, a.csc_src, a.rowflag, a.wordrank, 1e9f, OpMin{},
-      [&](uint64_t, uint32_t s, float we) { return __fadd_rn(dist[s], we); },
+      [&](uint64_t, uint32_t s, float we) {
+        return !use_sum || ((s_sum[s >> 12] >> ((s >> 6) & 63)) & 1ull) ? __fadd_rn(dist[s], we) : 1e9f;
+      },
       [&](uint32_t row, float m, bool) {
         const uint32_t v = a.row_dst[row];
         if (m < 1e9f && m < dist[v]) {
@@ -1457,6 +1476,7 @@ __global__ __launch_bounds__(kNT) void k_dsssp_end(DssspArgs a, uint32_t j) {
   const uint64_t words = (uint64_t(a.V) >> 6) + 1;
   uint64_t* fr = a.st.bm[step & 1];
   const uint64_t* nx = a.st.bm[(step + 1) & 1];
+  uint64_t* nsum = a.sum[(step + 1) & 1];
   SCount* nc = &a.st.cnt[(step + 1) % 3];
   unsigned long long nv = 0, ne = 0;
   for (uint64_t b0 = gw * kWave; b0 < words; b0 += nw * kWave) {
@@ -1468,6 +1488,7 @@ __global__ __launch_bounds__(kNT) void k_dsssp_end(DssspArgs a, uint32_t j) {
     }
     nv += __popcll(mine);
     uint64_t nz = __ballot(mine != 0);
+    if (lane == 0) nsum[b0 >> 6] = nz;  // the next step's summary word
     while (nz) {  // wave-uniform
       int idx[kEndWords];
 #pragma unroll
@@ -1515,6 +1536,7 @@ __global__ void k_dsssp_init(DssspArgs a, uint32_t root) {
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
     if (i < a.V) a.dist[i] = i == root ? 0.0f : 1e9f;
     if (i < 2 * words) a.st.bm[0][i] = i == (root >> 6) ? 1ull << (root & 63) : 0ull;  // bm[0], bm[1] contiguous
+    if (i < a.nsum) a.sum[0][i] = i == (root >> 12) ? 1ull << ((root >> 6) & 63) : 0ull;
   }
   if (blockIdx.x != 0) return;
   unsigned long long* z = reinterpret_cast<unsigned long long*>(a.st.cnt);
@@ -2093,7 +2115,8 @@ int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_i
   if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
   const uint32_t V = g->V;
   const uint64_t words = words_of(V);
-  EG_TRY(g->e_dsssp.resize(2 * words * 8 + 64 + 3 * sizeof(SCount) + 64));
+  const uint64_t nsum = (words >> 6) + 1;
+  EG_TRY(g->e_dsssp.resize(2 * words * 8 + 64 + 3 * sizeof(SCount) + 64 + 2 * nsum * 8));
   DssspArgs a{};  // zeroed padding: the bytes key the captured step graph
   SbfsState& st = a.st;
   st.bm[0] = g->e_dsssp.as<uint64_t>();
@@ -2101,6 +2124,9 @@ int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_i
   st.cnt = reinterpret_cast<SCount*>((reinterpret_cast<uintptr_t>(st.bm[1] + words) + 63) & ~uintptr_t(63));
   st.steps = reinterpret_cast<uint32_t*>(st.cnt + 3);
   st.base = st.steps + 1;
+  a.sum[0] = reinterpret_cast<uint64_t*>(st.cnt + 3) + 8;  // after steps and base, 64 bytes on
+  a.sum[1] = a.sum[0] + nsum;
+  a.nsum = uint32_t(nsum);
   a.V = V;
   a.E = g->E;
   a.policy = policy;
