@@ -121,9 +121,9 @@ def lib():
         "eg_bfs_level": ([vp, i32, u32, vp, vp, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64),
                           C.POINTER(i32), vp], i32),
         "eg_bfs": ([vp, u32, i32, vp, vp, C.POINTER(u32), C.POINTER(u64), vp], i32),
-        "eg_cc_iteration": ([vp, i32, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64), vp], i32),
+        "eg_cc_iteration": ([vp, i32, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), vp], i32),
         "eg_cc": ([vp, i32, vp, C.POINTER(u32), vp], i32),
-        "eg_sssp_iteration": ([vp, i32, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64), vp], i32),
+        "eg_sssp_iteration": ([vp, i32, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), vp], i32),
         "eg_sssp": ([vp, u32, i32, vp, C.POINTER(u32), vp], i32),
         "eg_ppr_slots": ([vp, vp, vp, vp, vp], i32),
         "eg_ppr_forward": ([vp, vp, vp, f64, i32, vp, vp], i32),
@@ -254,6 +254,42 @@ def memcpy_d2d(d_dst: int, d_src: int, nbytes: int, stream=None):
 
 def synchronize(stream=None):
     check(lib().eg_stream_synchronize(_stream(stream)))
+
+
+def device_fill(d_ptr: int, n: int, value, dtype, stream=None):
+    """fill_vertex_array (core/graph.hpp:289-294): n copies of value."""
+    v = np.array([value], dtype=dtype)
+    check(lib().eg_device_fill(C.c_void_p(d_ptr), n, v.ctypes.data_as(C.c_void_p), v.itemsize, _stream(stream)))
+
+
+def device_iota(d_ptr: int, n: int, stream=None):
+    """d[i] = i for i < n (cc.cpp:35-41's label)."""
+    check(lib().eg_device_iota(C.c_void_p(d_ptr), n, _stream(stream)))
+
+
+# BitmapType (core/bitmap.hpp:23-68) words in HBM: (V >> 6) + 1 u64
+def bitmap_clear(bm: DeviceBuffer, V: int, stream=None):
+    check(lib().eg_bitmap_clear(C.c_void_p(bm.ptr), V, _stream(stream)))
+
+
+def bitmap_fill(bm: DeviceBuffer, V: int, stream=None):
+    check(lib().eg_bitmap_fill(C.c_void_p(bm.ptr), V, _stream(stream)))
+
+
+def bitmap_set_bit(bm: DeviceBuffer, v: int, stream=None):
+    check(lib().eg_bitmap_set_bit(C.c_void_p(bm.ptr), v, _stream(stream)))
+
+
+def bitmap_count(bm: DeviceBuffer, V: int, stream=None) -> int:
+    c = C.c_uint64()
+    check(lib().eg_bitmap_count(C.c_void_p(bm.ptr), V, C.byref(c), _stream(stream)))
+    return c.value
+
+
+def bitmap_ids(words: np.ndarray, V: int) -> np.ndarray:
+    """Set bits of host bitmap words as ascending vertex ids (below V)."""
+    bits = np.unpackbits(words.view(np.uint8), bitorder="little")[:V]
+    return np.flatnonzero(bits)
 
 
 # ------------------------------------------------------------ generators --
@@ -453,6 +489,51 @@ class Graph:
         it = C.c_uint32()
         check(lib().eg_cc(self._h, policy, C.c_void_p(label.ptr), C.byref(it), _stream(stream)))
         return (label.download(np.uint32, self.V) if download else None), it.value
+
+    # ------ one process_edges / process_vertices step at a time: the path
+    # the C++ facade's Graph<EdgeData>::process_edges takes -------------------
+    def bitmap_degree_sum(self, bm: DeviceBuffer, stream=None) -> int:
+        """Sum of out-degrees over the set bits (the mode switch input, core/graph.hpp:1484-1490)."""
+        s = C.c_uint64()
+        check(lib().eg_bitmap_degree_sum(self._h, C.c_void_p(bm.ptr), C.byref(s), _stream(stream)))
+        return s.value
+
+    def bfs_level(self, policy, level, frontier, nxt, visited, parent, depth, frontier_edges, stream=None):
+        """One BFS level (bfs.cpp:41-86): returns (new vertices, their out-edges, ran sparse)."""
+        nv, ne, sp = C.c_uint64(), C.c_uint64(), C.c_int()
+        check(lib().eg_bfs_level(self._h, policy, level, C.c_void_p(frontier.ptr), C.c_void_p(nxt.ptr),
+                                 C.c_void_p(visited.ptr), C.c_void_p(parent.ptr),
+                                 C.c_void_p(depth.ptr if depth is not None else None), frontier_edges, C.byref(nv),
+                                 C.byref(ne), C.byref(sp), _stream(stream)))
+        return nv.value, ne.value, bool(sp.value)
+
+    def cc_iteration(self, policy, active, nxt, label, active_edges, stream=None):
+        """One CC step (cc.cpp:45-80): returns (new active vertices, their out-edges, ran sparse)."""
+        nv, ne, sp = C.c_uint64(), C.c_uint64(), C.c_int()
+        check(lib().eg_cc_iteration(self._h, policy, C.c_void_p(active.ptr), C.c_void_p(nxt.ptr),
+                                    C.c_void_p(label.ptr), active_edges, C.byref(nv), C.byref(ne), C.byref(sp),
+                                    _stream(stream)))
+        return nv.value, ne.value, bool(sp.value)
+
+    def sssp_iteration(self, policy, active, nxt, dist, active_edges, stream=None):
+        """One SSSP step (sssp.cpp:38-83): returns (new active vertices, their out-edges, ran sparse)."""
+        nv, ne, sp = C.c_uint64(), C.c_uint64(), C.c_int()
+        check(lib().eg_sssp_iteration(self._h, policy, C.c_void_p(active.ptr), C.c_void_p(nxt.ptr),
+                                      C.c_void_p(dist.ptr), active_edges, C.byref(nv), C.byref(ne), C.byref(sp),
+                                      _stream(stream)))
+        return nv.value, ne.value, bool(sp.value)
+
+    def pr_init(self, x: DeviceBuffer, x32: DeviceBuffer = None, stream=None):
+        check(lib().eg_pr_init(self._h, C.c_void_p(x.ptr), C.c_void_p(x32.ptr if x32 else None), _stream(stream)))
+
+    def pr_pull(self, precision: int, x: DeviceBuffer, nxt: DeviceBuffer, stream=None):
+        check(lib().eg_pr_pull(self._h, precision, C.c_void_p(x.ptr), C.c_void_p(nxt.ptr), _stream(stream)))
+
+    def pr_apply(self, x: DeviceBuffer, nxt: DeviceBuffer, nxt32: DeviceBuffer, last: bool, delta: DeviceBuffer,
+                 stream=None):
+        check(lib().eg_pr_apply(self._h, C.c_void_p(x.ptr), C.c_void_p(nxt.ptr),
+                                C.c_void_p(nxt32.ptr if nxt32 else None), int(bool(last)), C.c_void_p(delta.ptr),
+                                _stream(stream)))
 
     # ------------------------------------------------ personalised PageRank --
     def ppr_slots(self, stream=None):

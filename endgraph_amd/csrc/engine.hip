@@ -1933,6 +1933,7 @@ int bfs_level(eg_graph* g, int policy, uint32_t level, const uint64_t* d_frontie
               uint64_t* d_visited, uint32_t* d_parent, uint32_t* d_depth, uint64_t frontier_edges, int64_t frontier_n,
               uint64_t* h_new_vertices, uint64_t* h_new_edges, int* h_was_sparse, hipStream_t s) {
   EG_TRY(ensure_engine(g, s));
+  if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
   Scratch sc;
   EG_TRY(get_scratch(g, 0, &sc));
   const bool sparse = choose_sparse(g, policy, frontier_edges, true);
@@ -2043,13 +2044,17 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
 
 // ---- CC (toolkits/cc.cpp) ----
 int eg_cc_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t* d_next, uint32_t* d_label,
-                    uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream) {
+                    uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, int* h_was_sparse,
+                    void* stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   EG_TRY(ensure_engine(g, s));
+  if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
   Scratch sc;
   EG_TRY(get_scratch(g, 0, &sc));
   EG_TRY(fill<uint64_t>(d_next, words_of(g->V), 0ull, s));
-  if (choose_sparse(g, policy, active_edges, false)) {
+  const bool sparse = choose_sparse(g, policy, active_edges, false);
+  if (h_was_sparse) *h_was_sparse = sparse ? 1 : 0;
+  if (sparse) {
     uint32_t fn = 0, total = 0;
     EG_TRY(make_frontier(g, d_active, sc, &fn, &total, s));
     EG_TRY(expand(g, sc, fn, total, CcPushOp{g->e_out_dst.as<uint32_t>(), d_label, d_next}, s));
@@ -2075,7 +2080,7 @@ int eg_cc(eg_graph* g, int policy, uint32_t* d_label, uint32_t* h_iterations, vo
   uint64_t nv = g->V, ne = g->E;
   uint32_t it = 0;
   while (nv > 0) {
-    EG_TRY(eg_cc_iteration(g, policy, a, b, d_label, ne, &nv, &ne, stream));
+    EG_TRY(eg_cc_iteration(g, policy, a, b, d_label, ne, &nv, &ne, nullptr, stream));
     std::swap(a, b);
     ++it;
   }
@@ -2085,14 +2090,18 @@ int eg_cc(eg_graph* g, int policy, uint32_t* d_label, uint32_t* h_iterations, vo
 
 // ---- SSSP (toolkits/sssp.cpp) ----
 int eg_sssp_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t* d_next, float* d_dist,
-                      uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream) {
+                      uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, int* h_was_sparse,
+                      void* stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   EG_TRY(ensure_engine(g, s));
   if (g->rec != 12) return fail(EG_ERR_STATE, "SSSP needs a graph of EdgeType<float> (12-byte) records");
+  if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
   Scratch sc;
   EG_TRY(get_scratch(g, 0, &sc));
   EG_TRY(fill<uint64_t>(d_next, words_of(g->V), 0ull, s));
-  if (choose_sparse(g, policy, active_edges, true)) {
+  const bool sparse = choose_sparse(g, policy, active_edges, true);
+  if (h_was_sparse) *h_was_sparse = sparse ? 1 : 0;
+  if (sparse) {
     uint32_t fn = 0, total = 0;
     EG_TRY(make_frontier(g, d_active, sc, &fn, &total, s));
     EG_TRY(expand(g, sc, fn, total,

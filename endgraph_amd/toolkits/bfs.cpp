@@ -1,7 +1,9 @@
 // BFS drop-in: toolkits/bfs.cpp of the reference on the MI355X library.
 // Parents follow the minimum-id rule (the reference's CAS race leaves the
 // parent open).  An optional last argument "auto" enables the push/pull
-// switch for this rooted program (the reference always pushes).
+// switch for this rooted program (the reference always pushes).  With
+// EG_DUMP=<path> in the environment the first run also writes its parents
+// there with dump_vertex_array (the reference's raw V * 4-byte format).
 #include <cstdlib>
 
 #include "graph.hpp"
@@ -37,6 +39,7 @@ void compute(Graph<Empty>* graph, UINT root) {
 
   exec_time += get_time();
 
+  dump_result(graph, parent);
   std::vector<UINT> par = graph->gather_vertex_array(parent, 0);
   UINT found_vertices = 0;
   for (UINT v_i = 0; v_i < graph->vertices_num; v_i++)

@@ -1,5 +1,7 @@
 // CC drop-in: toolkits/cc.cpp of the reference on the MI355X library
 // (symmetrised graph, minimum-label propagation, undirected mode switch).
+// With EG_DUMP=<path> in the environment the first run also writes its labels
+// there with dump_vertex_array (the reference's raw V * 4-byte format).
 #include <cstdlib>
 
 #include "graph.hpp"
@@ -27,6 +29,7 @@ void compute(Graph<Empty>* graph) {
 
   exec_time += get_time();
 
+  dump_result(graph, label);
   std::vector<UINT> lab = graph->gather_vertex_array(label, 0);
   std::vector<UINT> count(graph->vertices_num, 0);
   for (UINT v_i = 0; v_i < graph->vertices_num; v_i++) count[lab[v_i]] += 1;

@@ -1,7 +1,9 @@
 // PageRank drop-in: toolkits/pagerank.cpp of the reference on the MI355X
 // library.  Same compute() structure and printouts; the lambdas become the
 // PageRank programs of the facade.  An optional last argument "fp32c" runs
-// the FP32 gathers with compensated sums (residual and update in FP64).
+// the FP32 gathers with compensated sums (residual and update in FP64).  With
+// EG_DUMP=<path> in the environment the first run also writes its ranks
+// there with dump_vertex_array (the reference's raw V * 8-byte format).
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -36,6 +38,7 @@ void compute(Graph<Empty>* graph, int iterations, bool fp32c) {
 
   exec_time += get_time();
 
+  dump_result(graph, curr);
   std::vector<double> pr = graph->gather_vertex_array(curr, 0);
   double pr_sum = 0;
   UINT max_v_i = 0;

@@ -1,5 +1,7 @@
 // SSSP drop-in: toolkits/sssp.cpp of the reference on the MI355X library
-// (float weights, INF = 1e9, write_min relaxation).
+// (float weights, INF = 1e9, write_min relaxation).  With EG_DUMP=<path> in
+// the environment the first run also writes its distances there with
+// dump_vertex_array (the reference's raw V * 4-byte format).
 #include <cstdlib>
 
 #include "graph.hpp"
@@ -32,6 +34,7 @@ void compute(Graph<Weight>* graph, UINT root) {
 
   exec_time += get_time();
 
+  dump_result(graph, distance);
   std::vector<Weight> dist = graph->gather_vertex_array(distance, 0);
   UINT max_v_i = root;
   for (UINT v_i = 0; v_i < graph->vertices_num; v_i++)

@@ -266,9 +266,13 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
 int eg_traversal_view_prepare(eg_graph* g, double* h_ms, void* stream);
 
 /* CC, toolkits/cc.cpp:22-112 (graph built with symmetrize=1): label =
- * smallest id of the component.  One iteration or the whole loop. */
+ * smallest id of the component.  One iteration or the whole loop.
+ * eg_cc_iteration / eg_sssp_iteration / eg_bfs_level take the active set's
+ * out-edge sum (the mode switch input, core/graph.hpp:1484-1490) and report
+ * in *h_was_sparse (optional) the mode the step ran in. */
 int eg_cc_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t* d_next, uint32_t* d_label,
-                    uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream);
+                    uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, int* h_was_sparse,
+                    void* stream);
 int eg_cc(eg_graph* g, int policy, uint32_t* d_label, uint32_t* h_iterations, void* stream);
 
 /* SSSP, toolkits/sssp.cpp:24-108 (12-byte records): float distances, 1e9
@@ -279,7 +283,8 @@ int eg_cc(eg_graph* g, int policy, uint32_t* d_label, uint32_t* h_iterations, vo
  * out-edges) and returns when it is done.  Distances do not depend on the
  * policy or the step order (the least fixpoint of the relaxation). */
 int eg_sssp_iteration(eg_graph* g, int policy, const uint64_t* d_active, uint64_t* d_next, float* d_dist,
-                      uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, void* stream);
+                      uint64_t active_edges, uint64_t* h_new_active, uint64_t* h_new_edges, int* h_was_sparse,
+                      void* stream);
 int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_iterations, void* stream);
 
 /* Personalized PageRank with per-edge weights and its adjoint (no reference

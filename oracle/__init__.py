@@ -57,6 +57,7 @@ def lib():
         L.orc_bfs.argtypes = [vp, u32, i32, vp, vp, vp]
         L.orc_cc.argtypes = [vp, vp, vp]
         L.orc_sssp.argtypes = [vp, u32, vp, vp]
+        L.orc_csr_batch.argtypes = [vp, i32, i32, vp, vp, vp, vp, i32]
         _lib = L
     return _lib
 
@@ -72,6 +73,20 @@ def optimal_offset(a: np.ndarray, k: int) -> np.ndarray:
     out = np.zeros(k + 1, dtype=np.uint32)
     _check(lib().orc_optimal_offset(a.ctypes.data, a.size, k, out.ctypes.data), "optimal_offset")
     return out
+
+
+def csr_batch(edges: np.ndarray, vertices, edge_counts, threads: int = 0):
+    """prepare_push_mode at P = S = 1 (core/graph.hpp:1248-1355) of every graph
+    of a batch stored back to back: (CSR records at the input offsets, row
+    pointers of vertices[i] + 1 entries each, back to back)."""
+    edges = np.ascontiguousarray(edges)
+    nv = np.ascontiguousarray(vertices, dtype=np.uint32)
+    ne = np.ascontiguousarray(edge_counts, dtype=np.uint64)
+    out = np.empty_like(edges)
+    rowptr = np.empty(int(nv.astype(np.uint64).sum()) + nv.size, dtype=np.uint64)
+    _check(lib().orc_csr_batch(edges.ctypes.data, edges.dtype.itemsize, nv.size, nv.ctypes.data, ne.ctypes.data,
+                               out.ctypes.data, rowptr.ctypes.data, int(threads)), "csr_batch")
+    return out, rowptr
 
 
 def _arr(ptr, n, dtype):

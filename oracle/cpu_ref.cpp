@@ -399,6 +399,31 @@ inline void set_bit(std::vector<uint64_t>& bm, uint32_t v) {
   __atomic_fetch_or(&bm[v >> 6], 1ull << (v & 63), __ATOMIC_RELAXED);
 }
 
+// write_min (core/atomic.hpp:35-43): CAS loop, true iff this call lowered *p.
+inline bool write_min_u32(uint32_t* p, uint32_t v) {
+  uint32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+  while (v < cur)
+    if (__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return true;
+  return false;
+}
+// write_min on a non-negative float: its bits order as the values do.
+inline bool write_min_f32(float* p, float v) {
+  uint32_t vb;
+  std::memcpy(&vb, &v, 4);
+  return write_min_u32(reinterpret_cast<uint32_t*>(p), vb);
+}
+
+// Set bits of a bitmap as an ascending vertex list.
+std::vector<uint32_t> bitmap_list(const std::vector<uint64_t>& bm, uint32_t V) {
+  std::vector<uint32_t> out;
+  for (size_t w = 0; w < bm.size(); ++w)
+    for (uint64_t b = bm[w]; b; b &= b - 1) {
+      const uint32_t v = uint32_t(w * 64 + __builtin_ctzll(b));
+      if (v < V) out.push_back(v);
+    }
+  return out;
+}
+
 // PageRank, toolkits/pagerank.cpp:26-103, always dense (use_iteration).
 // Partials of partition i for a destination owned by partition m are
 // combined in the receive order of core/graph.hpp:1756-1801:
@@ -484,15 +509,19 @@ void for_out_edges(const Graph& g, uint32_t u, F&& f) {
 }
 
 // BFS, toolkits/bfs.cpp:22-112 with R12 (min-id parent).  policy 0 =
-// reference (always push, core/graph.hpp:1497), 1 = direction-optimising.
-// Level-synchronous: parents and depths do not depend on the policy.
+// reference (always push, core/graph.hpp:1497), 1 = direction-optimising
+// (push iff the frontier's out-degree sum < E/20).  Level-synchronous and
+// OpenMP-parallel like the reference: the push's cas on parent (bfs.cpp:54)
+// becomes write_min among the vertices not yet visited, so parents and
+// depths do not depend on the policy, the thread count or the schedule.
 int bfs_impl(const Graph& g, uint32_t root, int policy, uint32_t* parent, uint32_t* depth,
              uint32_t* levels_out) {
   const uint32_t V = g.V;
   const int T = g.threads;
   if (root >= V) return -2;
   std::vector<uint64_t> visited((V >> 6) + 1, 0), active_in((V >> 6) + 1, 0), active_out((V >> 6) + 1, 0);
-  for (uint32_t v = 0; v < V; ++v) {
+#pragma omp parallel for num_threads(T)
+  for (int64_t v = 0; v < int64_t(V); ++v) {
     parent[v] = V;
     depth[v] = UINT32_MAX;
   }
@@ -504,21 +533,22 @@ int bfs_impl(const Graph& g, uint32_t root, int policy, uint32_t* parent, uint32
   uint32_t level = 0;
   while (active_vertices > 0) {
     std::fill(active_out.begin(), active_out.end(), 0);
+    const std::vector<uint32_t> front = bitmap_list(active_in, V);
     bool push = true;
     if (policy == 1) {
       uint64_t active_edges = 0;
-      for (uint32_t v = 0; v < V; ++v)
-        if (bit(active_in, v)) active_edges += g.out_deg[v];
+      for (uint32_t v : front) active_edges += g.out_deg[v];
       push = active_edges < g.E / 20;
     }
     if (push) {
-      // min-id rule: the smallest frontier vertex reaching dst wins
-      for (uint32_t u = 0; u < V; ++u) {
-        if (!bit(active_in, u)) continue;
+      const int64_t nf = int64_t(front.size());
+#pragma omp parallel for num_threads(T) schedule(dynamic, 64)
+      for (int64_t k = 0; k < nf; ++k) {
+        const uint32_t u = front[k];
         for_out_edges<Rec8>(g, u, [&](const Rec8& e) {
-          uint32_t dst = e.dst;
-          if (!bit(visited, dst) && parent[dst] == V) {
-            parent[dst] = u;
+          const uint32_t dst = e.dst;
+          if (!bit(visited, dst)) {
+            write_min_u32(&parent[dst], u);
             set_bit(active_out, dst);
           }
         });
@@ -537,23 +567,19 @@ int bfs_impl(const Graph& g, uint32_t root, int policy, uint32_t* parent, uint32
             if (bit(active_in, adj[e].src)) best = std::min(best, adj[e].src);
           if (best < V) {
             // several sub-partitions may propose: keep the minimum
-            uint32_t cur = __atomic_load_n(&parent[dst], __ATOMIC_RELAXED);
-            while (best < cur && !__atomic_compare_exchange_n(&parent[dst], &cur, best, false,
-                                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
-            }
+            write_min_u32(&parent[dst], best);
             set_bit(active_out, dst);
           }
         }
       }
     }
     ++level;
-    active_vertices = 0;
-    for (uint32_t v = 0; v < V; ++v)
-      if (bit(active_out, v)) {
-        set_bit(visited, v);
-        depth[v] = level;
-        ++active_vertices;
-      }
+    const std::vector<uint32_t> next = bitmap_list(active_out, V);
+    for (uint32_t v : next) {
+      set_bit(visited, v);
+      depth[v] = level;
+    }
+    active_vertices = next.size();
     std::swap(active_in, active_out);
   }
   if (levels_out) *levels_out = level;
@@ -561,29 +587,33 @@ int bfs_impl(const Graph& g, uint32_t root, int policy, uint32_t* parent, uint32
 }
 
 // CC, toolkits/cc.cpp:22-112 on the symmetrised graph; mode switch of
-// core/graph.hpp:1484-1492 with R4.
+// core/graph.hpp:1484-1492 with R4.  Both modes run on OpenMP threads with
+// the reference's write_min (cc.cpp:54, :75); labels converge to the
+// smallest id of each component whatever the interleaving.
 int cc_impl(const Graph& g, uint32_t* label, uint32_t* iters_out) {
   const uint32_t V = g.V;
+  const int T = g.threads;
   std::vector<uint64_t> active_in((V >> 6) + 1, 0), active_out((V >> 6) + 1, 0);
-  for (uint32_t v = 0; v < V; ++v) {
-    label[v] = v;
-    set_bit(active_in, v);
-  }
+#pragma omp parallel for num_threads(T)
+  for (int64_t v = 0; v < int64_t(V); ++v) label[v] = uint32_t(v);
+  for (uint32_t v = 0; v < V; ++v) set_bit(active_in, v);
   uint64_t active_vertices = V;
   uint32_t it = 0;
   while (active_vertices > 0) {
     std::fill(active_out.begin(), active_out.end(), 0);
+    const std::vector<uint32_t> front = bitmap_list(active_in, V);
     uint64_t active_edges = 0;
-    for (uint32_t v = 0; v < V; ++v)
-      if (bit(active_in, v)) active_edges += g.out_deg[v];
+    for (uint32_t v : front) active_edges += g.out_deg[v];
     const bool sparse = active_edges < g.E / 20;
     if (sparse) {
-      for (uint32_t u = 0; u < V; ++u) {
-        if (!bit(active_in, u)) continue;
-        uint32_t msg = label[u];
+      const int64_t nf = int64_t(front.size());
+#pragma omp parallel for num_threads(T) schedule(dynamic, 64)
+      for (int64_t k = 0; k < nf; ++k) {
+        const uint32_t u = front[k];
+        const uint32_t msg = __atomic_load_n(&label[u], __ATOMIC_RELAXED);
         for_out_edges<Rec8>(g, u, [&](const Rec8& e) {
-          if (msg < label[e.dst]) {
-            label[e.dst] = msg;
+          if (msg < __atomic_load_n(&label[e.dst], __ATOMIC_RELAXED)) {
+            write_min_u32(&label[e.dst], msg);
             set_bit(active_out, e.dst);
           }
         });
@@ -592,12 +622,15 @@ int cc_impl(const Graph& g, uint32_t* label, uint32_t* iters_out) {
       for (int q = 0; q < g.P * g.S; ++q) {
         const auto& ci = g.csc_cindex[q];
         const Rec8* adj = csc_edges<Rec8>(g, q);
-        for (size_t k = 0; k + 1 < ci.size(); ++k) {
-          uint32_t dst = ci[k].vertex;
+        const int64_t nd = int64_t(ci.size()) - 1;
+#pragma omp parallel for num_threads(T) schedule(dynamic, 256)
+        for (int64_t k = 0; k < nd; ++k) {
+          const uint32_t dst = ci[k].vertex;
           uint32_t msg = dst;
-          for (uint64_t e = ci[k].index; e < ci[k + 1].index; ++e) msg = std::min(msg, label[adj[e].src]);
-          if (msg < dst && msg < label[dst]) {
-            label[dst] = msg;
+          for (uint64_t e = ci[k].index; e < ci[k + 1].index; ++e)
+            msg = std::min(msg, __atomic_load_n(&label[adj[e].src], __ATOMIC_RELAXED));
+          if (msg < dst && msg < __atomic_load_n(&label[dst], __ATOMIC_RELAXED)) {
+            write_min_u32(&label[dst], msg);
             set_bit(active_out, dst);
           }
         }
@@ -612,27 +645,34 @@ int cc_impl(const Graph& g, uint32_t* label, uint32_t* iters_out) {
   return 0;
 }
 
-// SSSP, toolkits/sssp.cpp:24-108: float distances, INF = 1e9, push relax.
+// SSSP, toolkits/sssp.cpp:24-108: float distances, INF = 1e9, push relax
+// with write_min (sssp.cpp:53) on OpenMP threads.  One fp32 add per
+// relaxation, rounded to nearest: the add is monotone, so the fixpoint is the
+// minimum over paths of the left-fold sums whatever the order of updates.
 int sssp_impl(const Graph& g, uint32_t root, float* dist, uint32_t* iters_out) {
   const uint32_t V = g.V;
+  const int T = g.threads;
   if (root >= V) return -2;
   std::vector<uint64_t> active_in((V >> 6) + 1, 0), active_out((V >> 6) + 1, 0);
-  for (uint32_t v = 0; v < V; ++v) dist[v] = 1e9f;
+#pragma omp parallel for num_threads(T)
+  for (int64_t v = 0; v < int64_t(V); ++v) dist[v] = 1e9f;
   dist[root] = 0.0f;
   set_bit(active_in, root);
   uint64_t active_vertices = 1;
   uint32_t it = 0;
   while (active_vertices > 0) {
     std::fill(active_out.begin(), active_out.end(), 0);
-    for (uint32_t u = 0; u < V; ++u) {
-      if (!bit(active_in, u)) continue;
-      float msg = dist[u];
+    const std::vector<uint32_t> front = bitmap_list(active_in, V);
+    const int64_t nf = int64_t(front.size());
+#pragma omp parallel for num_threads(T) schedule(dynamic, 64)
+    for (int64_t k = 0; k < nf; ++k) {
+      const uint32_t u = front[k];
+      uint32_t mb = __atomic_load_n(reinterpret_cast<const uint32_t*>(&dist[u]), __ATOMIC_RELAXED);
+      float msg;
+      std::memcpy(&msg, &mb, 4);
       for_out_edges<Rec12>(g, u, [&](const Rec12& e) {
         volatile float relax = msg + e.w;  // one fp32 add, round to nearest
-        if (relax < dist[e.dst]) {
-          dist[e.dst] = relax;
-          set_bit(active_out, e.dst);
-        }
+        if (write_min_f32(&dist[e.dst], relax)) set_bit(active_out, e.dst);
       });
     }
     active_vertices = 0;
@@ -642,6 +682,32 @@ int sssp_impl(const Graph& g, uint32_t root, float* dist, uint32_t* iters_out) {
   }
   if (iters_out) *iters_out = it;
   return 0;
+}
+
+// Batched CSR at P = S = 1: prepare_push_mode (core/graph.hpp:1248-1355) of
+// every graph on its own, i.e. a stable counting sort of its records by src
+// (:1262) and the row pointer of its dense index (:1312-1353, every entry
+// defined).  Graphs run in parallel, one thread each.
+template <class E>
+void csr_batch_impl(const E* edges, int n, const uint32_t* nv, const uint64_t* ne, E* out, uint64_t* rowptr,
+                    int threads) {
+  std::vector<uint64_t> eoff(size_t(n) + 1, 0), voff(size_t(n) + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    eoff[i + 1] = eoff[i] + ne[i];
+    voff[i + 1] = voff[i] + uint64_t(nv[i]) + 1;
+  }
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (int i = 0; i < n; ++i) {
+    const E* in = edges + eoff[i];
+    E* o = out + eoff[i];
+    uint64_t* rp = rowptr + voff[i];
+    const uint32_t V = nv[i];
+    std::fill(rp, rp + V + 1, 0);
+    for (uint64_t j = 0; j < ne[i]; ++j) rp[in[j].src + 1]++;
+    for (uint32_t v = 0; v < V; ++v) rp[v + 1] += rp[v];
+    std::vector<uint64_t> at(rp, rp + V);
+    for (uint64_t j = 0; j < ne[i]; ++j) o[at[in[j].src]++] = in[j];
+  }
 }
 
 }  // namespace
@@ -723,6 +789,20 @@ int orc_sssp(void* h, uint32_t root, float* dist, uint32_t* iters) {
   auto* g = static_cast<Graph*>(h);
   if (g->rec != 12) return -1;
   return sssp_impl(*g, root, dist, iters);
+}
+// n graphs back to back in edges (graph i: local ids < nv[i], ne[i]
+// records); out gets every graph's CSR list at its input offset, rowptr the
+// row pointers (nv[i] + 1 entries each) back to back.  Ids are not checked
+// beyond the caller's guarantee.
+int orc_csr_batch(const void* edges, int rec_bytes, int n, const uint32_t* nv, const uint64_t* ne, void* out,
+                  uint64_t* rowptr, int threads) {
+  if (n < 0 || (rec_bytes != 8 && rec_bytes != 12)) return -1;
+  const int T = threads > 0 ? threads : omp_get_max_threads();
+  if (rec_bytes == 8)
+    csr_batch_impl(static_cast<const Rec8*>(edges), n, nv, ne, static_cast<Rec8*>(out), rowptr, T);
+  else
+    csr_batch_impl(static_cast<const Rec12*>(edges), n, nv, ne, static_cast<Rec12*>(out), rowptr, T);
+  return 0;
 }
 
 }  // extern "C"

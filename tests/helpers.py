@@ -167,16 +167,24 @@ def dijkstra_f32(V, src, dst, w, root):
     return dist
 
 
-def pagerank_dense(V, src, dst, iterations, d=0.85):
-    """Dense FP64 power iteration of toolkits/pagerank.cpp's update."""
+def pagerank_dense(V, src, dst, iterations, d=0.85, with_deltas=False):
+    """Dense FP64 power iteration of toolkits/pagerank.cpp's update.  With
+    with_deltas, also the printed delta(i) of every iteration: 1 before the
+    first (the init returns 1 per vertex), then sum over vertices of
+    |x' - x| * deg (|x' - x| where deg = 0) divided by V, and 0 after the
+    last, undivided iteration (its process_vertices returns 0)."""
     deg = np.bincount(src, minlength=V).astype(np.float64)
     A = np.zeros((V, V))
     np.add.at(A, (dst, src), 1.0)  # A[v, u] = multiplicity of u->v
     x = np.where(deg > 0, 1.0 / np.maximum(deg, 1), 1.0)
+    deltas = [1.0]
     for it in range(iterations):
         y = (1 - d) + d * (A @ x)
         if it == iterations - 1:
             x = y
+            deltas.append(0.0)
         else:
-            x = np.where(deg > 0, y / np.maximum(deg, 1), y)
-    return x
+            xn = np.where(deg > 0, y / np.maximum(deg, 1), y)
+            deltas.append(float((np.abs(xn - x) * np.where(deg > 0, deg, 1.0)).sum() / V))
+            x = xn
+    return (x, np.array(deltas)) if with_deltas else x

@@ -1,6 +1,8 @@
 """Batched (64-root) BFS and SSSP: every root's column equals the single-root
 program from that root, bit for bit (parents under the min-id rule, depths,
-levels, traversed edges, fp32 distances)."""
+levels, traversed edges, fp32 distances), and four columns of every batched
+SSSP equal the oracle's distances (sssp.cpp:24-108), so an error the batched
+and the single-root programs shared would still show."""
 import numpy as np
 import pytest
 
@@ -42,13 +44,17 @@ def _check_bfs(g, roots, policy):
         assert (par[:, r] == g.V).all() and (dep[:, r] == 0xFFFFFFFF).all()
 
 
-def _check_sssp(g, roots, policy):
+def _check_sssp(g, roots, policy, og=None):
     dist, steps = g.sssp_multi(roots, policy)
     for r, root in enumerate(roots):
         d1, _ = g.sssp(int(root), policy)
         assert np.array_equal(dist[:, r].view(np.uint32), d1.view(np.uint32)), f"dist column {r} (root {root})"
     for r in range(len(roots), 64):
         assert (dist[:, r] == np.float32(1e9)).all()
+    if og is not None:
+        for r in sorted({0, len(roots) // 3, 2 * len(roots) // 3, len(roots) - 1}):
+            o_dist, _ = og.sssp(int(roots[r]))
+            assert np.array_equal(dist[:, r].view(np.uint32), o_dist.view(np.uint32)), f"oracle column {r}"
 
 
 @pytest.mark.parametrize("scale,P,S", [(12, 1, 1), (14, 1, 1), (13, 4, 2)])
@@ -63,12 +69,15 @@ def test_bfs_multi_equals_single(eg, scale, P, S, policy):
 
 @pytest.mark.parametrize("scale,P,S", [(12, 1, 1), (14, 1, 1), (13, 4, 2)])
 @pytest.mark.parametrize("policy", [0, 1, 3])
-def test_sssp_multi_equals_single(eg, scale, P, S, policy):
+def test_sssp_multi_equals_single(eg, oracle_lib, scale, P, S, policy):
+    """Scales 13 and 14 have more than 4096 vertices, so the sources span
+    several words of the single-root program's active-set summary."""
     print(f"sssp_multi scale={scale} P={P} S={S} policy={policy}", flush=True)
     edges, V = _rmat(eg, scale, seed=22)
     g, _ = _graph(eg, edges, V, P, S)
+    og = oracle_lib.OracleGraph(edges, V, P=P, S=S, eta=1, m=1, threads=8)
     roots = np.random.default_rng(scale * 7 + policy).integers(0, V, 64).astype(np.uint32)
-    _check_sssp(g, roots, policy)
+    _check_sssp(g, roots, policy, og)
 
 
 def test_multi_few_and_duplicate_roots(eg):

@@ -1,5 +1,7 @@
 """The C++ drop-in toolkits (endgraph_amd/toolkits) on a binary edge file,
-checked against the oracle through their printed results."""
+checked against the oracle array by array: each toolkit writes its result
+(parents, labels, distances, ranks) with dump_vertex_array when EG_DUMP names
+a file, and its printed figures are checked too."""
 import os
 import re
 import subprocess
@@ -15,12 +17,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "endgraph_amd", "toolkits", "_build")
 
 
-def run(tool, *args):
+def run(tool, *args, dump=None):
     exe = os.path.join(BIN, tool)
     if not os.path.exists(exe):
         pytest.fail(f"toolkit {tool} not built (make -C endgraph_amd/toolkits)")
     print(f"run {tool} {' '.join(map(str, args))}", flush=True)
-    p = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=120)
+    env = dict(os.environ)
+    if dump:
+        env["EG_DUMP"] = str(dump)
+    p = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=120, env=env)
     assert p.returncode == 0, p.stderr + p.stdout
     return p.stdout
 
@@ -39,46 +44,52 @@ def graph_file(tmp_path_factory):
 
 
 @pytest.mark.parametrize("P,S,policy", [(1, 1, "reference"), (3, 2, "auto")])
-def test_bfs_toolkit(oracle_lib, graph_file, P, S, policy):
+def test_bfs_toolkit(oracle_lib, graph_file, tmp_path, P, S, policy):
     V, edges, _, f8, _ = graph_file
     og = oracle_lib.OracleGraph(edges, V, P=P, S=S, eta=1, m=1)
     root = int(np.argmax(og.out_degree()))
-    out = run("bfs", f8, V, root, 1, P, S, policy)
+    out = run("bfs", f8, V, root, 1, P, S, policy, dump=tmp_path / "parent.bin")
     found = [int(x) for x in re.findall(r"found_vertices = (\d+)", out)]
     parent, depth, _ = og.bfs(root)
     assert found and all(f == int((parent < V).sum()) for f in found)
+    assert np.array_equal(np.fromfile(tmp_path / "parent.bin", np.uint32), parent)
 
 
 @pytest.mark.parametrize("P,S", [(1, 1), (4, 1)])
-def test_cc_toolkit(oracle_lib, graph_file, P, S):
+def test_cc_toolkit(oracle_lib, graph_file, tmp_path, P, S):
     V, edges, _, f8, _ = graph_file
     og = oracle_lib.OracleGraph(edges[:3000], V, P=P, S=S, eta=1, m=1, symmetrize=True)
     small = os.path.join(os.path.dirname(f8), f"small{P}.bin")
     edges[:3000].tofile(small)
-    out = run("cc", small, V, 0, 1, P, S)
+    out = run("cc", small, V, 0, 1, P, S, dump=tmp_path / "label.bin")
     comps = [int(x) for x in re.findall(r"components = (\d+)", out)]
     label, _ = og.cc()
     assert comps and all(c == len(np.unique(label)) for c in comps)
+    assert np.array_equal(np.fromfile(tmp_path / "label.bin", np.uint32), label)
 
 
-def test_sssp_toolkit(oracle_lib, graph_file):
+def test_sssp_toolkit(oracle_lib, graph_file, tmp_path):
     V, _, wedges, _, f12 = graph_file
     og = oracle_lib.OracleGraph(wedges, V, P=2, S=1, eta=1, m=1)
     root = int(np.argmax(og.out_degree()))
-    out = run("sssp", f12, V, root, 1, 2, 1)
+    out = run("sssp", f12, V, root, 1, 2, 1, dump=tmp_path / "dist.bin")
     dist, _ = og.sssp(root)
+    got = np.fromfile(tmp_path / "dist.bin", np.float32)
+    assert np.array_equal(got.view(np.uint32), dist.view(np.uint32))
     reach = dist < 1e9
     vmax = int(np.flatnonzero(reach & (dist == dist[reach].max()))[0])
     m = re.findall(r"distance\[(\d+)\]=([0-9.]+)", out)
-    assert m and all(int(v) == vmax and abs(float(x) - float(dist[vmax])) < 1e-5 for v, x in m)
+    assert m and all(int(v) == vmax and float(x) == float(f"{dist[vmax]:f}") for v, x in m)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32c"])
-def test_pagerank_toolkit(oracle_lib, graph_file, prec):
+def test_pagerank_toolkit(oracle_lib, graph_file, tmp_path, prec):
     V, edges, _, f8, _ = graph_file
     og = oracle_lib.OracleGraph(edges, V, P=2, S=1, eta=1, m=10)
     pr, deltas = og.pagerank(10)
-    out = run("pagerank", f8, V, 10, 1, 2, 1, prec)
+    out = run("pagerank", f8, V, 10, 1, 2, 1, prec, dump=tmp_path / "pr.bin")
+    got = np.fromfile(tmp_path / "pr.bin", np.float64)
+    assert np.abs(got - pr).sum() / np.abs(pr).sum() <= (1e-6 if prec == "fp32c" else 1e-12)
     sums = [float(x) for x in re.findall(r"pr_sum=([0-9.]+)", out)]
     assert sums and all(abs(s - pr.sum()) <= 1e-6 * pr.sum() + 1e-6 for s in sums)
     d = [float(x) for x in re.findall(r"delta\(\d+\)=([0-9.]+)", out)[:10]]

@@ -173,3 +173,60 @@ def test_pagerank_matches_dense(oracle_lib, P, S):
     exp = H.pagerank_dense(V, edges["src"].astype(np.int64), edges["dst"].astype(np.int64), 20)
     assert np.abs(pr - exp).sum() / np.abs(exp).sum() < 1e-13
     assert deltas[0] == 1.0 and deltas[-1] == 0.0
+
+
+@pytest.mark.parametrize("P,S", [(1, 1), (4, 2)])
+def test_pagerank_deltas_match_dense(oracle_lib, P, S):
+    """Every delta(i) of the oracle equals the dense formulation's: a dropped
+    degree factor, a missing /V or the last iteration divided would show."""
+    edges, V = _small_graph(15, V=700, E=6000)
+    og = _build(oracle_lib, edges, V, P, S)
+    pr, deltas = og.pagerank(12)
+    exp, exp_d = H.pagerank_dense(V, edges["src"].astype(np.int64), edges["dst"].astype(np.int64), 12,
+                                  with_deltas=True)
+    assert deltas.shape == exp_d.shape
+    np.testing.assert_allclose(deltas, exp_d, rtol=1e-11, atol=0)
+    assert np.abs(pr - exp).sum() / np.abs(exp).sum() < 1e-13
+
+
+def test_pagerank_dense_formula_pins():
+    """The dense formulation itself on a 3-cycle with a sink, by hand: every
+    vertex of the cycle has degree 1, the sink (vertex 3) none."""
+    src, dst = np.array([0, 1, 2, 2]), np.array([1, 2, 0, 3])
+    # degrees 1, 1, 2, 0; x0 = 1, 1, 1/2, 1
+    x, d = H.pagerank_dense(4, src, dst, 2, with_deltas=True)
+    x0 = np.array([1.0, 1.0, 0.5, 1.0])
+    s1 = np.array([x0[2], x0[0], x0[1], x0[2]])
+    y1 = 0.15 + 0.85 * s1
+    x1 = y1 / np.array([1.0, 1.0, 2.0, 1.0])
+    assert d[1] == pytest.approx((np.abs(x1 - x0) * np.array([1, 1, 2, 1])).sum() / 4, rel=1e-15)
+    s2 = np.array([x1[2], x1[0], x1[1], x1[2]])
+    assert np.allclose(x, 0.15 + 0.85 * s2, rtol=1e-15) and d[2] == 0.0 and d[0] == 1.0
+
+
+def test_csr_batch_matches_stable_argsort(oracle_lib):
+    """The batched restatement of prepare_push_mode at P = S = 1: each graph's
+    records stably sorted by src, and its row pointer = searchsorted."""
+    rng = np.random.default_rng(16)
+    n = 9
+    nv = rng.integers(1, 3000, n).astype(np.uint32)
+    nv[2] = 1
+    ne = (nv.astype(np.float64) * rng.uniform(0, 12, n)).astype(np.uint64)
+    ne[4] = 0
+    parts = []
+    for i in range(n):
+        e = np.zeros(int(ne[i]), H.REC8)
+        e["src"] = rng.integers(0, nv[i], int(ne[i]))
+        e["dst"] = rng.integers(0, nv[i], int(ne[i]))
+        parts.append(e)
+    allrec = np.concatenate(parts)
+    got, rp = oracle_lib.csr_batch(allrec, nv, ne, threads=3)
+    eo, vo = 0, 0
+    for i in range(n):
+        e = parts[i]
+        order = np.argsort(e["src"], kind="stable")
+        assert np.array_equal(got[eo:eo + e.shape[0]].view(np.uint8), e[order].view(np.uint8))
+        exp_rp = np.searchsorted(e["src"][order].astype(np.int64), np.arange(int(nv[i]) + 1))
+        assert np.array_equal(rp[vo:vo + int(nv[i]) + 1], exp_rp.astype(np.uint64))
+        eo += e.shape[0]
+        vo += int(nv[i]) + 1
