@@ -96,6 +96,7 @@ def lib():
         "eg_batch_csr_rebuild": ([vp, vp, vp], i32),
         "eg_batch_csr_get": ([vp, i32, pp, C.POINTER(u64), pp, C.POINTER(u32)], i32),
         "eg_batch_destroy": ([vp], i32),
+        "eg_batch_csr_build_noalloc": ([vp, i32, i32, vp, vp, vp, vp, vp], i32),
         "eg_device_fill": ([vp, u64, vp, i32, vp], i32),
         "eg_device_iota": ([vp, u32, vp], i32),
         "eg_bitmap_or": ([vp, vp, u32, vp], i32),
@@ -309,6 +310,17 @@ def gen_er_batch(d_edges: int, vertices, edges, seed=4, rec_bytes=8, stream=None
     e = np.ascontiguousarray(edges, dtype=np.uint64)
     check(lib().eg_gen_er_batch(C.c_void_p(d_edges), int(v.size), v.ctypes.data_as(C.c_void_p),
                                 e.ctypes.data_as(C.c_void_p), seed, rec_bytes, _stream(stream)))
+
+
+def batch_csr_build_noalloc(d_edges: int, vertices, edges, d_out_list: int, d_out_rowptr: int, rec_bytes: int = 8,
+                            stream=None):
+    """The allocation-free batched CSR build: results into the caller's
+    buffers (graph g's row pointers start at entry sum(vertices[:g]) + g)."""
+    v = np.ascontiguousarray(vertices, dtype=np.uint32)
+    e = np.ascontiguousarray(edges, dtype=np.uint64)
+    check(lib().eg_batch_csr_build_noalloc(C.c_void_p(d_edges), rec_bytes, int(v.size), v.ctypes.data_as(C.c_void_p),
+                                           e.ctypes.data_as(C.c_void_p), C.c_void_p(d_out_list),
+                                           C.c_void_p(d_out_rowptr), _stream(stream)))
 
 
 class BatchCSR:

@@ -27,6 +27,9 @@ int fail(int status, const char* fmt, ...) {
 
 int DevBuf::resize(uint64_t want) {
   if (want <= bytes && ptr) return EG_OK;
+  if (!owned)
+    return fail(EG_ERR_ARG, "internal: a scratch view of %llu bytes cannot grow to %llu", (unsigned long long)bytes,
+                (unsigned long long)want);
   release();
   if (want == 0) want = 1;
   void* p = nullptr;
@@ -41,9 +44,10 @@ int DevBuf::resize(uint64_t want) {
 }
 
 void DevBuf::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr && owned) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
+  owned = true;
 }
 
 namespace {

@@ -131,12 +131,22 @@ inline int counting_bits(uint64_t n) {  // core/graph.hpp:1087-1092
 }
 
 // Device memory owned by the library.  hipMalloc'd buffers are 256-byte
-// aligned; resize() only grows.
+// aligned; resize() only grows.  borrow() turns the buffer into a view of
+// memory someone else keeps (the allocation-free entry point's static
+// scratch): resize() then never allocates and fails when the view is too
+// small, and release() frees nothing.
 struct DevBuf {
   void* ptr = nullptr;
   uint64_t bytes = 0;
+  bool owned = true;
   int resize(uint64_t want);
   void release();
+  void borrow(void* p, uint64_t n) {
+    release();
+    ptr = p;
+    bytes = n;
+    owned = false;
+  }
   ~DevBuf() { release(); }
   template <class T>
   T* as() const { return static_cast<T*>(ptr); }
@@ -146,10 +156,13 @@ struct DevBuf {
   void swap(DevBuf& o) {
     void* p = ptr;
     uint64_t b = bytes;
+    bool w = owned;
     ptr = o.ptr;
     bytes = o.bytes;
+    owned = o.owned;
     o.ptr = p;
     o.bytes = b;
+    o.owned = w;
   }
 };
 

@@ -627,19 +627,36 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
 // positions of `out` (tmp = ping-pong; in, out, tmp distinct; in unchanged).
 // The caller zeroes w.err before and checks it after synchronising.
 template <class R, class KeyFn>
-int rx_sort(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w,
-            hipStream_t s) {
+int rx_sort_with(const SortVariant& v, RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key,
+                 int key_bits, RxWork& w, hipStream_t s) {
   const int nseg = int(seg.size()) - 1;
   if (nseg < 1 || seg.back() == seg.front()) return EG_OK;
   if (seg.front() != 0) return fail(EG_ERR_ARG, "internal: sort segments must start at 0");
   if (seg.back() >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "sort of more than 2^32-1 records");
-  const SortVariant v = sort_variant(int(sizeof(R)));
   if (v.nt == 256) return rx_sort_nt<256, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
   if constexpr (sizeof(R) <= 8) {
     if (v.nt >= 1024) return rx_sort_nt<1024, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
   }
   if (v.bits >= 9) return rx_sort_nt<512, 9>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
   return rx_sort_nt<512, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
+}
+
+template <class R, class KeyFn>
+int rx_sort(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w,
+            hipStream_t s) {
+  return rx_sort_with(sort_variant(int(sizeof(R))), in, out, tmp, seg, key, key_bits, w, s);
+}
+
+// Device workspace a three-launch sort of these segments takes with tiles of
+// `tile` records: the digit table (every tile, widest digit) and its scan
+// temporaries, in bytes; the segment tables are RxWork::resize_segs's.
+inline uint64_t rx_three_launch_table_bytes(const std::vector<uint64_t>& seg, uint64_t tile) {
+  uint64_t tiles = 0;
+  for (size_t k = 0; k + 1 < seg.size(); ++k) tiles += ceil_div(seg[k + 1] - seg[k], tile);
+  return (tiles << kRxMaxBits) * 4;
+}
+inline uint64_t rx_three_launch_scan_bytes(const std::vector<uint64_t>& seg, uint64_t tile) {
+  return scan_tmp_elems(rx_three_launch_table_bytes(seg, tile) / 4) * 4;
 }
 
 // plain input

@@ -183,6 +183,20 @@ int eg_batch_csr_rebuild(eg_batch* b, const void* d_edges, void* stream);
 int eg_batch_csr_get(const eg_batch* b, int g, const void** d_list, uint64_t* n_edges, const uint64_t** d_rowptr,
                      uint32_t* n_vertices);
 int eg_batch_destroy(eg_batch* b);
+/* Allocation-free batched CSR build.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The result
+ * equals eg_batch_csr_build's: d_out_list (E records, at the input offsets)
+ * gets every graph's CSR list and d_out_rowptr (sum of h_vertices[g] + 1
+ * u64) the row pointers, graph g's h_vertices[g] + 1 of them starting at
+ * entry (vertices of graphs < g) + g.  Every temporary lives in one 32 MiB
+ * device array of the library, one per GPU, held until the process ends; a
+ * batch whose temporaries do not fit (about E * rec_bytes + 2 KiB per 4096
+ * records and per graph) is rejected with EG_ERR_ARG and the size it needs.
+ * Calls on one GPU share that array: calls that overlap on one GPU (other
+ * streams or threads) race.  Synchronises `stream`. */
+int eg_batch_csr_build_noalloc(const void* d_edges, int rec_bytes, int n_graphs, const uint32_t* h_vertices,
+                               const uint64_t* h_edges, void* d_out_list, uint64_t* d_out_rowptr, void* stream);
 
 /* ---- engine (Gemini process_vertices / process_edges) --------------------
  * The reference's process_edges takes four std::function lambdas
