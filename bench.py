@@ -10,6 +10,15 @@ boundaries, per-rank shuffle sort, two-level CSC and CSR with their indexes
 (the reference's timed prep region).  Inputs are generated on the GPU before
 the timed region.
 
+Beside the headline line's value: BFS, PageRank and CC on the same graph;
+the batched programs (R-MAT 22, 64 roots; 256 ER graphs) and the PPR
+adjoint; R-MAT 16 PageRank; and the twitter-shaped graph (41,652,230
+vertices, 1,468,365,182 edges): its build, BFS and FP32C-vs-FP64 PageRank
+with the FP32C ranks' relative L1 against the FP64 oracle.  The host-CPU
+baseline is the oracle (oracle/cpu_ref.cpp, OpenMP) on every host core this
+process may use: the full R-MAT 24 build (the line's cpu_baseline), its BFS,
+PageRank and CC, R-MAT 16 PageRank and the twitter-shaped build.
+
 Prints one JSON line (rank 0).  Multi-GPU runs are not part of this build.
 """
 from __future__ import annotations
@@ -181,21 +190,185 @@ def batching_extras(eg):
     return out
 
 
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def timed(fn):
+    t0 = time.perf_counter()
+    r = fn()
+    return r, time.perf_counter() - t0
+
+
+def build_algorithmic_bytes(g, E, rec=8):
+    """What a build must move at least: the COO list read once, the CSC and
+    CSR lists written once, out- and in-degree, every sub-partition's dense
+    CSR index (u64[V+1], the reference's layout) and bitmap, and the
+    compressed CSC index entries (12 B each, sentinels included)."""
+    V, Q = g.V, g.P * g.S
+    cidx = sum(g.csc_ptrs(p, s)[3] for p in range(g.P) for s in range(g.S))
+    return E * rec * 3 + V * 8 + Q * (V + 1) * 8 + Q * ((V >> 6) + 1) * 8 + cidx * 12
+
+
+def pagerank_block(eg, g, iters, E, oracle_pr=None):
+    """Plain FP64 (built CSC) against compensated FP32 on the degree-ordered
+    view, 20 iterations each, median of 3 after a warm-up; the view's
+    one-time derivation reported and counted in a second ratio."""
+    view_ms = g.prepare_pagerank_view()
+    V = g.V
+    out64, out32, scratch = eg.DeviceBuffer(V * 8), eg.DeviceBuffer(V * 8), eg.DeviceBuffer(V * 8)
+    t64 = median_time(lambda: g.pagerank(iters, eg.PR_FP64, out64))
+    t32 = median_time(lambda: g.pagerank(iters, eg.PR_FP32C, out32))
+    t64o = median_time(lambda: g.pagerank(iters, eg.PR_FP64 | eg.PR_ORDERED, scratch))
+    t32p = median_time(lambda: g.pagerank(iters, eg.PR_FP32C | eg.PR_PLAIN, scratch))
+    x64 = out64.download(np.float64, V)
+    x32 = out32.download(np.float64, V)
+    gt = lambda t: round(iters * E / t / 1e9, 3)
+    out = {"iterations": iters, "view_prepare_ms": round(view_ms, 2),
+           "ms_fp64": round(t64 * 1e3, 2), "ms_fp32c": round(t32 * 1e3, 2),
+           "gteps_fp64": gt(t64), "gteps_fp32c": gt(t32),
+           "speedup_fp32c": round(t64 / t32, 3),
+           "speedup_fp32c_view_counted": round(t64 / (t32 + view_ms * 1e-3), 3),
+           "l1_fp32c_vs_fp64_gpu": float(np.abs(x32 - x64).sum() / np.abs(x64).sum()),
+           "gteps_fp64_ordered": gt(t64o), "gteps_fp32c_plain": gt(t32p),
+           "layouts": "fp64 = plain FP64 on the built CSC; fp32c = compensated FP32 on the degree-ordered view; "
+                      "view_counted adds the view's one-time derivation to one 20-iteration FP32C run"}
+    if oracle_pr is not None:
+        n1 = np.abs(oracle_pr).sum()
+        out["l1_fp32c_vs_fp64_oracle"] = float(np.abs(x32 - oracle_pr).sum() / n1)
+        out["l1_fp64_vs_fp64_oracle"] = float(np.abs(x64 - oracle_pr).sum() / n1)
+    log(f"[bench] pagerank: fp64 {gt(t64)} GTEPS, fp32c {gt(t32)} GTEPS ({t64 / t32:.2f}x, "
+        f"{out['speedup_fp32c_view_counted']:.2f}x with the {view_ms:.1f} ms view)")
+    return out, x32
+
+
+def bfs_block(eg, g, root):
+    V = g.V
+    parent, depth = eg.DeviceBuffer(V * 4), eg.DeviceBuffer(V * 4)
+    res = {}
+
+    def run():
+        res["b"] = g.bfs(root, eg.POLICY_AUTO, parent, depth, download=False)
+
+    tb = median_time(run)
+    levels, trav = res["b"]
+    log(f"[bench] bfs from {root}: {tb * 1e3:.2f} ms, {trav / tb / 1e9:.2f} GTEPS")
+    return {"root": root, "policy": "auto", "levels": levels, "edges_traversed": trav,
+            "ms": round(tb * 1e3, 3), "gteps": round(trav / tb / 1e9, 3)}
+
+
+def cpu_block(name, seconds, work, unit_div, unit, cores):
+    return {"value": round(work / seconds / unit_div, 3), "unit": unit, "s": round(seconds, 3), "cores": cores,
+            "workload": name}
+
+
+def twitter_block(eg, args, cores):
+    """The twitter-shaped graph (41,652,230 vertices, 1,468,365,182 edges,
+    seed 3, P = 1): build Medges/s, BFS GTEPS, FP32C vs FP64 PageRank with
+    the FP32C ranks' relative L1 against the FP64 oracle (computed on the host,
+    untimed) and the oracle's build timed on every host core."""
+    import oracle
+    desc, V, E, gen, P, S, eta, m = WORKLOADS["twitter"]
+    edges = eg.DeviceBuffer(E * 8)
+    eg.gen_powerlaw(edges.ptr, E, V, seed=gen[1])
+    g = eg.Graph(V, P, S, eta, m, 8)
+    if not args.no_autotune:
+        g.autotune_sort(edges.ptr, E)
+    for _ in range(1):
+        g.build(edges.ptr, E)
+    eg.synchronize()
+    t0 = time.perf_counter()
+    steps = 3
+    for _ in range(steps):
+        g.build(edges.ptr, E)
+    eg.synchronize()
+    tb = (time.perf_counter() - t0) / steps
+    out = {"workload": desc, "vertices": V, "edges": E, "build_ms": round(tb * 1e3, 2),
+           "build_medges_per_s": round(E / tb / 1e6, 1), "build_steps": steps,
+           "sort_variant": dict(zip(("scheme", "threads", "digit_bits"), eg.get_sort_variant(8)))}
+    log(f"[bench] twitter build: {tb * 1e3:.1f} ms, {E / tb / 1e6:.0f} Medges/s")
+    prep_ms = g.prepare_engine()
+    tv_ms = g.prepare_traversal_view()
+    deg = g.out_degree()
+    out["bfs"] = bfs_block(eg, g, int(np.argmax(deg)))
+    del deg
+    # oracle on the host: the CPU build (timed) and the FP64 reference ranks
+    host = edges.download(eg.REC8, E)
+    del edges
+    og = oracle.OracleGraph(host, V, P=P, S=S, eta=eta, m=m, threads=cores)
+    del host
+    ref, t_pr_cpu = timed(lambda: og.pagerank(20)[0])
+    cpu_build = og.build_seconds
+    del og
+    pr, _ = pagerank_block(eg, g, 20, E, oracle_pr=ref)
+    out["pagerank"] = pr
+    out["engine_ready_ms"] = round(tb * 1e3 + prep_ms + tv_ms + pr["view_prepare_ms"], 2)
+    out["engine_ready_parts_ms"] = {"build": round(tb * 1e3, 2), "engine_prepare": round(prep_ms, 2),
+                                    "traversal_view": round(tv_ms, 2), "pagerank_view": pr["view_prepare_ms"]}
+    out["cpu"] = {"build": cpu_block("oracle build (split_graph-equivalent region)", cpu_build, E, 1e6,
+                                     "Medges/s", cores),
+                  "pagerank_fp64": cpu_block("oracle FP64 PageRank, 20 iterations", t_pr_cpu, 20 * E, 1e9, "GTEPS",
+                                             cores)}
+    out["build_vs_cpu"] = round(out["build_medges_per_s"] / out["cpu"]["build"]["value"], 1)
+    out["targets"] = {"build_vs_host_cpu_ge_20x": out["build_vs_cpu"] >= 20,
+                      "fp32c_vs_fp64_ge_1.6x": pr["speedup_fp32c"] >= 1.6,
+                      "fp32c_vs_fp64_ge_1.6x_view_counted": pr["speedup_fp32c_view_counted"] >= 1.6,
+                      "fp32c_l1_vs_fp64_oracle_le_1e-6": pr["l1_fp32c_vs_fp64_oracle"] <= 1e-6}
+    log(f"[bench] twitter: build {out['build_vs_cpu']}x the host CPU ({cores} cores); "
+        f"FP32C L1 vs FP64 oracle {pr['l1_fp32c_vs_fp64_oracle']:.2e}")
+    del g
+    return out
+
+
+def rmat16_block(eg, cores):
+    """R-MAT scale 16 (seed 1, P = 1): 20 PageRank iterations in FP64 on the
+    GPU and on the host (the reference's plain CPU configuration)."""
+    import oracle
+    desc, V, E, gen, P, S, eta, m = WORKLOADS["rmat16"]
+    edges = eg.DeviceBuffer(E * 8)
+    eg.gen_rmat(edges.ptr, E, gen[1], seed=gen[2])
+    g = eg.Graph(V, P, S, eta, m, 8)
+    tb = median_time(lambda: g.build(edges.ptr, E))
+    og = oracle.OracleGraph(edges.download(eg.REC8, E), V, P=P, S=S, eta=eta, m=m, threads=cores)
+    ref, t_cpu = timed(lambda: og.pagerank(20)[0])
+    out64 = eg.DeviceBuffer(V * 8)
+    t64 = median_time(lambda: g.pagerank(20, eg.PR_FP64, out64))
+    x = out64.download(np.float64, V)
+    return {"workload": desc, "build_ms": round(tb * 1e3, 3), "cpu_build_s": round(og.build_seconds, 4),
+            "pagerank_fp64_ms": round(t64 * 1e3, 3), "pagerank_fp64_gteps": round(20 * E / t64 / 1e9, 3),
+            "cpu_pagerank_fp64": cpu_block("oracle FP64 PageRank, 20 iterations", t_cpu, 20 * E, 1e9, "GTEPS", cores),
+            "l1_fp64_vs_oracle": float(np.abs(x - ref).sum() / np.abs(ref).sum())}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-engine", action="store_true", help="skip the BFS / PageRank figures")
+    ap.add_argument("--no-engine", action="store_true", help="skip the BFS / PageRank / CC figures")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batching", action="store_true", help="skip the batched BFS/SSSP/CSR figures")
+    ap.add_argument("--no-twitter", action="store_true", help="skip the twitter-shaped graph's block")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: two builds of the workload, then a 1 GiB device copy (the "
                          "calibration of FETCH_SIZE / WRITE_SIZE); prints no bench line")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default sort variant")
-    ap.add_argument("--cpu-sample-edges", type=int, default=1 << 25)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every host core this process may use")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -206,6 +379,7 @@ def main():
 
     import endgraph_amd as eg
 
+    cores = args.cpu_threads or host_cores()
     desc, V, E, gen, P, S, eta, m = WORKLOADS[args.workload]
     log(f"[bench] workload {args.workload}: V={V} E={E} P={P} S={S}")
     edges = eg.DeviceBuffer(E * 8)
@@ -228,8 +402,8 @@ def main():
     variant = None
     if not args.no_autotune:
         sc, nt, bits, ms = g.autotune_sort(edges.ptr, E)  # untimed: every variant builds twice
-        variant = {"scheme": ["three-launch", "one-pass", "three-launch-xcd"][sc], "threads": nt, "tile_records": nt * 16,
-                   "digit_bits": bits, "autotune_best_ms": round(ms, 2)}
+        variant = {"scheme": ["three-launch", "one-pass", "three-launch-xcd"][sc], "threads": nt,
+                   "tile_records": nt * 16, "digit_bits": bits, "autotune_best_ms": round(ms, 2)}
         log(f"[bench] autotuned sort: {variant}")
     for i in range(args.warmup):
         g.build(edges.ptr, E)
@@ -255,12 +429,15 @@ def main():
                           "gbs": (b / n) / (ms / n * 1e-3) / 1e9}
     dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
     dk = kernels[dom]
-    traffic = None
+    traffic, traffic_src = None, None
     tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        tj = json.load(open(tf))
+        traffic = tj.get("hbm_bytes_per_launch")
+        traffic_src = (f"copied from {os.path.relpath(tf, ROOT)} (calibrated rocprofv3 --pmc FETCH_SIZE and "
+                       f"WRITE_SIZE of {tj.get('source', 'a separate run')}), not measured in this run")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(dk["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dk["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "frac": round(dk["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "bytes_per_launch": int(dk["bytes_per_launch"]), "avg_launch_ms": round(dk["avg_ms"], 4),
                 "launches": dk["launches"]}
     # what a plain streaming copy of the same bytes reaches on this device
@@ -278,52 +455,47 @@ def main():
     del ca, cb
     roofline["copy_gbs"] = round(copy_gbs, 1)
     roofline["frac_of_copy"] = round(dk["gbs"] / copy_gbs, 4)
+    bb = build_algorithmic_bytes(g, E)
+    roofline["build_level"] = {"algorithmic_bytes_per_build": int(bb),
+                               "achieved_gbs": round(bb / (ms_per_step * 1e-3) / 1e9, 1),
+                               "frac": round(bb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                               "bytes": "COO read + CSC and CSR written + degrees + dense CSR indexes and bitmaps + "
+                                        "compressed CSC index"}
     log(f"[bench] {dom}: {dk['gbs']:.0f} GB/s = {dk['gbs'] / HBM_PEAK_GBS:.3f} of peak, "
-        f"{dk['gbs'] / copy_gbs:.3f} of a streaming copy ({copy_gbs:.0f} GB/s)")
+        f"{dk['gbs'] / copy_gbs:.3f} of a streaming copy ({copy_gbs:.0f} GB/s); build as a whole "
+        f"{roofline['build_level']['frac']:.3f} of peak")
 
     extra = {"kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in d.items()}
                          for k, d in kernels.items()}}
+    cpu_extra = {"host": {"cores": cores, "model": host_cpu_model()}}
+    og = None
+    if not args.no_cpu_baseline:
+        import oracle
+        host = edges.download(eg.REC8, E)
+        og = oracle.OracleGraph(host, V, P=P, S=S, eta=eta, m=m, threads=cores)
+        cpu_extra["build"] = cpu_block("oracle build (split_graph-equivalent region), full list", og.build_seconds,
+                                       E, 1e6, "Medges/s", cores)
+        log(f"[bench] cpu build: {cpu_extra['build']['value']} Medges/s on {cores} threads")
 
     if not args.no_engine:
         prep_ms = g.prepare_engine()
+        tv_ms = g.prepare_traversal_view()
         extra["engine_prepare_ms"] = round(prep_ms, 2)
-        extra["traversal_view_prepare_ms"] = round(g.prepare_traversal_view(), 2)
-        deg = g.out_degree()
-        root = int(np.argmax(deg))
-        parent = eg.DeviceBuffer(V * 4)
-        depth = eg.DeviceBuffer(V * 4)
-        res = {}
-
-        def run_bfs():
-            res["bfs"] = g.bfs(root, eg.POLICY_AUTO, parent, depth, download=False)
-
-        tb = median_time(run_bfs)
-        levels, trav = res["bfs"]
-        extra["bfs"] = {"root": root, "policy": "auto", "levels": levels, "edges_traversed": trav,
-                        "ms": round(tb * 1e3, 3), "gteps": round(trav / tb / 1e9, 3)}
-        log(f"[bench] bfs: {tb * 1e3:.2f} ms, {trav / tb / 1e9:.2f} GTEPS")
-        iters = 20
-        view_ms = g.prepare_pagerank_view()  # one-time degree-ordered view (FP32C's default layout)
-        out64 = eg.DeviceBuffer(V * 8)
-        out32 = eg.DeviceBuffer(V * 8)
-        scratch = eg.DeviceBuffer(V * 8)
-        t64 = median_time(lambda: g.pagerank(iters, eg.PR_FP64, out64))
-        t32 = median_time(lambda: g.pagerank(iters, eg.PR_FP32C, out32))
-        t64o = median_time(lambda: g.pagerank(iters, eg.PR_FP64 | eg.PR_ORDERED, scratch))
-        t32p = median_time(lambda: g.pagerank(iters, eg.PR_FP32C | eg.PR_PLAIN, scratch))
-        x64 = out64.download(np.float64, V)
-        x32 = out32.download(np.float64, V)
-        l1 = float(np.abs(x32 - x64).sum() / np.abs(x64).sum())
-        gt = lambda t: round(iters * E / t / 1e9, 3)
-        extra["pagerank"] = {
-            "iterations": iters, "view_prepare_ms": round(view_ms, 2),
-            "gteps_fp64": gt(t64), "gteps_fp32c": gt(t32), "speedup_fp32c": round(t64 / t32, 3),
-            "l1_fp32c_vs_fp64_gpu": l1, "ms_fp64": round(t64 * 1e3, 2), "ms_fp32c": round(t32 * 1e3, 2),
-            "layouts": "fp64 = plain FP64 on the built CSC; fp32c = compensated FP32 on the degree-ordered view",
-            "gteps_fp64_ordered": gt(t64o), "gteps_fp32c_plain": gt(t32p),
-            "speedup_fp32c_vs_fp64_ordered": round(t64o / t32, 3)}
-        log(f"[bench] pagerank: fp64 {gt(t64)} GTEPS, fp32c {gt(t32)} GTEPS ({t64 / t32:.2f}x), L1 {l1:.2e}; "
-            f"fp64 ordered {gt(t64o)}, fp32c plain {gt(t32p)}; view {view_ms:.1f} ms")
+        extra["traversal_view_prepare_ms"] = round(tv_ms, 2)
+        root = int(np.argmax(g.out_degree()))
+        extra["bfs"] = bfs_block(eg, g, root)
+        ref = None
+        if og is not None:
+            (o_par, _, _), t_bfs = timed(lambda: og.bfs(root, 1))
+            cpu_extra["bfs"] = cpu_block("oracle BFS (direction-optimising) from the same root", t_bfs,
+                                         extra["bfs"]["edges_traversed"], 1e9, "GTEPS", cores)
+            del o_par
+            ref, t_pr = timed(lambda: og.pagerank(20)[0])
+            cpu_extra["pagerank_fp64"] = cpu_block("oracle FP64 PageRank, 20 iterations", t_pr, 20 * E, 1e9,
+                                                   "GTEPS", cores)
+        extra["pagerank"], _ = pagerank_block(eg, g, 20, E, oracle_pr=ref)
+        extra["engine_ready_ms"] = round(ms_per_step + prep_ms + tv_ms + extra["pagerank"]["view_prepare_ms"], 2)
+    del og
 
     if not args.no_engine and args.workload == "rmat24":
         # CC on the same list symmetrised with reverse edges (2E records), as
@@ -346,26 +518,46 @@ def main():
         log(f"[bench] cc (symmetrised): build {t_sym * 1e3:.1f} ms, cc {tc * 1e3:.2f} ms, "
             f"{res['cc'][1]} iterations, {comps} components")
         del gs, label
+        if not args.no_cpu_baseline:
+            import oracle
+            ogs = oracle.OracleGraph(edges.download(eg.REC8, E), V, P=P, S=S, eta=eta, m=m, symmetrize=True,
+                                     threads=cores)
+            (lab, it), t_cc = timed(lambda: ogs.cc())
+            cpu_extra["cc"] = cpu_block(f"oracle CC on the symmetrised list ({it} iterations)", t_cc, it * 2 * E,
+                                        1e9, "GTEPS", cores)
+            cpu_extra["cc_symmetrised_build"] = cpu_block("oracle build of the symmetrised list", ogs.build_seconds,
+                                                          2 * E, 1e6, "Medges/s", cores)
+            del ogs, lab
 
     if not args.no_engine and args.workload == "rmat24":
         extra["file_load"] = file_load_extras(eg, edges, E)
 
+    del g, edges
     if not args.no_engine and not args.no_batching:
         extra["batching"] = batching_extras(eg)
+    if not args.no_engine and not args.no_cpu_baseline and args.workload == "rmat24":
+        extra["rmat16"] = rmat16_block(eg, cores)
+        log(f"[bench] rmat16: pagerank fp64 {extra['rmat16']['pagerank_fp64_gteps']} GTEPS vs cpu "
+            f"{extra['rmat16']['cpu_pagerank_fp64']['value']}")
+    if not args.no_engine and not args.no_twitter and not args.no_cpu_baseline and args.workload == "rmat24":
+        extra["twitter"] = twitter_block(eg, args, cores)
 
     cpu = None
-    if not args.no_cpu_baseline:
-        import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        n = min(args.cpu_sample_edges, E)
-        sample = edges.download(eg.REC8, n)
-        og = oracle.OracleGraph(sample, V, P=P, S=S, eta=eta, m=m, threads=threads)
-        secs = og.build_seconds
-        del og
-        cpu = {"value": round(n / secs / 1e6, 2), "unit": "Medges/s", "cores": threads, "kind": "port",
-               "sample": f"first {n:,} of the {E:,} records, same V/P/S/eta/m; oracle/cpu_ref.cpp build "
-                         f"(timed region = split_graph equivalent), {secs:.2f} s"}
-        log(f"[bench] cpu baseline: {cpu['value']} Medges/s on {threads} threads")
+    if "build" in cpu_extra:
+        cb = cpu_extra["build"]
+        cpu = {"value": cb["value"], "unit": "Medges/s", "cores": cores, "kind": "port",
+               "sample": f"the whole workload: all {E:,} records, same V/P/S/eta/m; oracle/cpu_ref.cpp build "
+                         f"(the split_graph-equivalent region) on {cores} threads of a {cpu_extra['host']['model']}, "
+                         f"{cb['s']:.2f} s"}
+        extra["cpu"] = cpu_extra
+        extra["gpu_vs_cpu"] = {"build": round(medges / cb["value"], 1)}
+        if "bfs" in cpu_extra:
+            extra["gpu_vs_cpu"]["bfs"] = round(extra["bfs"]["gteps"] / cpu_extra["bfs"]["value"], 1)
+        if "pagerank_fp64" in cpu_extra:
+            extra["gpu_vs_cpu"]["pagerank_fp64"] = round(extra["pagerank"]["gteps_fp64"] /
+                                                         cpu_extra["pagerank_fp64"]["value"], 1)
+        if "cc" in cpu_extra:
+            extra["gpu_vs_cpu"]["cc"] = round(extra["cc"]["gteps"] / cpu_extra["cc"]["value"], 1)
 
     line = {
         "metric": "CSR+CSC build Medges/s; PageRank GTEPS FP32-vs-FP64 at L1<=1e-6; BFS GTEPS",
