@@ -292,20 +292,15 @@ int ensure_engine(eg_graph* g, hipStream_t s) {
 // times.  Renumbering vertices by descending out-degree (ties by id) packs
 // the most-read values into the fewest cache lines: on a twitter-shaped graph
 // the top 2^20 sources hold 85% of the edges, 4 MiB of FP32 values, where
-// the original ids spread them over one cache line each.  Within a row the
-// edges are ordered by new source id, so the 64 lanes of a gather step read
-// ascending, mostly neighbouring values (one cache line serves many lanes in
-// the long rows of high in-degree).  Each row adds the same terms as the
-// built CSC; only their order differs, which moves an FP64 row sum by a few
-// ulps and a compensated FP32 row sum not at all at the FP32 rounding.
+// the original ids spread them over one cache line each.  The view is the
+// traversal view relabelled: one row per destination (its new id), the
+// sources of a row in ascending original id, each replaced by its new id.
+// Each row adds the same terms as the built CSC; only their order differs,
+// which moves an FP64 row sum by a few ulps and a compensated FP32 row sum
+// not at all at the FP32 rounding.
 struct KeyDstField {
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
-};
-struct KeyDstSrc {
-  int vb;  // bits of a vertex id
-  template <class R>
-  __device__ __forceinline__ uint64_t operator()(const R& r) const { return (uint64_t(r.dst) << vb) | r.src; }
 };
 
 // record v = (v, E - deg[v]): ascending key = descending degree
@@ -324,78 +319,63 @@ __global__ void k_view_perm(const Rec8* __restrict__ sorted, uint32_t V, const u
   }
 }
 
-template <class R>
-__global__ void k_view_relabel(const R* __restrict__ csc, uint64_t E, const uint32_t* __restrict__ perm,
-                               Rec8* __restrict__ out) {
-  for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += uint64_t(gridDim.x) * blockDim.x) {
-    const R r = csc[e];
-    out[e] = Rec8{perm[r.src], perm[r.dst]};
+// out[i] = perm[in[i]], four ids per thread step with 16-byte accesses
+__global__ void k_relabel(const uint32_t* __restrict__ in, uint64_t n, const uint32_t* __restrict__ perm,
+                          uint32_t* __restrict__ out) {
+  const uint64_t n4 = n / 4;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint4 v = reinterpret_cast<const uint4*>(in)[i];
+    reinterpret_cast<uint4*>(out)[i] = uint4{perm[v.x], perm[v.y], perm[v.z], perm[v.w]};
   }
+  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < n - n4 * 4) out[n4 * 4 + t] = perm[in[n4 * 4 + t]];
 }
 
 template <class R>
 int view_impl(eg_graph* g, hipStream_t s) {
+  EG_TRY(ensure_tview(g, s));
   const uint64_t E = g->E;
   const uint32_t V = g->V;
-  const uint64_t nwords = ceil_div(E, 64);
   EG_TRY(g->pv_perm.resize(uint64_t(V) * 4));
   EG_TRY(g->pv_inv.resize(uint64_t(V) * 4));
   EG_TRY(g->pv_deg.resize(uint64_t(V) * 4));
-  EG_TRY(g->pv_src.resize(std::max<uint64_t>(E, 1) * 4));
-  EG_TRY(g->pv_rowflag.resize((nwords + 1) * 8));
-  EG_TRY(g->pv_wordrank.resize((nwords + 2) * 4));
-  const uint64_t n = std::max<uint64_t>(E, V);
-  DevBuf a, b, c;  // sort input, output, ping-pong
-  EG_TRY(a.resize(std::max<uint64_t>(n, 1) * 8));
-  EG_TRY(b.resize(std::max<uint64_t>(n, 1) * 8));
-  EG_TRY(c.resize(std::max<uint64_t>(n, 1) * 8));
-  Rec8* ra = a.as<Rec8>();
-  Rec8* rb = b.as<Rec8>();
-  RxWork w;
-  EG_TRY(w.resize_segs(1));
-  EG_HIP(hipMemsetAsync(w.err.ptr, 0, 4, s));
-  // new ids: stable sort of (v, E - deg) by the second field
+  EG_TRY(g->pv_src.resize((std::max<uint64_t>(E, 1) + 4) * 4));
+  EG_TRY(g->pv_row_dst.resize((g->tv_R + 1) * 4));
+  // new ids: a stable sort of (v, E - deg) by the second field, in the
+  // build's sort temporaries (free after the build and the traversal view)
+  const uint64_t vb = std::max<uint64_t>(V, 1) * 8;
+  EG_TRY(g->t1.resize(vb));
+  EG_TRY(g->t2.resize(vb));
+  EG_TRY(g->t3.resize(vb));
+  Rec8* ra = g->t1.as<Rec8>();
+  Rec8* rb = g->t2.as<Rec8>();
+  EG_TRY(g->sortw.resize_segs(1));
+  EG_TRY(fill<uint32_t>(g->sortw.err.as<uint32_t>(), 1, 0u, s));
   hipLaunchKernelGGL(k_deg_keys, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
                      (const uint32_t*)g->out_deg.as<uint32_t>(), V, E, ra);
   EG_HIP(hipGetLastError());
-  EG_TRY(rx_sort((const Rec8*)ra, rb, c.as<Rec8>(), std::vector<uint64_t>{0, V}, KeyDstField{},
-                 std::max(1, counting_bits(E)), w, s));
+  EG_TRY(rx_sort((const Rec8*)ra, rb, g->t3.as<Rec8>(), std::vector<uint64_t>{0, V}, KeyDstField{},
+                 std::max(1, counting_bits(E)), g->sortw, s));
+  uint32_t* perm = g->pv_perm.as<uint32_t>();
   hipLaunchKernelGGL(k_view_perm, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s, (const Rec8*)rb, V,
-                     (const uint32_t*)g->out_deg.as<uint32_t>(), g->pv_perm.as<uint32_t>(), g->pv_inv.as<uint32_t>(),
+                     (const uint32_t*)g->out_deg.as<uint32_t>(), perm, g->pv_inv.as<uint32_t>(),
                      g->pv_deg.as<uint32_t>());
   EG_HIP(hipGetLastError());
-  uint32_t* wordrank = g->pv_wordrank.as<uint32_t>();
-  uint64_t* rowflag = g->pv_rowflag.as<uint64_t>();
   if (E) {
-    // relabelled CSC sorted by (new destination, new source)
-    hipLaunchKernelGGL((k_view_relabel<R>), dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, g->csc_list.as<R>(), E,
-                       (const uint32_t*)g->pv_perm.as<uint32_t>(), ra);
-    EG_HIP(hipGetLastError());
-    const int vb = std::max(1, counting_bits(V - 1));
-    EG_TRY(rx_sort((const Rec8*)ra, rb, c.as<Rec8>(), std::vector<uint64_t>{0, E}, KeyDstSrc{vb}, 2 * vb, w, s));
-    hipLaunchKernelGGL((k_prep_csc<Rec8>), dim3(grid_for(nwords, kNT / kWave, 65536)), dim3(kNT), 0, s,
-                       (const Rec8*)rb, E, g->pv_src.as<uint32_t>(), (float*)nullptr, rowflag,
-                       (const uint16_t*)nullptr);
-    EG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_popc_words, dim3(grid_for(nwords, kNT, 65536)), dim3(kNT), 0, s, rowflag, nwords, wordrank);
+    hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(E, 4), kNT, 65536)), dim3(kNT), 0, s,
+                       (const uint32_t*)g->tv_src.as<uint32_t>(), E, (const uint32_t*)perm, g->pv_src.as<uint32_t>());
     EG_HIP(hipGetLastError());
   }
-  EG_TRY(fill<uint32_t>(wordrank + nwords, 1, 0u, s));
-  EG_TRY(exclusive_scan_u32(wordrank, wordrank, nwords + 1, g->scan_tmp.as<uint32_t>(), s));
-  uint32_t R_h = 0;
-  EG_HIP(hipMemcpyAsync(&R_h, wordrank + nwords, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  g->pv_R = R_h;
-  EG_TRY(g->pv_row_dst.resize((uint64_t(R_h) + 1) * 4));
-  if (E) {
-    hipLaunchKernelGGL((k_prep_rows<Rec8>), dim3(grid_for(nwords, kNT / kWave, 65536)), dim3(kNT), 0, s,
-                       (const Rec8*)rb, E, (const uint64_t*)rowflag, (const uint32_t*)wordrank, (uint32_t*)nullptr,
+  if (g->tv_R) {
+    hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(g->tv_R, 4), kNT, 65536)), dim3(kNT), 0, s,
+                       (const uint32_t*)g->tv_row_dst.as<uint32_t>(), g->tv_R, (const uint32_t*)perm,
                        g->pv_row_dst.as<uint32_t>());
     EG_HIP(hipGetLastError());
   }
+  g->pv_R = uint32_t(g->tv_R);
   uint32_t sort_err = 0;
-  EG_HIP(hipMemcpyAsync(&sort_err, w.err.ptr, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+  EG_HIP(hipMemcpyAsync(&sort_err, g->sortw.err.ptr, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
   if (sort_err) return fail(EG_ERR_HIP, "radix sort look-back exceeded its spin limit");
   return EG_OK;
 }
@@ -1713,9 +1693,9 @@ PullView plain_view(const eg_graph* g) {
                   g->e_rowflag.as<uint64_t>(), g->Q > 1};
 }
 
-PullView ordered_view(const eg_graph* g) {
-  return PullView{g->E, g->pv_R, g->pv_src.as<uint32_t>(), g->pv_wordrank.as<uint32_t>(),
-                  g->pv_row_dst.as<uint32_t>(), g->pv_rowflag.as<uint64_t>(), 0};
+PullView ordered_view(const eg_graph* g) {  // rows of the traversal view, relabelled
+  return PullView{g->E, g->pv_R, g->pv_src.as<uint32_t>(), g->tv_wordrank.as<uint32_t>(),
+                  g->pv_row_dst.as<uint32_t>(), g->tv_rowflag.as<uint64_t>(), 0};
 }
 
 int pr_pull(const PullView& pv, uint32_t V, int precision, const void* x, double* next, hipStream_t s) {

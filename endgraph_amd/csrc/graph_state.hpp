@@ -100,23 +100,26 @@ struct eg_graph {
   eg::DevBuf e_bm[3];                // three bitmaps of (V>>6)+1 words
   eg::DevBuf e_pr;                   // PageRank work vectors
   // degree-ordered PageRank view: vertices renumbered by descending
-  // out-degree (ties by id), one row per destination in new-id order
+  // out-degree (ties by id); the traversal view's rows (and row bits) with
+  // every source and destination replaced by its new id
   bool pv_ready = false;
   uint32_t pv_R = 0;
   double pv_prepare_ms = 0;
   eg::DevBuf pv_perm;                // u32[V]  old id -> new id
   eg::DevBuf pv_inv;                 // u32[V]  new id -> old id
   eg::DevBuf pv_deg;                 // u32[V]  out-degree by new id
-  eg::DevBuf pv_src;                 // u32[E]  new source id of every edge, rows in new-id order
-  eg::DevBuf pv_rowflag;             // u64[ceil(E/64)] row-start bits
-  eg::DevBuf pv_wordrank;            // u32[ceil(E/64)+1]
-  eg::DevBuf pv_row_dst;             // u32[R]  new destination id of each row
+  eg::DevBuf pv_src;                 // u32[E + 4] new source id of every traversal-view edge
+  eg::DevBuf pv_row_dst;             // u32[R]  new destination id of each traversal-view row
   // traversal view for the batched (64-root) programs: the CSC merged over
   // sub-partitions with each destination's in-edges sorted by source
   bool tv_ready = false;
+  uint64_t tv_R = 0;                 // rows: destinations with in-edges
   eg::DevBuf tv_ptr;                 // u32[V+1] row pointer (prefix of in-degree)
   eg::DevBuf tv_src;                 // u32[E + 4] (4 words of padding)
   eg::DevBuf tv_w;                   // f32[E] (12-byte records only)
+  eg::DevBuf tv_rowflag;             // u64[ceil(E/64)]: bit e set iff edge e starts a row
+  eg::DevBuf tv_wordrank;            // u32[ceil(E/64)+1]: rows started before word w
+  eg::DevBuf tv_row_dst;             // u32[R] destination of each row
   eg::DevBuf tv_items;               // u64 (v << 32 | chunk): in-edge chunks of at most 512 edges
   uint64_t tv_nitems = 0;
   eg::DevBuf tv_big;                 // first chunks of the vertices with more than kTvSmallRow in-edges
