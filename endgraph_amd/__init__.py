@@ -80,6 +80,7 @@ def lib():
         "eg_graph_create": ([u32, i32, i32, u64, u32, i32, pp], i32),
         "eg_graph_destroy": ([vp], i32),
         "eg_graph_build": ([vp, vp, u64, i32, vp], i32),
+        "eg_graph_build_file": ([vp, C.c_char_p, vp, u64, i32, C.POINTER(u64), vp], i32),
         "eg_graph_get_info": ([vp, C.POINTER(_Info)], i32),
         "eg_graph_partition_offset": ([vp, vp], i32),
         "eg_graph_local_partition_offset": ([vp, i32, vp], i32),
@@ -384,6 +385,15 @@ class Graph:
     # load_directed / load_undirected_from_directed (core/graph.hpp:1358, 629)
     def build(self, d_edges: int, E0: int, symmetrize: bool = False, stream=None):
         check(lib().eg_graph_build(self._h, C.c_void_p(d_edges), E0, int(bool(symmetrize)), _stream(stream)))
+
+    def build_file(self, path: str, d_edges: int, capacity_records: int, symmetrize: bool = False, stream=None) -> int:
+        """load_directed / load_undirected_from_directed from a binary edge file:
+        the records land in d_edges rank slice by rank slice while the earlier
+        slices' first sorts run; returns the record count."""
+        n = C.c_uint64()
+        check(lib().eg_graph_build_file(self._h, str(path).encode(), C.c_void_p(d_edges), capacity_records,
+                                        int(bool(symmetrize)), C.byref(n), _stream(stream)))
+        return n.value
 
     def autotune_sort(self, d_edges: int, E0: int, symmetrize: bool = False, stream=None):
         """Time every sort variant on this input; keep the fastest (graph left built)."""

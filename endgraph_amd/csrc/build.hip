@@ -16,6 +16,9 @@
 //                everything by the composite key (q, key).
 //   indexes      compressed CSC index (:1147-1188), dense CSR index and
 //                bitmap (:1312-1353).
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -45,10 +48,11 @@ int fill(T* p, uint64_t n, T v, hipStream_t s) {
 // Per-rank symmetrisation (core/graph.hpp:1057-1078): rank r's list is its
 // slice followed by the reversed copy of that slice.
 template <class R>
-__global__ void k_symmetrize(const R* __restrict__ in, R* __restrict__ out, uint64_t E0, int P) {
+__global__ void k_symmetrize(const R* __restrict__ in, R* __restrict__ out, uint64_t E0, int P, uint64_t o_begin,
+                             uint64_t o_end) {
   const uint64_t base = E0 / P;
-  const uint64_t n_out = 2 * E0;
-  for (uint64_t o = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < n_out; o += uint64_t(gridDim.x) * blockDim.x) {
+  for (uint64_t o = o_begin + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < o_end;
+       o += uint64_t(gridDim.x) * blockDim.x) {
     uint64_t r = base ? o / (2 * base) : uint64_t(P - 1);
     if (r > uint64_t(P - 1)) r = P - 1;
     const uint64_t n_r = base + (r == uint64_t(P - 1) ? E0 % P : 0);
@@ -662,15 +666,20 @@ unsigned grid_for(uint64_t n, uint64_t per_block = kNT, uint64_t cap = 65536) {
   return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, per_block), cap)));
 }
 
-// Stable sort of every rank slice (one segment each) by key.
+// First record of rank slice r in the logical list (mult = 2 when each slice
+// is followed by its reversed copy); r = P gives the list's end.
+inline uint64_t slice_start(uint64_t E0, int P, uint64_t mult, int r) {
+  return r >= P ? mult * E0 : mult * (E0 / P) * uint64_t(r);
+}
+
+// Stable sort of the rank slices r0..r1-1 (one segment each) by key.
 template <class R, class KeyFn>
-int rank_sorts(eg_graph* g, const R* L, R* out, R* tmp, uint64_t E0, int P, uint64_t mult, KeyFn key, int bits,
-               hipStream_t s) {
-  const uint64_t base = E0 / P;
-  std::vector<uint64_t> seg(P + 1);
-  for (int r = 0; r < P; ++r) seg[r] = mult * base * r;
-  seg[P] = mult * E0;
-  return rx_sort(L, out, tmp, seg, key, bits, g->sortw, s);
+int rank_sorts(eg_graph* g, const R* L, R* out, R* tmp, uint64_t E0, int P, uint64_t mult, int r0, int r1, KeyFn key,
+               int bits, hipStream_t s) {
+  const uint64_t b0 = slice_start(E0, P, mult, r0);
+  std::vector<uint64_t> seg(r1 - r0 + 1);
+  for (int r = r0; r <= r1; ++r) seg[r - r0] = slice_start(E0, P, mult, r) - b0;
+  return rx_sort(L + b0, out + b0, tmp + b0, seg, key, bits, g->sortw, s);
 }
 
 template <class R, class KeyFn>
@@ -698,10 +707,13 @@ int run_degree(const R* a, uint64_t n, uint32_t* deg, hipStream_t s) {
 //   CSC = A sorted by (q(src), dst)  (P = 1: B sorted by q(src))
 //   CSR = B sorted by (q(dst), src)  (P = 1: A sorted by q(dst))
 // At P = 1 and S = 1 the q pass is the identity and A, B are the results.
+// The first sorts are per rank slice, so a caller that receives the list
+// slice by slice (the file reader) can run them for each slice as it lands:
+// build_setup, build_slices for every slice, build_finish.
 template <class R>
-int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) {
+int build_setup(eg_graph* g, uint64_t E0, int sym, hipStream_t s) {
   const uint32_t V = g->V;
-  const int P = g->P, S = g->S, Q = g->Q;
+  const int P = g->P, Q = g->Q;
   const uint64_t E = sym ? 2 * E0 : E0;
   const uint64_t rb = sizeof(R);
   g->built = false;
@@ -741,7 +753,46 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   EG_TRY(g->tile_cnt.resize((ceil_div(E, kChunk) + 2) * 4));
   EG_TRY(g->err.resize(4));
   if (sym) EG_TRY(g->sym.resize(Emax * rb));
+  // the first sorts check every id once (only mid-build read-back: build_finish)
+  EG_TRY(fill<uint32_t>(g->err.as<uint32_t>(), 1, 0u, s));
+  EG_TRY(g->sortw.resize_segs(1));
+  EG_TRY(fill<uint32_t>(g->sortw.err.as<uint32_t>(), 1, 0u, s));
+  return EG_OK;
+}
 
+// Rank slices r0..r1-1 of the input: their reversed copies (symmetrised
+// builds) and their two first sorts.
+template <class R>
+int build_slices(eg_graph* g, const R* d_in, uint64_t E0, int sym, int r0, int r1, hipStream_t s) {
+  const int P = g->P;
+  const uint64_t mult = sym ? 2 : 1;
+  const R* L = d_in;
+  if (sym) {
+    const uint64_t o0 = slice_start(E0, P, 2, r0), o1 = slice_start(E0, P, 2, r1);
+    if (o1 > o0)
+      hipLaunchKernelGGL((k_symmetrize<R>), dim3(grid_for(o1 - o0, kNT * 4, 65536)), dim3(kNT), 0, s, d_in,
+                         g->sym.as<R>(), E0, P, o0, o1);
+    EG_HIP(hipGetLastError());
+    L = g->sym.as<R>();
+  }
+  R* A = g->t1.as<R>();
+  R* B = g->t2.as<R>();
+  R* T = P > 1 ? g->t3.as<R>() : g->csc_list.as<R>();  // ping-pong of the first sorts
+  EG_TRY(rank_sorts(g, L, A, T, E0, P, mult, r0, r1, KeyChecked<true>{g->V, g->err.as<uint32_t>()}, g->vbits, s));
+  EG_TRY(rank_sorts(g, L, B, T, E0, P, mult, r0, r1, KeyChecked<false>{g->V, nullptr}, g->vbits, s));  // checked above
+  return EG_OK;
+}
+
+template <class R>
+int build_finish(eg_graph* g, hipStream_t s) {
+  const uint32_t V = g->V;
+  const int P = g->P, S = g->S, Q = g->Q;
+  const uint64_t E0 = g->E0, E = g->E;
+  const int sym = g->symmetrized;
+  const uint64_t rb = sizeof(R);
+  const uint64_t pages = g->pages;
+  const uint64_t idx_n = uint64_t(Q) * (V + 1);
+  const uint64_t words = (V >> 6) + 1;
   eg_graph::Meta* meta = g->dmeta();
   uint32_t* err = g->err.as<uint32_t>();
   uint32_t* out_deg = g->out_deg.as<uint32_t>();
@@ -749,25 +800,8 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   uint16_t* qmap = g->qmap.as<uint16_t>();
   uint32_t* stmp32 = g->scan_tmp.as<uint32_t>();
   const uint64_t mult = sym ? 2 : 1;
-
-  const R* L = d_in;
-  if (sym) {
-    hipLaunchKernelGGL((k_symmetrize<R>), dim3(grid_for(E, kNT * 4, 65536)), dim3(kNT), 0, s, d_in, g->sym.as<R>(),
-                       E0, P);
-    EG_HIP(hipGetLastError());
-    L = g->sym.as<R>();
-  }
-
-  // --- A, B: first sorts (they also validate every id: the only mid-build
-  // synchronisation) ---
-  EG_TRY(fill<uint32_t>(err, 1, 0u, s));
-  EG_TRY(g->sortw.resize_segs(1));
-  EG_TRY(fill<uint32_t>(g->sortw.err.as<uint32_t>(), 1, 0u, s));
   R* A = g->t1.as<R>();
   R* B = g->t2.as<R>();
-  R* T = P > 1 ? g->t3.as<R>() : g->csc_list.as<R>();  // ping-pong of the first sorts
-  EG_TRY(rank_sorts(g, L, A, T, E0, P, mult, KeyChecked<true>{V, err}, g->vbits, s));
-  EG_TRY(rank_sorts(g, L, B, T, E0, P, mult, KeyChecked<false>{V, nullptr}, g->vbits, s));  // checked above
   uint32_t h_err = 0;
   EG_HIP(hipMemcpyAsync(&h_err, err, 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
@@ -925,6 +959,13 @@ int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) 
   return EG_OK;
 }
 
+template <class R>
+int build_impl(eg_graph* g, const R* d_in, uint64_t E0, int sym, hipStream_t s) {
+  EG_TRY(build_setup<R>(g, E0, sym, s));
+  EG_TRY(build_slices<R>(g, d_in, E0, sym, 0, g->P, s));
+  return build_finish<R>(g, s);
+}
+
 }  // namespace
 }  // namespace eg
 
@@ -965,6 +1006,48 @@ int eg_graph_build(eg_graph* g, const void* d_edges, uint64_t E0, int symmetrize
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (g->rec == 8) return build_impl<Rec8>(g, static_cast<const Rec8*>(d_edges), E0, symmetrize ? 1 : 0, s);
   return build_impl<Rec12>(g, static_cast<const Rec12*>(d_edges), E0, symmetrize ? 1 : 0, s);
+}
+
+int eg_graph_build_file(eg_graph* g, const char* path, void* d_edges, uint64_t capacity_records, int symmetrize,
+                        uint64_t* h_records, void* stream) {
+  if (!g) return fail(EG_ERR_ARG, "eg_graph_build_file: graph is NULL");
+  uint64_t E0 = 0;
+  EG_TRY(eg_edge_file_records(path, g->rec, &E0));
+  if (E0 > capacity_records)
+    return fail(EG_ERR_ARG, "%s holds %llu records, the buffer %llu", path, (unsigned long long)E0,
+                (unsigned long long)capacity_records);
+  if (!d_edges && E0) return fail(EG_ERR_ARG, "eg_graph_build_file: d_edges is NULL");
+  const int sym = symmetrize ? 1 : 0;
+  if ((sym ? 2 * E0 : E0) >= (uint64_t(1) << 32))
+    return fail(EG_ERR_ARG, "edge count %llu exceeds 2^32-1", (unsigned long long)(sym ? 2 * E0 : E0));
+  const int fd = ::open(path, O_RDONLY);
+  if (fd < 0) return fail(EG_ERR_ARG, "cannot open %s", path);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::vector<uint64_t> bounds(g->P + 1);
+  for (int r = 0; r <= g->P; ++r) bounds[r] = slice_start(E0, g->P, 1, r) * uint64_t(g->rec);
+  int rc;
+  if (g->rec == 8) {
+    rc = build_setup<Rec8>(g, E0, sym, s);
+    if (rc == EG_OK)
+      rc = upload_file_ranges(fd, bounds, static_cast<char*>(d_edges), s, [&](int r) {
+        return build_slices<Rec8>(g, static_cast<const Rec8*>(d_edges), E0, sym, r, r + 1, s);
+      });
+    if (rc == EG_OK) rc = build_finish<Rec8>(g, s);
+  } else {
+    rc = build_setup<Rec12>(g, E0, sym, s);
+    if (rc == EG_OK)
+      rc = upload_file_ranges(fd, bounds, static_cast<char*>(d_edges), s, [&](int r) {
+        return build_slices<Rec12>(g, static_cast<const Rec12*>(d_edges), E0, sym, r, r + 1, s);
+      });
+    if (rc == EG_OK) rc = build_finish<Rec12>(g, s);
+  }
+  ::close(fd);
+  if (rc != EG_OK) {
+    (void)hipStreamSynchronize(s);  // nothing queued may outlive a failed call's buffers
+    return rc;
+  }
+  if (h_records) *h_records = E0;
+  return EG_OK;
 }
 
 int eg_autotune_sort(eg_graph* g, const void* d_edges, uint64_t E0, int symmetrize, int* h_scheme, int* h_threads,

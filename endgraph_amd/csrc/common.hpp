@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
+#include <vector>
 
 #include "../../include/endgraph.h"
 
@@ -192,5 +194,12 @@ uint64_t scan_tmp_elems(uint64_t n);
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
 int exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* tmp, hipStream_t s);
 int inclusive_max_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* tmp, hipStream_t s);
+
+// ---------------------------------------------------------- file reader --
+// Host file bytes -> device, range by range (io.hip): after the copies of
+// range k are queued, stream s waits for them and on_range(k) queues the
+// work that reads them, while the next range is being read.
+int upload_file_ranges(int fd, const std::vector<uint64_t>& bounds, char* d, hipStream_t s,
+                       const std::function<int(int)>& on_range);
 
 }  // namespace eg

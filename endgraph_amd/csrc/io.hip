@@ -15,6 +15,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -87,6 +88,7 @@ struct UploadPool {
   bool ready = false;
   void* buf[kIoThreads][2] = {};
   hipEvent_t ev[kIoThreads][2] = {};
+  hipEvent_t done[kIoThreads] = {};  // a range's last copy on each stream
   hipStream_t st[kIoThreads] = {};
   // creates what is still missing (a failed attempt keeps what it made)
   int init() {
@@ -106,6 +108,10 @@ struct UploadPool {
           return fail(EG_ERR_HIP, "event creation for the file reader failed");
         }
       }
+      if (!done[t] && hipEventCreateWithFlags(&done[t], hipEventDisableTiming) != hipSuccess) {
+        done[t] = nullptr;
+        return fail(EG_ERR_HIP, "event creation for the file reader failed");
+      }
     }
     ready = true;
     return EG_OK;
@@ -117,9 +123,15 @@ UploadPool& upload_pool() {
   return *p;
 }
 
-// host file [0, n) -> device with kIoThreads readers
-int upload_file(int fd, uint64_t n, char* d, hipStream_t s) {
-  if (!n) return EG_OK;
+}  // namespace
+
+// Host file -> device, range by range: the bytes [bounds[k], bounds[k+1])
+// are read by kIoThreads readers (chunk i of the range to reader i mod
+// kIoThreads), then stream s is made to wait for their copies and
+// on_range(k) runs (it queues work on s that reads range k) while range k+1
+// is being read.  Returns with every copy queued; s is ordered after them.
+int upload_file_ranges(int fd, const std::vector<uint64_t>& bounds, char* d, hipStream_t s,
+                       const std::function<int(int)>& on_range) {
   UploadPool& pool = upload_pool();
   std::lock_guard<std::mutex> lock(pool.busy);
   EG_TRY(pool.init());
@@ -129,37 +141,55 @@ int upload_file(int fd, uint64_t n, char* d, hipStream_t s) {
   EG_HIP(hipEventRecord(after_caller, s));
   for (int t = 0; t < kIoThreads; ++t) EG_HIP(hipStreamWaitEvent(pool.st[t], after_caller, 0));
   (void)hipEventDestroy(after_caller);
-  const uint64_t chunks = ceil_div(n, kUpChunk);
-  int status[kIoThreads];
-  std::string msg[kIoThreads];
-  auto reader = [&](int t) {
-    status[t] = EG_OK;
-    for (uint64_t k = uint64_t(t), j = 0; k < chunks; k += kIoThreads, ++j) {
-      const int b = int(j & 1);
-      const uint64_t off = k * kUpChunk, len = std::min(kUpChunk, n - off);
-      hipError_t e = hipEventSynchronize(pool.ev[t][b]);  // the copy that last used buffer b is done
-      if (e == hipSuccess && full_pread(fd, static_cast<char*>(pool.buf[t][b]), len, off) != EG_OK) {
-        status[t] = EG_ERR_ARG;
-        msg[t] = eg_last_error();
-        return;
+  uint64_t used[kIoThreads] = {};  // copies issued per reader: its buffers alternate across ranges
+  for (size_t k = 0; k + 1 < bounds.size(); ++k) {
+    const uint64_t r0 = bounds[k], n = bounds[k + 1] - r0;
+    const uint64_t chunks = ceil_div(n, kUpChunk);
+    int status[kIoThreads];
+    std::string msg[kIoThreads];
+    auto reader = [&](int t) {
+      status[t] = EG_OK;
+      for (uint64_t c = uint64_t(t); c < chunks; c += kIoThreads) {
+        const int b = int(used[t]++ & 1);
+        const uint64_t off = r0 + c * kUpChunk, len = std::min(kUpChunk, bounds[k + 1] - off);
+        hipError_t e = hipEventSynchronize(pool.ev[t][b]);  // the copy that last used buffer b is done
+        if (e == hipSuccess && full_pread(fd, static_cast<char*>(pool.buf[t][b]), len, off) != EG_OK) {
+          status[t] = EG_ERR_ARG;
+          msg[t] = eg_last_error();
+          return;
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(d + off, pool.buf[t][b], len, hipMemcpyHostToDevice, pool.st[t]);
+        if (e == hipSuccess) e = hipEventRecord(pool.ev[t][b], pool.st[t]);
+        if (e != hipSuccess) {
+          status[t] = EG_ERR_HIP;
+          msg[t] = hipGetErrorString(e);
+          return;
+        }
       }
-      if (e == hipSuccess) e = hipMemcpyAsync(d + off, pool.buf[t][b], len, hipMemcpyHostToDevice, pool.st[t]);
-      if (e == hipSuccess) e = hipEventRecord(pool.ev[t][b], pool.st[t]);
-      if (e != hipSuccess) {
-        status[t] = EG_ERR_HIP;
-        msg[t] = hipGetErrorString(e);
-        return;
-      }
+    };
+    const int nt = int(std::min<uint64_t>(kIoThreads, chunks));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(reader, t);
+    if (nt > 0) reader(0);
+    for (auto& x : th) x.join();
+    for (int t = 0; t < nt; ++t)
+      if (status[t] != EG_OK) return fail(status[t], "file reader: %s", msg[t].c_str());
+    for (int t = 0; t < nt; ++t) {
+      EG_HIP(hipEventRecord(pool.done[t], pool.st[t]));
+      EG_HIP(hipStreamWaitEvent(s, pool.done[t], 0));
     }
-  };
-  const int nt = int(std::min<uint64_t>(kIoThreads, chunks));
-  std::vector<std::thread> th;
-  for (int t = 1; t < nt; ++t) th.emplace_back(reader, t);
-  reader(0);
-  for (auto& x : th) x.join();
-  for (int t = 0; t < nt; ++t) EG_HIP(hipStreamSynchronize(pool.st[t]));
-  for (int t = 0; t < nt; ++t)
-    if (status[t] != EG_OK) return fail(status[t], "file reader: %s", msg[t].c_str());
+    EG_TRY(on_range(int(k)));
+  }
+  return EG_OK;
+}
+
+namespace {
+
+// host file [0, n) -> device with kIoThreads readers
+int upload_file(int fd, uint64_t n, char* d, hipStream_t s) {
+  if (!n) return EG_OK;
+  EG_TRY(upload_file_ranges(fd, std::vector<uint64_t>{0, n}, d, s, [](int) { return EG_OK; }));
+  EG_HIP(hipStreamSynchronize(s));
   return EG_OK;
 }
 

@@ -188,14 +188,7 @@ class Graph {
     check(eg_graph_build(g_, d_edges, n, symmetric ? 1 : 0, nullptr));
     t += get_time();
     prep_time = t;
-    eg_graph_info info;
-    check(eg_graph_get_info(g_, &info));
-    edges_num = info.edges_num;
-    const UINT* in_deg = nullptr;
-    check(eg_graph_degree(g_, &degree_dev, &in_deg));
-    degree.resize(vertices);
-    check(eg_memcpy_d2h(degree.data(), degree_dev, uint64_t(vertices) * 4, nullptr));
-    prep_data = 0;
+    after_build(vertices);
   }
 
   template <typename T>
@@ -300,6 +293,9 @@ class Graph {
   eg_graph* g_ = nullptr;
   double* delta_ = nullptr;
 
+  // The file is read rank slice by rank slice and each slice's first sorts
+  // run while the next is read (eg_graph_build_file); prep_time covers the
+  // whole call, the read included, since the two overlap.
   void load(const std::string& path, UINT vertices, bool symmetric) {
     const int rec = int(sizeof(EdgeType<EdgeData>));
     uint64_t n = 0;
@@ -307,13 +303,31 @@ class Graph {
     void* d = nullptr;
     check(eg_device_alloc(&d, n ? n * uint64_t(rec) : 1));
     try {
-      check(eg_load_edge_file(path.c_str(), rec, d, n, &n, nullptr));
-      build(d, n, vertices, symmetric);
+      vertices_num = vertices;
+      if (g_) eg_graph_destroy(g_);
+      g_ = nullptr;
+      check(eg_graph_create(vertices, partitions_num, sockets_num, eta, UINT(max_iterations), rec, &g_));
+      double t = -get_time();
+      check(eg_graph_build_file(g_, path.c_str(), d, n, symmetric ? 1 : 0, &n, nullptr));
+      t += get_time();
+      prep_time = t;
+      after_build(vertices);
     } catch (...) {
       eg_device_free(d);
       throw;
     }
     eg_device_free(d);
+  }
+
+  void after_build(UINT vertices) {
+    eg_graph_info info;
+    check(eg_graph_get_info(g_, &info));
+    edges_num = info.edges_num;
+    const UINT* in_deg = nullptr;
+    check(eg_graph_degree(g_, &degree_dev, &in_deg));
+    degree.resize(vertices);
+    check(eg_memcpy_d2h(degree.data(), degree_dev, uint64_t(vertices) * 4, nullptr));
+    prep_data = 0;
   }
 
   void ensure_engine() { check(eg_engine_prepare(g_, nullptr, nullptr)); }

@@ -121,6 +121,15 @@ int eg_graph_destroy(eg_graph* g);
  * directions (CSC: prepare_pull_mode :1098; CSR: prepare_push_mode :1248),
  * the degree arrays and the partition offsets.  Synchronises `stream`. */
 int eg_graph_build(eg_graph* g, const void* d_edges, uint64_t E0, int symmetrize, void* stream);
+/* load_directed / load_undirected_from_directed from a binary edge file
+ * (preload_graph_file, core/graph.hpp:1011-1084, then the build): the file's
+ * records (eg_edge_file_records of them, at most capacity_records) are read
+ * into d_edges rank slice by rank slice (:1024-1030), and each slice's
+ * reversed copy and first sorts run on the GPU while the next slice is read;
+ * the result equals eg_graph_build on the same records.  With P = 1 the one
+ * slice is read whole first.  Synchronises `stream`. */
+int eg_graph_build_file(eg_graph* g, const char* path, void* d_edges, uint64_t capacity_records, int symmetrize,
+                        uint64_t* h_records, void* stream);
 
 typedef struct eg_graph_info {
   uint32_t vertices_num;
