@@ -144,6 +144,9 @@ def batching_extras(eg):
     g = eg.Graph(V, 1, 1, 1, 1, 12)
     g.build(buf.ptr, E)
     g.prepare_engine()
+    tune, tms = g.autotune_engine(n_roots=64, seed=4)  # untimed; its own sample roots
+    out["engine_tune"] = {"picked": tune, "best_ms": [round(x, 3) for x in tms]}
+    log(f"[bench] engine tune (R-MAT 22 weighted): {tune}")
     roots = np.random.default_rng(4).integers(0, V, 64).astype(np.uint32)
     parent = eg.DeviceBuffer(V * 64 * 4)
     depth = eg.DeviceBuffer(V * 64 * 4)
@@ -322,6 +325,9 @@ def twitter_block(eg, args, cores):
     log(f"[bench] twitter build: {tb * 1e3:.1f} ms, {E / tb / 1e6:.0f} Medges/s")
     prep_ms = g.prepare_engine()
     tv_ms = g.prepare_traversal_view()
+    if not args.no_autotune:
+        tune, tms = g.autotune_engine(n_roots=16, seed=3)  # untimed
+        out["engine_tune"] = {"picked": tune, "best_ms": [round(x, 3) for x in tms]}
     deg = g.out_degree()
     out["bfs"] = bfs_block(eg, g, int(np.argmax(deg)))
     del deg
@@ -523,6 +529,10 @@ def main():
         tv_ms = g.prepare_traversal_view()
         extra["engine_prepare_ms"] = round(prep_ms, 2)
         extra["traversal_view_prepare_ms"] = round(tv_ms, 2)
+        if not args.no_autotune:
+            tune, tms = g.autotune_engine(n_roots=16, seed=2)  # untimed
+            extra["engine_tune"] = {"picked": tune, "best_ms": [round(x, 3) for x in tms]}
+            log(f"[bench] engine tune: {tune}")
         root = int(np.argmax(g.out_degree()))
         extra["bfs"] = bfs_block(eg, g, root)
         ref = None

@@ -41,6 +41,16 @@ def build_library(jobs: int = 8) -> str:
     return LIB_PATH
 
 
+class EngineTune(C.Structure):
+    """eg_engine_tune: per-graph sizes and auto-policy thresholds of the traversals."""
+    _fields_ = [("bfs_dense_div", C.c_uint32), ("sssp_dense_div", C.c_uint32), ("bfs_multi_dense_div", C.c_uint32),
+                ("sssp_multi_dense_div", C.c_uint32), ("hub_degree", C.c_uint32), ("hub_chunk", C.c_uint32),
+                ("multi_item_edges", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class _Info(C.Structure):
     _fields_ = [("vertices_num", C.c_uint32), ("partitions", C.c_int), ("sockets", C.c_int),
                 ("record_bytes", C.c_int), ("edges_num", C.c_uint64), ("pages_num", C.c_uint32),
@@ -132,6 +142,9 @@ def lib():
         "eg_ppr_adjoint": ([vp, vp, vp, vp, f64, i32, vp, vp, vp], i32),
         "eg_bfs_multi": ([vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(u32), vp], i32),
         "eg_sssp_multi": ([vp, vp, i32, i32, vp, C.POINTER(u32), vp], i32),
+        "eg_get_engine_tune": ([vp, C.POINTER(EngineTune)], i32),
+        "eg_set_engine_tune": ([vp, C.POINTER(EngineTune)], i32),
+        "eg_autotune_engine": ([vp, i32, u64, C.POINTER(EngineTune), C.POINTER(f64), vp], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -394,6 +407,25 @@ class Graph:
         check(lib().eg_graph_build_file(self._h, str(path).encode(), C.c_void_p(d_edges), capacity_records,
                                         int(bool(symmetrize)), C.byref(n), _stream(stream)))
         return n.value
+
+    def engine_tune(self) -> dict:
+        t = EngineTune()
+        check(lib().eg_get_engine_tune(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_engine_tune(self, **kw):
+        t = EngineTune()
+        check(lib().eg_get_engine_tune(self._h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(lib().eg_set_engine_tune(self._h, C.byref(t)))
+
+    def autotune_engine(self, n_roots: int = 16, seed: int = 1, stream=None):
+        """Time the traversals' thresholds and sizes on this graph; keep the fastest. Returns (tune, best ms)."""
+        t = EngineTune()
+        ms = (C.c_double * 5)()
+        check(lib().eg_autotune_engine(self._h, n_roots, seed, C.byref(t), ms, _stream(stream)))
+        return t.as_dict(), list(ms)
 
     def autotune_sort(self, d_edges: int, E0: int, symmetrize: bool = False, stream=None):
         """Time every sort variant on this input; keep the fastest (graph left built)."""

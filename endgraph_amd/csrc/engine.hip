@@ -152,14 +152,13 @@ __global__ void k_soa_csr(const R* __restrict__ lst, uint64_t E, uint32_t* __res
   }
 }
 
-// Single-root BFS push: vertices of out-degree above kSbfsHub ("hubs") are
-// cut into work items of kSbfsItemEdges out-edges, listed once per graph.
-constexpr uint32_t kSbfsItemEdges = 1024;  // out-edges per push work item of a hub
-constexpr uint32_t kSbfsHub = 4096;        // out-degree above which a vertex is cut into items
-
-__global__ void k_hub_item_count(const uint32_t* __restrict__ deg, uint32_t V, uint32_t* __restrict__ cnt) {
+// Single-root BFS / SSSP push: vertices of out-degree above the graph's hub
+// threshold (tune.hub, 4096 by default) are cut into work items of
+// tune.hub_chunk out-edges, listed once per graph.
+__global__ void k_hub_item_count(const uint32_t* __restrict__ deg, uint32_t V, uint32_t hub, uint32_t chunk,
+                                 uint32_t* __restrict__ cnt) {
   for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
-    cnt[v] = deg[v] > kSbfsHub ? uint32_t(ceil_div(deg[v], kSbfsItemEdges)) : 0u;
+    cnt[v] = deg[v] > hub ? uint32_t(ceil_div(deg[v], chunk)) : 0u;
 }
 
 __global__ void k_hub_items(const uint32_t* __restrict__ off, uint32_t V, uint64_t* __restrict__ items) {
@@ -252,27 +251,41 @@ int prepare_impl(eg_graph* g, hipStream_t s) {
   hipLaunchKernelGGL(k_deg8, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s, (const uint32_t*)g->out_deg.as<uint32_t>(),
                      V, g->e_deg8.as<uint8_t>());
   EG_HIP(hipGetLastError());
-  {  // the hubs' push items: a count per vertex, its exclusive prefix, the items
-    DevBuf off;
-    EG_TRY(off.resize((uint64_t(V) + 1) * 4));
-    uint32_t* o = off.as<uint32_t>();
-    hipLaunchKernelGGL(k_hub_item_count, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
-                       (const uint32_t*)g->out_deg.as<uint32_t>(), V, o);
-    EG_HIP(hipGetLastError());
-    EG_TRY(fill<uint32_t>(o + V, 1, 0u, s));
-    EG_TRY(exclusive_scan_u32(o, o, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
-    uint32_t n = 0;
-    EG_HIP(hipMemcpyAsync(&n, o + V, 4, hipMemcpyDeviceToHost, s));
-    EG_HIP(hipStreamSynchronize(s));
-    EG_TRY(g->e_hub_items.resize(std::max<uint64_t>(n, 1) * 8));
-    g->e_nhub_items = n;
-    hipLaunchKernelGGL(k_hub_items, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s, (const uint32_t*)o, V,
-                       g->e_hub_items.as<uint64_t>());
-    EG_HIP(hipGetLastError());
-    EG_HIP(hipStreamSynchronize(s));  // `off` is freed on scope exit
-  }
+  g->e_hub_for[0] = g->e_hub_for[1] = 0;
+  EG_TRY(ensure_hub_items(g, s));
   return EG_OK;
 }
+
+}  // namespace
+
+// The hubs' push items for the graph's tune: a count per vertex, its
+// exclusive prefix (in the engine scratch, V * 4 + 4096 bytes or more), the
+// items; rebuilt only when the tune changed.
+int ensure_hub_items(eg_graph* g, hipStream_t s) {
+  const uint32_t V = g->V, hub = g->tune.hub, chunk = g->tune.hub_chunk;
+  if (g->e_hub_for[0] == hub && g->e_hub_for[1] == chunk) return EG_OK;
+  if (chunk == 0) return fail(EG_ERR_ARG, "hub chunk size must be positive");
+  if (g->e_scratch.bytes < (uint64_t(V) + 1) * 4) EG_TRY(g->e_scratch.resize((uint64_t(V) + 1) * 4 + 4096));
+  uint32_t* o = g->e_scratch.as<uint32_t>();
+  hipLaunchKernelGGL(k_hub_item_count, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
+                     (const uint32_t*)g->out_deg.as<uint32_t>(), V, hub, chunk, o);
+  EG_HIP(hipGetLastError());
+  EG_TRY(fill<uint32_t>(o + V, 1, 0u, s));
+  EG_TRY(exclusive_scan_u32(o, o, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
+  uint32_t n = 0;
+  EG_HIP(hipMemcpyAsync(&n, o + V, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  EG_TRY(g->e_hub_items.resize(std::max<uint64_t>(n, 1) * 8));
+  g->e_nhub_items = n;
+  hipLaunchKernelGGL(k_hub_items, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s, (const uint32_t*)o, V,
+                     g->e_hub_items.as<uint64_t>());
+  EG_HIP(hipGetLastError());
+  g->e_hub_for[0] = hub;
+  g->e_hub_for[1] = chunk;
+  return EG_OK;
+}
+
+namespace {
 
 int ensure_engine(eg_graph* g, hipStream_t s) {
   if (!g) return fail(EG_ERR_ARG, "graph is NULL");
@@ -905,11 +918,10 @@ __global__ __launch_bounds__(kNT) void k_bfs_pull_items(const uint64_t* __restri
 // eg_bfs's levels as steps of a captured graph (no host round trip per
 // level): the level end counts the next frontier and its out-edges (which
 // pick the next step's direction on the device); a push step walks the
-// frontier bitmap for the vertices up to kSbfsHub out-edges and the graph's
+// frontier bitmap for the vertices up to the hub threshold and the graph's
 // list of hub items for the others.  The frontier and next bitmaps
 // alternate by step parity, and the level end clears the bitmap the next
 // step writes into.
-constexpr uint32_t kSbfsLaneEdges = 16;    // out-degree up to which a vertex runs in one lane
 
 // Counters of one step's frontier.  The frontier's out-edges are added by
 // every block of the level end, so they are spread over kNeSlots cache
@@ -946,6 +958,7 @@ struct SbfsArgs {
   uint64_t n_big, n_big_rest;
   const uint64_t* hub_items;
   uint64_t n_hub_items;
+  uint32_t dense_div, hub, hub_chunk;  // the graph's tune
   SbfsState st;
   uint32_t *parent, *depth;
 };
@@ -959,13 +972,12 @@ __device__ __forceinline__ SCountView count_of(const A& a, uint32_t step) {
   return r;
 }
 
-// auto policy: dense once the frontier's out-edges reach E / 20 (twitter-
-// shaped BFS: E/10 4.73 ms, E/20 3.97, E/40 6.38; RMAT-24 within 1%)
-constexpr uint64_t kSbfsDenseDiv = 20;
-
-__device__ __forceinline__ bool sbfs_sparse(int policy, unsigned long long ne, uint64_t E) {
+// auto policy: dense once the frontier's out-edges reach E / div (the
+// graph's tune; default 20: twitter-shaped BFS E/10 4.73 ms, E/20 3.97,
+// E/40 6.38; RMAT-24 within 1%)
+__device__ __forceinline__ bool sbfs_sparse(int policy, unsigned long long ne, uint64_t E, uint32_t div) {
   if (policy == EG_POLICY_PULL) return false;
-  if (policy == EG_POLICY_AUTO) return ne < E / kSbfsDenseDiv;
+  if (policy == EG_POLICY_AUTO) return ne < E / div;
   return true;  // reference (rooted programs push, R11) and push
 }
 
@@ -1020,7 +1032,7 @@ __device__ __forceinline__ void sbfs_relax_wave(const SbfsArgs& a, uint64_t* __r
 }
 
 // Sparse step: frontier vertices of out-degree up to kSbfsLaneEdges run in
-// their lane, up to kSbfsHub with the whole wave; hubs' items are strided
+// their lane, up to the hub threshold with the whole wave; hubs' items are strided
 // over the waves.
 template <bool kDerive>
 __device__ __forceinline__ void sbfs_push(const SbfsArgs& a, uint32_t step) {
@@ -1042,7 +1054,7 @@ __device__ __forceinline__ void sbfs_push(const SbfsArgs& a, uint32_t step) {
       const uint32_t e0 = a.out_ptr[u], e1 = a.out_ptr[u + 1];
       if (e1 - e0 <= kSbfsLaneEdges)
         sbfs_relax<kDerive>(a, nx, u, e0, 1, e1 - e0);
-      else if (e1 - e0 <= kSbfsHub)
+      else if (e1 - e0 <= a.hub)
         has_mid = true;
     }
     uint64_t wm = __ballot(has_mid);
@@ -1052,7 +1064,7 @@ __device__ __forceinline__ void sbfs_push(const SbfsArgs& a, uint32_t step) {
       const bool set = (shfl_t(mine, jj) >> lane) & 1ull;
       const uint32_t u = uint32_t((b0 + jj) * 64 + lane);
       const uint32_t e0 = set ? a.out_ptr[u] : 0u, e1 = set ? a.out_ptr[u + 1] : 0u;
-      uint64_t mid = __ballot(e1 - e0 > kSbfsLaneEdges && e1 - e0 <= kSbfsHub);
+      uint64_t mid = __ballot(e1 - e0 > kSbfsLaneEdges && e1 - e0 <= a.hub);
       while (mid) {  // wave-uniform
         const int l = __ffsll((unsigned long long)mid) - 1;
         mid &= mid - 1;
@@ -1078,8 +1090,8 @@ __device__ __forceinline__ void sbfs_push(const SbfsArgs& a, uint32_t step) {
       lm &= lm - 1;
       const uint64_t t = shfl_t(it, l);
       const uint32_t u = uint32_t(t >> 32);
-      const uint32_t e0 = a.out_ptr[u] + uint32_t(t) * kSbfsItemEdges;
-      sbfs_relax_wave<kDerive>(a, nx, u, e0, min(e0 + kSbfsItemEdges, a.out_ptr[u + 1]), lane);
+      const uint32_t e0 = a.out_ptr[u] + uint32_t(t) * a.hub_chunk;
+      sbfs_relax_wave<kDerive>(a, nx, u, e0, min(e0 + a.hub_chunk, a.out_ptr[u + 1]), lane);
     }
   }
 }
@@ -1087,7 +1099,7 @@ __device__ __forceinline__ void sbfs_push(const SbfsArgs& a, uint32_t step) {
 __global__ __launch_bounds__(kNT) void k_sbfs_push(SbfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const SCountView c = count_of(a, step);
-  if (!c.nv || !sbfs_sparse(a.policy, c.ne, a.E)) return;
+  if (!c.nv || !sbfs_sparse(a.policy, c.ne, a.E, a.dense_div)) return;
   if (sbfs_derive(c.ne, a.V))
     sbfs_push<true>(a, step);
   else
@@ -1098,7 +1110,7 @@ __global__ __launch_bounds__(kNT) void k_sbfs_push(SbfsArgs a, uint32_t j) {
 __global__ __launch_bounds__(kNT) void k_sbfs_pull_rows(SbfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const SCountView c = count_of(a, step);
-  if (!c.nv || sbfs_sparse(a.policy, c.ne, a.E)) return;
+  if (!c.nv || sbfs_sparse(a.policy, c.ne, a.E, a.dense_div)) return;
   bfs_pull_rows(a.V, a.in_ptr, a.in_src, a.st.bm[step & 1], a.st.vis, a.parent, a.st.bm[(step + 1) & 1]);
 }
 
@@ -1106,7 +1118,7 @@ template <int kRest>
 __global__ __launch_bounds__(kNT) void k_sbfs_pull_items(SbfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const SCountView c = count_of(a, step);
-  if (!c.nv || sbfs_sparse(a.policy, c.ne, a.E)) return;
+  if (!c.nv || sbfs_sparse(a.policy, c.ne, a.E, a.dense_div)) return;
   bfs_pull_items<!kRest>(kRest ? a.big_rest : a.big, kRest ? a.n_big_rest : a.n_big, a.in_ptr, a.in_src,
                  a.st.bm[step & 1], a.st.vis, a.parent, a.st.bm[(step + 1) & 1]);
 }
@@ -1143,7 +1155,7 @@ __global__ __launch_bounds__(kNT) void k_sbfs_end(SbfsArgs a, uint32_t j) {
   uint64_t* fr = a.st.bm[step & 1];
   uint64_t* nx = a.st.bm[(step + 1) & 1];
   SCount* nc = &a.st.cnt[(step + 1) % 3];
-  const bool derive = sbfs_sparse(a.policy, c.ne, a.E) && sbfs_derive(c.ne, a.V);
+  const bool derive = sbfs_sparse(a.policy, c.ne, a.E, a.dense_div) && sbfs_derive(c.ne, a.V);
   unsigned long long nv = 0, ne = 0;
   // the vertices of up to kEndWords words b0 + idx[k] (idx[k] < 0: none),
   // whose bits lane idx[k] holds in `mine`: depth stores and degree loads
@@ -1273,7 +1285,7 @@ __global__ void k_sbfs_init(SbfsArgs a, uint32_t root) {
 
 // ------------------------------------------ single-root SSSP, device-driven
 // eg_sssp's iterations (sssp.cpp:38-85) as steps of a captured graph, like
-// eg_bfs: a push step walks the active bitmap (vertices up to kSbfsHub
+// eg_bfs: a push step walks the active bitmap (vertices up to the hub threshold
 // out-edges) and the hub items, a dense step is the pull above, and the
 // step end counts the next active set and its out-edges (the mode switch
 // input) and clears the bitmap the next step writes into.
@@ -1281,11 +1293,9 @@ __global__ void k_sbfs_init(SbfsArgs a, uint32_t root) {
 // vertices' out-edges reach E/80.  A push step's improving relaxations are
 // atomics and most of its edges improve, while the dense pass streams;
 // 64 R-MAT 22 runs: E/2 594 ms, E/4 344, E/8 244, E/20 198, E/40 189,
-// E/80 187, E/160 190.
-constexpr uint64_t kDssspDenseDiv = 80;
-
-__device__ __forceinline__ bool dsssp_sparse(int policy, unsigned long long ne, uint64_t E) {
-  return policy == EG_POLICY_AUTO ? ne < E / kDssspDenseDiv : sbfs_sparse(policy, ne, E);
+// E/80 187, E/160 190 (the default; the graph's tune sets it).
+__device__ __forceinline__ bool dsssp_sparse(int policy, unsigned long long ne, uint64_t E, uint32_t div) {
+  return sbfs_sparse(policy, ne, E, div);
 }
 
 struct DssspArgs {
@@ -1297,6 +1307,7 @@ struct DssspArgs {
   const uint8_t* deg8;
   const uint64_t* hub_items;
   uint64_t n_hub_items;
+  uint32_t dense_div, hub, hub_chunk;  // the graph's tune
   const uint32_t *csc_src, *wordrank, *row_dst;
   const float* csc_w;
   const uint64_t* rowflag;
@@ -1347,7 +1358,7 @@ __device__ __forceinline__ void dsssp_relax_wave(const DssspArgs& a, uint64_t* _
 __global__ __launch_bounds__(kNT) void k_dsssp_push(DssspArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const SCountView c = count_of(a, step);
-  if (!c.nv || !dsssp_sparse(a.policy, c.ne, a.E)) return;
+  if (!c.nv || !dsssp_sparse(a.policy, c.ne, a.E, a.dense_div)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
@@ -1363,7 +1374,7 @@ __global__ __launch_bounds__(kNT) void k_dsssp_push(DssspArgs a, uint32_t j) {
       const uint32_t e0 = a.out_ptr[u], e1 = a.out_ptr[u + 1];
       if (e1 - e0 <= kSbfsLaneEdges)
         dsssp_relax(a, nx, a.dist[u], e0, 1, e1 - e0);
-      else if (e1 - e0 <= kSbfsHub)
+      else if (e1 - e0 <= a.hub)
         has_mid = true;
     }
     uint64_t wm = __ballot(has_mid);
@@ -1373,7 +1384,7 @@ __global__ __launch_bounds__(kNT) void k_dsssp_push(DssspArgs a, uint32_t j) {
       const bool set = (shfl_t(mine, jj) >> lane) & 1ull;
       const uint32_t u = uint32_t((b0 + jj) * 64 + lane);
       const uint32_t e0 = set ? a.out_ptr[u] : 0u, e1 = set ? a.out_ptr[u + 1] : 0u;
-      const bool is_mid = e1 - e0 > kSbfsLaneEdges && e1 - e0 <= kSbfsHub;
+      const bool is_mid = e1 - e0 > kSbfsLaneEdges && e1 - e0 <= a.hub;
       const float du = is_mid ? a.dist[u] : 0.0f;
       uint64_t mid = __ballot(is_mid);
       while (mid) {  // wave-uniform
@@ -1397,8 +1408,8 @@ __global__ __launch_bounds__(kNT) void k_dsssp_push(DssspArgs a, uint32_t j) {
       lm &= lm - 1;
       const uint64_t t = shfl_t(it, l);
       const uint32_t u = uint32_t(t >> 32);
-      const uint32_t e0 = a.out_ptr[u] + uint32_t(t) * kSbfsItemEdges;
-      dsssp_relax_wave(a, nx, a.dist[u], e0, min(e0 + kSbfsItemEdges, a.out_ptr[u + 1]), lane);
+      const uint32_t e0 = a.out_ptr[u] + uint32_t(t) * a.hub_chunk;
+      dsssp_relax_wave(a, nx, a.dist[u], e0, min(e0 + a.hub_chunk, a.out_ptr[u + 1]), lane);
     }
   }
 }
@@ -1413,7 +1424,7 @@ __global__ __launch_bounds__(kNT) void k_dsssp_push(DssspArgs a, uint32_t j) {
 __global__ __launch_bounds__(kNT) void k_dsssp_pull(DssspArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const SCountView c = count_of(a, step);
-  if (!c.nv || dsssp_sparse(a.policy, c.ne, a.E)) return;  // block-uniform
+  if (!c.nv || dsssp_sparse(a.policy, c.ne, a.E, a.dense_div)) return;  // block-uniform
   __shared__ uint64_t s_sum[kSumMax];
   const bool use_sum = a.nsum <= kSumMax;
   if (use_sum) {
@@ -1991,8 +2002,12 @@ int eg_bfs(eg_graph* g, uint32_t root, int policy, uint32_t* d_parent, uint32_t*
   a.out_dst = g->e_out_dst.as<uint32_t>();
   a.deg = g->out_deg.as<uint32_t>();
   a.deg8 = g->e_deg8.as<uint8_t>();
+  EG_TRY(ensure_hub_items(g, s));
   a.hub_items = g->e_hub_items.as<uint64_t>();
   a.n_hub_items = g->e_nhub_items;
+  a.dense_div = g->tune.sbfs_div;
+  a.hub = g->tune.hub;
+  a.hub_chunk = g->tune.hub_chunk;
   if (pulls) {
     a.in_ptr = g->tv_ptr.as<uint32_t>();
     a.in_src = g->tv_src.as<uint32_t>();
@@ -2124,8 +2139,12 @@ int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_i
   a.out_w = g->e_out_w.as<float>();
   a.deg = g->out_deg.as<uint32_t>();
   a.deg8 = g->e_deg8.as<uint8_t>();
+  EG_TRY(ensure_hub_items(g, s));
   a.hub_items = g->e_hub_items.as<uint64_t>();
   a.n_hub_items = g->e_nhub_items;
+  a.dense_div = g->tune.dsssp_div;
+  a.hub = g->tune.hub;
+  a.hub_chunk = g->tune.hub_chunk;
   a.csc_src = g->e_csc_src.as<uint32_t>();
   a.csc_w = g->e_csc_w.as<float>();
   a.rowflag = g->e_rowflag.as<uint64_t>();

@@ -49,6 +49,19 @@ struct MultiGraph {
   }
 };
 
+// Sizes and auto-policy thresholds of the device-driven traversals, per
+// graph (eg_autotune_engine; the defaults are the sweeps' picks on R-MAT and
+// the twitter-shaped graph).  None changes a result, only the schedule.
+struct EngineTune {
+  uint32_t sbfs_div = 20;     // single-root BFS: dense once the frontier's out-edges reach E / sbfs_div
+  uint32_t dsssp_div = 80;    // single-root SSSP: the same for the active set
+  uint32_t mbfs_div = 20;     // batched BFS
+  uint32_t msssp_div = 2;     // batched SSSP
+  uint32_t hub = 4096;        // single-root push: out-degree above which a vertex is cut into chunks
+  uint32_t hub_chunk = 1024;  // out-edges per such chunk
+  uint32_t m_item = 4096;     // batched push: out-edges per work item
+};
+
 struct eg_graph {
   // configuration (Graph(int eta), core/graph.hpp:202; P ranks and S sockets)
   uint32_t V = 0;
@@ -96,6 +109,8 @@ struct eg_graph {
   eg::DevBuf e_deg8;                 // u8[V]    out-degree saturated at 255 (BFS level ends)
   eg::DevBuf e_hub_items;            // u64 (v << 32 | chunk): out-edge chunks of the BFS push hubs
   uint64_t e_nhub_items = 0;
+  uint32_t e_hub_for[2] = {0, 0};    // (hub, hub_chunk) the list was made for
+  EngineTune tune;
   eg::DevBuf e_scratch;              // frontier list, counters
   eg::DevBuf e_bm[3];                // three bitmaps of (V>>6)+1 words
   eg::DevBuf e_pr;                   // PageRank work vectors
@@ -169,6 +184,8 @@ struct eg_graph {
 namespace eg {
 // Derives the engine's view after a build (first engine call; engine.hip).
 int engine_ensure(eg_graph* g, hipStream_t s);
+// The single-root push's hub chunks for the graph's current tune (engine.hip).
+int ensure_hub_items(eg_graph* g, hipStream_t s);
 // Rows of in-edges by destination with sources ascending and the pull work
 // items over them (first traversal that needs it; multi.hip).
 int ensure_tview(eg_graph* g, hipStream_t s);
@@ -177,5 +194,8 @@ int ensure_tview(eg_graph* g, hipStream_t s);
 // at their first source; twitter-shaped BFS 3.76 ms at 32, 3.50 at 128,
 // 3.43 at 512; RMAT-24 0.78 -> 0.75 ms).
 constexpr uint32_t kTvPullEdges = 512;
+// Out-degree up to which a single-root push relaxes a frontier vertex in one
+// lane; the hub threshold of EngineTune may not go below it.
+constexpr uint32_t kSbfsLaneEdges = 16;
 constexpr uint32_t kTvSmallRow = 512;
 }  // namespace eg

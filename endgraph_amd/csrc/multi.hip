@@ -36,7 +36,6 @@ namespace {
 
 constexpr int kNT = 256;
 constexpr int kEndStrides = 4;         // grid strides a level-end wave takes per step
-constexpr uint32_t kItemEdges = 4096;  // out-edges per push work item
 constexpr uint32_t kPullEdges = kTvPullEdges;  // in-edges per pull work item
 constexpr uint32_t kUnreached = 0xffffffffu;
 constexpr float kInfDist = 1e9f;       // toolkits/sssp.cpp INF
@@ -89,13 +88,11 @@ __device__ __forceinline__ bool sparse_level(int policy, unsigned long long acti
 }
 
 // auto policy of the batched SSSP: dense once the active out-edges reach
-// E / kMssspDenseDiv.  Its pull skips the sources no root has active and
-// gathers 64-root distance rows, so pushes pay until half of E (64 R-MAT 22
-// roots: E/20 47.7 ms, E/5 46.1, E/3 45.4, E/2 44.4, E/1 106, push only 109).
-constexpr uint64_t kMssspDenseDiv = 2;
-// auto policy of the batched BFS: dense once the frontier's out-edges reach
-// E / kMbfsDenseDiv (64 R-MAT 22 roots: E/5..E/20 within 1%, E/40 7% slower)
-constexpr uint64_t kMbfsDenseDiv = 20;
+// E / tune.msssp_div.  Its pull skips the sources no root has active and
+// gathers 64-root distance rows, so pushes pay until half of E (default 2;
+// 64 R-MAT 22 roots: E/20 47.7 ms, E/5 46.1, E/3 45.4, E/2 44.4, E/1 106,
+// push only 109).  The batched BFS goes dense at E / tune.mbfs_div (default
+// 20; 64 R-MAT 22 roots: E/5..E/20 within 1%, E/40 7% slower).
 
 template <class T>
 __global__ void k_fill_t(T* __restrict__ p, uint64_t n, T v) {
@@ -286,9 +283,10 @@ struct MultiState {
   MCount* cnt;              // ring of 3 by step: read at s, written at s-1, cleared at s-2
   uint32_t* steps;          // steps that had a nonempty frontier
   uint32_t* base;           // first step of the queued batch (step = *base + j)
+  uint32_t item_edges;      // out-edges per push work item (the graph's tune.m_item)
 };
 
-uint64_t item_cap(const eg_graph* g) { return uint64_t(g->V) + g->E / kItemEdges + 1; }
+uint64_t item_cap(const eg_graph* g) { return uint64_t(g->V) + g->E / g->tune.m_item + 1; }
 
 int multi_state(eg_graph* g, MultiState* st) {
   const uint64_t V = g->V;
@@ -304,12 +302,14 @@ int multi_state(eg_graph* g, MultiState* st) {
   st->cnt = reinterpret_cast<MCount*>(st->items[1] + ic);
   st->steps = reinterpret_cast<uint32_t*>(st->cnt + 3);
   st->base = st->steps + 1;
+  st->item_edges = g->tune.m_item;
   return EG_OK;
 }
 
 // Roots: lane r < n sets bit r of its root; work items and counters of the
 // first step come from the distinct roots.  One wave.
-__global__ void k_multi_roots(const uint32_t* __restrict__ roots, int n, const uint32_t* __restrict__ deg,
+__global__ void k_multi_roots(const uint32_t* __restrict__ roots, int n, uint32_t item_edges,
+                              const uint32_t* __restrict__ deg,
                               uint64_t* __restrict__ vis, uint64_t* __restrict__ fr, uint64_t* __restrict__ items,
                               MCount* __restrict__ cnt, uint32_t* __restrict__ parent, uint32_t* __restrict__ depth,
                               float* __restrict__ dist) {
@@ -328,7 +328,7 @@ __global__ void k_multi_roots(const uint32_t* __restrict__ roots, int n, const u
   const uint32_t d = deg[v];
   atomicAdd(&cnt[0].nv, 1ull);
   atomicAdd(&cnt[0].ne, (unsigned long long)d);
-  const uint32_t k = uint32_t(ceil_div(d, kItemEdges));
+  const uint32_t k = uint32_t(ceil_div(d, item_edges));
   const uint32_t at = atomicAdd(&cnt[0].items, k);
   for (uint32_t j = 0; j < k; ++j) items[at + j] = (uint64_t(v) << 32) | j;
 }
@@ -342,7 +342,7 @@ __global__ void k_multi_roots(const uint32_t* __restrict__ roots, int n, const u
 template <class Visit>
 __device__ __forceinline__ void level_end(uint32_t V, const uint32_t* __restrict__ deg, uint64_t* __restrict__ nx,
                                           uint64_t* __restrict__ fr, MCount* nc, uint64_t* __restrict__ items,
-                                          Visit visit) {
+                                          uint32_t item_edges, Visit visit) {
   // A wave takes kEndStrides grid strides of 64 vertices per step, with the
   // mask loads of all of them, then the degree loads, in flight together;
   // the block reserves its work items with one atomic (a block scan).
@@ -372,7 +372,7 @@ __device__ __forceinline__ void level_end(uint32_t V, const uint32_t* __restrict
         roots |= nb[r];
         ++nv;
         ne += d[r];
-        ksum += uint32_t(ceil_div(d[r], kItemEdges));
+        ksum += uint32_t(ceil_div(d[r], item_edges));
       }
   }
   uint32_t tot = 0;
@@ -397,7 +397,7 @@ __device__ __forceinline__ void level_end(uint32_t V, const uint32_t* __restrict
       if (v < V) {
         if (nb[r]) {
           nx[v] = 0;
-          const uint32_t k = uint32_t(ceil_div(d[r], kItemEdges));
+          const uint32_t k = uint32_t(ceil_div(d[r], item_edges));
           for (uint32_t q = 0; q < k; ++q) items[at + q] = (v << 32) | q;
           at += k;
         }
@@ -445,6 +445,7 @@ struct BfsArgs {
   uint32_t V;
   uint64_t E;
   int policy;
+  uint32_t dense_div;  // the graph's tune.mbfs_div
   uint64_t lanes;  // bits of the live roots
   const uint32_t *out_ptr, *out_dst, *deg, *in_ptr, *in_src;
   const uint64_t* pull_items;  // in-edge chunks (v << 32 | chunk)
@@ -458,7 +459,7 @@ struct BfsArgs {
 __global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
-  if (!c.nv || !sparse_level(a.policy, c.ne, a.E, kMbfsDenseDiv)) return;
+  if (!c.nv || !sparse_level(a.policy, c.ne, a.E, a.dense_div)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
@@ -467,8 +468,8 @@ __global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t j) {
     const uint64_t it = items[i];
     const uint32_t u = uint32_t(it >> 32), ch = uint32_t(it);
     const uint64_t fm = a.st.fr[u];
-    const uint32_t e0 = a.out_ptr[u] + ch * kItemEdges;
-    const uint32_t e1 = min(e0 + kItemEdges, a.out_ptr[u + 1]);
+    const uint32_t e0 = a.out_ptr[u] + ch * a.st.item_edges;
+    const uint32_t e1 = min(e0 + a.st.item_edges, a.out_ptr[u + 1]);
     for (uint32_t e = e0 + lane; e < e1; e += kWave) {
       const uint32_t v = a.out_dst[e];
       uint64_t bits = fm & ~a.st.vis[v];
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(kNT) void k_mbfs_push(BfsArgs a, uint32_t j) {
 __global__ __launch_bounds__(kNT) void k_mbfs_pull(BfsArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
-  if (!c.nv || sparse_level(a.policy, c.ne, a.E, kMbfsDenseDiv)) return;
+  if (!c.nv || sparse_level(a.policy, c.ne, a.E, a.dense_div)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
@@ -559,7 +560,7 @@ __global__ __launch_bounds__(kNT) void k_mbfs_end(BfsArgs a, uint32_t j) {
   step_bookkeeping(a.st, step, live);
   if (!live) return;  // block-uniform
   const int lane = threadIdx.x & (kWave - 1);
-  level_end(a.V, a.deg, a.st.nx, a.st.fr, &a.st.cnt[(step + 1) % 3], a.st.items[(step + 1) & 1],
+  level_end(a.V, a.deg, a.st.nx, a.st.fr, &a.st.cnt[(step + 1) % 3], a.st.items[(step + 1) & 1], a.st.item_edges,
             [&](uint64_t v0, uint64_t v, uint64_t nb) {
               if (nb) a.st.vis[v] |= nb;
               // depth rows: lane = root, one 256-byte row per vertex
@@ -579,6 +580,7 @@ struct SsspArgs {
   uint32_t V;
   uint64_t E;
   int policy;
+  uint32_t dense_div;  // the graph's tune.msssp_div
   uint64_t lanes;
   const uint32_t *out_ptr, *out_dst, *deg, *in_ptr, *in_src;
   const float *out_w, *in_w;
@@ -593,7 +595,7 @@ struct SsspArgs {
 __global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
-  if (!c.nv || !sparse_level(a.policy, c.ne, a.E, kMssspDenseDiv)) return;
+  if (!c.nv || !sparse_level(a.policy, c.ne, a.E, a.dense_div)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
@@ -602,8 +604,8 @@ __global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t j) {
     const uint64_t it = items[i];
     const uint32_t u = uint32_t(it >> 32), ch = uint32_t(it);
     const uint64_t am = a.st.fr[u];
-    const uint32_t e0 = a.out_ptr[u] + ch * kItemEdges;
-    const uint32_t e1 = min(e0 + kItemEdges, a.out_ptr[u + 1]);
+    const uint32_t e0 = a.out_ptr[u] + ch * a.st.item_edges;
+    const uint32_t e1 = min(e0 + a.st.item_edges, a.out_ptr[u + 1]);
     for (uint32_t e = e0 + lane; e < e1; e += kWave) {
       const uint32_t v = a.out_dst[e];
       const float w = a.out_w[e];
@@ -627,7 +629,7 @@ __global__ __launch_bounds__(kNT) void k_msssp_push(SsspArgs a, uint32_t j) {
 __global__ __launch_bounds__(kNT) void k_msssp_pull(SsspArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const MCount c = a.st.cnt[step % 3];
-  if (!c.nv || sparse_level(a.policy, c.ne, a.E, kMssspDenseDiv)) return;
+  if (!c.nv || sparse_level(a.policy, c.ne, a.E, a.dense_div)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
@@ -692,7 +694,7 @@ __global__ __launch_bounds__(kNT) void k_msssp_end(SsspArgs a, uint32_t j) {
   const bool live = a.st.cnt[step % 3].nv != 0;
   step_bookkeeping(a.st, step, live);
   if (!live) return;  // block-uniform
-  level_end(a.V, a.deg, a.st.nx, a.st.fr, &a.st.cnt[(step + 1) % 3], a.st.items[(step + 1) & 1],
+  level_end(a.V, a.deg, a.st.nx, a.st.fr, &a.st.cnt[(step + 1) % 3], a.st.items[(step + 1) & 1], a.st.item_edges,
             [](uint64_t, uint64_t, uint64_t) {});
 }
 
@@ -759,7 +761,7 @@ int start_multi(eg_graph* g, const uint32_t* h_roots, int n, MultiState* st, uin
   EG_TRY(g->e_multi_roots.resize(64 * 4 + 64 * 4 + 64 * 8 + 256));
   EG_HIP(hipMemcpyAsync(g->e_multi_roots.ptr, h_roots, size_t(n) * 4, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_multi_roots, dim3(1), dim3(64), 0, s, (const uint32_t*)g->e_multi_roots.as<uint32_t>(), n,
-                     (const uint32_t*)g->out_deg.as<uint32_t>(), parent ? st->vis : nullptr, st->fr, st->items[0],
+                     st->item_edges, (const uint32_t*)g->out_deg.as<uint32_t>(), parent ? st->vis : nullptr, st->fr, st->items[0],
                      st->cnt, parent, depth, dist);
   EG_HIP(hipGetLastError());
   return EG_OK;
@@ -798,6 +800,7 @@ int eg_bfs_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, 
   a.V = g->V;
   a.E = g->E;
   a.policy = policy;
+  a.dense_div = g->tune.mbfs_div;
   a.lanes = n_roots == 64 ? ~0ull : ((1ull << n_roots) - 1);
   a.out_ptr = g->e_out_ptr.as<uint32_t>();
   a.out_dst = g->e_out_dst.as<uint32_t>();
@@ -849,6 +852,7 @@ int eg_sssp_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy,
   a.V = g->V;
   a.E = g->E;
   a.policy = policy;
+  a.dense_div = g->tune.msssp_div;
   a.lanes = n_roots == 64 ? ~0ull : ((1ull << n_roots) - 1);
   a.out_ptr = g->e_out_ptr.as<uint32_t>();
   a.out_dst = g->e_out_dst.as<uint32_t>();

@@ -341,6 +341,28 @@ int eg_ppr_adjoint(eg_graph* g, const double* d_w, const double* d_x, const doub
  * root; h_steps (optional) the number of levels / rounds run.
  * The first call after a build derives a source-sorted copy of the CSC
  * (the pull side of the batched programs). */
+/* Sizes and auto-policy thresholds of the device-driven traversals, per
+ * graph (no reference counterpart: the reference's schedule is its fixed
+ * thread chunks, and its mode switch a fixed edges/20, core/graph.hpp:1491).
+ * Under EG_POLICY_AUTO eg_bfs goes dense once the frontier's out-edges reach
+ * E / bfs_dense_div (eg_sssp: sssp_dense_div; the batched programs:
+ * bfs_multi_dense_div, sssp_multi_dense_div); the single-root push cuts
+ * vertices of more than hub_degree out-edges into chunks of hub_chunk; the
+ * batched push's work items hold multi_item_edges out-edges.  No value
+ * changes a result.  Defaults: 20, 80, 20, 2, 4096, 1024, 4096.
+ * eg_autotune_engine times the programs from n_roots sample roots (the
+ * max-out-degree vertex and seeded vertices with out-edges) over candidate
+ * values, keeps the fastest for g and returns them (h_ms, optional: the best
+ * times of single BFS, single BFS+SSSP hub split, single SSSP, batched BFS,
+ * batched SSSP, in ms).  Synchronises. */
+typedef struct eg_engine_tune {
+  uint32_t bfs_dense_div, sssp_dense_div, bfs_multi_dense_div, sssp_multi_dense_div;
+  uint32_t hub_degree, hub_chunk, multi_item_edges;
+} eg_engine_tune;
+int eg_get_engine_tune(const eg_graph* g, eg_engine_tune* out);
+int eg_set_engine_tune(eg_graph* g, const eg_engine_tune* tune);
+int eg_autotune_engine(eg_graph* g, int n_roots, uint64_t seed, eg_engine_tune* h_out, double* h_ms, void* stream);
+
 int eg_bfs_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, uint32_t* d_parent, uint32_t* d_depth,
                  uint32_t* h_levels, uint64_t* h_edges, uint32_t* h_steps, void* stream);
 int eg_sssp_multi(eg_graph* g, const uint32_t* h_roots, int n_roots, int policy, float* d_dist, uint32_t* h_steps,
