@@ -1314,10 +1314,13 @@ struct DssspArgs {
   SbfsState st;  // bm[2], cnt, steps, base (vis and total unused)
   float* dist;
   uint64_t* sum[2];  // active summary of step s in sum[s & 1]: bit w set iff word w of its bitmap is not empty
-  uint32_t nsum;     // summary words, ((V >> 6) >> 6) + 1
+  uint32_t nsum;     // summary words, ((V - 1) >> 12) + 1: bit w of word w >> 6 for bitmap words w < V/64 rounded up
 };
 
-constexpr uint32_t kSumMax = 4096;  // summary words a dense step stages in LDS (V up to 2^24)
+// Summary words a dense step stages in LDS (32 KB: V up to 2^24); larger
+// graphs run the dense step without the summary, and reserve no LDS for it.
+constexpr uint32_t kSumMax = 4096;
+inline uint32_t summary_words(uint32_t V) { return ((V ? V - 1 : 0) >> 12) + 1; }
 
 // Sparse slot (sssp.cpp:44-58) for the out-edges e = first + k * stride,
 // k < n <= kRelaxBatch, of active vertex u at distance du: targets, weights
@@ -1425,7 +1428,7 @@ __global__ __launch_bounds__(kNT) void k_dsssp_pull(DssspArgs a, uint32_t j) {
   const uint32_t step = *a.st.base + j;
   const SCountView c = count_of(a, step);
   if (!c.nv || dsssp_sparse(a.policy, c.ne, a.E, a.dense_div)) return;  // block-uniform
-  __shared__ uint64_t s_sum[kSumMax];
+  extern __shared__ uint64_t s_sum[];  // a.nsum words when they fit kSumMax (launch with that many), else none
   const bool use_sum = a.nsum <= kSumMax;
   if (use_sum) {
     const uint64_t* sum = a.sum[step & 1];
@@ -1479,7 +1482,9 @@ __global__ __launch_bounds__(kNT) void k_dsssp_end(DssspArgs a, uint32_t j) {
     }
     nv += __popcll(mine);
     uint64_t nz = __ballot(mine != 0);
-    if (lane == 0) nsum[b0 >> 6] = nz;  // the next step's summary word
+    // the next step's summary word; the bitmap's last word may lie past
+    // the summary (V a multiple of 64: that word holds no vertex and is empty)
+    if (lane == 0 && (b0 >> 6) < a.nsum) nsum[b0 >> 6] = nz;
     while (nz) {  // wave-uniform
       int idx[kEndWords];
 #pragma unroll
@@ -2119,7 +2124,7 @@ int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_i
   if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
   const uint32_t V = g->V;
   const uint64_t words = words_of(V);
-  const uint64_t nsum = (words >> 6) + 1;
+  const uint64_t nsum = summary_words(V);
   EG_TRY(g->e_dsssp.resize(2 * words * 8 + 64 + 3 * sizeof(SCount) + 64 + 2 * nsum * 8));
   DssspArgs a{};  // zeroed padding: the bytes key the captured step graph
   SbfsState& st = a.st;
@@ -2155,7 +2160,7 @@ int eg_sssp(eg_graph* g, uint32_t root, int policy, float* d_dist, uint32_t* h_i
                      root);
   EG_HIP(hipGetLastError());
   const StepLaunch<DssspArgs> ls[3] = {{k_dsssp_push, 2048},
-                                       {k_dsssp_pull, row_grid(g->E)},
+                                       {k_dsssp_pull, row_grid(g->E), nsum <= kSumMax ? unsigned(nsum * 8) : 0u},
                                        {k_dsssp_end, grid_for(ceil_div(words, uint64_t(kEndWords)), kNT / kWave, 2048)}};
   const unsigned long long* const nv[3] = {&st.cnt[0].nv, &st.cnt[1].nv, &st.cnt[2].nv};
   uint32_t steps = 0;

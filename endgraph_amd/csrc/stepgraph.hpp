@@ -41,6 +41,7 @@ template <class Args>
 struct StepLaunch {
   void (*fn)(Args, uint32_t);
   unsigned grid;
+  unsigned lds = 0;  // dynamic LDS bytes per block
 };
 
 // Runs steps until the frontier count of the next step (nv_ring[step % 3])
@@ -70,12 +71,15 @@ int run_step_graph(MultiGraph& mg, const Args& a, const StepLaunch<Args> (&ls)[N
     key.insert(key.end(), reinterpret_cast<const char*>(&fn), reinterpret_cast<const char*>(&fn) + sizeof(fn));
     key.insert(key.end(), reinterpret_cast<const char*>(&ls[i].grid),
                reinterpret_cast<const char*>(&ls[i].grid) + sizeof(ls[i].grid));
+    key.insert(key.end(), reinterpret_cast<const char*>(&ls[i].lds),
+               reinterpret_cast<const char*>(&ls[i].lds) + sizeof(ls[i].lds));
   }
   if (!mg.exec || mg.key != key) {
     mg.reset_exec();
     EG_HIP(hipStreamBeginCapture(mg.stream, hipStreamCaptureModeThreadLocal));
     for (uint32_t j = 0; j < kStepBatch; ++j)
-      for (int i = 0; i < N; ++i) hipLaunchKernelGGL(ls[i].fn, dim3(ls[i].grid), dim3(kStepNT), 0, mg.stream, a, j);
+      for (int i = 0; i < N; ++i)
+        hipLaunchKernelGGL(ls[i].fn, dim3(ls[i].grid), dim3(kStepNT), ls[i].lds, mg.stream, a, j);
     hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, mg.stream, d_base, kStepBatch, ring, mg.mail);
     hipGraph_t graph = nullptr;
     const hipError_t ce = hipStreamEndCapture(mg.stream, &graph);

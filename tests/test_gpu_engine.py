@@ -243,3 +243,23 @@ def test_pagerank_dense_small_closed_form(eg, oracle_lib):
     x, _ = g.pagerank(15, eg.PR_FP64)
     exp = H.pagerank_dense(V, edges["src"].astype(np.int64), edges["dst"].astype(np.int64), 15)
     assert np.abs(x - exp).sum() / np.abs(exp).sum() < 1e-13
+
+
+@pytest.mark.parametrize("V", [1 << 24, (1 << 24) + 1])
+@pytest.mark.parametrize("policy", [1, 3])
+def test_sssp_dense_summary_at_its_limit(eg, oracle_lib, V, policy):
+    """The single-root SSSP's dense steps skip sources in empty words of the
+    active bitmap through a summary staged in LDS: 4096 words cover exactly
+    2^24 vertices (used), one vertex more runs the dense step without it."""
+    print(f"sssp summary limit V={V} policy={policy}", flush=True)
+    E = 1 << 22
+    buf = eg.DeviceBuffer(E * 12)
+    eg.gen_rmat(buf.ptr, E, 24, seed=51, rec_bytes=12, w_lo=0.0, w_hi=1.0)
+    edges = buf.download(eg.REC12, E)
+    g = eg.Graph(V, 1, 1, 1, 1, 12)
+    g.build(buf.ptr, E)
+    og = oracle_lib.OracleGraph(edges, V, P=1, S=1, eta=1, m=1, threads=8)
+    root = int(np.argmax(og.out_degree()))
+    dist, _ = g.sssp(root, policy)
+    o_dist, _ = og.sssp(root)
+    assert np.array_equal(dist.view(np.uint32), o_dist.view(np.uint32))
