@@ -188,3 +188,18 @@ def pagerank_dense(V, src, dst, iterations, d=0.85, with_deltas=False):
             deltas.append(float((np.abs(xn - x) * np.where(deg > 0, deg, 1.0)).sum() / V))
             x = xn
     return (x, np.array(deltas)) if with_deltas else x
+
+
+def widest_paths(V, src, dst, w, root):
+    """Maximum-bottleneck widths from root by synchronous relaxation to the
+    fixpoint: width[v] = max(width[v], max over edges u->v of min(width[u],
+    w)); root 1e9, unreached 0 (float32, min/max only: exact)."""
+    width = np.zeros(V, np.float32)
+    width[root] = np.float32(1e9)
+    w = w.astype(np.float32)
+    while True:
+        new = width.copy()
+        np.maximum.at(new, dst, np.minimum(width[src], w))
+        if np.array_equal(new, width):
+            return width
+        width = new

@@ -94,3 +94,24 @@ def test_pagerank_toolkit(oracle_lib, graph_file, tmp_path, prec):
     assert sums and all(abs(s - pr.sum()) <= 1e-6 * pr.sum() + 1e-6 for s in sums)
     d = [float(x) for x in re.findall(r"delta\(\d+\)=([0-9.]+)", out)[:10]]
     assert np.allclose(d, deltas[:10], rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("P,S,policy", [(1, 1, "reference"), (2, 1, "auto"), (3, 2, "pull"), (1, 1, "push")])
+def test_widest_toolkit_on_the_generic_engine(oracle_lib, graph_file, tmp_path, P, S, policy):
+    """A fifth program written only against the facade's generic device
+    engine (toolkits/widest.hip: the four signal/slot functors, no library
+    kernel of its own): widths bit-exact against the numpy relaxation, in
+    every mode (reference and push: sparse steps only; pull: dense steps
+    only; auto: both)."""
+    V, _, wedges, _, f12 = graph_file
+    deg = np.bincount(wedges["src"], minlength=V)
+    root = int(np.argmax(deg))
+    out = run("widest", f12, V, root, 1, P, S, policy, dump=tmp_path / "width.bin")
+    exp = H.widest_paths(V, wedges["src"].astype(np.int64), wedges["dst"].astype(np.int64), wedges["w"], root)
+    got = np.fromfile(tmp_path / "width.bin", np.float32)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    reached = [int(x) for x in re.findall(r"reached = (\d+)", out)]
+    assert reached and all(r == int((exp > 0).sum()) for r in reached)
+    if policy == "auto":
+        steps = re.findall(r"active\(\d+\)", out)
+        assert len(steps) > 2

@@ -230,3 +230,13 @@ def test_csr_batch_matches_stable_argsort(oracle_lib):
         assert np.array_equal(rp[vo:vo + int(nv[i]) + 1], exp_rp.astype(np.uint64))
         eo += e.shape[0]
         vo += int(nv[i]) + 1
+
+
+def test_widest_paths_reference_hand_case():
+    """The numpy widest-path reference on a hand graph: two routes to 3
+    (bottlenecks 0.5 and 0.25), a dead end 4, an unreachable 5."""
+    src = np.array([0, 1, 0, 2, 3, 5])
+    dst = np.array([1, 3, 2, 3, 4, 0])
+    w = np.array([0.5, 0.75, 0.25, 1.0, 0.125, 1.0], np.float32)
+    width = H.widest_paths(6, src, dst, w, 0)
+    assert width.tolist() == [1e9, 0.5, 0.25, 0.5, 0.125, 0.0]
