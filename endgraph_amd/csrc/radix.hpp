@@ -17,8 +17,8 @@
 //
 // Ranking inside a tile is stable: wave w owns a contiguous run of the tile,
 // handled in rounds of 64 records; the lanes holding the same digit find each
-// other with one ballot per digit bit, and per-wave running counters in LDS
-// order the rounds.  The tile is then staged in LDS in digit order, so each
+// other through a per-wave LDS table of lane masks, and per-wave running
+// counters in LDS order the rounds.  The tile is then staged in LDS in digit order, so each
 // digit's run is written contiguously (bigger tiles => longer runs => fewer
 // partially written lines).  Variants are chosen per record size by
 // select_sort_variant (see eg_autotune_sort).
@@ -196,11 +196,14 @@ __device__ __forceinline__ void rx_rank(const RxIn<R>& in, int64_t sh, const RxT
   for (int k = 0; k < kRxIPT; ++k) {
     const bool valid = first + uint32_t(k) * kWave < x.count;
     const uint32_t d = dig[k];
-    // The compiler sees one lane: to it the three accesses of record k and
-    // those of record k+1 touch unrelated addresses, while across lanes they
-    // alias.  Each access completes (s_waitcnt lgkmcnt(0)) before the next is
-    // issued, so the read sees every lane's OR and the clear follows every
-    // lane's read; the empty asm statements keep the compiler from moving
+    // The compiler sees one lane: to it the accesses of record k and those
+    // of record k+1 touch unrelated addresses, while across lanes they alias.
+    // Three waits (s_waitcnt lgkmcnt(0)) order them without relying on the
+    // LDS completing a wave's accesses in issue order: the read sees every
+    // lane's OR, the clear follows every lane's read, and the clear has
+    // landed before record k+1's OR of the same entry (the wait after the
+    // count read; the compiler emitted it only inside the branch of the
+    // count write).  The empty asm statements keep the compiler from moving
     // memory accesses across those points.
     uint64_t peers = 0;
     if (valid) __hip_atomic_fetch_or(tab + d, uint64_t(1) << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
@@ -210,10 +213,11 @@ __device__ __forceinline__ void rx_rank(const RxIn<R>& in, int64_t sh, const RxT
     __builtin_amdgcn_s_waitcnt(kLgkmZero);
     __asm__ volatile("" ::: "memory");
     if (valid) __hip_atomic_store(tab + d, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    const uint32_t prior = wcnt[w][d];
+    __builtin_amdgcn_s_waitcnt(kLgkmZero);
     __asm__ volatile("" ::: "memory");
     const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(peers >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(peers), 0u));
     const uint32_t total = uint32_t(__popcll(peers));
-    const uint32_t prior = wcnt[w][d];
     rank[k] = prior + below;
     if (valid && below + 1 == total) wcnt[w][d] = prior + total;  // highest peer updates
   }
