@@ -242,12 +242,11 @@ def build_algorithmic_bytes(g, E, rec=8):
     return E * rec * 3 + V * 8 + Q * (V + 1) * 8 + Q * ((V >> 6) + 1) * 8 + cidx * 12
 
 
-def pagerank_block(eg, g, iters, E, tv_ms, oracle_pr=None):
+def pagerank_block(eg, g, iters, E, oracle_pr=None):
     """Plain FP64 (built CSC) against compensated FP32 on the degree-ordered
-    view, 20 iterations each, median of 3 after a warm-up.  The ordered view
-    relabels the traversal view, so FP32C's one-time derivation is both
-    (tv_ms: the traversal view's, already derived for BFS); the second ratio
-    counts them in one 20-iteration run."""
+    view (the built CSC relabelled), 20 iterations each, median of 3 after a
+    warm-up.  The second ratio adds the view's one-time derivation to one
+    20-iteration run."""
     view_ms = g.prepare_pagerank_view()
     V = g.V
     out64, out32, scratch = eg.DeviceBuffer(V * 8), eg.DeviceBuffer(V * 8), eg.DeviceBuffer(V * 8)
@@ -262,19 +261,18 @@ def pagerank_block(eg, g, iters, E, tv_ms, oracle_pr=None):
            "ms_fp64": round(t64 * 1e3, 2), "ms_fp32c": round(t32 * 1e3, 2),
            "gteps_fp64": gt(t64), "gteps_fp32c": gt(t32),
            "speedup_fp32c": round(t64 / t32, 3),
-           "traversal_view_prepare_ms": round(tv_ms, 2),
-           "speedup_fp32c_view_counted": round(t64 / (t32 + (view_ms + tv_ms) * 1e-3), 3),
+           "speedup_fp32c_view_counted": round(t64 / (t32 + view_ms * 1e-3), 3),
            "l1_fp32c_vs_fp64_gpu": float(np.abs(x32 - x64).sum() / np.abs(x64).sum()),
            "gteps_fp64_ordered": gt(t64o), "gteps_fp32c_plain": gt(t32p),
-           "layouts": "fp64 = plain FP64 on the built CSC; fp32c = compensated FP32 on the degree-ordered view; "
-                      "view_counted adds both views' one-time derivation (traversal view, then its degree-ordered "
-                      "relabelling) to one 20-iteration FP32C run"}
+           "layouts": "fp64 = plain FP64 on the built CSC; fp32c = compensated FP32 on the degree-ordered view "
+                      "(the built CSC with vertices renumbered by descending out-degree inside each sub-partition's "
+                      "source range); view_counted adds the view's one-time derivation to one 20-iteration FP32C run"}
     if oracle_pr is not None:
         n1 = np.abs(oracle_pr).sum()
         out["l1_fp32c_vs_fp64_oracle"] = float(np.abs(x32 - oracle_pr).sum() / n1)
         out["l1_fp64_vs_fp64_oracle"] = float(np.abs(x64 - oracle_pr).sum() / n1)
     log(f"[bench] pagerank: fp64 {gt(t64)} GTEPS, fp32c {gt(t32)} GTEPS ({t64 / t32:.2f}x, "
-        f"{out['speedup_fp32c_view_counted']:.2f}x with the {tv_ms:.1f} + {view_ms:.1f} ms views)")
+        f"{out['speedup_fp32c_view_counted']:.2f}x with the {view_ms:.1f} ms view counted)")
     return out, x32
 
 
@@ -296,6 +294,62 @@ def bfs_block(eg, g, root):
 def cpu_block(name, seconds, work, unit_div, unit, cores):
     return {"value": round(work / seconds / unit_div, 3), "unit": unit, "s": round(seconds, 3), "cores": cores,
             "workload": name}
+
+
+def rocpd_summary(root, pattern=None):
+    """Markdown table of one profiled probe from rocprofv3's databases under
+    root: root/kt (--kernel-trace --stats) for calls and average durations,
+    and one directory per --pmc run for the counters, averaged per kernel
+    over its dispatches.  FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are
+    scaled to bytes by what they read for the probe's 1 GiB device copy
+    (k_copy16: 1 GiB read and 1 GiB written)."""
+    import glob
+    import re
+    import sqlite3
+
+    def short(n):
+        n = re.sub(r"\(anonymous namespace\)::", "", n)
+        n = re.sub(r"^void ", "", n)
+        return re.sub(r"^eg::", "", n).split("(")[0]
+
+    stats, vgprs = {}, {}
+    for db in glob.glob(os.path.join(root, "kt", "*.db"))[:1]:
+        con = sqlite3.connect(db)
+        for name, calls, tot, avg in con.execute("select name, total_calls, total_duration, average from top_kernels"):
+            stats[short(name)] = (calls, tot / 1e3, avg / 1e3)  # the view's durations are in us
+        for name, v, a in con.execute("select name, max(vgpr_count), max(accum_vgpr_count) from kernels group by name"):
+            vgprs[short(name)] = (v or 0) + (a or 0)
+    cnt = {}
+    for db in sorted(glob.glob(os.path.join(root, "*", "*.db"))):
+        if os.sep + "kt" + os.sep in db:
+            continue
+        con = sqlite3.connect(db)
+        q = ("select kernel_name, counter_name, avg(v) from (select dispatch_id, kernel_name, counter_name, "
+             "sum(value) v from counters_collection group by dispatch_id, counter_name) "
+             "group by kernel_name, counter_name")
+        for kn, cn, v in con.execute(q):
+            cnt.setdefault(short(kn), {})[cn] = v
+    copy = cnt.get("k_copy16", {})
+    fs = (1 << 30) / (copy["FETCH_SIZE"] * 1024) if copy.get("FETCH_SIZE") else None
+    ws = (1 << 30) / (copy["WRITE_SIZE"] * 1024) if copy.get("WRITE_SIZE") else None
+    lines = [f"calibration (k_copy16, 1 GiB read + 1 GiB written): FETCH_SIZE x {fs}, WRITE_SIZE x {ws}", "",
+             "| kernel | calls | avg ms | VGPRs | FETCH GB/dispatch | WRITE GB/dispatch | L2 hit % | VALU / wave | "
+             "VMEM rd / wave | wait / busy |", "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    fmt = lambda x, f="{:.3f}": f.format(x) if x is not None else "-"
+    for k in sorted(set(stats) | set(cnt), key=lambda k: -stats.get(k, (0, 0, 0))[1]):
+        if pattern and not re.search(pattern, k):
+            continue
+        c, st = cnt.get(k, {}), stats.get(k, (0, 0.0, 0.0))
+        f, w, h, m, waves = (c.get("FETCH_SIZE"), c.get("WRITE_SIZE"), c.get("TCC_HIT_sum"), c.get("TCC_MISS_sum"),
+                             c.get("SQ_WAVES"))
+        lines.append(
+            f"| `{k}` | {st[0]} | {st[2]:.3f} | {vgprs.get(k, '-')} | {fmt(f * 1024 * fs / 1e9 if f and fs else None)} | "
+            f"{fmt(w * 1024 * ws / 1e9 if w and ws else None)} | "
+            f"{fmt(100 * h / (h + m) if h is not None and m else None, '{:.1f}')} | "
+            f"{fmt(c['SQ_INSTS_VALU'] / waves if waves and c.get('SQ_INSTS_VALU') else None, '{:.0f}')} | "
+            f"{fmt(c['SQ_INSTS_VMEM_RD'] / waves if waves and c.get('SQ_INSTS_VMEM_RD') else None, '{:.0f}')} | "
+            f"{fmt(c['SQ_WAIT_INST_ANY'] / c['SQ_BUSY_CYCLES'] if c.get('SQ_BUSY_CYCLES') and c.get('SQ_WAIT_INST_ANY') else None, '{:.2f}')} |")
+    return "\n".join(lines)
 
 
 def twitter_block(eg, args, cores):
@@ -339,7 +393,7 @@ def twitter_block(eg, args, cores):
     ref, t_pr_cpu = timed(lambda: og.pagerank(20)[0])
     cpu_build = og.build_seconds
     del og
-    pr, _ = pagerank_block(eg, g, 20, E, tv_ms, oracle_pr=ref)
+    pr, _ = pagerank_block(eg, g, 20, E, oracle_pr=ref)
     out["pagerank"] = pr
     out["engine_ready_ms"] = round(tb * 1e3 + prep_ms + tv_ms + pr["view_prepare_ms"], 2)
     out["engine_ready_parts_ms"] = {"build": round(tb * 1e3, 2), "engine_prepare": round(prep_ms, 2),
@@ -392,6 +446,10 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: two builds of the workload, then a 1 GiB device copy (the "
                          "calibration of FETCH_SIZE / WRITE_SIZE); prints no bench line")
+    ap.add_argument("--rocpd-summary", metavar="DIR",
+                    help="print the markdown summary of a profiled probe's rocprofv3 databases under DIR "
+                         "(DIR/kt: kernel trace; DIR/<name>: one --pmc run each) and exit")
+    ap.add_argument("--kernels", metavar="REGEX", help="with --rocpd-summary: only kernels matching REGEX")
     ap.add_argument("--engine-probe", action="store_true",
                     help="for rocprofv3 runs: build the workload and its views, then two runs each of plain FP64 "
                          "and ordered FP32C PageRank (20 iterations) and of BFS from the max-out-degree vertex, "
@@ -399,6 +457,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default sort variant")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every host core this process may use")
     args = ap.parse_args()
+    if args.rocpd_summary:
+        print(rocpd_summary(args.rocpd_summary, args.kernels))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -435,12 +496,21 @@ def main():
         g.prepare_pagerank_view()
         root = int(np.argmax(g.out_degree()))
         out = eg.DeviceBuffer(V * 8)
+        pulls = {}
         for _ in range(2):
-            g.pagerank(20, eg.PR_FP64, out)
-            g.pagerank(20, eg.PR_FP32C, out)
-            g.bfs(root, eg.POLICY_AUTO, download=False)
+            for name, prec in (("fp64_plain", eg.PR_FP64), ("fp32c_ordered", eg.PR_FP32C)):
+                eg.profile_reset()
+                eg.profile_enable(True)
+                g.pagerank(20, prec, out)
+                eg.profile_enable(False)
+                ms, n, b = eg.profile_read("pr_pull")
+                pulls[name] = {"launches": n, "avg_ms": round(ms / n, 4), "algorithmic_bytes_per_launch": b // n,
+                               "gbs": round(b / n / (ms / n * 1e-3) / 1e9, 1)}
+            levels, trav = g.bfs(root, eg.POLICY_AUTO, download=False)
         del out
         eg.synchronize()
+        print(json.dumps({"probe": "engine", "workload": desc, "vertices": V, "edges": E, "pulls": pulls,
+                          "bfs": {"root": root, "levels": levels, "edges_traversed": trav}}), flush=True)
         a, b = eg.DeviceBuffer(1 << 30), eg.DeviceBuffer(1 << 30)
         eg.memcpy_d2d(b.ptr, a.ptr, 1 << 30)
         eg.synchronize()
@@ -544,7 +614,7 @@ def main():
             ref, t_pr = timed(lambda: og.pagerank(20)[0])
             cpu_extra["pagerank_fp64"] = cpu_block("oracle FP64 PageRank, 20 iterations", t_pr, 20 * E, 1e9,
                                                    "GTEPS", cores)
-        extra["pagerank"], _ = pagerank_block(eg, g, 20, E, tv_ms, oracle_pr=ref)
+        extra["pagerank"], _ = pagerank_block(eg, g, 20, E, oracle_pr=ref)
         extra["engine_ready_ms"] = round(ms_per_step + prep_ms + tv_ms + extra["pagerank"]["view_prepare_ms"], 2)
     del og
 

@@ -221,12 +221,12 @@ enum eg_policy { EG_POLICY_REFERENCE = 0, EG_POLICY_AUTO = 1, EG_POLICY_PUSH = 2
 enum eg_precision { EG_PR_FP64 = 0, EG_PR_FP32C = 1 };
 /* Layout flags for eg_pagerank, OR-ed into the precision (no reference
  * counterpart).  EG_PR_ORDERED runs the pull on a degree-ordered view:
- * vertices renumbered by descending out-degree so the most-gathered values
- * share cache lines; one row per destination, its in-edges in ascending
- * original source id (the traversal view relabelled: the same terms as the
- * CSC row, summed in another order).  EG_PR_PLAIN runs it on the built CSC
- * in original ids.  Defaults: FP64 plain, FP32C ordered.  The view (and the
- * traversal view it relabels) is derived on first use after a build. */
+ * vertices renumbered by descending out-degree inside each sub-partition's
+ * source range, so the most-gathered values share cache lines; the built
+ * CSC's rows with every id replaced by its new id (the same terms in the same
+ * order).  EG_PR_PLAIN runs it on the built CSC in original ids.  Defaults:
+ * FP64 plain, FP32C ordered.  The view is derived on first use after a
+ * build. */
 enum eg_pr_layout { EG_PR_ORDERED = 0x10, EG_PR_PLAIN = 0x20 };
 
 /* Derives the engine's view of the built arrays (SoA source/target ids,

@@ -204,6 +204,24 @@ def test_pagerank_fp64_and_fp32c(eg, oracle_lib, scale, P, S):
             assert np.allclose(d, ref_d, rtol=1e-9, atol=1e-15), name
 
 
+@pytest.mark.parametrize("P,S", [(1, 1), (3, 1), (2, 3)])
+@pytest.mark.parametrize("tv_first", [False, True])
+def test_pagerank_ordered_view(eg, oracle_lib, P, S, tv_first):
+    """The ordered view (the built CSC renumbered inside each sub-partition's
+    source range), derived before or after a traversal view, against the
+    FP64 oracle, with several sub-partitions per partition too."""
+    print(f"pagerank ordered view P={P} S={S} tv_first={tv_first}", flush=True)
+    edges, V = _rmat(eg, 13)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, P, S, m=20)
+    if tv_first:
+        g.prepare_traversal_view()
+    ref, ref_d = og.pagerank(20)
+    for prec, tol in ((eg.PR_FP32C, PR_FP32C_TOL), (eg.PR_FP64 | eg.PR_ORDERED, PR_FP64_TOL)):
+        x, d = g.pagerank(20, prec)
+        assert np.abs(x - ref).sum() / np.abs(ref).sum() <= tol
+    assert np.allclose(d, ref_d, rtol=1e-9, atol=1e-15)
+
+
 def test_pagerank_layout_flags_rejected(eg):
     edges = np.zeros(4, dtype=eg.REC8)
     edges["src"] = [0, 1, 2, 3]
