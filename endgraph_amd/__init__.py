@@ -145,6 +145,9 @@ def lib():
         "eg_get_engine_tune": ([vp, C.POINTER(EngineTune)], i32),
         "eg_set_engine_tune": ([vp, C.POINTER(EngineTune)], i32),
         "eg_autotune_engine": ([vp, i32, u64, C.POINTER(EngineTune), C.POINTER(f64), vp], i32),
+        "eg_prep_data": ([vp, C.POINTER(u64)], i32),
+        "eg_mode_sparse": ([vp, i32, i32, u64, C.POINTER(i32)], i32),
+        "eg_dense_messages": ([vp, i32, vp, vp, vp, C.POINTER(u64), vp], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -213,6 +216,7 @@ def to_host(d_ptr: int, dtype, count: int, stream=None) -> np.ndarray:
 
 
 POLICY_REFERENCE, POLICY_AUTO, POLICY_PUSH, POLICY_PULL = 0, 1, 2, 3
+PROGRAM_PAGERANK, PROGRAM_BFS, PROGRAM_CC, PROGRAM_SSSP = 0, 1, 2, 3
 PR_FP64, PR_FP32C = 0, 1
 # layout flags OR-ed into the precision of Graph.pagerank (defaults: FP64
 # plain, FP32C degree-ordered)
@@ -546,6 +550,26 @@ class Graph:
 
     # ------ one process_edges / process_vertices step at a time: the path
     # the C++ facade's Graph<EdgeData>::process_edges takes -------------------
+    def prep_data(self) -> int:
+        """Graph::prep_data (core/graph.hpp:129): bytes the reference's ranks would exchange while preprocessing."""
+        n = C.c_uint64()
+        check(lib().eg_prep_data(self._h, C.byref(n)))
+        return n.value
+
+    def mode_sparse(self, policy: int, program: int, active_edges: int) -> bool:
+        out = C.c_int()
+        check(lib().eg_mode_sparse(self._h, policy, program, active_edges, C.byref(out)))
+        return bool(out.value)
+
+    def dense_messages(self, program: int, active: DeviceBuffer = None, visited: DeviceBuffer = None,
+                       values: DeviceBuffer = None, stream=None) -> int:
+        """Messages a dense step of `program` sends between partitions (exe_data, core/graph.hpp:1736)."""
+        n = C.c_uint64()
+        ptr = lambda b: C.c_void_p(b.ptr) if b is not None else None
+        check(lib().eg_dense_messages(self._h, program, ptr(active), ptr(visited), ptr(values), C.byref(n),
+                                      _stream(stream)))
+        return n.value
+
     def bitmap_degree_sum(self, bm: DeviceBuffer, stream=None) -> int:
         """Sum of out-degrees over the set bits (the mode switch input, core/graph.hpp:1484-1490)."""
         s = C.c_uint64()

@@ -842,10 +842,10 @@ int build_finish(eg_graph* g, hipStream_t s) {
     std::vector<uint64_t> rank_start(P + 1);
     for (int r = 0; r < P; ++r) rank_start[r] = mult * (E0 / P) * r;
     rank_start[P] = E;
-    EG_TRY(g->blk.resize((uint64_t(P) + 1) * 8 + uint64_t(P) * (Q + 1) * 8 + 2 * (uint64_t(P) * Q + 1) * 8 + 64));
+    EG_TRY(g->blk.resize((uint64_t(P) + 1) * 8 + 2 * uint64_t(P) * (Q + 1) * 8 + 2 * (uint64_t(P) * Q + 1) * 8 + 64));
     uint64_t* d_rank = g->blk.as<uint64_t>();
-    uint64_t* bpos = d_rank + P + 1;
-    uint64_t* tab_dst = bpos + uint64_t(P) * (Q + 1);
+    uint64_t* bpos_dir = d_rank + P + 1;  // block bounds of each shuffle: by source, then by destination
+    uint64_t* tab_dst = bpos_dir + 2 * uint64_t(P) * (Q + 1);
     uint64_t* tab_src = tab_dst + uint64_t(P) * Q + 1;
     EG_HIP(hipMemcpyAsync(d_rank, rank_start.data(), (P + 1) * 8, hipMemcpyHostToDevice, s));
     EG_HIP(hipStreamSynchronize(s));
@@ -855,6 +855,7 @@ int build_finish(eg_graph* g, hipStream_t s) {
     for (int dir = 0; dir < 2; ++dir) {
       const bool pull = dir == 0;
       R* in = pull ? A : B;
+      uint64_t* bpos = bpos_dir + uint64_t(dir) * P * (Q + 1);
       const unsigned bb = unsigned(ceil_div(uint64_t(P) * (Q + 1), kNT));
       if (pull)
         hipLaunchKernelGGL((k_block_bounds<R, true>), dim3(bb), dim3(kNT), 0, s, (const R*)in, (const uint64_t*)d_rank,
@@ -942,10 +943,26 @@ int build_finish(eg_graph* g, hipStream_t s) {
   // --- small tables back to the host ---
   eg_graph::Meta h;
   uint32_t sort_err = 0;
+  std::vector<uint64_t> bpos_h(P > 1 ? 2 * uint64_t(P) * (Q + 1) : 0);
   EG_HIP(hipMemcpyAsync(&h, meta, sizeof(h), hipMemcpyDeviceToHost, s));
   EG_HIP(hipMemcpyAsync(&sort_err, g->sortw.err.ptr, 4, hipMemcpyDeviceToHost, s));
+  if (P > 1)
+    EG_HIP(hipMemcpyAsync(bpos_h.data(), g->blk.as<uint64_t>() + P + 1, bpos_h.size() * 8, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
   if (sort_err) return fail(EG_ERR_HIP, "radix sort look-back exceeded its spin limit");
+  // prep_data: every rank's allreduce of each degree array (V x 4 B; the
+  // build keeps out- and in-degrees, core/graph.hpp:716 counts the one it
+  // computes), and per shuffle the records of rank slice r owned by another
+  // partition (core/graph.hpp:906-908; block (r, q) holds rank r's records
+  // owned by sub-partition q, of partition q / S)
+  g->prep_data = 2 * uint64_t(P) * V * 4;
+  for (int dir = 0; dir < 2 && P > 1; ++dir)
+    for (int r = 0; r < P; ++r)
+      for (int q = 0; q < Q; ++q) {
+        const uint64_t* b = bpos_h.data() + uint64_t(dir) * P * (Q + 1) + uint64_t(r) * (Q + 1);
+        if (q / S != r) g->prep_data += (b[q + 1] - b[q]) * rb;
+      }
+  g->pr_cross_rows = ~0ull;
   g->page_offset.assign(h.page_offset, h.page_offset + P + 1);
   g->partition_offset.assign(h.partition_offset, h.partition_offset + P + 1);
   g->local_partition_offset.assign(h.local_partition_offset, h.local_partition_offset + P * (S + 1));

@@ -1594,6 +1594,108 @@ __global__ __launch_bounds__(kNT) void k_sssp_pull(uint64_t E, const uint32_t* _
       w);
 }
 
+// ------------------------------------------- messages between partitions --
+// exe_data of a dense step (core/graph.hpp:1736): the messages that the
+// dense signals of a partition's rows send to the other partitions' masters.
+// Per pull row, the value its program's dense signal tests: PageRank emits
+// always (pagerank.cpp:64-70), BFS when the destination is unvisited and a
+// source active (bfs.cpp:59-67), CC when a source's label is below the
+// destination id (cc.cpp:62-72), SSSP when a relaxation is below 1e9
+// (sssp.cpp:62-74).
+template <int kProg>
+__global__ __launch_bounds__(kNT) void k_msg_rows(uint64_t E, const uint32_t* __restrict__ src,
+                                                   const float* __restrict__ w, const uint64_t* __restrict__ rowflag,
+                                                   const uint32_t* __restrict__ wordrank,
+                                                   const uint64_t* __restrict__ active, const void* __restrict__ values,
+                                                   uint32_t* __restrict__ acc) {
+  auto put_min = [&](uint32_t row, uint32_t v, bool shared) {
+    if (shared)
+      atomicMin(&acc[row], v);
+    else
+      acc[row] = v;
+  };
+  if constexpr (kProg == EG_PROGRAM_SSSP) {
+    const float* dist = static_cast<const float*>(values);
+    row_reduce_wave<uint32_t>(
+        first_word(), E, src, rowflag, wordrank, 0xFFFFFFFFu, OpMin{},
+        [&](uint64_t, uint32_t s, float we) { return __float_as_uint(__fadd_rn(dist[s], we)); }, put_min, w);
+  } else if constexpr (kProg == EG_PROGRAM_CC) {
+    const uint32_t* label = static_cast<const uint32_t*>(values);
+    row_reduce_wave<uint32_t>(first_word(), E, src, rowflag, wordrank, 0xFFFFFFFFu, OpMin{},
+                              [&](uint64_t, uint32_t s) { return label[s]; }, put_min);
+  } else {
+    row_reduce_wave<uint32_t>(
+        first_word(), E, src, rowflag, wordrank, 0u, OpOr{},
+        [&](uint64_t, uint32_t s) {
+          return kProg == EG_PROGRAM_BFS ? uint32_t((active[s >> 6] >> (s & 63)) & 1ull) : 1u;
+        },
+        [&](uint32_t row, uint32_t v, bool shared) {
+          if (shared)
+            atomicOr(&acc[row], v);
+          else
+            acc[row] = v;
+        });
+  }
+}
+
+// Rows whose destination another partition owns and whose signal emits.
+template <int kProg>
+__global__ __launch_bounds__(kNT) void k_msg_count(uint64_t R, const uint32_t* __restrict__ acc,
+                                                    const uint32_t* __restrict__ row_dst,
+                                                    const uint32_t* __restrict__ row_start,
+                                                    const uint32_t* __restrict__ src, const uint16_t* __restrict__ qmap,
+                                                    int S, const uint64_t* __restrict__ visited,
+                                                    unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long wt[kNT / kWave];
+  unsigned long long n = 0;
+  for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < R; r += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t d = row_dst[r];
+    const uint32_t s0 = src[row_start[r]];
+    if (qmap[d >> kPageShift] / S == qmap[s0 >> kPageShift] / S) continue;
+    const uint32_t a = acc[r];
+    bool emits = true;
+    if constexpr (kProg == EG_PROGRAM_BFS) emits = a != 0 && !((visited[d >> 6] >> (d & 63)) & 1ull);
+    if constexpr (kProg == EG_PROGRAM_CC) emits = a < d;
+    if constexpr (kProg == EG_PROGRAM_SSSP) emits = __uint_as_float(a) < 1e9f;
+    n += emits ? 1 : 0;
+  }
+  n = wave_reduce(n, OpSum{});
+  if ((threadIdx.x & (kWave - 1)) == 0) wt[threadIdx.x / kWave] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kNT / kWave; ++i) t += wt[i];
+    atomicAdd(out, t);
+  }
+}
+
+template <int kProg>
+int count_messages(eg_graph* g, const uint64_t* active, const uint64_t* visited, const void* values, uint64_t* h_count,
+                   hipStream_t s) {
+  const uint64_t R = g->R;
+  EG_TRY(g->e_msg.resize((R + 1) * 4 + 64));
+  uint32_t* acc = g->e_msg.as<uint32_t>();
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(g->e_msg.as<char>() + ((R * 4 + 63) & ~63ull));
+  EG_TRY(fill<unsigned long long>(cnt, 1, 0ull, s));
+  if (R) {
+    const uint32_t id = kProg == EG_PROGRAM_SSSP || kProg == EG_PROGRAM_CC ? 0xFFFFFFFFu : 0u;
+    EG_TRY(fill<uint32_t>(acc, R, id, s));
+    hipLaunchKernelGGL((k_msg_rows<kProg>), dim3(row_grid(g->E)), dim3(kNT), 0, s, g->E,
+                       (const uint32_t*)g->e_csc_src.as<uint32_t>(), (const float*)g->e_csc_w.as<float>(),
+                       (const uint64_t*)g->e_rowflag.as<uint64_t>(), (const uint32_t*)g->e_wordrank.as<uint32_t>(),
+                       active, values, acc);
+    EG_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_msg_count<kProg>), dim3(grid_for(R, kNT, 4096)), dim3(kNT), 0, s, R, (const uint32_t*)acc,
+                       (const uint32_t*)g->e_row_dst.as<uint32_t>(), (const uint32_t*)g->e_row_start.as<uint32_t>(),
+                       (const uint32_t*)g->e_csc_src.as<uint32_t>(), (const uint16_t*)g->qmap.as<uint16_t>(), g->S,
+                       visited, cnt);
+    EG_HIP(hipGetLastError());
+  }
+  EG_HIP(hipMemcpyAsync(h_count, cnt, 8, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
 }  // namespace
 }  // namespace eg
 
@@ -1913,6 +2015,47 @@ int eg_pagerank(eg_graph* g, int iterations, int precision, double* d_out, doubl
   }
   EG_HIP(hipStreamSynchronize(s));
   return EG_OK;
+}
+
+int eg_prep_data(const eg_graph* g, uint64_t* h_bytes) {
+  if (!g || !h_bytes) return fail(EG_ERR_ARG, "eg_prep_data: NULL argument");
+  if (!g->built) return fail(EG_ERR_STATE, "graph has not been built");
+  *h_bytes = g->prep_data;
+  return EG_OK;
+}
+
+int eg_mode_sparse(const eg_graph* g, int policy, int program, uint64_t active_edges, int* h_sparse) {
+  if (!g || !h_sparse) return fail(EG_ERR_ARG, "eg_mode_sparse: NULL argument");
+  if (policy < EG_POLICY_REFERENCE || policy > EG_POLICY_PULL) return fail(EG_ERR_ARG, "unknown policy %d", policy);
+  if (program < EG_PROGRAM_PAGERANK || program > EG_PROGRAM_SSSP) return fail(EG_ERR_ARG, "unknown program %d", program);
+  *h_sparse = program == EG_PROGRAM_PAGERANK ? 0 : choose_sparse(g, policy, active_edges, program != EG_PROGRAM_CC);
+  return EG_OK;
+}
+
+int eg_dense_messages(eg_graph* g, int program, const uint64_t* d_active, const uint64_t* d_visited,
+                      const void* d_values, uint64_t* h_count, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EG_TRY(ensure_engine(g, s));
+  if (!h_count) return fail(EG_ERR_ARG, "eg_dense_messages: h_count is NULL");
+  switch (program) {
+    case EG_PROGRAM_PAGERANK:
+      if (g->pr_cross_rows == ~0ull) EG_TRY(count_messages<EG_PROGRAM_PAGERANK>(g, nullptr, nullptr, nullptr,
+                                                                               &g->pr_cross_rows, s));
+      *h_count = g->pr_cross_rows;
+      return EG_OK;
+    case EG_PROGRAM_BFS:
+      if (!d_active || !d_visited) return fail(EG_ERR_ARG, "BFS messages need the active and visited bitmaps");
+      return count_messages<EG_PROGRAM_BFS>(g, d_active, d_visited, nullptr, h_count, s);
+    case EG_PROGRAM_CC:
+      if (!d_values) return fail(EG_ERR_ARG, "CC messages need the labels");
+      return count_messages<EG_PROGRAM_CC>(g, nullptr, nullptr, d_values, h_count, s);
+    case EG_PROGRAM_SSSP:
+      if (g->rec != 12) return fail(EG_ERR_STATE, "SSSP needs a graph of EdgeType<float> (12-byte) records");
+      if (!d_values) return fail(EG_ERR_ARG, "SSSP messages need the distances");
+      return count_messages<EG_PROGRAM_SSSP>(g, nullptr, nullptr, d_values, h_count, s);
+    default:
+      return fail(EG_ERR_ARG, "unknown program %d", program);
+  }
 }
 
 int eg_pagerank_view_prepare(eg_graph* g, double* h_ms, void* stream) {

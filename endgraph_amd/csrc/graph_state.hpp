@@ -77,6 +77,10 @@ struct eg_graph {
   uint64_t E0 = 0;  // records in the input list
   uint64_t E = 0;   // records after symmetrisation
   int vbits = 1, qbits = 0;
+  // bytes the reference's ranks would exchange while preprocessing
+  // (prep_data, core/graph.hpp:716, :908)
+  uint64_t prep_data = 0;
+  uint64_t pr_cross_rows = ~0ull;    // pull rows whose destination another partition owns (~0: not counted yet)
 
   eg::DevBuf out_deg, in_deg;        // u32[V]
   eg::DevBuf page_prefix;            // u64[pages+1]: page weights, then their exclusive prefix
@@ -112,6 +116,7 @@ struct eg_graph {
   uint32_t e_hub_for[2] = {0, 0};    // (hub, hub_chunk) the list was made for
   EngineTune tune;
   eg::DevBuf e_scratch;              // frontier list, counters
+  eg::DevBuf e_msg;                  // per pull row values and a counter (eg_dense_messages)
   eg::DevBuf e_bm[3];                // three bitmaps of (V>>6)+1 words
   eg::DevBuf e_pr;                   // PageRank work vectors
   // degree-ordered PageRank view: vertices renumbered by descending

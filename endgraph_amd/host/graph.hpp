@@ -258,6 +258,7 @@ class Graph {
   R process_edges(const programs::PageRankPull& p, VertexSubset* /*active: all, dense*/,
                   VertexSubset* /*dense_selective*/ = nullptr) {
     ensure_engine();
+    if (partitions_num > 1) exe_data += 12 * dense_messages(EG_PROGRAM_PAGERANK, nullptr, nullptr, nullptr);
     if (p.curr32)
       check(eg_pr_pull(g_, EG_PR_FP32C, p.curr32, p.next, nullptr));
     else
@@ -268,24 +269,30 @@ class Graph {
   R process_edges(const programs::BfsLevel& p, VertexSubset* active, VertexSubset* /*dense_selective*/ = nullptr) {
     ensure_engine();
     uint64_t nv = 0, ne = 0;
-    check(eg_bfs_level(g_, policy, p.level, active->data, p.active_out->data, p.visited->data, p.parent, p.depth,
-                       active_edges(active), &nv, &ne, nullptr, nullptr));
+    const uint64_t ae = active_edges(active);
+    // bfs.cpp passes visited as dense_selective
+    count_step(EG_PROGRAM_BFS, ae, active, p.visited, nullptr, 8, true);
+    check(eg_bfs_level(g_, policy, p.level, active->data, p.active_out->data, p.visited->data, p.parent, p.depth, ae,
+                       &nv, &ne, nullptr, nullptr));
     return R(nv);
   }
   template <typename R, typename M>
   R process_edges(const programs::CcStep& p, VertexSubset* active, VertexSubset* /*dense_selective*/ = nullptr) {
     ensure_engine();
     uint64_t nv = 0, ne = 0;
-    check(eg_cc_iteration(g_, policy, active->data, p.active_out->data, p.label, active_edges(active), &nv, &ne,
-                          nullptr, nullptr));
+    const uint64_t ae = active_edges(active);
+    count_step(EG_PROGRAM_CC, ae, active, nullptr, p.label, 8, false);
+    check(eg_cc_iteration(g_, policy, active->data, p.active_out->data, p.label, ae, &nv, &ne, nullptr, nullptr));
     return R(nv);
   }
   template <typename R, typename M>
   R process_edges(const programs::SsspStep& p, VertexSubset* active, VertexSubset* /*dense_selective*/ = nullptr) {
     ensure_engine();
     uint64_t nv = 0, ne = 0;
-    check(eg_sssp_iteration(g_, policy, active->data, p.active_out->data, p.distance, active_edges(active), &nv, &ne,
-                            nullptr, nullptr));
+    const uint64_t ae = active_edges(active);
+    count_step(EG_PROGRAM_SSSP, ae, active, nullptr, p.distance, 8, false);
+    check(eg_sssp_iteration(g_, policy, active->data, p.active_out->data, p.distance, ae, &nv, &ne, nullptr,
+                            nullptr));
     return R(nv);
   }
 
@@ -327,7 +334,40 @@ class Graph {
     check(eg_graph_degree(g_, &degree_dev, &in_deg));
     degree.resize(vertices);
     check(eg_memcpy_d2h(degree.data(), degree_dev, uint64_t(vertices) * 4, nullptr));
-    prep_data = 0;
+    uint64_t bytes = 0;
+    check(eg_prep_data(g_, &bytes));
+    prep_data = bytes;
+  }
+
+  uint64_t dense_messages(int program, VertexSubset* active, VertexSubset* visited, const void* values) {
+    uint64_t n = 0;
+    check(eg_dense_messages(g_, program, active ? active->data : nullptr, visited ? visited->data : nullptr, values,
+                            &n, nullptr));
+    return n;
+  }
+
+  // exe_data of one step before it runs (core/graph.hpp:1560, :1671, :1736):
+  // sparse, each partition's messages (one per active vertex: every toolkit's
+  // sparse signal emits) counted once per other partition, as a count, as
+  // the reference adds it; dense, the dense_selective bitmap words sent to
+  // the other partitions, then the messages' bytes.  One partition sends
+  // nothing.
+  void count_step(int program, uint64_t ae, VertexSubset* active, VertexSubset* dense_selective, const void* values,
+                  uint64_t msg_bytes, bool bitmap) {
+    if (partitions_num <= 1) return;
+    int sparse = 0;
+    check(eg_mode_sparse(g_, policy, program, ae, &sparse));
+    if (sparse) {
+      exe_data += uint64_t(partitions_num - 1) * active->count();
+      return;
+    }
+    if (bitmap) {
+      std::vector<uint32_t> off(partitions_num + 1);
+      check(eg_graph_partition_offset(g_, off.data()));
+      for (int p = 0; p < partitions_num; ++p)
+        exe_data += uint64_t(partitions_num - 1) * ((off[p + 1] - off[p]) / 64) * sizeof(unsigned long);
+    }
+    exe_data += msg_bytes * dense_messages(program, active, dense_selective, values);
   }
 
   void ensure_engine() { check(eg_engine_prepare(g_, nullptr, nullptr)); }

@@ -45,7 +45,8 @@ void compute(Graph<Empty>* graph, UINT root) {
   for (UINT v_i = 0; v_i < graph->vertices_num; v_i++)
     if (par[v_i] < graph->vertices_num) found_vertices += 1;
   printf("found_vertices = %u\n", found_vertices);
-  printf("pre time: %lf s, exec time: %lf\n", graph->prep_time, exec_time);
+  printf("algo data: %lu MB, pre data: %lu MB\npre time: %lf s, exec time: %lf\n", graph->exe_data >> 20,
+         graph->prep_data >> 20, graph->prep_time, exec_time);
 
   graph->dealloc_vertex_array(parent);
   delete active_in;

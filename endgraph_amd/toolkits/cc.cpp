@@ -37,7 +37,8 @@ void compute(Graph<Empty>* graph) {
   for (UINT v_i = 0; v_i < graph->vertices_num; v_i++)
     if (count[v_i] > 0) components += 1;
   printf("components = %u\n", components);
-  printf("pre time: %lf s, exec time: %lf s \n", graph->prep_time, exec_time);
+  printf("algo data: %lu MB, pre data: %lu MB\npre time: %lf s, exec time: %lf s \n", graph->exe_data >> 20,
+         graph->prep_data >> 20, graph->prep_time, exec_time);
 
   graph->dealloc_vertex_array(label);
   delete active_in;

@@ -48,7 +48,8 @@ void compute(Graph<Empty>* graph, int iterations, bool fp32c) {
   }
   printf("pr_sum=%lf\n", pr_sum);
   printf("pr[%u]=%lf\n", max_v_i, pr[max_v_i]);
-  printf("pre time: %lf s exec time: %lf s\n", graph->prep_time, exec_time);
+  printf("algo data: %lu MB, pre data: %lu MB\npre time: %lf s exec time: %lf s\n", graph->exe_data >> 20,
+         graph->prep_data >> 20, graph->prep_time, exec_time);
 
   graph->dealloc_vertex_array(curr);
   graph->dealloc_vertex_array(next);

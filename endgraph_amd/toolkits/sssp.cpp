@@ -40,7 +40,8 @@ void compute(Graph<Weight>* graph, UINT root) {
   for (UINT v_i = 0; v_i < graph->vertices_num; v_i++)
     if (dist[v_i] < 1e9 && dist[v_i] > dist[max_v_i]) max_v_i = v_i;
   printf("distance[%u]=%f\n", max_v_i, dist[max_v_i]);
-  printf("pre time: %lfs, exec time: %lf s\n", graph->prep_time, exec_time);
+  printf("algo data: %lu MB, pre data: %lu MB\npre time: %lfs, exec time: %lf s\n", graph->exe_data >> 20,
+         graph->prep_data >> 20, graph->prep_time, exec_time);
 
   graph->dealloc_vertex_array(distance);
   delete active_in;

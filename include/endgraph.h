@@ -228,6 +228,33 @@ enum eg_precision { EG_PR_FP64 = 0, EG_PR_FP32C = 1 };
  * FP64 plain, FP32C ordered.  The view is derived on first use after a
  * build. */
 enum eg_pr_layout { EG_PR_ORDERED = 0x10, EG_PR_PLAIN = 0x20 };
+/* The four toolkit programs, for the mode rule and the message counts. */
+enum eg_program { EG_PROGRAM_PAGERANK = 0, EG_PROGRAM_BFS = 1, EG_PROGRAM_CC = 2, EG_PROGRAM_SSSP = 3 };
+
+/* The reference's byte counters (Graph::prep_data, Graph::exe_data,
+ * core/graph.hpp:128-129), printed by its toolkits as "algo data / pre data".
+ * The P partitions share one GPU here, so nothing is sent: these count what
+ * the reference's ranks would exchange.
+ * eg_prep_data: every rank's allreduce of each degree array (V x 4 B; the
+ * build keeps out- and in-degrees, :716 counts the one it computes) plus,
+ * for each shuffle (by source for the CSC, by destination for the CSR), the
+ * records of each rank slice owned by another partition (:906-908).
+ * eg_mode_sparse: whether a step of `program` with active_edges active
+ * out-edges runs sparse under `policy` (the rule every step entry point
+ * applies; PageRank is always dense).
+ * eg_dense_messages: the messages a dense step of `program` sends between
+ * partitions (:1736, MsgUnit of 4 + sizeof(message) bytes each): pull rows
+ * (one per sub-partition and destination) whose destination another
+ * partition owns and whose dense signal emits, given the state at the step's
+ * start: PageRank always (pagerank.cpp:64-70; counted once per build), BFS
+ * when the destination is not in d_visited and a source is in d_active
+ * (bfs.cpp:59-67), CC when a source's label (d_values, u32) is below the
+ * destination id (cc.cpp:62-72), SSSP when a source's distance (d_values,
+ * f32) plus the weight is below 1e9 (sssp.cpp:62-74).  Synchronises. */
+int eg_prep_data(const eg_graph* g, uint64_t* h_bytes);
+int eg_mode_sparse(const eg_graph* g, int policy, int program, uint64_t active_edges, int* h_sparse);
+int eg_dense_messages(eg_graph* g, int program, const uint64_t* d_active, const uint64_t* d_visited,
+                      const void* d_values, uint64_t* h_count, void* stream);
 
 /* Derives the engine's view of the built arrays (SoA source/target ids,
  * row-start bits, a CSR merged over sub-partitions).  Runs on the first
