@@ -305,12 +305,17 @@ int ensure_engine(eg_graph* g, hipStream_t s) {
 // times.  Renumbering vertices by descending out-degree (ties by id) packs
 // the most-read values into the fewest cache lines: on a twitter-shaped graph
 // the top 2^20 sources hold 85% of the edges, 4 MiB of FP32 values, where
-// the original ids spread them over one cache line each.  The renumbering
-// stays inside each sub-partition's source range, so a sub-partition's rows
-// still gather from one contiguous range (the locality that splitting rows
-// by sub-partition buys; merged rows fetched 1.7x the bytes on R-MAT 24 at
-// P = 8).  The view is the built CSC with every id replaced by its new id:
-// no sort of the edges, the CSC rows' terms in the CSC's order.
+// the original ids spread them over one cache line each.  The view keeps one
+// row per destination and replaces every id by its new id.  With one
+// sub-partition the rows are the built CSC's (no sort; the CSC's terms in
+// the CSC's order).  With several they are the traversal view's (each
+// destination's sub-partition rows merged, sources in ascending original
+// id): the CSC's rows would need an atomic add per row, one per
+// sub-partition that reaches a destination (R-MAT 24 at P = 8: 41 ms for 20
+// FP32C iterations on them against 36 ms on merged rows, though the merged
+// rows fetch 8.4 GB per pull against 2.2 GB).  Merged rows hold the CSC rows'
+// terms in another order, which moves an FP64 row sum by a few ulps and a
+// compensated FP32 row sum not at all at the FP32 rounding.
 struct KeyDstField {
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
@@ -346,24 +351,20 @@ __global__ void k_relabel(const uint32_t* __restrict__ in, uint64_t n, const uin
 
 template <class R>
 int view_impl(eg_graph* g, hipStream_t s) {
+  const bool from_csc = g->Q == 1;
+  if (!from_csc) EG_TRY(ensure_tview(g, s));
   const uint64_t E = g->E;
   const uint32_t V = g->V;
-  const uint64_t rows = g->R;
-  // the sub-partitions' source ranges (contiguous, in sub-partition order):
-  // new ids are assigned inside each
-  std::vector<uint64_t> segs(1, 0);
-  for (int p = 0; p < g->P; ++p)
-    for (int q = 1; q <= g->S; ++q) segs.push_back(g->local_partition_offset[uint64_t(p) * (g->S + 1) + q]);
-  for (size_t i = 1; i < segs.size(); ++i)
-    if (segs[i] < segs[i - 1]) return fail(EG_ERR_STATE, "internal: sub-partition ranges out of order");
-  if (segs.back() != V) return fail(EG_ERR_STATE, "internal: sub-partition ranges do not end at the vertex count");
+  const uint64_t rows = from_csc ? g->R : g->tv_R;
+  const uint32_t* in_src = from_csc ? g->e_csc_src.as<uint32_t>() : g->tv_src.as<uint32_t>();
+  const uint32_t* in_row_dst = from_csc ? g->e_row_dst.as<uint32_t>() : g->tv_row_dst.as<uint32_t>();
   EG_TRY(g->pv_perm.resize(uint64_t(V) * 4));
   EG_TRY(g->pv_inv.resize(uint64_t(V) * 4));
   EG_TRY(g->pv_deg.resize(uint64_t(V) * 4));
   EG_TRY(g->pv_src.resize((std::max<uint64_t>(E, 1) + 4) * 4));
   EG_TRY(g->pv_row_dst.resize((rows + 1) * 4));
-  // new ids: a stable sort of (v, E - deg) by the second field within each
-  // source range, in the build's sort temporaries (free after the build;
+  // new ids: a stable sort of (v, E - deg) by the second field, in the
+  // build's sort temporaries (free after the build and the traversal view;
   // growing them only, they keep the E records the traversal view's sort uses)
   const uint64_t vb = std::max<uint64_t>(V, 1) * 8;
   EG_TRY(g->t1.resize(vb));
@@ -371,28 +372,26 @@ int view_impl(eg_graph* g, hipStream_t s) {
   EG_TRY(g->t3.resize(vb));
   Rec8* ra = g->t1.as<Rec8>();
   Rec8* rb = g->t2.as<Rec8>();
-  EG_TRY(g->sortw.resize_segs(int(segs.size()) - 1));
+  EG_TRY(g->sortw.resize_segs(1));
   EG_TRY(fill<uint32_t>(g->sortw.err.as<uint32_t>(), 1, 0u, s));
   hipLaunchKernelGGL(k_deg_keys, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
                      (const uint32_t*)g->out_deg.as<uint32_t>(), V, E, ra);
   EG_HIP(hipGetLastError());
-  EG_TRY(rx_sort((const Rec8*)ra, rb, g->t3.as<Rec8>(), segs, KeyDstField{}, std::max(1, counting_bits(E)), g->sortw,
-                 s));
+  EG_TRY(rx_sort((const Rec8*)ra, rb, g->t3.as<Rec8>(), std::vector<uint64_t>{0, V}, KeyDstField{},
+                 std::max(1, counting_bits(E)), g->sortw, s));
   uint32_t* perm = g->pv_perm.as<uint32_t>();
   hipLaunchKernelGGL(k_view_perm, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s, (const Rec8*)rb, V,
                      (const uint32_t*)g->out_deg.as<uint32_t>(), perm, g->pv_inv.as<uint32_t>(),
                      g->pv_deg.as<uint32_t>());
   EG_HIP(hipGetLastError());
   if (E) {
-    hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(E, 4), kNT, 65536)), dim3(kNT), 0, s,
-                       (const uint32_t*)g->e_csc_src.as<uint32_t>(), E, (const uint32_t*)perm,
-                       g->pv_src.as<uint32_t>());
+    hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(E, 4), kNT, 65536)), dim3(kNT), 0, s, in_src, E,
+                       (const uint32_t*)perm, g->pv_src.as<uint32_t>());
     EG_HIP(hipGetLastError());
   }
   if (rows) {
-    hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(rows, 4), kNT, 65536)), dim3(kNT), 0, s,
-                       (const uint32_t*)g->e_row_dst.as<uint32_t>(), rows, (const uint32_t*)perm,
-                       g->pv_row_dst.as<uint32_t>());
+    hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(rows, 4), kNT, 65536)), dim3(kNT), 0, s, in_row_dst, rows,
+                       (const uint32_t*)perm, g->pv_row_dst.as<uint32_t>());
     EG_HIP(hipGetLastError());
   }
   g->pv_R = uint32_t(rows);
@@ -1821,9 +1820,12 @@ PullView plain_view(const eg_graph* g) {
                   g->e_rowflag.as<uint64_t>(), g->Q > 1};
 }
 
-PullView ordered_view(const eg_graph* g) {  // the built CSC's rows, relabelled
-  return PullView{g->E, g->pv_R, g->pv_src.as<uint32_t>(), g->e_wordrank.as<uint32_t>(),
-                  g->pv_row_dst.as<uint32_t>(), g->e_rowflag.as<uint64_t>(), g->Q > 1};
+PullView ordered_view(const eg_graph* g) {  // one row per destination, relabelled
+  const bool from_csc = g->Q == 1;
+  const DevBuf& wr = from_csc ? g->e_wordrank : g->tv_wordrank;
+  const DevBuf& rf = from_csc ? g->e_rowflag : g->tv_rowflag;
+  return PullView{g->E, g->pv_R, g->pv_src.as<uint32_t>(), wr.as<uint32_t>(), g->pv_row_dst.as<uint32_t>(),
+                  rf.as<uint64_t>(), 0};
 }
 
 int pr_pull(const PullView& pv, uint32_t V, int precision, const void* x, double* next, hipStream_t s) {

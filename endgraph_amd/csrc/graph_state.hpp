@@ -128,8 +128,8 @@ struct eg_graph {
   eg::DevBuf pv_perm;                // u32[V]  old id -> new id
   eg::DevBuf pv_inv;                 // u32[V]  new id -> old id
   eg::DevBuf pv_deg;                 // u32[V]  out-degree by new id
-  eg::DevBuf pv_src;                 // u32[E + 4] new source id of every edge of the CSC
-  eg::DevBuf pv_row_dst;             // u32[R]  new destination id of each CSC row
+  eg::DevBuf pv_src;                 // u32[E + 4] new source id of every edge of the view's rows
+  eg::DevBuf pv_row_dst;             // u32[R]  new destination id of each row
   // traversal view for the batched (64-root) programs: the CSC merged over
   // sub-partitions with each destination's in-edges sorted by source
   bool tv_ready = false;

@@ -221,12 +221,14 @@ enum eg_policy { EG_POLICY_REFERENCE = 0, EG_POLICY_AUTO = 1, EG_POLICY_PUSH = 2
 enum eg_precision { EG_PR_FP64 = 0, EG_PR_FP32C = 1 };
 /* Layout flags for eg_pagerank, OR-ed into the precision (no reference
  * counterpart).  EG_PR_ORDERED runs the pull on a degree-ordered view:
- * vertices renumbered by descending out-degree inside each sub-partition's
- * source range, so the most-gathered values share cache lines; the built
- * CSC's rows with every id replaced by its new id (the same terms in the same
+ * vertices renumbered by descending out-degree, so the most-gathered values
+ * share cache lines; one row per destination.  With P = S = 1 its rows are
+ * the built CSC's, relabelled (the same terms in the same order, no sort);
+ * otherwise the traversal view's, relabelled (a destination's in-edges
+ * merged in ascending original source id: the CSC rows' terms in another
  * order).  EG_PR_PLAIN runs it on the built CSC in original ids.  Defaults:
- * FP64 plain, FP32C ordered.  The view is derived on first use after a
- * build. */
+ * FP64 plain, FP32C ordered.  The view (and, with P*S > 1, the traversal
+ * view it relabels) is derived on first use after a build. */
 enum eg_pr_layout { EG_PR_ORDERED = 0x10, EG_PR_PLAIN = 0x20 };
 /* The four toolkit programs, for the mode rule and the message counts. */
 enum eg_program { EG_PROGRAM_PAGERANK = 0, EG_PROGRAM_BFS = 1, EG_PROGRAM_CC = 2, EG_PROGRAM_SSSP = 3 };

@@ -207,9 +207,9 @@ def test_pagerank_fp64_and_fp32c(eg, oracle_lib, scale, P, S):
 @pytest.mark.parametrize("P,S", [(1, 1), (3, 1), (2, 3)])
 @pytest.mark.parametrize("tv_first", [False, True])
 def test_pagerank_ordered_view(eg, oracle_lib, P, S, tv_first):
-    """The ordered view (the built CSC renumbered inside each sub-partition's
-    source range), derived before or after a traversal view, against the
-    FP64 oracle, with several sub-partitions per partition too."""
+    """The ordered view (the built CSC's rows relabelled at P = S = 1, the
+    traversal view's otherwise), derived before or after a traversal view,
+    against the FP64 oracle, with several sub-partitions per partition too."""
     print(f"pagerank ordered view P={P} S={S} tv_first={tv_first}", flush=True)
     edges, V = _rmat(eg, 13)
     g, og, _ = _pair(eg, oracle_lib, edges, V, P, S, m=20)
