@@ -672,14 +672,16 @@ inline uint64_t slice_start(uint64_t E0, int P, uint64_t mult, int r) {
   return r >= P ? mult * E0 : mult * (E0 / P) * uint64_t(r);
 }
 
-// Stable sort of the rank slices r0..r1-1 (one segment each) by key.
-template <class R, class KeyFn>
-int rank_sorts(eg_graph* g, const R* L, R* out, R* tmp, uint64_t E0, int P, uint64_t mult, int r0, int r1, KeyFn key,
-               int bits, hipStream_t s) {
+// Stable sorts of the rank slices r0..r1-1 (one segment each) by source
+// into a and by destination into b; the two sorts share their first read of
+// L (the first upsweep counts both first digits).
+template <class R, class KeyA, class KeyB>
+int rank_sorts(eg_graph* g, const R* L, R* a, R* b, R* tmp, uint64_t E0, int P, uint64_t mult, int r0, int r1,
+               KeyA key_a, KeyB key_b, int bits, hipStream_t s) {
   const uint64_t b0 = slice_start(E0, P, mult, r0);
   std::vector<uint64_t> seg(r1 - r0 + 1);
   for (int r = r0; r <= r1; ++r) seg[r - r0] = slice_start(E0, P, mult, r) - b0;
-  return rx_sort(L + b0, out + b0, tmp + b0, seg, key, bits, g->sortw, s);
+  return rx_sort_pair(L + b0, a + b0, b + b0, tmp + b0, seg, key_a, key_b, bits, g->sortw, s);
 }
 
 template <class R, class KeyFn>
@@ -778,8 +780,9 @@ int build_slices(eg_graph* g, const R* d_in, uint64_t E0, int sym, int r0, int r
   R* A = g->t1.as<R>();
   R* B = g->t2.as<R>();
   R* T = P > 1 ? g->t3.as<R>() : g->csc_list.as<R>();  // ping-pong of the first sorts
-  EG_TRY(rank_sorts(g, L, A, T, E0, P, mult, r0, r1, KeyChecked<true>{g->V, g->err.as<uint32_t>()}, g->vbits, s));
-  EG_TRY(rank_sorts(g, L, B, T, E0, P, mult, r0, r1, KeyChecked<false>{g->V, nullptr}, g->vbits, s));  // checked above
+  // the ids are range-checked once, by the source sort's first upsweep
+  EG_TRY(rank_sorts(g, L, A, B, T, E0, P, mult, r0, r1, KeyChecked<true>{g->V, g->err.as<uint32_t>()},
+                    KeyChecked<false>{g->V, nullptr}, g->vbits, s));
   return EG_OK;
 }
 

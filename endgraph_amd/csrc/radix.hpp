@@ -24,6 +24,7 @@
 // select_sort_variant (see eg_autotune_sort).
 #pragma once
 
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -293,16 +294,26 @@ __device__ __forceinline__ void rx_dstart(int nb, uint32_t tot, uint32_t* wt, ui
 // Histogram order is free, so a full tile of 8-byte records at an even
 // record address is read as 16-byte pairs; each wave counts into its own
 // LDS histogram (fewer lanes of a skewed digit on one counter), and the
-// wave histograms are added at the end.
-template <int NT, int BM, class R, class KeyFn>
+// wave histograms are added at the end.  With a second key (Key2 other than
+// RxNoKey) the same read also counts the first digit of a second sort of
+// the same records into hist2 (two sorts of one list share their first
+// upsweep).
+struct RxNoKey {};
+
+template <int NT, int BM, class R, class KeyFn, class Key2 = RxNoKey>
 __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __restrict__ seg,
                                               const uint32_t* __restrict__ ts, int nseg, KeyFn key, int shift,
-                                              int bits, uint32_t* __restrict__ hist, int xcd_order) {
+                                              int bits, uint32_t* __restrict__ hist, int xcd_order, Key2 key2 = {},
+                                              uint32_t* __restrict__ hist2 = nullptr) {
   constexpr int TILE = NT * kRxIPT;
   constexpr int W = NT / kWave;
+  constexpr bool kTwo = !std::is_same<Key2, RxNoKey>::value;
   __shared__ uint32_t h[W][1 << BM];
+  __shared__ uint32_t h2[kTwo ? W : 1][kTwo ? (1 << BM) : 1];
   const int nb = 1 << bits;
   for (int i = threadIdx.x; i < W * nb; i += NT) h[i / nb][i % nb] = 0;
+  if constexpr (kTwo)
+    for (int i = threadIdx.x; i < W * nb; i += NT) h2[i / nb][i % nb] = 0;
   __syncthreads();
   const uint32_t t = xcd_order ? rx_block_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const RxTile x = rx_tile<TILE>(seg, ts, nseg, t);
@@ -333,14 +344,22 @@ __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __rest
   for (int k = 0; k < kRxIPT; ++k) {
     const uint32_t d = uint32_t(key(r[k]) >> shift) & mask;
     // full tiles count every record; otherwise record k of this thread is li
-    if (pairs || uint32_t(k) * NT + threadIdx.x < x.count) atomicAdd(&hw[d], 1u);
+    const bool live = pairs || uint32_t(k) * NT + threadIdx.x < x.count;
+    if (live) atomicAdd(&hw[d], 1u);
+    if constexpr (kTwo)
+      if (live) atomicAdd(&h2[threadIdx.x / kWave][uint32_t(key2(r[k]) >> shift) & mask], 1u);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < nb; d += NT) {
-    uint32_t c = 0;
+    uint32_t c = 0, c2 = 0;
 #pragma unroll
-    for (int ww = 0; ww < W; ++ww) c += h[ww][d];
-    hist[uint64_t(ts[x.s]) * nb + uint64_t(d) * x.nt + x.lt] = c;
+    for (int ww = 0; ww < W; ++ww) {
+      c += h[ww][d];
+      if constexpr (kTwo) c2 += h2[ww][d];
+    }
+    const uint64_t at = uint64_t(ts[x.s]) * nb + uint64_t(d) * x.nt + x.lt;
+    hist[at] = c;
+    if constexpr (kTwo) hist2[at] = c2;
   }
 }
 
@@ -524,7 +543,7 @@ SortVariant& sort_variant(int rec_bytes);
 
 // Device workspace (grow-only, owned by the caller).
 struct RxWork {
-  DevBuf table, ghist, status, ticket, segs, err, scan, tshift;
+  DevBuf table, table2, ghist, status, ticket, segs, err, scan, tshift;
   uint32_t epoch = 0;
   int resize_segs(int nseg) {
     EG_TRY(segs.resize((uint64_t(nseg) + 1) * 12 + 16));
@@ -534,9 +553,14 @@ struct RxWork {
   }
 };
 
-template <int NT, int BM, class R, class KeyFn>
+// First pass of a three-launch sort: its own upsweep, its own upsweep that
+// also counts a second sort's first digit into w.table2 (key2), or no
+// upsweep, the counts being in w.table2 already.
+enum RxFirst { kFirstOwn = 0, kFirstAlsoSecond = 1, kFirstCounted = 2 };
+
+template <int NT, int BM, class R, class KeyFn, class Key2 = RxNoKey>
 int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, int scheme,
-               RxWork& w, hipStream_t s) {
+               RxWork& w, hipStream_t s, RxFirst first = kFirstOwn, Key2 key2 = {}) {
   static_assert((1 << BM) <= NT, "one digit per thread in the count/scan steps");
   static_assert(NT * kRxIPT * sizeof(R) + (NT / kWave) * (4 << BM) + (8 << BM) <= 160 * 1024, "LDS of one block");
   static_assert((NT / kWave) * (8 << BM) <= NT * kRxIPT * sizeof(R), "peer tables fit the staging buffer");
@@ -566,21 +590,32 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
     const int xcd = scheme == kThreeLaunchXcd;
     const uint64_t tn = uint64_t(tiles) << kRxMaxBits;
     EG_TRY(w.table.resize(tn * 4));
+    if (first == kFirstAlsoSecond) EG_TRY(w.table2.resize(tn * 4));
+    if (first == kFirstCounted && w.table2.bytes < tn * 4)
+      return fail(EG_ERR_STATE, "internal: the first digit's counts were not made for this sort");
     if (w.scan.bytes < scan_tmp_elems(tn) * 4) EG_TRY(w.scan.resize(scan_tmp_elems(tn) * 4));
     for (int p = 0; p < plan.passes; ++p) {
       R* dst = ((plan.passes - 1 - p) % 2 == 0) ? out : tmp;
-      prof_begin(s);
-      hipLaunchKernelGGL((k_rx_up<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, (const uint64_t*)d_seg,
-                         (const uint32_t*)d_ts, nseg, p == 0 ? key : quiet, plan.shift[p], plan.bits[p],
-                         w.table.as<uint32_t>(), xcd);
-      prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
-      EG_HIP(hipGetLastError());
-      EG_TRY(exclusive_scan_u32(w.table.as<uint32_t>(), w.table.as<uint32_t>(), uint64_t(tiles) << plan.bits[p],
-                                w.scan.as<uint32_t>(), s));
+      uint32_t* tab = p == 0 && first == kFirstCounted ? w.table2.as<uint32_t>() : w.table.as<uint32_t>();
+      if (p == 0 && first == kFirstAlsoSecond) {
+        prof_begin(s);
+        hipLaunchKernelGGL((k_rx_up<NT, BM, R, KeyFn, Key2>), dim3(tiles), dim3(NT), 0, s, src,
+                           (const uint64_t*)d_seg, (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p],
+                           tab, xcd, key2, w.table2.as<uint32_t>());
+        prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
+        EG_HIP(hipGetLastError());
+      } else if (!(p == 0 && first == kFirstCounted)) {
+        prof_begin(s);
+        hipLaunchKernelGGL((k_rx_up<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, (const uint64_t*)d_seg,
+                           (const uint32_t*)d_ts, nseg, p == 0 ? key : quiet, plan.shift[p], plan.bits[p], tab, xcd,
+                           RxNoKey{}, nullptr);
+        prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
+        EG_HIP(hipGetLastError());
+      }
+      EG_TRY(exclusive_scan_u32(tab, tab, uint64_t(tiles) << plan.bits[p], w.scan.as<uint32_t>(), s));
       prof_begin(s);
       hipLaunchKernelGGL((k_rx_down<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, (const uint64_t*)d_seg,
-                         (const uint32_t*)d_ts, nseg, quiet, plan.shift[p], plan.bits[p],
-                         (const uint32_t*)w.table.as<uint32_t>(), xcd);
+                         (const uint32_t*)d_ts, nseg, quiet, plan.shift[p], plan.bits[p], (const uint32_t*)tab, xcd);
       prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
       EG_HIP(hipGetLastError());
       src = RxIn<R>{dst, nullptr, nullptr, nullptr, 0};
@@ -649,6 +684,40 @@ template <class R, class KeyFn>
 int rx_sort(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w,
             hipStream_t s) {
   return rx_sort_with(sort_variant(int(sizeof(R))), in, out, tmp, seg, key, key_bits, w, s);
+}
+
+// Two stable sorts of the same records (plain input) by key_a into out_a and
+// by key_b into out_b, each as rx_sort would do it.  Three-launch variants
+// read the input once for both first digits (the first key's first upsweep
+// also counts the second's); the one-pass variant runs the two sorts.
+template <int NT, int BM, class R, class KeyA, class KeyB>
+int rx_sort_pair_nt(const R* in, R* out_a, R* out_b, R* tmp, const std::vector<uint64_t>& seg, KeyA key_a,
+                    KeyB key_b, int key_bits, int scheme, RxWork& w, hipStream_t s) {
+  const RxIn<R> src{in, nullptr, nullptr, nullptr, 0};
+  if (scheme == kOnePass) {
+    EG_TRY((rx_sort_nt<NT, BM>(src, out_a, tmp, seg, key_a, key_bits, scheme, w, s)));
+    return rx_sort_nt<NT, BM>(src, out_b, tmp, seg, key_b, key_bits, scheme, w, s);
+  }
+  EG_TRY((rx_sort_nt<NT, BM>(src, out_a, tmp, seg, key_a, key_bits, scheme, w, s, kFirstAlsoSecond,
+                            rx_quiet(key_b))));
+  return rx_sort_nt<NT, BM>(src, out_b, tmp, seg, key_b, key_bits, scheme, w, s, kFirstCounted);
+}
+
+template <class R, class KeyA, class KeyB>
+int rx_sort_pair(const R* in, R* out_a, R* out_b, R* tmp, const std::vector<uint64_t>& seg, KeyA key_a, KeyB key_b,
+                 int key_bits, RxWork& w, hipStream_t s) {
+  const SortVariant v = sort_variant(int(sizeof(R)));
+  const int nseg = int(seg.size()) - 1;
+  if (nseg < 1 || seg.back() == seg.front()) return EG_OK;
+  if (seg.front() != 0) return fail(EG_ERR_ARG, "internal: sort segments must start at 0");
+  if (seg.back() >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "sort of more than 2^32-1 records");
+  if (v.nt == 256) return rx_sort_pair_nt<256, 8>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+  if constexpr (sizeof(R) <= 8) {
+    if (v.nt >= 1024)
+      return rx_sort_pair_nt<1024, 8>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+  }
+  if (v.bits >= 9) return rx_sort_pair_nt<512, 9>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+  return rx_sort_pair_nt<512, 8>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
 }
 
 // Device workspace a three-launch sort of these segments takes with tiles of
