@@ -129,6 +129,7 @@ def lib():
         "eg_pr_apply": ([vp, vp, vp, vp, i32, vp, vp], i32),
         "eg_pagerank": ([vp, i32, i32, vp, vp, vp], i32),
         "eg_pagerank_view_prepare": ([vp, C.POINTER(f64), vp], i32),
+        "eg_pagerank_view_prepare_layout": ([vp, i32, C.POINTER(f64), vp], i32),
         "eg_traversal_view_prepare": ([vp, C.POINTER(f64), vp], i32),
         "eg_bfs_level": ([vp, i32, u32, vp, vp, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64),
                           C.POINTER(i32), vp], i32),
@@ -220,7 +221,7 @@ PROGRAM_PAGERANK, PROGRAM_BFS, PROGRAM_CC, PROGRAM_SSSP = 0, 1, 2, 3
 PR_FP64, PR_FP32C = 0, 1
 # layout flags OR-ed into the precision of Graph.pagerank (defaults: FP64
 # plain, FP32C degree-ordered)
-PR_ORDERED, PR_PLAIN = 0x10, 0x20
+PR_ORDERED, PR_PLAIN, PR_SORTED = 0x10, 0x20, 0x40
 
 
 def set_sort_variant(rec_bytes: int, scheme: int, threads: int, digit_bits: int = 8):
@@ -509,10 +510,11 @@ class Graph:
         check(lib().eg_engine_prepare(self._h, C.byref(ms), _stream(stream)))
         return ms.value
 
-    def prepare_pagerank_view(self, stream=None) -> float:
-        """Derive the degree-ordered PageRank view now; returns its wall ms."""
+    def prepare_pagerank_view(self, layout: int = PR_ORDERED, stream=None) -> float:
+        """Derive the degree-ordered PageRank view now (PR_ORDERED: the graph's
+        default form; PR_SORTED: the sorted-row form); returns its wall ms."""
         ms = C.c_double(0)
-        check(lib().eg_pagerank_view_prepare(self._h, C.byref(ms), _stream(stream)))
+        check(lib().eg_pagerank_view_prepare_layout(self._h, layout, C.byref(ms), _stream(stream)))
         return ms.value
 
     def prepare_traversal_view(self, stream=None) -> float:

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <type_traits>
 
 #include "graph_state.hpp"
 #include "rowreduce.hpp"
@@ -306,16 +308,21 @@ int ensure_engine(eg_graph* g, hipStream_t s) {
 // the most-read values into the fewest cache lines: on a twitter-shaped graph
 // the top 2^20 sources hold 85% of the edges, 4 MiB of FP32 values, where
 // the original ids spread them over one cache line each.  The view keeps one
-// row per destination and replaces every id by its new id.  With one
-// sub-partition the rows are the built CSC's (no sort; the CSC's terms in
-// the CSC's order).  With several they are the traversal view's (each
-// destination's sub-partition rows merged, sources in ascending original
-// id): the CSC's rows would need an atomic add per row, one per
-// sub-partition that reaches a destination (R-MAT 24 at P = 8: 41 ms for 20
-// FP32C iterations on them against 36 ms on merged rows, though the merged
-// rows fetch 8.4 GB per pull against 2.2 GB).  Merged rows hold the CSC rows'
-// terms in another order, which moves an FP64 row sum by a few ulps and a
-// compensated FP32 row sum not at all at the FP32 rounding.
+// row per destination, ids renumbered, in one of two forms:
+//   relabelled CSC (P = S = 1)  the built CSC's rows with every id replaced
+//       by its new id: one gather per edge to build, the CSC's terms in the
+//       CSC's order;
+//   sorted rows  the renumbered graph's rows with sources ascending by new
+//       id (a row copy and one sort by destination to build): the lanes of a
+//       gather then share cache lines, and the vector memory unit takes one
+//       line per cycle (twitter-shaped FP32C pull 8.2 -> 7.5 ms, R-MAT 24
+//       P = 8 1.67 -> 1.62 ms; the build of the view 17.5 -> 50 ms and 13 ->
+//       9 ms against the relabelled CSC, resp. the relabelled traversal view).
+// The default is the relabelled CSC with one sub-partition and sorted rows
+// otherwise (the CSC's rows there are one per sub-partition and destination,
+// which would need an atomic add per row); EG_PR_SORTED asks for sorted rows.
+// Both hold the CSC rows' terms; a different order moves an FP64 row sum by
+// a few ulps and a compensated FP32 row sum not at all at the FP32 rounding.
 struct KeyDstField {
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
@@ -349,23 +356,22 @@ __global__ void k_relabel(const uint32_t* __restrict__ in, uint64_t n, const uin
   if (t < n - n4 * 4) out[n4 * 4 + t] = perm[in[n4 * 4 + t]];
 }
 
-template <class R>
-int view_impl(eg_graph* g, hipStream_t s) {
-  const bool from_csc = g->Q == 1;
-  if (!from_csc) EG_TRY(ensure_tview(g, s));
+int sorted_rows(eg_graph* g, hipStream_t s);
+
+// The view in form kViewCsc (P = S = 1 only: the built CSC's rows, every id
+// relabelled) or kViewSorted (rows sorted by new source id).
+enum ViewForm { kViewNone = -1, kViewCsc = 0, kViewSorted = 1 };
+
+int view_impl(eg_graph* g, int form, hipStream_t s) {
   const uint64_t E = g->E;
   const uint32_t V = g->V;
-  const uint64_t rows = from_csc ? g->R : g->tv_R;
-  const uint32_t* in_src = from_csc ? g->e_csc_src.as<uint32_t>() : g->tv_src.as<uint32_t>();
-  const uint32_t* in_row_dst = from_csc ? g->e_row_dst.as<uint32_t>() : g->tv_row_dst.as<uint32_t>();
   EG_TRY(g->pv_perm.resize(uint64_t(V) * 4));
   EG_TRY(g->pv_inv.resize(uint64_t(V) * 4));
   EG_TRY(g->pv_deg.resize(uint64_t(V) * 4));
   EG_TRY(g->pv_src.resize((std::max<uint64_t>(E, 1) + 4) * 4));
-  EG_TRY(g->pv_row_dst.resize((rows + 1) * 4));
   // new ids: a stable sort of (v, E - deg) by the second field, in the
-  // build's sort temporaries (free after the build and the traversal view;
-  // growing them only, they keep the E records the traversal view's sort uses)
+  // build's sort temporaries (free after the build; growing them only, they
+  // keep the E records the traversal view's sort uses)
   const uint64_t vb = std::max<uint64_t>(V, 1) * 8;
   EG_TRY(g->t1.resize(vb));
   EG_TRY(g->t2.resize(vb));
@@ -384,17 +390,26 @@ int view_impl(eg_graph* g, hipStream_t s) {
                      (const uint32_t*)g->out_deg.as<uint32_t>(), perm, g->pv_inv.as<uint32_t>(),
                      g->pv_deg.as<uint32_t>());
   EG_HIP(hipGetLastError());
-  if (E) {
-    hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(E, 4), kNT, 65536)), dim3(kNT), 0, s, in_src, E,
-                       (const uint32_t*)perm, g->pv_src.as<uint32_t>());
-    EG_HIP(hipGetLastError());
+  if (form == kViewSorted) {
+    EG_TRY(sorted_rows(g, s));
+  } else {
+    if (g->Q != 1) return fail(EG_ERR_STATE, "internal: the relabelled-CSC view needs one sub-partition");
+    const uint64_t rows = g->R;
+    EG_TRY(g->pv_row_dst.resize((rows + 1) * 4));
+    if (E) {
+      hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(E, 4), kNT, 65536)), dim3(kNT), 0, s,
+                         (const uint32_t*)g->e_csc_src.as<uint32_t>(), E, (const uint32_t*)perm,
+                         g->pv_src.as<uint32_t>());
+      EG_HIP(hipGetLastError());
+    }
+    if (rows) {
+      hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(rows, 4), kNT, 65536)), dim3(kNT), 0, s,
+                         (const uint32_t*)g->e_row_dst.as<uint32_t>(), rows, (const uint32_t*)perm,
+                         g->pv_row_dst.as<uint32_t>());
+      EG_HIP(hipGetLastError());
+    }
+    g->pv_R = uint32_t(rows);
   }
-  if (rows) {
-    hipLaunchKernelGGL(k_relabel, dim3(grid_for(ceil_div(rows, 4), kNT, 65536)), dim3(kNT), 0, s, in_row_dst, rows,
-                       (const uint32_t*)perm, g->pv_row_dst.as<uint32_t>());
-    EG_HIP(hipGetLastError());
-  }
-  g->pv_R = uint32_t(rows);
   uint32_t sort_err = 0;
   EG_HIP(hipMemcpyAsync(&sort_err, g->sortw.err.ptr, 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
@@ -402,13 +417,21 @@ int view_impl(eg_graph* g, hipStream_t s) {
   return EG_OK;
 }
 
-int ensure_view(eg_graph* g, hipStream_t s) {
+// The ordered view of eg_pagerank's layout flags: EG_PR_SORTED the sorted-row
+// form, EG_PR_ORDERED (or the FP32C default) the graph's default form.
+int view_form_of(const eg_graph* g, int layout) {
+  if (layout == EG_PR_SORTED || g->Q > 1) return kViewSorted;
+  return kViewCsc;
+}
+
+int ensure_view(eg_graph* g, int form, hipStream_t s) {
   EG_TRY(ensure_engine(g, s));
-  if (g->pv_ready) return EG_OK;
+  if (g->pv_ready && g->pv_form == form) return EG_OK;
+  g->pv_ready = false;
   auto t0 = std::chrono::steady_clock::now();
-  int rc = g->rec == 8 ? view_impl<Rec8>(g, s) : view_impl<Rec12>(g, s);
-  if (rc != EG_OK) return rc;
+  EG_TRY(view_impl(g, form, s));
   g->pv_prepare_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  g->pv_form = form;
   g->pv_ready = true;
   return EG_OK;
 }
@@ -619,6 +642,13 @@ __device__ __forceinline__ uint32_t last_le_u32(const uint32_t* a, uint32_t lo, 
   return lo;
 }
 
+// An Op with `kSlot` takes (list index, vertex, edge, slot) instead of
+// (vertex, edge).
+template <class Op, class = void>
+struct ExpandSlot : std::false_type {};
+template <class Op>
+struct ExpandSlot<Op, std::void_t<decltype(Op::kSlot)>> : std::true_type {};
+
 template <class Op>
 __global__ __launch_bounds__(kNT) void k_expand(const uint32_t* __restrict__ flist, const uint32_t* __restrict__ foff,
                                                 uint32_t fn, const uint32_t* __restrict__ out_ptr, Op op) {
@@ -638,7 +668,7 @@ __global__ __launch_bounds__(kNT) void k_expand(const uint32_t* __restrict__ fli
   if (in_lds)
     for (uint32_t i = threadIdx.x; i < hi - lo + 2; i += kNT) win[i] = foff[lo + i];
   __syncthreads();
-  uint32_t u[8], e[8];
+  uint32_t u[8], e[8], ix[8];
   bool ok[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -653,12 +683,114 @@ __global__ __launch_bounds__(kNT) void k_expand(const uint32_t* __restrict__ fli
       i = last_le_u32(foff, lo, hi + 1, ok[k] ? t : t0);
       start = foff[i];
     }
+    ix[k] = i;
     u[k] = flist[i];
     e[k] = out_ptr[u[k]] + ((ok[k] ? t : t0) - start);
   }
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    if (ok[k]) op(u[k], e[k]);
+  for (int k = 0; k < 8; ++k) {
+    if (!ok[k]) continue;
+    if constexpr (ExpandSlot<Op>::value)
+      op(ix[k], u[k], e[k], t0 + uint32_t(k) * kNT + threadIdx.x);
+    else
+      op(u[k], e[k]);
+  }
+}
+
+// Sorted-row form: the renumbered graph's rows, sources ascending by new id
+// inside each row, so the lanes of one gather touch neighbouring lines (the
+// vector memory unit takes one cache line per cycle).  The merged CSR's rows
+// are copied in new-id order as (new source, destination) records (the slot
+// of an out-edge is its new source's offset plus its rank), then sorted
+// stably by destination.
+struct RowCopyOp {
+  static constexpr bool kSlot = true;
+  const uint32_t* out_dst;
+  Rec8* rec;
+  __device__ __forceinline__ void operator()(uint32_t i, uint32_t, uint32_t e, uint32_t t) const {
+    rec[t] = Rec8{i, out_dst[e]};
+  }
+};
+
+__global__ void k_copy_u32_v(const uint32_t* __restrict__ a, uint32_t* __restrict__ b, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    b[i] = a[i];
+}
+
+__global__ __launch_bounds__(kNT) void k_view_soa(const Rec8* __restrict__ lst, uint64_t E, uint32_t* __restrict__ src,
+                                                   uint64_t* __restrict__ rowflag, uint32_t* __restrict__ cnt) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t nwords = ceil_div(E, 64);
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t wd = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave; wd < nwords; wd += nw) {
+    const uint64_t e = wd * 64 + lane;
+    bool head = false;
+    if (e < E) {
+      const Rec8 r = lst[e];
+      src[e] = r.src;
+      head = e == 0 || lst[e - 1].dst != r.dst;
+    }
+    const uint64_t m = __ballot(head);
+    if (lane == 0) {
+      rowflag[wd] = m;
+      cnt[wd] = uint32_t(__popcll(m));
+    }
+  }
+}
+
+__global__ void k_view_row_dst(const Rec8* __restrict__ lst, uint64_t E, const uint64_t* __restrict__ rowflag,
+                               const uint32_t* __restrict__ wordrank, const uint32_t* __restrict__ perm,
+                               uint32_t* __restrict__ row_dst) {
+  for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t f = rowflag[e >> 6];
+    if ((f >> (e & 63)) & 1ull)
+      row_dst[wordrank[e >> 6] + uint32_t(__popcll(f & ((1ull << (e & 63)) - 1)))] = perm[lst[e].dst];
+  }
+}
+
+// Sorted-row form from the new ids (perm, inv, pv_deg made by view_impl).
+int sorted_rows(eg_graph* g, hipStream_t s) {
+  const uint64_t E = g->E;
+  const uint32_t V = g->V;
+  if (!E) return EG_OK;
+  const uint64_t nwords = ceil_div(E, 64);
+  uint32_t* noff = g->e_scratch.as<uint32_t>();  // (V + 1) * 4 bytes or more: out-degree prefix in new order
+  hipLaunchKernelGGL(k_copy_u32_v, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
+                     (const uint32_t*)g->pv_deg.as<uint32_t>(), noff, uint64_t(V));
+  EG_HIP(hipGetLastError());
+  EG_TRY(fill<uint32_t>(noff + V, 1, 0u, s));
+  EG_TRY(exclusive_scan_u32(noff, noff, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
+  EG_TRY(g->t1.resize(E * 8));
+  EG_TRY(g->t2.resize(E * 8));
+  EG_TRY(g->t3.resize(E * 8));
+  Rec8* rec = g->t1.as<Rec8>();
+  Rec8* sorted = g->t2.as<Rec8>();
+  hipLaunchKernelGGL((k_expand<RowCopyOp>), dim3(unsigned(ceil_div(E, kExpSlots))), dim3(kNT), 0, s,
+                     (const uint32_t*)g->pv_inv.as<uint32_t>(), (const uint32_t*)noff, V,
+                     (const uint32_t*)g->e_out_ptr.as<uint32_t>(), RowCopyOp{g->e_out_dst.as<uint32_t>(), rec});
+  EG_HIP(hipGetLastError());
+  EG_TRY(g->sortw.resize_segs(1));
+  EG_TRY(fill<uint32_t>(g->sortw.err.as<uint32_t>(), 1, 0u, s));
+  EG_TRY(rx_sort((const Rec8*)rec, sorted, g->t3.as<Rec8>(), std::vector<uint64_t>{0, E}, KeyDstField{}, g->vbits,
+                 g->sortw, s));
+  EG_TRY(g->pv_rowflag.resize((nwords + 1) * 8));
+  EG_TRY(g->pv_wordrank.resize((nwords + 2) * 4));
+  uint32_t* wr = g->pv_wordrank.as<uint32_t>();
+  hipLaunchKernelGGL(k_view_soa, dim3(grid_for(nwords, kNT / kWave, 65536)), dim3(kNT), 0, s, (const Rec8*)sorted, E,
+                     g->pv_src.as<uint32_t>(), g->pv_rowflag.as<uint64_t>(), wr);
+  EG_HIP(hipGetLastError());
+  EG_TRY(fill<uint32_t>(wr + nwords, 1, 0u, s));
+  EG_TRY(exclusive_scan_u32(wr, wr, nwords + 1, g->scan_tmp.as<uint32_t>(), s));
+  uint32_t R = 0;
+  EG_HIP(hipMemcpyAsync(&R, wr + nwords, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  EG_TRY(g->pv_row_dst.resize((uint64_t(R) + 1) * 4));
+  hipLaunchKernelGGL(k_view_row_dst, dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, (const Rec8*)sorted, E,
+                     (const uint64_t*)g->pv_rowflag.as<uint64_t>(), (const uint32_t*)wr,
+                     (const uint32_t*)g->pv_perm.as<uint32_t>(), g->pv_row_dst.as<uint32_t>());
+  EG_HIP(hipGetLastError());
+  g->pv_R = R;
+  return EG_OK;
 }
 
 // Sparse slot of toolkits/bfs.cpp:52-58 with R12: an unvisited out-neighbour
@@ -1821,9 +1953,9 @@ PullView plain_view(const eg_graph* g) {
 }
 
 PullView ordered_view(const eg_graph* g) {  // one row per destination, relabelled
-  const bool from_csc = g->Q == 1;
-  const DevBuf& wr = from_csc ? g->e_wordrank : g->tv_wordrank;
-  const DevBuf& rf = from_csc ? g->e_rowflag : g->tv_rowflag;
+  const bool sorted = g->pv_form == kViewSorted;
+  const DevBuf& wr = sorted ? g->pv_wordrank : g->e_wordrank;
+  const DevBuf& rf = sorted ? g->pv_rowflag : g->e_rowflag;
   return PullView{g->E, g->pv_R, g->pv_src.as<uint32_t>(), wr.as<uint32_t>(), g->pv_row_dst.as<uint32_t>(),
                   rf.as<uint64_t>(), 0};
 }
@@ -1976,10 +2108,10 @@ int eg_pagerank(eg_graph* g, int iterations, int precision, double* d_out, doubl
   const int prec = precision & 0xF;
   const int layout = precision & ~0xF;
   if (prec != EG_PR_FP64 && prec != EG_PR_FP32C) return fail(EG_ERR_ARG, "unknown precision %d", prec);
-  if (layout != 0 && layout != EG_PR_ORDERED && layout != EG_PR_PLAIN)
-    return fail(EG_ERR_ARG, "precision flags: at most one of EG_PR_ORDERED, EG_PR_PLAIN");
-  const bool ordered = layout == EG_PR_ORDERED || (layout == 0 && prec == EG_PR_FP32C);
-  if (ordered) EG_TRY(ensure_view(g, s));
+  if (layout != 0 && layout != EG_PR_ORDERED && layout != EG_PR_PLAIN && layout != EG_PR_SORTED)
+    return fail(EG_ERR_ARG, "precision flags: at most one of EG_PR_ORDERED, EG_PR_PLAIN, EG_PR_SORTED");
+  const bool ordered = layout == EG_PR_ORDERED || layout == EG_PR_SORTED || (layout == 0 && prec == EG_PR_FP32C);
+  if (ordered) EG_TRY(ensure_view(g, view_form_of(g, layout), s));
   const PullView pv = ordered ? ordered_view(g) : plain_view(g);
   const uint32_t* deg = ordered ? g->pv_deg.as<uint32_t>() : g->out_deg.as<uint32_t>();
   Scratch sc;
@@ -2061,7 +2193,14 @@ int eg_dense_messages(eg_graph* g, int program, const uint64_t* d_active, const 
 }
 
 int eg_pagerank_view_prepare(eg_graph* g, double* h_ms, void* stream) {
-  EG_TRY(ensure_view(g, static_cast<hipStream_t>(stream)));
+  return eg_pagerank_view_prepare_layout(g, EG_PR_ORDERED, h_ms, stream);
+}
+
+int eg_pagerank_view_prepare_layout(eg_graph* g, int layout, double* h_ms, void* stream) {
+  if (!g) return fail(EG_ERR_ARG, "graph is NULL");
+  if (layout != EG_PR_ORDERED && layout != EG_PR_SORTED)
+    return fail(EG_ERR_ARG, "view layout must be EG_PR_ORDERED or EG_PR_SORTED");
+  EG_TRY(ensure_view(g, view_form_of(g, layout), static_cast<hipStream_t>(stream)));
   if (h_ms) *h_ms = g->pv_prepare_ms;
   return EG_OK;
 }

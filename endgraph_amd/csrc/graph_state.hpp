@@ -130,6 +130,9 @@ struct eg_graph {
   eg::DevBuf pv_deg;                 // u32[V]  out-degree by new id
   eg::DevBuf pv_src;                 // u32[E + 4] new source id of every edge of the view's rows
   eg::DevBuf pv_row_dst;             // u32[R]  new destination id of each row
+  eg::DevBuf pv_rowflag;             // u64[ceil(E/64)+1] row starts of the sorted-row form (the other form uses the engine's)
+  eg::DevBuf pv_wordrank;            // u32[ceil(E/64)+2]
+  int pv_form = -1;                  // the form pv_ready holds: 0 relabelled CSC rows, 1 rows sorted by new source id
   // traversal view for the batched (64-root) programs: the CSC merged over
   // sub-partitions with each destination's in-edges sorted by source
   bool tv_ready = false;

@@ -223,13 +223,15 @@ enum eg_precision { EG_PR_FP64 = 0, EG_PR_FP32C = 1 };
  * counterpart).  EG_PR_ORDERED runs the pull on a degree-ordered view:
  * vertices renumbered by descending out-degree, so the most-gathered values
  * share cache lines; one row per destination.  With P = S = 1 its rows are
- * the built CSC's, relabelled (the same terms in the same order, no sort);
- * otherwise the traversal view's, relabelled (a destination's in-edges
- * merged in ascending original source id: the CSC rows' terms in another
- * order).  EG_PR_PLAIN runs it on the built CSC in original ids.  Defaults:
- * FP64 plain, FP32C ordered.  The view (and, with P*S > 1, the traversal
- * view it relabels) is derived on first use after a build. */
-enum eg_pr_layout { EG_PR_ORDERED = 0x10, EG_PR_PLAIN = 0x20 };
+ * the built CSC's, relabelled (the same terms in the same order; one gather
+ * per edge to derive); otherwise the renumbered graph's rows with sources
+ * ascending by new id.  EG_PR_SORTED asks for that sorted-row form on any
+ * graph: costlier to derive (a row copy and a sort by destination) and
+ * faster per pull, since a gather's lanes share cache lines.  EG_PR_PLAIN
+ * runs the pull on the built CSC in original ids.  Defaults: FP64 plain,
+ * FP32C ordered.  The view is derived on first use after a build, and again
+ * when another form is asked for. */
+enum eg_pr_layout { EG_PR_ORDERED = 0x10, EG_PR_PLAIN = 0x20, EG_PR_SORTED = 0x40 };
 /* The four toolkit programs, for the mode rule and the message counts. */
 enum eg_program { EG_PROGRAM_PAGERANK = 0, EG_PROGRAM_BFS = 1, EG_PROGRAM_CC = 2, EG_PROGRAM_SSSP = 3 };
 
@@ -295,6 +297,9 @@ int eg_pr_apply(eg_graph* g, const double* d_x, double* d_next, float* d_next32,
                 void* stream);
 int eg_pagerank(eg_graph* g, int iterations, int precision, double* d_out, double* h_deltas, void* stream);
 int eg_pagerank_view_prepare(eg_graph* g, double* h_ms, void* stream);
+/* The same for layout EG_PR_ORDERED (the graph's default form) or
+ * EG_PR_SORTED (the sorted-row form). */
+int eg_pagerank_view_prepare_layout(eg_graph* g, int layout, double* h_ms, void* stream);
 
 /* BFS, toolkits/bfs.cpp:22-112.  Parent rule (no reference counterpart: the
  * reference's CAS race leaves it open): the smallest frontier id among a

@@ -194,6 +194,9 @@ def test_pagerank_fp64_and_fp32c(eg, oracle_lib, scale, P, S):
         "fp64 ordered": (eg.PR_FP64 | eg.PR_ORDERED, PR_FP64_TOL),
         "fp32c ordered": (eg.PR_FP32C, PR_FP32C_TOL),
         "fp32c plain": (eg.PR_FP32C | eg.PR_PLAIN, PR_FP32C_TOL),
+        "fp32c sorted rows": (eg.PR_FP32C | eg.PR_SORTED, PR_FP32C_TOL),
+        "fp64 sorted rows": (eg.PR_FP64 | eg.PR_SORTED, PR_FP64_TOL),
+        "fp32c ordered again": (eg.PR_FP32C, PR_FP32C_TOL),
     }
     for name, (prec, tol) in runs.items():
         x, d = g.pagerank(20, prec)
@@ -207,9 +210,10 @@ def test_pagerank_fp64_and_fp32c(eg, oracle_lib, scale, P, S):
 @pytest.mark.parametrize("P,S", [(1, 1), (3, 1), (2, 3)])
 @pytest.mark.parametrize("tv_first", [False, True])
 def test_pagerank_ordered_view(eg, oracle_lib, P, S, tv_first):
-    """The ordered view (the built CSC's rows relabelled at P = S = 1, the
-    traversal view's otherwise), derived before or after a traversal view,
-    against the FP64 oracle, with several sub-partitions per partition too."""
+    """The ordered view (the built CSC's rows relabelled at P = S = 1, rows
+    sorted by new source id otherwise), derived before or after a traversal
+    view, against the FP64 oracle, with several sub-partitions per partition
+    too."""
     print(f"pagerank ordered view P={P} S={S} tv_first={tv_first}", flush=True)
     edges, V = _rmat(eg, 13)
     g, og, _ = _pair(eg, oracle_lib, edges, V, P, S, m=20)
@@ -231,6 +235,8 @@ def test_pagerank_layout_flags_rejected(eg):
     g.build(buf.ptr, 4)
     with pytest.raises(eg.EndGraphError):
         g.pagerank(3, eg.PR_FP64 | eg.PR_ORDERED | eg.PR_PLAIN)
+    with pytest.raises(eg.EndGraphError):
+        g.pagerank(3, eg.PR_FP32C | eg.PR_SORTED | eg.PR_PLAIN)
     with pytest.raises(eg.EndGraphError):
         g.pagerank(3, 5)
 
