@@ -136,13 +136,9 @@ struct eg_graph {
   // traversal view for the batched (64-root) programs: the CSC merged over
   // sub-partitions with each destination's in-edges sorted by source
   bool tv_ready = false;
-  uint64_t tv_R = 0;                 // rows: destinations with in-edges
   eg::DevBuf tv_ptr;                 // u32[V+1] row pointer (prefix of in-degree)
   eg::DevBuf tv_src;                 // u32[E + 4] (4 words of padding)
   eg::DevBuf tv_w;                   // f32[E] (12-byte records only)
-  eg::DevBuf tv_rowflag;             // u64[ceil(E/64)]: bit e set iff edge e starts a row
-  eg::DevBuf tv_wordrank;            // u32[ceil(E/64)+1]: rows started before word w
-  eg::DevBuf tv_row_dst;             // u32[R] destination of each row
   eg::DevBuf tv_items;               // u64 (v << 32 | chunk): in-edge chunks of at most 512 edges
   uint64_t tv_nitems = 0;
   eg::DevBuf tv_big;                 // first chunks of the vertices with more than kTvSmallRow in-edges

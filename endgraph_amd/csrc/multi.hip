@@ -119,43 +119,14 @@ struct KeyDstOnly {
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
 };
 
-// SoA sources (and weights) of the sorted list and its row-start bits (a
-// row = one destination), one 64-edge word per wave step.
+// SoA sources (and weights) of the sorted list.
 template <class R>
 __global__ __launch_bounds__(kNT) void k_tv_soa(const R* __restrict__ lst, uint64_t E, uint32_t* __restrict__ src,
-                                                float* __restrict__ w, uint64_t* __restrict__ rowflag) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t nwords = ceil_div(E, 64);
-  const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
-  for (uint64_t wd = gw; wd < nwords; wd += nw) {
-    const uint64_t e = wd * 64 + lane;
-    bool head = false;
-    if (e < E) {
-      const R r = lst[e];
-      src[e] = r.src;
-      if constexpr (sizeof(R) == 12) w[e] = reinterpret_cast<const Rec12&>(r).w;
-      head = e == 0 || lst[e - 1].dst != r.dst;
-    }
-    const uint64_t m = __ballot(head);
-    if (lane == 0) rowflag[wd] = m;
-  }
-}
-
-__global__ void k_tv_popc(const uint64_t* __restrict__ words, uint64_t n, uint32_t* __restrict__ cnt) {
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-    cnt[i] = uint32_t(__popcll(words[i]));
-}
-
-// Rows are the destinations with in-edges, ascending: row r's destination is
-// the r-th vertex with a nonzero in-degree, at the rank its row-start bit has.
-__global__ void k_tv_row_dst(const uint32_t* __restrict__ ptr, uint32_t V, const uint64_t* __restrict__ rowflag,
-                             const uint32_t* __restrict__ wordrank, uint32_t* __restrict__ row_dst) {
-  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t e = ptr[v];
-    if (ptr[v + 1] == e) continue;
-    const uint64_t f = rowflag[e >> 6];
-    row_dst[wordrank[e >> 6] + uint32_t(__popcll(f & ((1ull << (e & 63)) - 1)))] = uint32_t(v);
+                                                float* __restrict__ w) {
+  for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += uint64_t(gridDim.x) * blockDim.x) {
+    const R r = lst[e];
+    src[e] = r.src;
+    if constexpr (sizeof(R) == 12) w[e] = reinterpret_cast<const Rec12&>(r).w;
   }
 }
 
@@ -224,11 +195,6 @@ int tview_impl(eg_graph* g, hipStream_t s) {
                        static_cast<uint64_t*>(l.items->ptr));
     EG_HIP(hipGetLastError());
   }
-  const uint64_t nwords = ceil_div(E, 64);
-  EG_TRY(g->tv_rowflag.resize((nwords + 1) * 8));
-  EG_TRY(g->tv_wordrank.resize((nwords + 2) * 4));
-  uint64_t* rowflag = g->tv_rowflag.as<uint64_t>();
-  uint32_t* wordrank = g->tv_wordrank.as<uint32_t>();
   if (E) {
     // the build's sort temporaries (t1, t2: E records each) are free after it
     EG_TRY(g->sortw.resize_segs(1));
@@ -236,22 +202,10 @@ int tview_impl(eg_graph* g, hipStream_t s) {
     R* sorted = g->t1.as<R>();
     EG_TRY(rx_sort(g->csr_list.as<R>(), sorted, g->t2.as<R>(), std::vector<uint64_t>{0, E}, KeyDstOnly{}, g->vbits,
                    g->sortw, s));
-    hipLaunchKernelGGL((k_tv_soa<R>), dim3(grid_for(nwords, kNT / kWave)), dim3(kNT), 0, s, (const R*)sorted, E,
-                       g->tv_src.as<uint32_t>(), g->tv_w.as<float>(), rowflag);
-    EG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_tv_popc, dim3(grid_for(nwords)), dim3(kNT), 0, s, (const uint64_t*)rowflag, nwords, wordrank);
+    hipLaunchKernelGGL((k_tv_soa<R>), dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, (const R*)sorted, E,
+                       g->tv_src.as<uint32_t>(), g->tv_w.as<float>());
     EG_HIP(hipGetLastError());
   }
-  EG_TRY(fill_t<uint32_t>(wordrank + nwords, 1, 0u, s));
-  EG_TRY(exclusive_scan_u32(wordrank, wordrank, nwords + 1, g->scan_tmp.as<uint32_t>(), s));
-  uint32_t R_h = 0;
-  EG_HIP(hipMemcpyAsync(&R_h, wordrank + nwords, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  g->tv_R = R_h;
-  EG_TRY(g->tv_row_dst.resize((uint64_t(R_h) + 1) * 4));
-  hipLaunchKernelGGL(k_tv_row_dst, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)ptr, V,
-                     (const uint64_t*)rowflag, (const uint32_t*)wordrank, g->tv_row_dst.as<uint32_t>());
-  EG_HIP(hipGetLastError());
   if (E) {
     uint32_t err = 0;
     EG_HIP(hipMemcpyAsync(&err, g->sortw.err.ptr, 4, hipMemcpyDeviceToHost, s));
