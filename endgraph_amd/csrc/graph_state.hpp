@@ -120,8 +120,9 @@ struct eg_graph {
   eg::DevBuf e_bm[3];                // three bitmaps of (V>>6)+1 words
   eg::DevBuf e_pr;                   // PageRank work vectors
   // degree-ordered PageRank view: vertices renumbered by descending
-  // out-degree (ties by id); the traversal view's rows (and row bits) with
-  // every source and destination replaced by its new id
+  // out-degree (ties by id); either the built CSC's rows with every source
+  // and destination replaced by its new id (row bits shared with the
+  // engine's), or the merged rows in new-id order with sources ascending
   bool pv_ready = false;
   uint32_t pv_R = 0;
   double pv_prepare_ms = 0;
