@@ -361,6 +361,7 @@ int sorted_rows(eg_graph* g, hipStream_t s);
 // The view in form kViewCsc (P = S = 1 only: the built CSC's rows, every id
 // relabelled) or kViewSorted (rows sorted by new source id).
 enum ViewForm { kViewNone = -1, kViewCsc = 0, kViewSorted = 1 };
+constexpr int kHotNT = 1024;  // threads per block of the FP32C pull with LDS-staged hot values
 
 int view_impl(eg_graph* g, int form, hipStream_t s) {
   const uint64_t E = g->E;
@@ -432,6 +433,17 @@ int ensure_view(eg_graph* g, int form, hipStream_t s) {
   EG_TRY(view_impl(g, form, s));
   g->pv_prepare_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   g->pv_form = form;
+  // The FP32C pull's LDS staging: as many hot values as one block's LDS
+  // holds (160 KB on gfx950: 40,960 sources), one block per CU (its 1024
+  // threads at the kernel's register count leave no room for a second).
+  int dev = 0, cus = 0, lds = 0;
+  EG_HIP(hipGetDevice(&dev));
+  EG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  EG_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const uint64_t waves = ceil_div(ceil_div(g->E, 64), kRowWordsPerWave);
+  g->pv_hot = uint32_t(std::min<uint64_t>(g->V, uint64_t(std::max(lds, 0)) / 4));
+  g->pv_hot_grid = unsigned(std::min<uint64_t>(ceil_div(waves, kHotNT / kWave), uint64_t(std::max(cus, 1))));
+  if (!g->E) g->pv_hot = 0;
   g->pv_ready = true;
   return EG_OK;
 }
@@ -497,6 +509,47 @@ __global__ __launch_bounds__(kNT) void k_pr_pull_f32c(uint64_t E, const uint32_t
         else
           *p = t;
       });
+}
+
+// The same pull on the degree-ordered view, where new ids [0, nhot) are the
+// highest out-degrees (on the twitter-shaped graph the 40,960 hottest of
+// 41.6 M sources carry about 70% of the edges): their values are staged once
+// per block in LDS, and a gather of one of them is an LDS read instead of a
+// cache-line request to the L2.  A persistent grid (one kHotNT-thread block
+// per CU, as many hot values as the LDS holds) whose waves take the K-word
+// ranges in turn; each range is reduced exactly as k_pr_pull_f32c does.
+__global__ __launch_bounds__(kHotNT) void k_pr_pull_f32c_hot(uint64_t E, const uint32_t* __restrict__ src,
+                                                             const uint64_t* __restrict__ rowflag,
+                                                             const uint32_t* __restrict__ wordrank,
+                                                             const uint32_t* __restrict__ row_dst,
+                                                             const float* __restrict__ curr,
+                                                             double* __restrict__ next, int all_atomic,
+                                                             uint32_t nhot, uint64_t nranges) {
+  extern __shared__ float hot[];
+  for (uint32_t i = threadIdx.x; i < nhot; i += kHotNT) hot[i] = curr[i];
+  __syncthreads();
+  constexpr uint64_t kWpb = kHotNT / kWave;
+  for (uint64_t rg = uint64_t(blockIdx.x) * kWpb + threadIdx.x / kWave; rg < nranges; rg += uint64_t(gridDim.x) * kWpb)
+    row_reduce_wave<F2, kRowWordsPerWave, NoEdgeValue, true>(
+        rg * kRowWordsPerWave, E, src, rowflag, wordrank, F2{0.f, 0.f}, OpF2{},
+        [&](uint64_t, uint32_t s) {
+          // Both loads in every lane, without branches: hot lanes read
+          // curr[0] (one line per wave), the others hot[0].  The value is
+          // a sum of the two, one of them +0 (exact): a select of two loads
+          // would be folded into one flat load through the selected address.
+          const bool h = s < nhot;
+          const float a = hot[h ? s : 0u];
+          const float b = curr[h ? 0u : s];
+          return F2{(h ? a : 0.f) + (h ? 0.f : b), 0.f};
+        },
+        [&](uint32_t row, F2 v, bool shared) {
+          double* p = &next[row_dst[row]];
+          const double t = double(v.s) + double(v.c);
+          if (shared || all_atomic)
+            atomicAdd(p, t);
+          else
+            *p = t;
+        });
 }
 
 // process_vertices of toolkits/pagerank.cpp:36-47 (init) and :79-98 (apply).
@@ -1944,12 +1997,14 @@ struct PullView {
   uint64_t E, R;
   const uint32_t *src, *wordrank, *row_dst;
   const uint64_t* rowflag;
-  int all_atomic;  // a destination has a row per sub-partition
+  int all_atomic;     // a destination has a row per sub-partition
+  uint32_t hot;       // ordered view: FP32C values of sources [0, hot) staged in LDS (0: no staging)
+  uint32_t hot_grid;  // ... by a persistent grid of this many blocks
 };
 
 PullView plain_view(const eg_graph* g) {
   return PullView{g->E, g->R, g->e_csc_src.as<uint32_t>(), g->e_wordrank.as<uint32_t>(), g->e_row_dst.as<uint32_t>(),
-                  g->e_rowflag.as<uint64_t>(), g->Q > 1};
+                  g->e_rowflag.as<uint64_t>(), g->Q > 1, 0, 0};
 }
 
 PullView ordered_view(const eg_graph* g) {  // one row per destination, relabelled
@@ -1957,7 +2012,7 @@ PullView ordered_view(const eg_graph* g) {  // one row per destination, relabell
   const DevBuf& wr = sorted ? g->pv_wordrank : g->e_wordrank;
   const DevBuf& rf = sorted ? g->pv_rowflag : g->e_rowflag;
   return PullView{g->E, g->pv_R, g->pv_src.as<uint32_t>(), wr.as<uint32_t>(), g->pv_row_dst.as<uint32_t>(),
-                  rf.as<uint64_t>(), 0};
+                  rf.as<uint64_t>(), 0, g->pv_hot, g->pv_hot_grid};
 }
 
 int pr_pull(const PullView& pv, uint32_t V, int precision, const void* x, double* next, hipStream_t s) {
@@ -1967,6 +2022,11 @@ int pr_pull(const PullView& pv, uint32_t V, int precision, const void* x, double
   if (precision == EG_PR_FP64) {
     hipLaunchKernelGGL(k_pr_pull_f64, dim3(row_grid(pv.E)), dim3(kNT), 0, s, pv.E, pv.src, pv.rowflag, pv.wordrank,
                        pv.row_dst, static_cast<const double*>(x), next, pv.all_atomic);
+  } else if (pv.hot) {
+    const uint64_t nranges = ceil_div(ceil_div(pv.E, 64), kRowWordsPerWave);
+    hipLaunchKernelGGL(k_pr_pull_f32c_hot, dim3(pv.hot_grid), dim3(kHotNT), size_t(pv.hot) * 4, s, pv.E, pv.src,
+                       pv.rowflag, pv.wordrank, pv.row_dst, static_cast<const float*>(x), next, pv.all_atomic,
+                       pv.hot, nranges);
   } else {
     hipLaunchKernelGGL(k_pr_pull_f32c, dim3(row_grid(pv.E)), dim3(kNT), 0, s, pv.E, pv.src, pv.rowflag, pv.wordrank,
                        pv.row_dst, static_cast<const float*>(x), next, pv.all_atomic);

@@ -134,6 +134,8 @@ struct eg_graph {
   eg::DevBuf pv_rowflag;             // u64[ceil(E/64)+1] row starts of the sorted-row form (the other form uses the engine's)
   eg::DevBuf pv_wordrank;            // u32[ceil(E/64)+2]
   int pv_form = -1;                  // the form pv_ready holds: 0 relabelled CSC rows, 1 rows sorted by new source id
+  uint32_t pv_hot = 0;               // FP32C pull: values of new ids [0, pv_hot) staged in LDS per block
+  uint32_t pv_hot_grid = 0;          // ... by a persistent grid of this many blocks
   // traversal view for the batched (64-root) programs: the CSC merged over
   // sub-partitions with each destination's in-edges sorted by source
   bool tv_ready = false;

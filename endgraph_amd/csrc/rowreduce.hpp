@@ -92,7 +92,12 @@ struct NoEdgeValue {};
 
 // ev (optional): a second per-edge stream (weights), loaded like the sources
 // (16-byte vectors when aligned) and handed to val(e, source, ev[e]).
-template <class T, int K = kRowWordsPerWave, class EV = NoEdgeValue, class Op, class Val, class Emit>
+// AllLanes: val is called for every lane, those past the end with source 0
+// (its value is then dropped), so that its loads and the arithmetic on them
+// are one straight-line block whose loads all issue before the first use;
+// val must accept source 0 (every graph with an edge has vertex 0).
+template <class T, int K = kRowWordsPerWave, class EV = NoEdgeValue, bool AllLanes = false, class Op, class Val,
+          class Emit>
 __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const uint32_t* __restrict__ src,
                                                 const uint64_t* __restrict__ rowflag,
                                                 const uint32_t* __restrict__ wordrank, T identity, Op op, Val val,
@@ -162,10 +167,15 @@ __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const u
     const bool ok = r * RW + lw < nwd;
 #pragma unroll
     for (int i = 0; i < IPL; ++i) {
-      if constexpr (kEV)
+      if constexpr (AllLanes) {
+        static_assert(!kEV, "AllLanes with a per-edge value stream is not needed");
+        const T t = val(e0 + i, sv[r][i]);  // sv is 0 past the end
+        v[r][i] = ok && e0 + i < E ? t : identity;
+      } else if constexpr (kEV) {
         v[r][i] = ok && e0 + i < E ? val(e0 + i, sv[r][i], evv[r][i]) : identity;
-      else
+      } else {
         v[r][i] = ok && e0 + i < E ? val(e0 + i, sv[r][i]) : identity;
+      }
     }
   }
   const uint64_t lt = (1ull << lane) - 1;

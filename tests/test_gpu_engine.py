@@ -226,6 +226,30 @@ def test_pagerank_ordered_view(eg, oracle_lib, P, S, tv_first):
     assert np.allclose(d, ref_d, rtol=1e-9, atol=1e-15)
 
 
+@pytest.mark.parametrize("kind,P", [("rmat17", 1), ("rmat17", 4), ("powerlaw", 1), ("powerlaw", 2)])
+def test_pagerank_fp32c_hot_and_cold_sources(eg, oracle_lib, kind, P):
+    """Graphs with more sources than the FP32C pull stages in LDS (40,960 on
+    gfx950), so that a wave's gathers mix LDS reads of the hottest sources
+    with memory reads of the others, on both view forms; edge counts not a
+    multiple of a wave's range."""
+    print(f"pagerank fp32c hot/cold kind={kind} P={P}", flush=True)
+    if kind == "rmat17":
+        edges, V = _rmat(eg, 17)
+        edges = edges[: edges.shape[0] - 1000 - 37]
+    else:
+        V, E = 300_000, 4_000_000 - 77
+        buf = eg.DeviceBuffer(E * 8)
+        eg.gen_powerlaw(buf.ptr, E, V, max_out=60_000.0, max_in=20_000.0, seed=5)
+        edges = buf.download(eg.REC8, E)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, P, 1, m=20)
+    ref, _ = og.pagerank(20)
+    for prec in (eg.PR_FP32C, eg.PR_FP32C | eg.PR_SORTED):
+        x, _ = g.pagerank(20, prec)
+        l1 = np.abs(x - ref).sum() / np.abs(ref).sum()
+        print(f"  relative L1 {prec:#x}: {l1:.3e}", flush=True)
+        assert l1 <= PR_FP32C_TOL
+
+
 def test_pagerank_layout_flags_rejected(eg):
     edges = np.zeros(4, dtype=eg.REC8)
     edges["src"] = [0, 1, 2, 3]
