@@ -247,12 +247,22 @@ def pagerank_block(eg, g, iters, E, oracle_pr=None):
     view, 20 iterations each, median of 3 after a warm-up.  The view's default
     form is the relabelled CSC with P = S = 1 and rows sorted by new source id
     otherwise; with P = S = 1 the sorted-row form (costlier to derive, faster
-    per pull) is timed too.  The view-counted ratios add the view's one-time
-    derivation to one 20-iteration run."""
-    view_ms = g.prepare_pagerank_view()
+    per pull) is timed too.  The view-counted ratio of the default form is
+    the plain FP64 run against an FP32C run that starts without the view and
+    derives it (the relabelled CSC's sources are then written by its first
+    pull); the sorted-row form adds its one-time derivation to one run."""
     V = g.V
     out64, out32, scratch = eg.DeviceBuffer(V * 8), eg.DeviceBuffer(V * 8), eg.DeviceBuffer(V * 8)
     t64 = median_time(lambda: g.pagerank(iters, eg.PR_FP64, out64))
+    firsts = []
+    for _ in range(3):
+        g.release_pagerank_view()  # untimed: frees the view
+        t0 = time.perf_counter()
+        g.pagerank(iters, eg.PR_FP32C, out32)  # synchronises
+        firsts.append(time.perf_counter() - t0)
+    t32_first = statistics.median(firsts)
+    g.release_pagerank_view()
+    view_ms = g.prepare_pagerank_view()  # the derivation on its own, for the record
     t32 = median_time(lambda: g.pagerank(iters, eg.PR_FP32C, out32))
     t64o = median_time(lambda: g.pagerank(iters, eg.PR_FP64 | eg.PR_ORDERED, scratch))
     t32p = median_time(lambda: g.pagerank(iters, eg.PR_FP32C | eg.PR_PLAIN, scratch))
@@ -265,11 +275,13 @@ def pagerank_block(eg, g, iters, E, oracle_pr=None):
            "ms_fp64": round(t64 * 1e3, 2), "ms_fp32c": round(t32 * 1e3, 2),
            "gteps_fp64": gt(t64), "gteps_fp32c": gt(t32),
            "speedup_fp32c": round(t64 / t32, 3),
-           "speedup_fp32c_view_counted": round(t64 / (t32 + view_ms * 1e-3), 3),
+           "ms_fp32c_first_run": round(t32_first * 1e3, 2),
+           "speedup_fp32c_view_counted": round(t64 / t32_first, 3),
            "l1_fp32c_vs_fp64_gpu": float(np.abs(x32 - x64).sum() / np.abs(x64).sum()),
            "gteps_fp64_ordered": gt(t64o), "gteps_fp32c_plain": gt(t32p),
            "layouts": "fp64 = plain FP64 on the built CSC; fp32c = compensated FP32 on the degree-ordered view "
-                      "(view_form); view_counted adds the view's one-time derivation to one 20-iteration FP32C run"}
+                      "(view_form); ms_fp32c_first_run = a 20-iteration FP32C run that starts without the view "
+                      "and derives it; view_counted = ms_fp64 / ms_fp32c_first_run"}
     if one:
         sorted_ms = g.prepare_pagerank_view(eg.PR_SORTED)
         t32s = median_time(lambda: g.pagerank(iters, eg.PR_FP32C | eg.PR_SORTED, scratch))
@@ -285,8 +297,9 @@ def pagerank_block(eg, g, iters, E, oracle_pr=None):
         out["l1_fp64_vs_fp64_oracle"] = float(np.abs(x64 - oracle_pr).sum() / n1)
         if one:
             out["sorted_rows"]["l1_fp32c_vs_fp64_oracle"] = float(np.abs(xs - oracle_pr).sum() / n1)
-    log(f"[bench] pagerank: fp64 {gt(t64)} GTEPS, fp32c {gt(t32)} GTEPS ({t64 / t32:.2f}x, "
-        f"{out['speedup_fp32c_view_counted']:.2f}x with the {view_ms:.1f} ms view counted)")
+    log(f"[bench] pagerank: fp64 {gt(t64)} GTEPS, fp32c {gt(t32)} GTEPS ({t64 / t32:.2f}x; "
+        f"{out['speedup_fp32c_view_counted']:.2f}x for a run that derives the view, {t32_first * 1e3:.1f} ms; "
+        f"the view alone {view_ms:.1f} ms)")
     if one:
         so = out["sorted_rows"]
         log(f"[bench] pagerank, sorted-row view: fp32c {so['gteps_fp32c']} GTEPS ({so['speedup_fp32c']:.2f}x, "

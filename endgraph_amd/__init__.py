@@ -130,6 +130,7 @@ def lib():
         "eg_pagerank": ([vp, i32, i32, vp, vp, vp], i32),
         "eg_pagerank_view_prepare": ([vp, C.POINTER(f64), vp], i32),
         "eg_pagerank_view_prepare_layout": ([vp, i32, C.POINTER(f64), vp], i32),
+        "eg_pagerank_view_release": ([vp], i32),
         "eg_traversal_view_prepare": ([vp, C.POINTER(f64), vp], i32),
         "eg_bfs_level": ([vp, i32, u32, vp, vp, vp, vp, vp, u64, C.POINTER(u64), C.POINTER(u64),
                           C.POINTER(i32), vp], i32),
@@ -516,6 +517,10 @@ class Graph:
         ms = C.c_double(0)
         check(lib().eg_pagerank_view_prepare_layout(self._h, layout, C.byref(ms), _stream(stream)))
         return ms.value
+
+    def release_pagerank_view(self):
+        """Free the degree-ordered view; the next ordered run derives it again."""
+        check(lib().eg_pagerank_view_release(self._h))
 
     def prepare_traversal_view(self, stream=None) -> float:
         """Derive the traversal view (in-edges by destination, sources ascending) now; returns its wall ms."""

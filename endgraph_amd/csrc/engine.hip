@@ -2265,6 +2265,21 @@ int eg_pagerank_view_prepare_layout(eg_graph* g, int layout, double* h_ms, void*
   return EG_OK;
 }
 
+int eg_pagerank_view_release(eg_graph* g) {
+  if (!g) return fail(EG_ERR_ARG, "graph is NULL");
+  EG_HIP(hipDeviceSynchronize());  // a run still queued may read the view
+  g->pv_ready = false;
+  g->pv_form = kViewNone;
+  g->pv_src.release();
+  g->pv_row_dst.release();
+  g->pv_rowflag.release();
+  g->pv_wordrank.release();
+  g->pv_perm.release();
+  g->pv_inv.release();
+  g->pv_deg.release();
+  return EG_OK;
+}
+
 int eg_traversal_view_prepare(eg_graph* g, double* h_ms, void* stream) {
   if (!g) return fail(EG_ERR_ARG, "graph is NULL");
   EG_TRY(ensure_tview(g, static_cast<hipStream_t>(stream)));

@@ -92,10 +92,13 @@ struct NoEdgeValue {};
 
 // ev (optional): a second per-edge stream (weights), loaded like the sources
 // (16-byte vectors when aligned) and handed to val(e, source, ev[e]).
-// AllLanes: val is called for every lane, those past the end with source 0
-// (its value is then dropped), so that its loads and the arithmetic on them
-// are one straight-line block whose loads all issue before the first use;
-// val must accept source 0 (every graph with an edge has vertex 0).
+// AllLanes: every load is unconditional (addresses of lanes past the end
+// clamped to the range's first word and the list's first edges, their
+// values then dropped) and val is called for every lane, those past the end
+// with source 0, so that the loads and the arithmetic on them are one
+// straight-line block whose loads all issue before the first use.  Needs src
+// 16-byte aligned and readable up to E rounded up to a multiple of 4, and
+// val accepting source 0 (every graph with an edge has vertex 0).
 template <class T, int K = kRowWordsPerWave, class EV = NoEdgeValue, bool AllLanes = false, class Op, class Val,
           class Emit>
 __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const uint32_t* __restrict__ src,
@@ -126,9 +129,23 @@ __device__ __forceinline__ void row_reduce_wave(uint64_t w0, uint64_t E, const u
     const int k = r * RW + lw;
     const uint64_t wd = w0 + k;
     const bool ok = k < nwd;
+    const uint64_t e0 = wd * 64 + pos0;
+    if constexpr (AllLanes) {
+      const uint64_t wdc = ok ? wd : w0;
+      const uint64_t fl = rowflag[wdc];
+      const uint32_t bs = wordrank[wdc];
+      f[r] = ok ? fl : 0ull;
+      base[r] = ok ? bs : 0u;
+      const bool in = ok && e0 < E;
+      const uint4 q = *reinterpret_cast<const uint4*>(src + (in ? e0 : 0));
+      sv[r][0] = in ? q.x : 0u;
+      sv[r][1] = in && e0 + 1 < E ? q.y : 0u;
+      sv[r][2] = in && e0 + 2 < E ? q.z : 0u;
+      sv[r][3] = in && e0 + 3 < E ? q.w : 0u;
+      continue;
+    }
     f[r] = ok ? rowflag[wd] : 0ull;
     base[r] = ok ? wordrank[wd] : 0u;
-    const uint64_t e0 = wd * 64 + pos0;
     if (ok && vec && e0 + IPL <= E) {
       const uint4 q = *reinterpret_cast<const uint4*>(src + e0);
       sv[r][0] = q.x;

@@ -119,7 +119,9 @@ int eg_graph_destroy(eg_graph* g);
  * reference's timed prep region, split_graph, :981-1008).  symmetrize=1
  * appends every rank slice's reversed copy (:1057-1078).  Builds both
  * directions (CSC: prepare_pull_mode :1098; CSR: prepare_push_mode :1248),
- * the degree arrays and the partition offsets.  Synchronises `stream`. */
+ * the degree arrays and the partition offsets.  Synchronises `stream`.
+ * Limit: E (2*E0 when symmetrised) < 2^32 records; the engine's edge
+ * positions are u32 (EG_ERR_ARG beyond). */
 int eg_graph_build(eg_graph* g, const void* d_edges, uint64_t E0, int symmetrize, void* stream);
 /* load_directed / load_undirected_from_directed from a binary edge file
  * (preload_graph_file, core/graph.hpp:1011-1084, then the build): the file's
@@ -290,7 +292,12 @@ int eg_bitmap_degree_sum(eg_graph* g, const uint64_t* d_bm, uint64_t* h_sum, voi
  * flag; d_out is in original ids either way); h_deltas (optional) gets
  * iterations+1 values, delta(i) of the reference (already divided by V).
  * eg_pagerank_view_prepare derives the degree-ordered view now (h_ms gets
- * its wall time) so that its one-time cost is kept out of a timed run. */
+ * its wall time) so that its one-time cost is kept out of a timed run.
+ * Reproducibility: a pull row cut by two wave ranges (or, with several
+ * sub-partitions, every row of the plain layout) is completed with FP64
+ * atomic adds, whose order varies between runs, so ranks may differ in the
+ * last bits from run to run (about 1e-19 relative); the integer programs
+ * are bit-reproducible. */
 int eg_pr_init(eg_graph* g, double* d_x, float* d_x32, void* stream);
 int eg_pr_pull(eg_graph* g, int precision, const void* d_x, double* d_next, void* stream);
 int eg_pr_apply(eg_graph* g, const double* d_x, double* d_next, float* d_next32, int last, double* d_delta,
@@ -300,6 +307,9 @@ int eg_pagerank_view_prepare(eg_graph* g, double* h_ms, void* stream);
 /* The same for layout EG_PR_ORDERED (the graph's default form) or
  * EG_PR_SORTED (the sorted-row form). */
 int eg_pagerank_view_prepare_layout(eg_graph* g, int layout, double* h_ms, void* stream);
+/* Frees the degree-ordered view (no reference counterpart); the next ordered
+ * run derives it again.  Synchronises the device. */
+int eg_pagerank_view_release(eg_graph* g);
 
 /* BFS, toolkits/bfs.cpp:22-112.  Parent rule (no reference counterpart: the
  * reference's CAS race leaves it open): the smallest frontier id among a

@@ -250,6 +250,31 @@ def test_pagerank_fp32c_hot_and_cold_sources(eg, oracle_lib, kind, P):
         assert l1 <= PR_FP32C_TOL
 
 
+@pytest.mark.parametrize("E", [1, 3, 4099, 200_003])
+def test_pagerank_fp32c_run_that_derives_the_view(eg, oracle_lib, E):
+    """An FP32C run that derives the ordered view, a later run on the view it
+    left, and a run on a view derived by an FP64 run after the view was
+    released agree (rows cut by wave ranges add their partials in any order:
+    1e-14) and meet the FP64 oracle; edge counts not a multiple of a lane's
+    four edges."""
+    print(f"pagerank fp32c deriving run E={E}", flush=True)
+    rng = np.random.default_rng(E)
+    V = max(2, E // 7)
+    edges = H.random_edges(rng, V, E, skew=True)
+    g, og, _ = _pair(eg, oracle_lib, edges, V, 1, 1, m=20)
+    ref, _ = og.pagerank(20)
+    n1 = np.abs(ref).sum()
+    x1, _ = g.pagerank(20, eg.PR_FP32C)   # derives the view
+    x2, _ = g.pagerank(20, eg.PR_FP32C)   # on the view it left
+    g.release_pagerank_view()
+    g.pagerank(3, eg.PR_FP64 | eg.PR_ORDERED)  # derives the view with the relabelling pass
+    x3, _ = g.pagerank(20, eg.PR_FP32C)
+    for x in (x1, x2, x3):
+        assert np.abs(x - ref).sum() / n1 <= PR_FP32C_TOL
+    assert np.abs(x1 - x2).sum() / n1 <= 1e-14
+    assert np.abs(x2 - x3).sum() / n1 <= 1e-14
+
+
 def test_pagerank_layout_flags_rejected(eg):
     edges = np.zeros(4, dtype=eg.REC8)
     edges["src"] = [0, 1, 2, 3]
