@@ -79,6 +79,8 @@ template <bool kSrc>
 struct KeyEnd {
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return kSrc ? r.src : r.dst; }
+  // a list kept as field arrays: the owner's array itself
+  __device__ __forceinline__ uint64_t operator()(uint32_t v) const { return v; }
 };
 
 // page_amount (core/graph.hpp:718-741): one block per page.  The weight
@@ -263,6 +265,7 @@ __global__ void k_q_offsets(eg_graph::Meta* meta, int Q) {
 // *err and sorts as V-1, so every access stays in bounds until the check.
 template <bool kSrc>
 struct KeyChecked {
+  static constexpr int kField = kSrc ? 0 : 1;
   uint32_t V;
   uint32_t* err;  // null: clamp only (the ids were checked in an earlier launch)
   template <class R>
@@ -271,6 +274,8 @@ struct KeyChecked {
     const uint32_t k = kSrc ? r.src : r.dst;
     return k < V ? k : V - 1;
   }
+  // the key of a field value alone (passes after the check only)
+  __device__ __forceinline__ uint64_t of_field(uint32_t k) const { return k < V ? k : V - 1; }
 };
 // the range check runs once per record: in the first upsweep of the first
 // rank sort only
@@ -288,12 +293,16 @@ struct KeyQOnly {
   }
 };
 struct KeySrc {
+  static constexpr int kField = 0;
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.src; }
+  __device__ __forceinline__ uint64_t of_field(uint32_t k) const { return k; }
 };
 struct KeyDst {
+  static constexpr int kField = 1;
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
+  __device__ __forceinline__ uint64_t of_field(uint32_t k) const { return k; }
 };
 // (q(owner) << vbits) | other endpoint.  kPull: owner = src, key = dst.
 template <bool kPull>
@@ -635,7 +644,7 @@ __global__ void k_block_bounds(const R* __restrict__ a, const uint64_t* __restri
   uint64_t lo = lo0, hi = hi0;  // first record whose owner >= vb
   while (lo < hi) {
     const uint64_t mid = lo + (hi - lo) / 2;
-    if ((kSrc ? a[mid].src : a[mid].dst) < vb)
+    if (KeyEnd<kSrc>{}(a[mid]) < vb)
       lo = mid + 1;
     else
       hi = mid;
@@ -682,6 +691,17 @@ int rank_sorts(eg_graph* g, const R* L, R* a, R* b, R* tmp, uint64_t E0, int P, 
   std::vector<uint64_t> seg(r1 - r0 + 1);
   for (int r = r0; r <= r1; ++r) seg[r - r0] = slice_start(E0, P, mult, r) - b0;
   return rx_sort_pair(L + b0, a + b0, b + b0, tmp + b0, seg, key_a, key_b, bits, g->sortw, s);
+}
+
+// The same with A, B and the ping-pong as field arrays (P > 1: A and B are
+// intermediate lists, read by key only in the degree and upsweep passes).
+template <class R, class KeyA, class KeyB>
+int rank_sorts_fields(eg_graph* g, const R* L, RxSoa<R> a, RxSoa<R> b, RxSoa<R> tmp, uint64_t E0, int P,
+                      uint64_t mult, int r0, int r1, KeyA key_a, KeyB key_b, int bits, hipStream_t s) {
+  const uint64_t b0 = slice_start(E0, P, mult, r0);
+  std::vector<uint64_t> seg(r1 - r0 + 1);
+  for (int r = r0; r <= r1; ++r) seg[r - r0] = slice_start(E0, P, mult, r) - b0;
+  return rx_sort_pair_fields(L + b0, a.at(b0), b.at(b0), tmp.at(b0), seg, key_a, key_b, bits, g->sortw, s);
 }
 
 template <class R, class KeyFn>
@@ -745,9 +765,11 @@ int build_setup(eg_graph* g, uint64_t E0, int sym, hipStream_t s) {
   const uint64_t words = (V >> 6) + 1;
   EG_TRY(g->csr_index.resize(idx_n * 8));
   EG_TRY(g->csr_bitmap.resize(uint64_t(Q) * words * 8));
-  EG_TRY(g->t1.resize(Emax * rb));
-  EG_TRY(g->t2.resize(Emax * rb));
-  if (P > 1 || Q > 1) EG_TRY(g->t3.resize(Emax * rb));
+  // P > 1: the rank-sorted lists and the sorts' ping-pong are field arrays
+  const uint64_t tb = P > 1 ? rx_soa_bytes<R>(Emax) : Emax * rb;
+  EG_TRY(g->t1.resize(tb));
+  EG_TRY(g->t2.resize(tb));
+  if (P > 1 || Q > 1) EG_TRY(g->t3.resize(tb));
   const uint64_t hist_n = 1;
   const uint64_t scan_n = std::max<uint64_t>({scan_tmp_elems(hist_n), scan_tmp_elems(pages + 1),
                                               scan_tmp_elems(ceil_div(E, kChunk) + 1)});
@@ -777,10 +799,18 @@ int build_slices(eg_graph* g, const R* d_in, uint64_t E0, int sym, int r0, int r
     EG_HIP(hipGetLastError());
     L = g->sym.as<R>();
   }
+  // the ids are range-checked once, by the source sort's first upsweep
+  if (P > 1) {
+    const uint64_t Emax = std::max<uint64_t>(g->E, 1);
+    EG_TRY(rank_sorts_fields(g, L, rx_soa_in<R>(g->t1.ptr, Emax), rx_soa_in<R>(g->t2.ptr, Emax),
+                             rx_soa_in<R>(g->t3.ptr, Emax), E0, P, mult, r0, r1,
+                             KeyChecked<true>{g->V, g->err.as<uint32_t>()}, KeyChecked<false>{g->V, nullptr},
+                             g->vbits, s));
+    return EG_OK;
+  }
   R* A = g->t1.as<R>();
   R* B = g->t2.as<R>();
-  R* T = P > 1 ? g->t3.as<R>() : g->csc_list.as<R>();  // ping-pong of the first sorts
-  // the ids are range-checked once, by the source sort's first upsweep
+  R* T = g->csc_list.as<R>();  // ping-pong of the first sorts
   EG_TRY(rank_sorts(g, L, A, B, T, E0, P, mult, r0, r1, KeyChecked<true>{g->V, g->err.as<uint32_t>()},
                     KeyChecked<false>{g->V, nullptr}, g->vbits, s));
   return EG_OK;
@@ -813,8 +843,15 @@ int build_finish(eg_graph* g, hipStream_t s) {
   // --- degrees from runs ---
   EG_TRY(fill<uint32_t>(out_deg, V, 0u, s));
   EG_TRY(fill<uint32_t>(in_deg, V, 0u, s));
-  EG_TRY((run_degree<R, true>(A, E, out_deg, s)));
-  EG_TRY((run_degree<R, false>(B, E, in_deg, s)));
+  const uint64_t Emax = std::max<uint64_t>(E, 1);
+  const RxSoa<R> fa = rx_soa_in<R>(g->t1.ptr, Emax), fb = rx_soa_in<R>(g->t2.ptr, Emax);
+  if (P > 1) {  // the owner's field array of each rank-sorted list
+    EG_TRY((run_degree<uint32_t, true>(fa.s, E, out_deg, s)));
+    EG_TRY((run_degree<uint32_t, false>(fb.d, E, in_deg, s)));
+  } else {
+    EG_TRY((run_degree<R, true>(A, E, out_deg, s)));
+    EG_TRY((run_degree<R, false>(B, E, in_deg, s)));
+  }
 
   // --- CPBT ---
   uint64_t* prefix = g->page_prefix.as<uint64_t>();
@@ -854,28 +891,30 @@ int build_finish(eg_graph* g, hipStream_t s) {
     EG_HIP(hipStreamSynchronize(s));
     const std::vector<uint64_t> seg_csc(qoff.begin(), qoff.begin() + Q + 1);
     const std::vector<uint64_t> seg_csr(qoff.begin() + Q + 1, qoff.end());
-    R* X = g->t3.as<R>();
+    const RxSoa<R> X = rx_soa_in<R>(g->t3.ptr, Emax);
     for (int dir = 0; dir < 2; ++dir) {
       const bool pull = dir == 0;
-      R* in = pull ? A : B;
+      const RxSoa<R> in = pull ? fa : fb;
       uint64_t* bpos = bpos_dir + uint64_t(dir) * P * (Q + 1);
       const unsigned bb = unsigned(ceil_div(uint64_t(P) * (Q + 1), kNT));
       if (pull)
-        hipLaunchKernelGGL((k_block_bounds<R, true>), dim3(bb), dim3(kNT), 0, s, (const R*)in, (const uint64_t*)d_rank,
-                           P, Q, S, (const eg_graph::Meta*)meta, bpos);
+        hipLaunchKernelGGL((k_block_bounds<uint32_t, true>), dim3(bb), dim3(kNT), 0, s, (const uint32_t*)in.s,
+                           (const uint64_t*)d_rank, P, Q, S, (const eg_graph::Meta*)meta, bpos);
       else
-        hipLaunchKernelGGL((k_block_bounds<R, false>), dim3(bb), dim3(kNT), 0, s, (const R*)in,
+        hipLaunchKernelGGL((k_block_bounds<uint32_t, false>), dim3(bb), dim3(kNT), 0, s, (const uint32_t*)in.d,
                            (const uint64_t*)d_rank, P, Q, S, (const eg_graph::Meta*)meta, bpos);
       EG_HIP(hipGetLastError());
       hipLaunchKernelGGL(k_block_table, dim3(1), dim3(kWave), 0, s, (const uint64_t*)bpos, P, Q, tab_dst, tab_src);
       EG_HIP(hipGetLastError());
-      // the first pass of the segmented sort reads the (q, r) blocks in place
+      // the first pass of the segmented sort reads the (q, r) blocks in
+      // place; its later passes ping-pong between X and the rank-sorted
+      // list it has consumed
       if (pull)
-        EG_TRY(rx_sort_blocks((const R*)in, tab_dst, tab_src, P * Q, g->csc_list.as<R>(), X, seg_csc, KeyDst{},
-                              g->vbits, g->sortw, s));
+        EG_TRY(rx_sort_blocks_fields(in, tab_dst, tab_src, P * Q, g->csc_list.as<R>(), X, in, seg_csc, KeyDst{},
+                                     g->vbits, g->sortw, s));
       else
-        EG_TRY(rx_sort_blocks((const R*)in, tab_dst, tab_src, P * Q, g->csr_list.as<R>(), X, seg_csr, KeySrc{},
-                              g->vbits, g->sortw, s));
+        EG_TRY(rx_sort_blocks_fields(in, tab_dst, tab_src, P * Q, g->csr_list.as<R>(), X, in, seg_csr, KeySrc{},
+                                     g->vbits, g->sortw, s));
     }
   } else if (Q == 1) {
     g->csr_list.swap(g->t1);  // A is the CSR, B the CSC

@@ -69,13 +69,84 @@ KeyFn rx_quiet(const KeyFn& k) {
 // record i is p[i + shift[t]] for tile t holding i, or, in a tile that
 // straddles a block boundary (shift[t] == kRxStraddle), p[bsrc[b] + i - bdst[b]]
 // for the last block b with bdst[b] <= i.
+// Records kept as field arrays (structure of arrays): src[i], dst[i] and,
+// for 12-byte records, w[i].  A sort whose intermediate lists are field
+// arrays reads only the key's array in its upsweeps (half the bytes of
+// 8-byte records); the record layout is kept where a list is a result.
+template <class R>
+struct RxSoa {
+  uint32_t* s = nullptr;
+  uint32_t* d = nullptr;
+  float* w = nullptr;  // 12-byte records only
+  RxSoa at(uint64_t off) const {
+    RxSoa o;
+    o.s = s + off;
+    o.d = d + off;
+    o.w = w ? w + off : nullptr;
+    return o;
+  }
+};
+
+// Field arrays of n records in one buffer, each array starting on a 256-byte
+// boundary: rx_soa_bytes<R>(n) bytes.
+inline uint64_t rx_soa_stride(uint64_t n) { return (n + 63) & ~uint64_t(63); }
+template <class R>
+uint64_t rx_soa_bytes(uint64_t n) {
+  return rx_soa_stride(n) * sizeof(R);
+}
+template <class R>
+RxSoa<R> rx_soa_in(void* buf, uint64_t n) {
+  const uint64_t st = rx_soa_stride(n);
+  RxSoa<R> o;
+  o.s = static_cast<uint32_t*>(buf);
+  o.d = o.s + st;
+  if constexpr (sizeof(R) == 12) o.w = reinterpret_cast<float*>(o.d + st);
+  return o;
+}
+
+template <class R>
+__device__ __forceinline__ R rx_soa_load(const RxSoa<R>& a, uint64_t j) {
+  R r;
+  r.src = a.s[j];
+  r.dst = a.d[j];
+  if constexpr (sizeof(R) == 12) r.w = a.w[j];
+  return r;
+}
+
+template <class R>
+__device__ __forceinline__ void rx_soa_store(const RxSoa<R>& a, uint64_t j, const R& r) {
+  a.s[j] = r.src;
+  a.d[j] = r.dst;
+  if constexpr (sizeof(R) == 12) a.w[j] = r.w;
+}
+
+// Field a key reads (0 src, 1 dst) when the key functor names one
+// (static constexpr int kField, with of_field(v) the key of field value v),
+// else -1: an upsweep over field arrays then reads that array only.
+template <class K, class = void>
+struct RxKeyField {
+  static constexpr int value = -1;
+};
+template <class K>
+struct RxKeyField<K, std::void_t<decltype(K::kField)>> {
+  static constexpr int value = K::kField;
+};
+
 template <class R>
 struct RxIn {
-  const R* p;
+  const R* p;            // records (record layout), or null for field arrays
   const int64_t* shift;  // per tile, or null for plain input
   const uint64_t* bdst;  // nblk + 1 block starts in sort order
   const uint64_t* bsrc;  // nblk block starts in p
   int nblk;
+  RxSoa<R> f;            // field arrays (kernels with SIN = true)
+};
+
+// Output of a pass: records, or field arrays (SOUT = true).
+template <class R>
+struct RxOut {
+  R* p = nullptr;
+  RxSoa<R> f;
 };
 
 constexpr int64_t kRxStraddle = INT64_MIN;
@@ -85,9 +156,10 @@ __device__ __forceinline__ int64_t rx_tile_shift(const RxIn<R>& in, uint32_t t) 
   return in.shift ? in.shift[t] : 0;
 }
 
+// position in the input's storage of record i of the sort order
 template <class R>
-__device__ __forceinline__ R rx_load(const RxIn<R>& in, int64_t sh, uint64_t i) {
-  if (sh != kRxStraddle) return in.p[uint64_t(int64_t(i) + sh)];
+__device__ __forceinline__ uint64_t rx_src_index(const RxIn<R>& in, int64_t sh, uint64_t i) {
+  if (sh != kRxStraddle) return uint64_t(int64_t(i) + sh);
   int lo = 0, hi = in.nblk;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -96,7 +168,24 @@ __device__ __forceinline__ R rx_load(const RxIn<R>& in, int64_t sh, uint64_t i) 
     else
       hi = mid;
   }
-  return in.p[in.bsrc[lo] + (i - in.bdst[lo])];
+  return in.bsrc[lo] + (i - in.bdst[lo]);
+}
+
+template <bool SIN, class R>
+__device__ __forceinline__ R rx_load(const RxIn<R>& in, int64_t sh, uint64_t i) {
+  const uint64_t j = rx_src_index(in, sh, i);
+  if constexpr (SIN)
+    return rx_soa_load(in.f, j);
+  else
+    return in.p[j];
+}
+
+template <bool SOUT, class R>
+__device__ __forceinline__ void rx_store(const RxOut<R>& out, uint64_t j, const R& r) {
+  if constexpr (SOUT)
+    rx_soa_store(out.f, j, r);
+  else
+    out.p[j] = r;
 }
 
 struct RxPassArgs {
@@ -168,7 +257,7 @@ constexpr unsigned kLgkmZero = 0xC07F;
 // wave has all its loads in flight at once.  peer_tab: (NT/64) << BM u64 of
 // LDS, free for the duration of the call.
 
-template <int NT, int BM, class R, class KeyFn>
+template <int NT, int BM, bool SIN, class R, class KeyFn>
 __device__ __forceinline__ void rx_rank(const RxIn<R>& in, int64_t sh, const RxTile& x, KeyFn key, int shift, int bits,
                                         R (&item)[kRxIPT], uint32_t (&dig)[kRxIPT], uint32_t (&rank)[kRxIPT],
                                         uint32_t (*wcnt)[1 << BM], uint32_t* tot, uint64_t* peer_tab) {
@@ -181,7 +270,7 @@ __device__ __forceinline__ void rx_rank(const RxIn<R>& in, int64_t sh, const RxT
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
     const uint32_t li = first + uint32_t(k) * kWave;
-    item[k] = rx_load(in, sh, x.base + (li < x.count ? li : 0u));
+    item[k] = rx_load<SIN>(in, sh, x.base + (li < x.count ? li : 0u));
   }
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) dig[k] = uint32_t(key(item[k]) >> shift) & mask;
@@ -237,8 +326,8 @@ __device__ __forceinline__ void rx_rank(const RxIn<R>& in, int64_t sh, const RxT
 
 // Stage in digit order and write every digit's run contiguously (LDS reads,
 // then digits, then stores, so the key lookups of a thread overlap).
-template <int NT, int BM, class R, class KeyFn>
-__device__ __forceinline__ void rx_scatter(R* __restrict__ out, const RxTile& x, KeyFn key, int shift, int bits,
+template <int NT, int BM, bool SOUT, class R, class KeyFn>
+__device__ __forceinline__ void rx_scatter(const RxOut<R>& out, const RxTile& x, KeyFn key, int shift, int bits,
                                            const R (&item)[kRxIPT], const uint32_t (&dig)[kRxIPT],
                                            const uint32_t (&rank)[kRxIPT], uint32_t (*wcnt)[1 << BM],
                                            uint32_t* goff, const uint32_t* dstart, R* stage) {
@@ -274,7 +363,7 @@ __device__ __forceinline__ void rx_scatter(R* __restrict__ out, const RxTile& x,
     const uint32_t lp = uint32_t(k) * NT + threadIdx.x;
     if (lp < x.count) {
       const uint64_t pos = uint64_t(goff[d[k]]) + lp;
-      if (pos < x.end) out[pos] = r[k];  // always true unless a look-back gave up
+      if (pos < x.end) rx_store<SOUT>(out, pos, r[k]);  // always true unless a look-back gave up
     }
   }
 }
@@ -300,7 +389,7 @@ __device__ __forceinline__ void rx_dstart(int nb, uint32_t tot, uint32_t* wt, ui
 // upsweep).
 struct RxNoKey {};
 
-template <int NT, int BM, class R, class KeyFn, class Key2 = RxNoKey>
+template <int NT, int BM, bool SIN, class R, class KeyFn, class Key2 = RxNoKey>
 __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __restrict__ seg,
                                               const uint32_t* __restrict__ ts, int nseg, KeyFn key, int shift,
                                               int bits, uint32_t* __restrict__ hist, int xcd_order, Key2 key2 = {},
@@ -320,9 +409,39 @@ __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __rest
   const int64_t sh = rx_tile_shift(in, t);
   const uint32_t mask = uint32_t(nb - 1);
   uint32_t* hw = h[threadIdx.x / kWave];
+  constexpr int kField = RxKeyField<KeyFn>::value;
+  if constexpr (SIN && kField >= 0 && !kTwo) {
+    // field arrays: read the key's array only, four keys per 16-byte load
+    // in full tiles
+    const uint32_t* fa = kField == 0 ? in.f.s : in.f.d;
+    uint32_t v[kRxIPT];
+    const uint32_t* p0 = fa + uint64_t(int64_t(x.base) + (sh == kRxStraddle ? 0 : sh));
+    const bool quads =
+        x.count == uint32_t(TILE) && sh != kRxStraddle && (reinterpret_cast<uintptr_t>(p0) & 15) == 0;
+    if (quads) {
+      const uint4* q4 = reinterpret_cast<const uint4*>(p0);
+#pragma unroll
+      for (int k = 0; k < kRxIPT / 4; ++k) {
+        const uint4 q = q4[uint32_t(k) * NT + threadIdx.x];
+        v[4 * k] = q.x;
+        v[4 * k + 1] = q.y;
+        v[4 * k + 2] = q.z;
+        v[4 * k + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kRxIPT; ++k) {
+        const uint32_t li = uint32_t(k) * NT + threadIdx.x;
+        v[k] = fa[rx_src_index(in, sh, x.base + (li < x.count ? li : 0u))];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kRxIPT; ++k)
+      if (quads || uint32_t(k) * NT + threadIdx.x < x.count) atomicAdd(&hw[uint32_t(key.of_field(v[k]) >> shift) & mask], 1u);
+  } else {
   R r[kRxIPT];
   bool pairs = false;
-  if constexpr (sizeof(R) == 8) {
+  if constexpr (sizeof(R) == 8 && !SIN) {
     pairs = x.count == uint32_t(TILE) && sh != kRxStraddle && ((int64_t(x.base) + sh) & 1) == 0;
     if (pairs) {
       const uint4* p = reinterpret_cast<const uint4*>(in.p + uint64_t(int64_t(x.base) + sh));
@@ -337,7 +456,7 @@ __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __rest
 #pragma unroll
     for (int k = 0; k < kRxIPT; ++k) {
       const uint32_t li = uint32_t(k) * NT + threadIdx.x;
-      r[k] = rx_load(in, sh, x.base + (li < x.count ? li : 0u));
+      r[k] = rx_load<SIN>(in, sh, x.base + (li < x.count ? li : 0u));
     }
   }
 #pragma unroll
@@ -348,6 +467,7 @@ __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __rest
     if (live) atomicAdd(&hw[d], 1u);
     if constexpr (kTwo)
       if (live) atomicAdd(&h2[threadIdx.x / kWave][uint32_t(key2(r[k]) >> shift) & mask], 1u);
+  }
   }
   __syncthreads();
   for (int d = threadIdx.x; d < nb; d += NT) {
@@ -363,8 +483,8 @@ __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __rest
   }
 }
 
-template <int NT, int BM, class R, class KeyFn>
-__global__ __launch_bounds__(NT) void k_rx_down(RxIn<R> in, R* __restrict__ out,
+template <int NT, int BM, bool SIN, bool SOUT, class R, class KeyFn>
+__global__ __launch_bounds__(NT) void k_rx_down(RxIn<R> in, RxOut<R> out,
                                                 const uint64_t* __restrict__ seg, const uint32_t* __restrict__ ts,
                                                 int nseg, KeyFn key, int shift, int bits,
                                                 const uint32_t* __restrict__ offs, int xcd_order) {
@@ -386,14 +506,14 @@ __global__ __launch_bounds__(NT) void k_rx_down(RxIn<R> in, R* __restrict__ out,
   uint32_t tot;
   // the peer table of the ranking lives in the staging buffer, which is
   // first written after the ranking's last barrier
-  rx_rank<NT, BM>(in, rx_tile_shift(in, t), x, key, shift, bits, item, dig, rank, wcnt, &tot,
-                  reinterpret_cast<uint64_t*>(stage));
+  rx_rank<NT, BM, SIN>(in, rx_tile_shift(in, t), x, key, shift, bits, item, dig, rank, wcnt, &tot,
+                       reinterpret_cast<uint64_t*>(stage));
   rx_dstart<NT>(nb, tot, wt, dstart);
-  rx_scatter<NT, BM>(out, x, key, shift, bits, item, dig, rank, wcnt, goff, dstart, stage);
+  rx_scatter<NT, BM, SOUT>(out, x, key, shift, bits, item, dig, rank, wcnt, goff, dstart, stage);
 }
 
 // -------------------------------------------------------------- one-pass --
-template <int NT, int BM, class R, class KeyFn>
+template <int NT, int BM, bool SIN, class R, class KeyFn>
 __global__ __launch_bounds__(NT) void k_rx_hist_all(RxIn<R> in, const uint64_t* __restrict__ seg,
                                                     const uint32_t* __restrict__ ts, int nseg, KeyFn key,
                                                     RxPassArgs pa, uint32_t* __restrict__ ghist) {
@@ -407,7 +527,7 @@ __global__ __launch_bounds__(NT) void k_rx_hist_all(RxIn<R> in, const uint64_t* 
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
     const uint32_t li = uint32_t(k) * NT + threadIdx.x;
-    r[k] = rx_load(in, sh, x.base + (li < x.count ? li : 0u));
+    r[k] = rx_load<SIN>(in, sh, x.base + (li < x.count ? li : 0u));
   }
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
@@ -425,8 +545,8 @@ __global__ __launch_bounds__(NT) void k_rx_hist_all(RxIn<R> in, const uint64_t* 
   }
 }
 
-template <int NT, int BM, class R, class KeyFn>
-__global__ __launch_bounds__(NT) void k_rx_onepass(RxIn<R> in, R* __restrict__ out,
+template <int NT, int BM, bool SIN, bool SOUT, class R, class KeyFn>
+__global__ __launch_bounds__(NT) void k_rx_onepass(RxIn<R> in, RxOut<R> out,
                                                    const uint64_t* __restrict__ seg, const uint32_t* __restrict__ ts,
                                                    int nseg, KeyFn key, int shift, int bits,
                                                    const uint32_t* __restrict__ dbase, uint64_t* __restrict__ status,
@@ -450,8 +570,8 @@ __global__ __launch_bounds__(NT) void k_rx_onepass(RxIn<R> in, R* __restrict__ o
   R item[kRxIPT];
   uint32_t dig[kRxIPT], rank[kRxIPT];
   uint32_t tot;
-  rx_rank<NT, BM>(in, rx_tile_shift(in, t), x, key, shift, bits, item, dig, rank, wcnt, &tot,
-                  reinterpret_cast<uint64_t*>(stage));
+  rx_rank<NT, BM, SIN>(in, rx_tile_shift(in, t), x, key, shift, bits, item, dig, rank, wcnt, &tot,
+                       reinterpret_cast<uint64_t*>(stage));
   // A status word is its own payload (epoch | flag | count): relaxed
   // device-scope atomics suffice; release/acquire would write back or
   // invalidate the L2 on every publish and poll.
@@ -485,7 +605,7 @@ __global__ __launch_bounds__(NT) void k_rx_onepass(RxIn<R> in, R* __restrict__ o
     }
   }
   rx_dstart<NT>(nb, tot, wt, dstart);
-  rx_scatter<NT, BM>(out, x, key, shift, bits, item, dig, rank, wcnt, goff, dstart, stage);
+  rx_scatter<NT, BM, SOUT>(out, x, key, shift, bits, item, dig, rank, wcnt, goff, dstart, stage);
 }
 
 namespace {  // internal linkage: this header is included by several sources
@@ -558,9 +678,70 @@ struct RxWork {
 // upsweep, the counts being in w.table2 already.
 enum RxFirst { kFirstOwn = 0, kFirstAlsoSecond = 1, kFirstCounted = 2 };
 
-template <int NT, int BM, class R, class KeyFn, class Key2 = RxNoKey>
-int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, int scheme,
-               RxWork& w, hipStream_t s, RxFirst first = kFirstOwn, Key2 key2 = {}) {
+// Storage of a sort's lists: records throughout (kRecords); the input in
+// records and every pass's output in field arrays (kToFields: the result is
+// an intermediate list of the build); the input and every intermediate in
+// field arrays and the result in records (kFromFields: passes alternate
+// between tmp and tmp2, both field arrays, the last writes out).
+enum RxStorage { kRecords = 0, kToFields = 1, kFromFields = 2 };
+
+template <class R>
+RxIn<R> rx_in_of(const RxOut<R>& o) {
+  RxIn<R> in{o.p, nullptr, nullptr, nullptr, 0, o.f};
+  return in;
+}
+
+template <int NT, int BM, bool SIN, class R, class KeyFn, class Key2>
+int rx_launch_up(const RxIn<R>& src, uint32_t tiles, const uint64_t* d_seg, const uint32_t* d_ts, int nseg,
+                 KeyFn key, int shift, int bits, uint32_t* tab, int xcd, Key2 key2, uint32_t* tab2, uint64_t n,
+                 hipStream_t s) {
+  constexpr bool kKeyOnly = SIN && RxKeyField<KeyFn>::value >= 0 && std::is_same<Key2, RxNoKey>::value;
+  prof_begin(s);
+  hipLaunchKernelGGL((k_rx_up<NT, BM, SIN, R, KeyFn, Key2>), dim3(tiles), dim3(NT), 0, s, src, d_seg, d_ts, nseg, key,
+                     shift, bits, tab, xcd, key2, tab2);
+  prof_end(PK_RADIX_UPSWEEP, n * (kKeyOnly ? 4 : sizeof(R)), s);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+template <int NT, int BM, bool SIN, bool SOUT, class R, class KeyFn>
+int rx_launch_down(const RxIn<R>& src, const RxOut<R>& dst, uint32_t tiles, const uint64_t* d_seg,
+                   const uint32_t* d_ts, int nseg, KeyFn key, int shift, int bits, const uint32_t* tab, int xcd,
+                   uint64_t n, hipStream_t s) {
+  prof_begin(s);
+  hipLaunchKernelGGL((k_rx_down<NT, BM, SIN, SOUT, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, d_seg, d_ts,
+                     nseg, key, shift, bits, tab, xcd);
+  prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+template <int NT, int BM, bool SIN, bool SOUT, class R, class KeyFn>
+int rx_launch_onepass(const RxIn<R>& src, const RxOut<R>& dst, uint32_t tiles, const uint64_t* d_seg,
+                      const uint32_t* d_ts, int nseg, KeyFn key, int shift, int bits, const uint32_t* dbase,
+                      RxWork& w, int p, uint32_t epoch, uint64_t n, hipStream_t s) {
+  prof_begin(s);
+  hipLaunchKernelGGL((k_rx_onepass<NT, BM, SIN, SOUT, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, d_seg, d_ts,
+                     nseg, key, shift, bits, dbase, w.status.as<uint64_t>(), w.ticket.as<uint32_t>() + p, epoch,
+                     w.err.as<uint32_t>());
+  prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
+  EG_HIP(hipGetLastError());
+  return EG_OK;
+}
+
+// pass p of `passes` (storage ST): the input / output is field arrays
+template <int ST>
+constexpr bool rx_pass_in_fields(int p) {
+  return ST == kFromFields || (ST == kToFields && p > 0);
+}
+template <int ST>
+constexpr bool rx_pass_out_fields(int p, int passes) {
+  return ST == kToFields || (ST == kFromFields && p != passes - 1);
+}
+
+template <int NT, int BM, int ST = kRecords, class R, class KeyFn, class Key2 = RxNoKey>
+int rx_sort_nt(RxIn<R> in, RxOut<R> out, RxOut<R> tmp, RxOut<R> tmp2, const std::vector<uint64_t>& seg, KeyFn key,
+               int key_bits, int scheme, RxWork& w, hipStream_t s, RxFirst first = kFirstOwn, Key2 key2 = {}) {
   static_assert((1 << BM) <= NT, "one digit per thread in the count/scan steps");
   static_assert(NT * kRxIPT * sizeof(R) + (NT / kWave) * (4 << BM) + (8 << BM) <= 160 * 1024, "LDS of one block");
   static_assert((NT / kWave) * (8 << BM) <= NT * kRxIPT * sizeof(R), "peer tables fit the staging buffer");
@@ -585,6 +766,13 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
     EG_HIP(hipGetLastError());
     in.shift = w.tshift.as<int64_t>();
   }
+  // destination of pass p: kFromFields alternates tmp, tmp2 and ends in out;
+  // the others alternate out and tmp so that the last pass writes out
+  auto dest = [&](int p) -> const RxOut<R>& {
+    if (p == plan.passes - 1) return out;
+    if (ST == kFromFields) return p % 2 == 0 ? tmp : tmp2;
+    return (plan.passes - 1 - p) % 2 == 0 ? out : tmp;
+  };
   RxIn<R> src = in;
   if (scheme == kThreeLaunch || scheme == kThreeLaunchXcd) {
     const int xcd = scheme == kThreeLaunchXcd;
@@ -595,30 +783,51 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
       return fail(EG_ERR_STATE, "internal: the first digit's counts were not made for this sort");
     if (w.scan.bytes < scan_tmp_elems(tn) * 4) EG_TRY(w.scan.resize(scan_tmp_elems(tn) * 4));
     for (int p = 0; p < plan.passes; ++p) {
-      R* dst = ((plan.passes - 1 - p) % 2 == 0) ? out : tmp;
+      const RxOut<R>& dst = dest(p);
       uint32_t* tab = p == 0 && first == kFirstCounted ? w.table2.as<uint32_t>() : w.table.as<uint32_t>();
+      const bool fin = rx_pass_in_fields<ST>(p), fout = rx_pass_out_fields<ST>(p, plan.passes);
       if (p == 0 && first == kFirstAlsoSecond) {
-        prof_begin(s);
-        hipLaunchKernelGGL((k_rx_up<NT, BM, R, KeyFn, Key2>), dim3(tiles), dim3(NT), 0, s, src,
-                           (const uint64_t*)d_seg, (const uint32_t*)d_ts, nseg, key, plan.shift[p], plan.bits[p],
-                           tab, xcd, key2, w.table2.as<uint32_t>());
-        prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
-        EG_HIP(hipGetLastError());
+        if (fin)
+          EG_TRY((rx_launch_up<NT, BM, ST == kFromFields>(src, tiles, d_seg, d_ts, nseg, key, plan.shift[p],
+                                                          plan.bits[p], tab, xcd, key2, w.table2.as<uint32_t>(), n,
+                                                          s)));
+        else
+          EG_TRY((rx_launch_up<NT, BM, false>(src, tiles, d_seg, d_ts, nseg, key, plan.shift[p], plan.bits[p], tab, xcd,
+                                              key2, w.table2.as<uint32_t>(), n, s)));
       } else if (!(p == 0 && first == kFirstCounted)) {
-        prof_begin(s);
-        hipLaunchKernelGGL((k_rx_up<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, (const uint64_t*)d_seg,
-                           (const uint32_t*)d_ts, nseg, p == 0 ? key : quiet, plan.shift[p], plan.bits[p], tab, xcd,
-                           RxNoKey{}, nullptr);
-        prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
-        EG_HIP(hipGetLastError());
+        const KeyFn kp = p == 0 ? key : quiet;
+        if constexpr (ST == kRecords) {
+          EG_TRY((rx_launch_up<NT, BM, false>(src, tiles, d_seg, d_ts, nseg, kp, plan.shift[p], plan.bits[p], tab, xcd,
+                                              RxNoKey{}, nullptr, n, s)));
+        } else {
+          if (fin)
+            EG_TRY((rx_launch_up<NT, BM, true>(src, tiles, d_seg, d_ts, nseg, kp, plan.shift[p], plan.bits[p], tab,
+                                               xcd, RxNoKey{}, nullptr, n, s)));
+          else
+            EG_TRY((rx_launch_up<NT, BM, false>(src, tiles, d_seg, d_ts, nseg, kp, plan.shift[p], plan.bits[p], tab,
+                                                xcd, RxNoKey{}, nullptr, n, s)));
+        }
       }
       EG_TRY(exclusive_scan_u32(tab, tab, uint64_t(tiles) << plan.bits[p], w.scan.as<uint32_t>(), s));
-      prof_begin(s);
-      hipLaunchKernelGGL((k_rx_down<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, (const uint64_t*)d_seg,
-                         (const uint32_t*)d_ts, nseg, quiet, plan.shift[p], plan.bits[p], (const uint32_t*)tab, xcd);
-      prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
-      EG_HIP(hipGetLastError());
-      src = RxIn<R>{dst, nullptr, nullptr, nullptr, 0};
+      if constexpr (ST == kRecords) {
+        EG_TRY((rx_launch_down<NT, BM, false, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                     plan.bits[p], tab, xcd, n, s)));
+      } else if constexpr (ST == kToFields) {
+        if (fin)
+          EG_TRY((rx_launch_down<NT, BM, true, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                     plan.bits[p], tab, xcd, n, s)));
+        else
+          EG_TRY((rx_launch_down<NT, BM, false, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                      plan.bits[p], tab, xcd, n, s)));
+      } else {
+        if (fout)
+          EG_TRY((rx_launch_down<NT, BM, true, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                     plan.bits[p], tab, xcd, n, s)));
+        else
+          EG_TRY((rx_launch_down<NT, BM, true, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                      plan.bits[p], tab, xcd, n, s)));
+      }
+      src = rx_in_of(dst);
     }
     return EG_OK;
   }
@@ -640,44 +849,68 @@ int rx_sort_nt(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, Key
                      w.ghist.as<uint32_t>(), nh);
   hipLaunchKernelGGL(k_rx_zero, dim3(1), dim3(64), 0, s, w.ticket.as<uint32_t>(), uint64_t(kRxMaxPasses));
   prof_begin(s);
-  hipLaunchKernelGGL((k_rx_hist_all<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, in, (const uint64_t*)d_seg,
-                     (const uint32_t*)d_ts, nseg, key, pa, w.ghist.as<uint32_t>());
+  hipLaunchKernelGGL((k_rx_hist_all<NT, BM, ST == kFromFields, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, in,
+                     (const uint64_t*)d_seg, (const uint32_t*)d_ts, nseg, key, pa, w.ghist.as<uint32_t>());
   prof_end(PK_RADIX_UPSWEEP, n * sizeof(R), s);
   EG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_rx_base, dim3(nseg), dim3(64), 0, s, w.ghist.as<uint32_t>(), (const uint64_t*)d_seg, nseg,
                      plan.passes);
   EG_HIP(hipGetLastError());
   for (int p = 0; p < plan.passes; ++p) {
-    R* dst = ((plan.passes - 1 - p) % 2 == 0) ? out : tmp;
+    const RxOut<R>& dst = dest(p);
     const uint32_t epoch = ++w.epoch;
-    prof_begin(s);
-    hipLaunchKernelGGL((k_rx_onepass<NT, BM, R, KeyFn>), dim3(tiles), dim3(NT), 0, s, src, dst, (const uint64_t*)d_seg,
-                       (const uint32_t*)d_ts, nseg, quiet, plan.shift[p], plan.bits[p],
-                       (const uint32_t*)w.ghist.as<uint32_t>() + uint64_t(p) * (1 << kRxMaxBits),
-                       w.status.as<uint64_t>(), w.ticket.as<uint32_t>() + p, epoch, w.err.as<uint32_t>());
-    prof_end(PK_RADIX_DOWNSWEEP, 2 * n * sizeof(R), s);
-    EG_HIP(hipGetLastError());
-    src = RxIn<R>{dst, nullptr, nullptr, nullptr, 0};
+    const uint32_t* db = (const uint32_t*)w.ghist.as<uint32_t>() + uint64_t(p) * (1 << kRxMaxBits);
+    const bool fin = rx_pass_in_fields<ST>(p), fout = rx_pass_out_fields<ST>(p, plan.passes);
+    if constexpr (ST == kRecords) {
+      EG_TRY((rx_launch_onepass<NT, BM, false, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                      plan.bits[p], db, w, p, epoch, n, s)));
+    } else if constexpr (ST == kToFields) {
+      if (fin)
+        EG_TRY((rx_launch_onepass<NT, BM, true, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                      plan.bits[p], db, w, p, epoch, n, s)));
+      else
+        EG_TRY((rx_launch_onepass<NT, BM, false, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                       plan.bits[p], db, w, p, epoch, n, s)));
+    } else {
+      if (fout)
+        EG_TRY((rx_launch_onepass<NT, BM, true, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                      plan.bits[p], db, w, p, epoch, n, s)));
+      else
+        EG_TRY((rx_launch_onepass<NT, BM, true, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                       plan.bits[p], db, w, p, epoch, n, s)));
+    }
+    src = rx_in_of(dst);
   }
   return EG_OK;
 }
 
 // Sort every segment [seg[k], seg[k+1]) of `in` stably by key into the same
-// positions of `out` (tmp = ping-pong; in, out, tmp distinct; in unchanged).
-// The caller zeroes w.err before and checks it after synchronising.
-template <class R, class KeyFn>
-int rx_sort_with(const SortVariant& v, RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key,
-                 int key_bits, RxWork& w, hipStream_t s) {
+// positions of `out` (tmp = ping-pong; in, out, tmp distinct; in unchanged;
+// tmp2: the second field-array temporary of kFromFields, which may be the
+// input's storage).  The caller zeroes w.err before and checks it after
+// synchronising.
+template <int ST = kRecords, class R, class KeyFn>
+int rx_sort_with(const SortVariant& v, RxIn<R> in, RxOut<R> out, RxOut<R> tmp, RxOut<R> tmp2,
+                 const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w, hipStream_t s) {
   const int nseg = int(seg.size()) - 1;
   if (nseg < 1 || seg.back() == seg.front()) return EG_OK;
   if (seg.front() != 0) return fail(EG_ERR_ARG, "internal: sort segments must start at 0");
   if (seg.back() >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "sort of more than 2^32-1 records");
-  if (v.nt == 256) return rx_sort_nt<256, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
+  if (v.nt == 256) return rx_sort_nt<256, 8, ST>(in, out, tmp, tmp2, seg, key, key_bits, v.scheme, w, s);
   if constexpr (sizeof(R) <= 8) {
-    if (v.nt >= 1024) return rx_sort_nt<1024, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
+    if (v.nt >= 1024) return rx_sort_nt<1024, 8, ST>(in, out, tmp, tmp2, seg, key, key_bits, v.scheme, w, s);
   }
-  if (v.bits >= 9) return rx_sort_nt<512, 9>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
-  return rx_sort_nt<512, 8>(in, out, tmp, seg, key, key_bits, v.scheme, w, s);
+  if (v.bits >= 9) return rx_sort_nt<512, 9, ST>(in, out, tmp, tmp2, seg, key, key_bits, v.scheme, w, s);
+  return rx_sort_nt<512, 8, ST>(in, out, tmp, tmp2, seg, key, key_bits, v.scheme, w, s);
+}
+
+template <class R, class KeyFn>
+int rx_sort_with(const SortVariant& v, RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key,
+                 int key_bits, RxWork& w, hipStream_t s) {
+  RxOut<R> o, t;
+  o.p = out;
+  t.p = tmp;
+  return rx_sort_with<kRecords>(v, in, o, t, RxOut<R>{}, seg, key, key_bits, w, s);
 }
 
 template <class R, class KeyFn>
@@ -690,34 +923,58 @@ int rx_sort(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn 
 // by key_b into out_b, each as rx_sort would do it.  Three-launch variants
 // read the input once for both first digits (the first key's first upsweep
 // also counts the second's); the one-pass variant runs the two sorts.
-template <int NT, int BM, class R, class KeyA, class KeyB>
-int rx_sort_pair_nt(const R* in, R* out_a, R* out_b, R* tmp, const std::vector<uint64_t>& seg, KeyA key_a,
-                    KeyB key_b, int key_bits, int scheme, RxWork& w, hipStream_t s) {
-  const RxIn<R> src{in, nullptr, nullptr, nullptr, 0};
+// ST: kRecords, or kToFields (the results as field arrays).
+template <int NT, int BM, int ST, class R, class KeyA, class KeyB>
+int rx_sort_pair_nt(const R* in, RxOut<R> out_a, RxOut<R> out_b, RxOut<R> tmp, const std::vector<uint64_t>& seg,
+                    KeyA key_a, KeyB key_b, int key_bits, int scheme, RxWork& w, hipStream_t s) {
+  const RxIn<R> src{in, nullptr, nullptr, nullptr, 0, {}};
   if (scheme == kOnePass) {
-    EG_TRY((rx_sort_nt<NT, BM>(src, out_a, tmp, seg, key_a, key_bits, scheme, w, s)));
-    return rx_sort_nt<NT, BM>(src, out_b, tmp, seg, key_b, key_bits, scheme, w, s);
+    EG_TRY((rx_sort_nt<NT, BM, ST>(src, out_a, tmp, RxOut<R>{}, seg, key_a, key_bits, scheme, w, s)));
+    return rx_sort_nt<NT, BM, ST>(src, out_b, tmp, RxOut<R>{}, seg, key_b, key_bits, scheme, w, s);
   }
-  EG_TRY((rx_sort_nt<NT, BM>(src, out_a, tmp, seg, key_a, key_bits, scheme, w, s, kFirstAlsoSecond,
-                            rx_quiet(key_b))));
-  return rx_sort_nt<NT, BM>(src, out_b, tmp, seg, key_b, key_bits, scheme, w, s, kFirstCounted);
+  EG_TRY((rx_sort_nt<NT, BM, ST>(src, out_a, tmp, RxOut<R>{}, seg, key_a, key_bits, scheme, w, s, kFirstAlsoSecond,
+                                rx_quiet(key_b))));
+  return rx_sort_nt<NT, BM, ST>(src, out_b, tmp, RxOut<R>{}, seg, key_b, key_bits, scheme, w, s, kFirstCounted);
 }
 
-template <class R, class KeyA, class KeyB>
-int rx_sort_pair(const R* in, R* out_a, R* out_b, R* tmp, const std::vector<uint64_t>& seg, KeyA key_a, KeyB key_b,
-                 int key_bits, RxWork& w, hipStream_t s) {
+template <int ST, class R, class KeyA, class KeyB>
+int rx_sort_pair_st(const R* in, RxOut<R> out_a, RxOut<R> out_b, RxOut<R> tmp, const std::vector<uint64_t>& seg,
+                    KeyA key_a, KeyB key_b, int key_bits, RxWork& w, hipStream_t s) {
   const SortVariant v = sort_variant(int(sizeof(R)));
   const int nseg = int(seg.size()) - 1;
   if (nseg < 1 || seg.back() == seg.front()) return EG_OK;
   if (seg.front() != 0) return fail(EG_ERR_ARG, "internal: sort segments must start at 0");
   if (seg.back() >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "sort of more than 2^32-1 records");
-  if (v.nt == 256) return rx_sort_pair_nt<256, 8>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+  if (v.nt == 256)
+    return rx_sort_pair_nt<256, 8, ST>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
   if constexpr (sizeof(R) <= 8) {
     if (v.nt >= 1024)
-      return rx_sort_pair_nt<1024, 8>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+      return rx_sort_pair_nt<1024, 8, ST>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
   }
-  if (v.bits >= 9) return rx_sort_pair_nt<512, 9>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
-  return rx_sort_pair_nt<512, 8>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+  if (v.bits >= 9)
+    return rx_sort_pair_nt<512, 9, ST>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+  return rx_sort_pair_nt<512, 8, ST>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+}
+
+template <class R, class KeyA, class KeyB>
+int rx_sort_pair(const R* in, R* out_a, R* out_b, R* tmp, const std::vector<uint64_t>& seg, KeyA key_a, KeyB key_b,
+                 int key_bits, RxWork& w, hipStream_t s) {
+  RxOut<R> a, b, t;
+  a.p = out_a;
+  b.p = out_b;
+  t.p = tmp;
+  return rx_sort_pair_st<kRecords>(in, a, b, t, seg, key_a, key_b, key_bits, w, s);
+}
+
+// The same with the two results (and the ping-pong) as field arrays.
+template <class R, class KeyA, class KeyB>
+int rx_sort_pair_fields(const R* in, RxSoa<R> out_a, RxSoa<R> out_b, RxSoa<R> tmp, const std::vector<uint64_t>& seg,
+                        KeyA key_a, KeyB key_b, int key_bits, RxWork& w, hipStream_t s) {
+  RxOut<R> a, b, t;
+  a.f = out_a;
+  b.f = out_b;
+  t.f = tmp;
+  return rx_sort_pair_st<kToFields>(in, a, b, t, seg, key_a, key_b, key_bits, w, s);
 }
 
 // Device workspace a three-launch sort of these segments takes with tiles of
@@ -736,7 +993,7 @@ inline uint64_t rx_three_launch_scan_bytes(const std::vector<uint64_t>& seg, uin
 template <class R, class KeyFn>
 int rx_sort(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w,
             hipStream_t s) {
-  return rx_sort(RxIn<R>{in, nullptr, nullptr, nullptr, 0}, out, tmp, seg, key, key_bits, w, s);
+  return rx_sort(RxIn<R>{in, nullptr, nullptr, nullptr, 0, {}}, out, tmp, seg, key, key_bits, w, s);
 }
 
 // Sort of a block view: the input in sort order is nblk contiguous blocks,
@@ -748,7 +1005,23 @@ template <class R, class KeyFn>
 int rx_sort_blocks(const R* in, const uint64_t* bdst, const uint64_t* bsrc, int nblk, R* out, R* tmp,
                    const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w, hipStream_t s) {
   if (nblk < 1) return fail(EG_ERR_ARG, "internal: a block view needs at least one block");
-  return rx_sort(RxIn<R>{in, nullptr, bdst, bsrc, nblk}, out, tmp, seg, key, key_bits, w, s);
+  return rx_sort(RxIn<R>{in, nullptr, bdst, bsrc, nblk, {}}, out, tmp, seg, key, key_bits, w, s);
+}
+
+// The same over field arrays: the blocks and the intermediate passes are
+// field arrays (tmp, and tmp2, which may be in's storage, since the first
+// pass has read in before the second writes tmp2); the result is records.
+template <class R, class KeyFn>
+int rx_sort_blocks_fields(RxSoa<R> in, const uint64_t* bdst, const uint64_t* bsrc, int nblk, R* out, RxSoa<R> tmp,
+                          RxSoa<R> tmp2, const std::vector<uint64_t>& seg, KeyFn key, int key_bits, RxWork& w,
+                          hipStream_t s) {
+  if (nblk < 1) return fail(EG_ERR_ARG, "internal: a block view needs at least one block");
+  RxOut<R> o, t, t2;
+  o.p = out;
+  t.f = tmp;
+  t2.f = tmp2;
+  return rx_sort_with<kFromFields>(sort_variant(int(sizeof(R))), RxIn<R>{nullptr, nullptr, bdst, bsrc, nblk, in}, o, t,
+                                   t2, seg, key, key_bits, w, s);
 }
 
 }  // namespace eg

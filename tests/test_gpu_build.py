@@ -37,6 +37,25 @@ def test_build_random_skewed(eg, oracle_lib, P, S, sym, weighted):
     assert_build_equal(g, og)
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_build_four_digit_passes(eg, oracle_lib, weighted):
+    """More than 2^24 vertices at P = 3: 25-bit keys take four 8-bit passes,
+    so the partition-local sorts pass their field arrays through both
+    temporaries twice before writing the records."""
+    print(f"build V > 2^24 P=3 weighted={weighted}", flush=True)
+    rng = np.random.default_rng(2024)
+    V = (1 << 24) + 5003
+    edges = H.random_edges(rng, V, 300_007, weighted=weighted)
+    keep = eg.get_sort_variant(edges.dtype.itemsize)
+    eg.set_sort_variant(edges.dtype.itemsize, 2, 512, 8)
+    try:
+        g, _ = _gpu_build(eg, edges, V, 3, 1, 1, 1, False)
+        og = oracle_lib.OracleGraph(edges, V, P=3, S=1, eta=1, m=1, threads=8)
+        assert_build_equal(g, og)
+    finally:
+        eg.set_sort_variant(edges.dtype.itemsize, *keep)
+
+
 @pytest.mark.parametrize("scale,P,sym", [(14, 1, False), (16, 1, False), (16, 8, False), (15, 4, True)])
 def test_build_rmat(eg, oracle_lib, scale, P, sym):
     print(f"build rmat scale={scale} P={P} sym={sym}", flush=True)
