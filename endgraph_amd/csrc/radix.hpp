@@ -345,9 +345,16 @@ __device__ __forceinline__ void rx_scatter(const RxOut<R>& out, const RxTile& x,
     goff[d] -= ds;  // no wrap: the global prefix of d counts this tile's smaller digits
   }
   __syncthreads();
+  // Every LDS read of a step is issued (unconditionally: the digits of lanes
+  // past the tile's end are those of a real record) before the first use,
+  // and the branches hold only the stores, so a thread's round trips overlap
+  // instead of one per record.
+  uint32_t at[kRxIPT];
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) at[k] = wcnt[w][dig[k]] + rank[k];
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k)
-    if (first + uint32_t(k) * kWave < x.count) stage[wcnt[w][dig[k]] + rank[k]] = item[k];
+    if (first + uint32_t(k) * kWave < x.count) stage[at[k]] = item[k];
   __syncthreads();
   R r[kRxIPT];
   uint32_t d[kRxIPT];
@@ -358,13 +365,15 @@ __device__ __forceinline__ void rx_scatter(const RxOut<R>& out, const RxTile& x,
   }
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) d[k] = uint32_t(key(r[k]) >> shift) & mask;
+  uint32_t go[kRxIPT];
+#pragma unroll
+  for (int k = 0; k < kRxIPT; ++k) go[k] = goff[d[k]];
 #pragma unroll
   for (int k = 0; k < kRxIPT; ++k) {
     const uint32_t lp = uint32_t(k) * NT + threadIdx.x;
-    if (lp < x.count) {
-      const uint64_t pos = uint64_t(goff[d[k]]) + lp;
-      if (pos < x.end) rx_store<SOUT>(out, pos, r[k]);  // always true unless a look-back gave up
-    }
+    const uint64_t pos = uint64_t(go[k]) + lp;
+    // pos < end always holds unless a look-back gave up
+    if (lp < x.count && pos < x.end) rx_store<SOUT>(out, pos, r[k]);
   }
 }
 
