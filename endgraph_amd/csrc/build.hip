@@ -683,18 +683,9 @@ inline uint64_t slice_start(uint64_t E0, int P, uint64_t mult, int r) {
 
 // Stable sorts of the rank slices r0..r1-1 (one segment each) by source
 // into a and by destination into b; the two sorts share their first read of
-// L (the first upsweep counts both first digits).
-template <class R, class KeyA, class KeyB>
-int rank_sorts(eg_graph* g, const R* L, R* a, R* b, R* tmp, uint64_t E0, int P, uint64_t mult, int r0, int r1,
-               KeyA key_a, KeyB key_b, int bits, hipStream_t s) {
-  const uint64_t b0 = slice_start(E0, P, mult, r0);
-  std::vector<uint64_t> seg(r1 - r0 + 1);
-  for (int r = r0; r <= r1; ++r) seg[r - r0] = slice_start(E0, P, mult, r) - b0;
-  return rx_sort_pair(L + b0, a + b0, b + b0, tmp + b0, seg, key_a, key_b, bits, g->sortw, s);
-}
-
-// The same with A, B and the ping-pong as field arrays (P > 1: A and B are
-// intermediate lists, read by key only in the degree and upsweep passes).
+// L (the first upsweep counts both first digits).  A, B and the ping-pong
+// are field arrays: with P > 1 A and B are intermediate lists, read by key
+// only in the degree and upsweep passes.
 template <class R, class KeyA, class KeyB>
 int rank_sorts_fields(eg_graph* g, const R* L, RxSoa<R> a, RxSoa<R> b, RxSoa<R> tmp, uint64_t E0, int P,
                       uint64_t mult, int r0, int r1, KeyA key_a, KeyB key_b, int bits, hipStream_t s) {
@@ -757,8 +748,11 @@ int build_setup(eg_graph* g, uint64_t E0, int sym, hipStream_t s) {
   EG_TRY(g->page_prefix.resize((pages + 1) * 8 + 2 * pages * 8));
   EG_TRY(g->meta.resize(sizeof(eg_graph::Meta)));
   EG_TRY(g->qmap.resize(pages * 2));
-  EG_TRY(g->csc_list.resize(Emax * rb));
-  EG_TRY(g->csr_list.resize(Emax * rb));
+  // P = 1: the two lists' buffers are also the first sorts' field-array
+  // ping-pong (they are free until the sorts within sub-partitions)
+  const uint64_t lb = P == 1 ? std::max(Emax * rb, rx_soa_bytes<R>(Emax)) : Emax * rb;
+  EG_TRY(g->csc_list.resize(lb));
+  EG_TRY(g->csr_list.resize(lb));
   const uint64_t cidx_cap = std::min<uint64_t>(E, uint64_t(Q) * V) + Q;
   EG_TRY(g->csc_cidx.resize(cidx_cap * sizeof(CIdx)));
   const uint64_t idx_n = uint64_t(Q) * (V + 1);
@@ -808,11 +802,16 @@ int build_slices(eg_graph* g, const R* d_in, uint64_t E0, int sym, int r0, int r
                              g->vbits, s));
     return EG_OK;
   }
-  R* A = g->t1.as<R>();
-  R* B = g->t2.as<R>();
-  R* T = g->csc_list.as<R>();  // ping-pong of the first sorts
-  EG_TRY(rank_sorts(g, L, A, B, T, E0, P, mult, r0, r1, KeyChecked<true>{g->V, g->err.as<uint32_t>()},
-                    KeyChecked<false>{g->V, nullptr}, g->vbits, s));
+  // P = 1: the whole list, records in and out, intermediate passes as field
+  // arrays in the CSC and CSR buffers
+  const uint64_t Emax = std::max<uint64_t>(g->E, 1);
+  const uint64_t b0 = slice_start(E0, P, mult, r0);
+  std::vector<uint64_t> seg(r1 - r0 + 1);
+  for (int r = r0; r <= r1; ++r) seg[r - r0] = slice_start(E0, P, mult, r) - b0;
+  EG_TRY(rx_sort_pair_through(L + b0, g->t1.as<R>() + b0, g->t2.as<R>() + b0,
+                              rx_soa_in<R>(g->csc_list.ptr, Emax).at(b0), rx_soa_in<R>(g->csr_list.ptr, Emax).at(b0),
+                              seg, KeyChecked<true>{g->V, g->err.as<uint32_t>()}, KeyChecked<false>{g->V, nullptr},
+                              g->vbits, g->sortw, s));
   return EG_OK;
 }
 

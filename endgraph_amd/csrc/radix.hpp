@@ -406,10 +406,18 @@ __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __rest
   constexpr int TILE = NT * kRxIPT;
   constexpr int W = NT / kWave;
   constexpr bool kTwo = !std::is_same<Key2, RxNoKey>::value;
-  __shared__ uint32_t h[W][1 << BM];
+  constexpr int kField = RxKeyField<KeyFn>::value;
+  // Key arrays of a later pass hold the keys in the previous digit's order,
+  // where a wave's lanes often share a digit (a vertex of high degree
+  // repeats): each 16-lane group counts into its own histogram, rows one
+  // bank apart, so the lanes of a group contend only among themselves.
+  constexpr bool kKeys = SIN && kField >= 0 && !kTwo;
+  constexpr int G = kKeys ? 4 : 1;         // histograms per wave
+  constexpr int HS = (1 << BM) + (G > 1);  // row stride in words
+  __shared__ uint32_t h[W * G * HS];
   __shared__ uint32_t h2[kTwo ? W : 1][kTwo ? (1 << BM) : 1];
   const int nb = 1 << bits;
-  for (int i = threadIdx.x; i < W * nb; i += NT) h[i / nb][i % nb] = 0;
+  for (int i = threadIdx.x; i < W * G * HS; i += NT) h[i] = 0;
   if constexpr (kTwo)
     for (int i = threadIdx.x; i < W * nb; i += NT) h2[i / nb][i % nb] = 0;
   __syncthreads();
@@ -417,9 +425,8 @@ __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __rest
   const RxTile x = rx_tile<TILE>(seg, ts, nseg, t);
   const int64_t sh = rx_tile_shift(in, t);
   const uint32_t mask = uint32_t(nb - 1);
-  uint32_t* hw = h[threadIdx.x / kWave];
-  constexpr int kField = RxKeyField<KeyFn>::value;
-  if constexpr (SIN && kField >= 0 && !kTwo) {
+  uint32_t* hw = h + ((threadIdx.x / kWave) * G + (threadIdx.x & (kWave - 1)) / (kWave / G)) * HS;
+  if constexpr (kKeys) {
     // field arrays: read the key's array only, four keys per 16-byte load
     // in full tiles
     const uint32_t* fa = kField == 0 ? in.f.s : in.f.d;
@@ -482,9 +489,10 @@ __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __rest
   for (int d = threadIdx.x; d < nb; d += NT) {
     uint32_t c = 0, c2 = 0;
 #pragma unroll
-    for (int ww = 0; ww < W; ++ww) {
-      c += h[ww][d];
-      if constexpr (kTwo) c2 += h2[ww][d];
+    for (int ww = 0; ww < W * G; ++ww) c += h[ww * HS + d];
+    if constexpr (kTwo) {
+#pragma unroll
+      for (int ww = 0; ww < W; ++ww) c2 += h2[ww][d];
     }
     const uint64_t at = uint64_t(ts[x.s]) * nb + uint64_t(d) * x.nt + x.lt;
     hist[at] = c;
@@ -691,8 +699,10 @@ enum RxFirst { kFirstOwn = 0, kFirstAlsoSecond = 1, kFirstCounted = 2 };
 // records and every pass's output in field arrays (kToFields: the result is
 // an intermediate list of the build); the input and every intermediate in
 // field arrays and the result in records (kFromFields: passes alternate
-// between tmp and tmp2, both field arrays, the last writes out).
-enum RxStorage { kRecords = 0, kToFields = 1, kFromFields = 2 };
+// between tmp and tmp2, both field arrays, the last writes out); the input
+// and the result in records, the intermediates in field arrays
+// (kThroughFields, tmp and tmp2 as in kFromFields).
+enum RxStorage { kRecords = 0, kToFields = 1, kFromFields = 2, kThroughFields = 3 };
 
 template <class R>
 RxIn<R> rx_in_of(const RxOut<R>& o) {
@@ -741,11 +751,11 @@ int rx_launch_onepass(const RxIn<R>& src, const RxOut<R>& dst, uint32_t tiles, c
 // pass p of `passes` (storage ST): the input / output is field arrays
 template <int ST>
 constexpr bool rx_pass_in_fields(int p) {
-  return ST == kFromFields || (ST == kToFields && p > 0);
+  return ST == kFromFields || ((ST == kToFields || ST == kThroughFields) && p > 0);
 }
 template <int ST>
 constexpr bool rx_pass_out_fields(int p, int passes) {
-  return ST == kToFields || (ST == kFromFields && p != passes - 1);
+  return ST == kToFields || ((ST == kFromFields || ST == kThroughFields) && p != passes - 1);
 }
 
 template <int NT, int BM, int ST = kRecords, class R, class KeyFn, class Key2 = RxNoKey>
@@ -779,7 +789,7 @@ int rx_sort_nt(RxIn<R> in, RxOut<R> out, RxOut<R> tmp, RxOut<R> tmp2, const std:
   // the others alternate out and tmp so that the last pass writes out
   auto dest = [&](int p) -> const RxOut<R>& {
     if (p == plan.passes - 1) return out;
-    if (ST == kFromFields) return p % 2 == 0 ? tmp : tmp2;
+    if (ST == kFromFields || ST == kThroughFields) return p % 2 == 0 ? tmp : tmp2;
     return (plan.passes - 1 - p) % 2 == 0 ? out : tmp;
   };
   RxIn<R> src = in;
@@ -796,13 +806,8 @@ int rx_sort_nt(RxIn<R> in, RxOut<R> out, RxOut<R> tmp, RxOut<R> tmp2, const std:
       uint32_t* tab = p == 0 && first == kFirstCounted ? w.table2.as<uint32_t>() : w.table.as<uint32_t>();
       const bool fin = rx_pass_in_fields<ST>(p), fout = rx_pass_out_fields<ST>(p, plan.passes);
       if (p == 0 && first == kFirstAlsoSecond) {
-        if (fin)
-          EG_TRY((rx_launch_up<NT, BM, ST == kFromFields>(src, tiles, d_seg, d_ts, nseg, key, plan.shift[p],
-                                                          plan.bits[p], tab, xcd, key2, w.table2.as<uint32_t>(), n,
-                                                          s)));
-        else
-          EG_TRY((rx_launch_up<NT, BM, false>(src, tiles, d_seg, d_ts, nseg, key, plan.shift[p], plan.bits[p], tab, xcd,
-                                              key2, w.table2.as<uint32_t>(), n, s)));
+        EG_TRY((rx_launch_up<NT, BM, ST == kFromFields>(src, tiles, d_seg, d_ts, nseg, key, plan.shift[p],
+                                                        plan.bits[p], tab, xcd, key2, w.table2.as<uint32_t>(), n, s)));
       } else if (!(p == 0 && first == kFirstCounted)) {
         const KeyFn kp = p == 0 ? key : quiet;
         if constexpr (ST == kRecords) {
@@ -828,13 +833,26 @@ int rx_sort_nt(RxIn<R> in, RxOut<R> out, RxOut<R> tmp, RxOut<R> tmp2, const std:
         else
           EG_TRY((rx_launch_down<NT, BM, false, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
                                                       plan.bits[p], tab, xcd, n, s)));
-      } else {
+      } else if constexpr (ST == kFromFields) {
         if (fout)
           EG_TRY((rx_launch_down<NT, BM, true, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
                                                      plan.bits[p], tab, xcd, n, s)));
         else
           EG_TRY((rx_launch_down<NT, BM, true, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
                                                       plan.bits[p], tab, xcd, n, s)));
+      } else {  // kThroughFields
+        if (fin && fout)
+          EG_TRY((rx_launch_down<NT, BM, true, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                     plan.bits[p], tab, xcd, n, s)));
+        else if (fin)
+          EG_TRY((rx_launch_down<NT, BM, true, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                      plan.bits[p], tab, xcd, n, s)));
+        else if (fout)
+          EG_TRY((rx_launch_down<NT, BM, false, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                      plan.bits[p], tab, xcd, n, s)));
+        else
+          EG_TRY((rx_launch_down<NT, BM, false, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                       plan.bits[p], tab, xcd, n, s)));
       }
       src = rx_in_of(dst);
     }
@@ -880,13 +898,26 @@ int rx_sort_nt(RxIn<R> in, RxOut<R> out, RxOut<R> tmp, RxOut<R> tmp2, const std:
       else
         EG_TRY((rx_launch_onepass<NT, BM, false, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
                                                        plan.bits[p], db, w, p, epoch, n, s)));
-    } else {
+    } else if constexpr (ST == kFromFields) {
       if (fout)
         EG_TRY((rx_launch_onepass<NT, BM, true, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
                                                       plan.bits[p], db, w, p, epoch, n, s)));
       else
         EG_TRY((rx_launch_onepass<NT, BM, true, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
                                                        plan.bits[p], db, w, p, epoch, n, s)));
+    } else {  // kThroughFields
+      if (fin && fout)
+        EG_TRY((rx_launch_onepass<NT, BM, true, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                      plan.bits[p], db, w, p, epoch, n, s)));
+      else if (fin)
+        EG_TRY((rx_launch_onepass<NT, BM, true, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                       plan.bits[p], db, w, p, epoch, n, s)));
+      else if (fout)
+        EG_TRY((rx_launch_onepass<NT, BM, false, true>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                       plan.bits[p], db, w, p, epoch, n, s)));
+      else
+        EG_TRY((rx_launch_onepass<NT, BM, false, false>(src, dst, tiles, d_seg, d_ts, nseg, quiet, plan.shift[p],
+                                                        plan.bits[p], db, w, p, epoch, n, s)));
     }
     src = rx_in_of(dst);
   }
@@ -932,50 +963,44 @@ int rx_sort(RxIn<R> in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn 
 // by key_b into out_b, each as rx_sort would do it.  Three-launch variants
 // read the input once for both first digits (the first key's first upsweep
 // also counts the second's); the one-pass variant runs the two sorts.
-// ST: kRecords, or kToFields (the results as field arrays).
+// ST: kToFields (the results as field arrays) or kThroughFields (records
+// in and out, the intermediate passes as field arrays).
 template <int NT, int BM, int ST, class R, class KeyA, class KeyB>
-int rx_sort_pair_nt(const R* in, RxOut<R> out_a, RxOut<R> out_b, RxOut<R> tmp, const std::vector<uint64_t>& seg,
-                    KeyA key_a, KeyB key_b, int key_bits, int scheme, RxWork& w, hipStream_t s) {
+int rx_sort_pair_nt(const R* in, RxOut<R> out_a, RxOut<R> out_b, RxOut<R> tmp, RxOut<R> tmp2,
+                    const std::vector<uint64_t>& seg, KeyA key_a, KeyB key_b, int key_bits, int scheme, RxWork& w,
+                    hipStream_t s) {
   const RxIn<R> src{in, nullptr, nullptr, nullptr, 0, {}};
   if (scheme == kOnePass) {
-    EG_TRY((rx_sort_nt<NT, BM, ST>(src, out_a, tmp, RxOut<R>{}, seg, key_a, key_bits, scheme, w, s)));
-    return rx_sort_nt<NT, BM, ST>(src, out_b, tmp, RxOut<R>{}, seg, key_b, key_bits, scheme, w, s);
+    EG_TRY((rx_sort_nt<NT, BM, ST>(src, out_a, tmp, tmp2, seg, key_a, key_bits, scheme, w, s)));
+    return rx_sort_nt<NT, BM, ST>(src, out_b, tmp, tmp2, seg, key_b, key_bits, scheme, w, s);
   }
-  EG_TRY((rx_sort_nt<NT, BM, ST>(src, out_a, tmp, RxOut<R>{}, seg, key_a, key_bits, scheme, w, s, kFirstAlsoSecond,
+  EG_TRY((rx_sort_nt<NT, BM, ST>(src, out_a, tmp, tmp2, seg, key_a, key_bits, scheme, w, s, kFirstAlsoSecond,
                                 rx_quiet(key_b))));
-  return rx_sort_nt<NT, BM, ST>(src, out_b, tmp, RxOut<R>{}, seg, key_b, key_bits, scheme, w, s, kFirstCounted);
+  return rx_sort_nt<NT, BM, ST>(src, out_b, tmp, tmp2, seg, key_b, key_bits, scheme, w, s, kFirstCounted);
 }
 
 template <int ST, class R, class KeyA, class KeyB>
-int rx_sort_pair_st(const R* in, RxOut<R> out_a, RxOut<R> out_b, RxOut<R> tmp, const std::vector<uint64_t>& seg,
-                    KeyA key_a, KeyB key_b, int key_bits, RxWork& w, hipStream_t s) {
+int rx_sort_pair_st(const R* in, RxOut<R> out_a, RxOut<R> out_b, RxOut<R> tmp, RxOut<R> tmp2,
+                    const std::vector<uint64_t>& seg, KeyA key_a, KeyB key_b, int key_bits, RxWork& w,
+                    hipStream_t s) {
   const SortVariant v = sort_variant(int(sizeof(R)));
   const int nseg = int(seg.size()) - 1;
   if (nseg < 1 || seg.back() == seg.front()) return EG_OK;
   if (seg.front() != 0) return fail(EG_ERR_ARG, "internal: sort segments must start at 0");
   if (seg.back() >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "sort of more than 2^32-1 records");
   if (v.nt == 256)
-    return rx_sort_pair_nt<256, 8, ST>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+    return rx_sort_pair_nt<256, 8, ST>(in, out_a, out_b, tmp, tmp2, seg, key_a, key_b, key_bits, v.scheme, w, s);
   if constexpr (sizeof(R) <= 8) {
     if (v.nt >= 1024)
-      return rx_sort_pair_nt<1024, 8, ST>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+      return rx_sort_pair_nt<1024, 8, ST>(in, out_a, out_b, tmp, tmp2, seg, key_a, key_b, key_bits, v.scheme, w, s);
   }
   if (v.bits >= 9)
-    return rx_sort_pair_nt<512, 9, ST>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
-  return rx_sort_pair_nt<512, 8, ST>(in, out_a, out_b, tmp, seg, key_a, key_b, key_bits, v.scheme, w, s);
+    return rx_sort_pair_nt<512, 9, ST>(in, out_a, out_b, tmp, tmp2, seg, key_a, key_b, key_bits, v.scheme, w, s);
+  return rx_sort_pair_nt<512, 8, ST>(in, out_a, out_b, tmp, tmp2, seg, key_a, key_b, key_bits, v.scheme, w, s);
 }
 
-template <class R, class KeyA, class KeyB>
-int rx_sort_pair(const R* in, R* out_a, R* out_b, R* tmp, const std::vector<uint64_t>& seg, KeyA key_a, KeyB key_b,
-                 int key_bits, RxWork& w, hipStream_t s) {
-  RxOut<R> a, b, t;
-  a.p = out_a;
-  b.p = out_b;
-  t.p = tmp;
-  return rx_sort_pair_st<kRecords>(in, a, b, t, seg, key_a, key_b, key_bits, w, s);
-}
-
-// The same with the two results (and the ping-pong) as field arrays.
+// Two sorts of the same records as above with the two results (and the
+// ping-pong) as field arrays.
 template <class R, class KeyA, class KeyB>
 int rx_sort_pair_fields(const R* in, RxSoa<R> out_a, RxSoa<R> out_b, RxSoa<R> tmp, const std::vector<uint64_t>& seg,
                         KeyA key_a, KeyB key_b, int key_bits, RxWork& w, hipStream_t s) {
@@ -983,7 +1008,21 @@ int rx_sort_pair_fields(const R* in, RxSoa<R> out_a, RxSoa<R> out_b, RxSoa<R> tm
   a.f = out_a;
   b.f = out_b;
   t.f = tmp;
-  return rx_sort_pair_st<kToFields>(in, a, b, t, seg, key_a, key_b, key_bits, w, s);
+  return rx_sort_pair_st<kToFields>(in, a, b, t, RxOut<R>{}, seg, key_a, key_b, key_bits, w, s);
+}
+
+// The same with records in and out and the intermediate passes in the field
+// arrays tmp and tmp2 (which must not overlap in, out_a or out_b).
+template <class R, class KeyA, class KeyB>
+int rx_sort_pair_through(const R* in, R* out_a, R* out_b, RxSoa<R> tmp, RxSoa<R> tmp2,
+                         const std::vector<uint64_t>& seg, KeyA key_a, KeyB key_b, int key_bits, RxWork& w,
+                         hipStream_t s) {
+  RxOut<R> a, b, t, t2;
+  a.p = out_a;
+  b.p = out_b;
+  t.f = tmp;
+  t2.f = tmp2;
+  return rx_sort_pair_st<kThroughFields>(in, a, b, t, t2, seg, key_a, key_b, key_bits, w, s);
 }
 
 // Device workspace a three-launch sort of these segments takes with tiles of
