@@ -115,20 +115,11 @@ int fill_t(T* p, uint64_t n, T v, hipStream_t s) {
 // every destination's in-edges in ascending source order: one sort of the
 // vertex-id width instead of one of twice that width over the CSC.
 struct KeyDstOnly {
+  static constexpr int kField = 1;
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
+  __device__ __forceinline__ uint64_t of_field(uint32_t k) const { return k; }
 };
-
-// SoA sources (and weights) of the sorted list.
-template <class R>
-__global__ __launch_bounds__(kNT) void k_tv_soa(const R* __restrict__ lst, uint64_t E, uint32_t* __restrict__ src,
-                                                float* __restrict__ w) {
-  for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += uint64_t(gridDim.x) * blockDim.x) {
-    const R r = lst[e];
-    src[e] = r.src;
-    if constexpr (sizeof(R) == 12) w[e] = reinterpret_cast<const Rec12&>(r).w;
-  }
-}
 
 __global__ void k_copy_u32(const uint32_t* __restrict__ a, uint32_t* __restrict__ b, uint64_t n) {
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
@@ -196,15 +187,20 @@ int tview_impl(eg_graph* g, hipStream_t s) {
     EG_HIP(hipGetLastError());
   }
   if (E) {
-    // the build's sort temporaries (t1, t2: E records each) are free after it
+    // The CSR sorted stably by destination, written as field arrays: the
+    // sources straight into tv_src (and the weights into tv_w), the
+    // destinations (implied by tv_ptr) into the build's sort temporary t1;
+    // t2 holds the intermediate passes.  Both are free after the build.
     EG_TRY(g->sortw.resize_segs(1));
     EG_TRY(fill_t<uint32_t>(g->sortw.err.as<uint32_t>(), 1, 0u, s));
-    R* sorted = g->t1.as<R>();
-    EG_TRY(rx_sort(g->csr_list.as<R>(), sorted, g->t2.as<R>(), std::vector<uint64_t>{0, E}, KeyDstOnly{}, g->vbits,
-                   g->sortw, s));
-    hipLaunchKernelGGL((k_tv_soa<R>), dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, (const R*)sorted, E,
-                       g->tv_src.as<uint32_t>(), g->tv_w.as<float>());
-    EG_HIP(hipGetLastError());
+    EG_TRY(g->t1.resize(E * 4));
+    EG_TRY(g->t2.resize(rx_soa_bytes<R>(E)));
+    RxSoa<R> out;
+    out.s = g->tv_src.as<uint32_t>();
+    out.d = g->t1.as<uint32_t>();
+    out.w = sizeof(R) == 12 ? g->tv_w.as<float>() : nullptr;
+    EG_TRY(rx_sort_to_fields(g->csr_list.as<R>(), out, rx_soa_in<R>(g->t2.ptr, E), std::vector<uint64_t>{0, E},
+                             KeyDstOnly{}, g->vbits, g->sortw, s));
   }
   if (E) {
     uint32_t err = 0;

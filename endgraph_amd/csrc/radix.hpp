@@ -1044,6 +1044,19 @@ int rx_sort(const R* in, R* out, R* tmp, const std::vector<uint64_t>& seg, KeyFn
   return rx_sort(RxIn<R>{in, nullptr, nullptr, nullptr, 0, {}}, out, tmp, seg, key, key_bits, w, s);
 }
 
+// Sort of records into field arrays: every pass writes field arrays (out or
+// tmp, alternating so that the last writes out), the later ones read only
+// the key's array in their upsweeps.
+template <class R, class KeyFn>
+int rx_sort_to_fields(const R* in, RxSoa<R> out, RxSoa<R> tmp, const std::vector<uint64_t>& seg, KeyFn key,
+                      int key_bits, RxWork& w, hipStream_t s) {
+  RxOut<R> o, t;
+  o.f = out;
+  t.f = tmp;
+  return rx_sort_with<kToFields>(sort_variant(int(sizeof(R))), RxIn<R>{in, nullptr, nullptr, nullptr, 0, {}}, o, t,
+                                 RxOut<R>{}, seg, key, key_bits, w, s);
+}
+
 // Sort of a block view: the input in sort order is nblk contiguous blocks,
 // block b = in[bsrc[b], bsrc[b] + bdst[b+1] - bdst[b]) placed at bdst[b]
 // (device tables; bdst[0] == 0, bdst[nblk] == the records).  The first pass
