@@ -576,6 +576,55 @@ __global__ __launch_bounds__(kNT) void k_csr_long_gaps(uint64_t* __restrict__ in
   }
 }
 
+// ---- one sub-partition (P = S = 1): both indexes from the degrees ----
+// The CSC holds the records by destination and the CSR by source, each in
+// one run per vertex, so a vertex's run starts at the exclusive prefix of
+// its degree: the compressed CSC index is {prefix of in-degree, v} for the
+// v with in-edges, then {E, V}; the dense CSR index is the prefix of the
+// out-degree over V + 1 entries (the run-scanning kernels give the same
+// arrays for Q > 1 one sub-partition at a time).  deg as u64 (entry V = 0)
+// and flags deg > 0 as u32 (entry V = 0), for the scans.
+__global__ void k_deg_u64(const uint32_t* __restrict__ deg, uint32_t V, uint64_t* __restrict__ out) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v <= V; v += uint64_t(gridDim.x) * blockDim.x)
+    out[v] = v < V ? deg[v] : 0ull;
+}
+__global__ void k_deg_flags(const uint32_t* __restrict__ deg, uint32_t V, uint32_t* __restrict__ out) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v <= V; v += uint64_t(gridDim.x) * blockDim.x)
+    out[v] = v < V && deg[v] > 0 ? 1u : 0u;
+}
+// pos, slot: the exclusive scans of k_deg_u64 and k_deg_flags over in_deg
+__global__ void k_cindex_from_deg(const uint32_t* __restrict__ in_deg, const uint64_t* __restrict__ pos,
+                                  const uint32_t* __restrict__ slot, uint32_t V, uint64_t E,
+                                  eg_graph::Meta* __restrict__ meta, CIdx* __restrict__ out) {
+  for (uint64_t v = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += uint64_t(gridDim.x) * blockDim.x)
+    if (in_deg[v]) {
+      CIdx c;
+      c.index = pos[v];
+      c.vertex = uint32_t(v);
+      out[slot[v]] = c;
+    }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CIdx c;
+    c.index = E;
+    c.vertex = V;
+    out[slot[V]] = c;
+    meta->cidx_off[0] = 0;
+    meta->cidx_cnt[0] = uint64_t(slot[V]) + 1;
+  }
+}
+// bit u of the CSR bitmap: u has out-edges; one thread per word
+__global__ void k_bitmap_from_deg(const uint32_t* __restrict__ deg, uint32_t V, uint64_t words,
+                                  uint64_t* __restrict__ bitmap) {
+  for (uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words; w += uint64_t(gridDim.x) * blockDim.x) {
+    uint64_t b = 0;
+    for (int i = 0; i < 64; ++i) {
+      const uint64_t u = w * 64 + i;
+      if (u < V && deg[u] > 0) b |= 1ull << i;
+    }
+    bitmap[w] = b;
+  }
+}
+
 // Rows of empty sub-partitions (no records, so no run writes them): all 0.
 __global__ __launch_bounds__(kNT) void k_csr_empty_rows(uint64_t* __restrict__ index, uint32_t V,
                                                         const eg_graph::Meta* __restrict__ meta) {
@@ -766,7 +815,8 @@ int build_setup(eg_graph* g, uint64_t E0, int sym, hipStream_t s) {
   if (P > 1 || Q > 1) EG_TRY(g->t3.resize(tb));
   const uint64_t hist_n = 1;
   const uint64_t scan_n = std::max<uint64_t>({scan_tmp_elems(hist_n), scan_tmp_elems(pages + 1),
-                                              scan_tmp_elems(ceil_div(E, kChunk) + 1)});
+                                              scan_tmp_elems(ceil_div(E, kChunk) + 1),
+                                              Q == 1 ? scan_tmp_elems(uint64_t(V) + 1) : 0});
   EG_TRY(g->scan_tmp.resize(scan_n * 8));
   EG_TRY(g->tile_cnt.resize((ceil_div(E, kChunk) + 2) * 4));
   EG_TRY(g->err.resize(4));
@@ -924,61 +974,90 @@ int build_finish(eg_graph* g, hipStream_t s) {
     EG_TRY(whole_sort(g, (const R*)A, g->csr_list.as<R>(), g->t3.as<R>(), E, KeyQOnly<false>{qmap}, qb, s));
   }
 
-  // --- compressed CSC index ---
-  {
-    const R* a = g->csc_list.as<R>();
-    KeyQ<true> key{qmap, g->vbits};
-    const uint64_t nch = ceil_div(E, kChunk);
-    uint32_t* tc = g->tile_cnt.as<uint32_t>();
+  if (Q == 1) {
+    // --- both indexes from the degrees (one sub-partition) ---
     CIdx* cidx = g->csc_cidx.as<CIdx>();
-    if (E) {
-      hipLaunchKernelGGL((k_run_count<R, KeyQ<true>>), dim3(chunk_grid(E)), dim3(kNT), 0, s, a, E, key, tc, nch);
-      EG_HIP(hipGetLastError());
-      EG_TRY(exclusive_scan_u32(tc, tc, nch + 1, stmp32, s));
-      prof_begin(s);
-      hipLaunchKernelGGL((k_cindex_write<R>), dim3(chunk_grid(E)), dim3(kNT), 0, s, a, E, key, (const uint32_t*)tc,
-                         meta, cidx);
-      prof_end(PK_CINDEX, E * rb, s);
-      EG_HIP(hipGetLastError());
-    } else {
-      EG_TRY(fill<uint32_t>(tc, 2, 0u, s));
-    }
-    hipLaunchKernelGGL((k_cindex_sentinel<R>), dim3(Q), dim3(kNT), 0, s, a, E, key, (const uint32_t*)tc, V, meta,
-                       cidx);
+    EG_TRY(g->idx_tmp.resize((uint64_t(V) + 1) * 12 + 64));
+    uint64_t* pos = g->idx_tmp.as<uint64_t>();
+    uint32_t* slot = reinterpret_cast<uint32_t*>(pos + uint64_t(V) + 1);
+    const unsigned gv = grid_for(uint64_t(V) + 1);
+    hipLaunchKernelGGL(k_deg_u64, dim3(gv), dim3(kNT), 0, s, (const uint32_t*)in_deg, V, pos);
     EG_HIP(hipGetLastError());
-  }
-
-  // --- dense CSR index + bitmap: runs write their gaps, long gaps go to a
-  // list written by a second launch, empty sub-partitions by a third ---
-  {
+    hipLaunchKernelGGL(k_deg_flags, dim3(gv), dim3(kNT), 0, s, (const uint32_t*)in_deg, V, slot);
+    EG_HIP(hipGetLastError());
+    EG_TRY(exclusive_scan_u64(pos, pos, uint64_t(V) + 1, g->scan_tmp.as<uint64_t>(), s));
+    EG_TRY(exclusive_scan_u32(slot, slot, uint64_t(V) + 1, stmp32, s));
+    prof_begin(s);
+    hipLaunchKernelGGL(k_cindex_from_deg, dim3(grid_for(V)), dim3(kNT), 0, s, (const uint32_t*)in_deg,
+                       (const uint64_t*)pos, (const uint32_t*)slot, V, E, meta, cidx);
+    prof_end(PK_CINDEX, uint64_t(V) * 16, s);
+    EG_HIP(hipGetLastError());
     uint64_t* index = g->csr_index.as<uint64_t>();
-    uint64_t* bitmap = g->csr_bitmap.as<uint64_t>();
-    // a gap of L > kShortGap entries takes ceil(L / kGapPiece) <= L / 65 + 1
-    // pieces and the gaps of a row add up to at most V + 1 entries, so the
-    // list cannot fill up (the fallback in put_gap is never taken)
-    const uint64_t max_pieces = 2 * idx_n / (kShortGap + 1) + 1;
-    if (max_pieces >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "CSR index of %llu entries is too large",
-                                                       (unsigned long long)idx_n);
-    const uint32_t max_gaps = uint32_t(max_pieces);
-    EG_TRY(g->idx_tmp.resize(uint64_t(max_gaps) * sizeof(GapPiece) + 64));
-    GapPiece* gaps = g->idx_tmp.as<GapPiece>();
-    uint32_t* ngaps = reinterpret_cast<uint32_t*>(gaps + max_gaps);
-    EG_TRY(fill<uint64_t>(bitmap, uint64_t(Q) * words, uint64_t(0), s));
-    EG_TRY(fill<uint32_t>(ngaps, 1, 0u, s));
-    if (E) {
-      prof_begin(s);
-      hipLaunchKernelGGL((k_csr_gaps<R>), dim3(chunk_grid(E)), dim3(kNT), 0, s, g->csr_list.as<R>(), E,
-                         KeyQ<false>{qmap, g->vbits}, V, (const eg_graph::Meta*)meta, index,
-                         reinterpret_cast<unsigned long long*>(bitmap), gaps, ngaps, max_gaps);
-      prof_end(PK_CSR_INDEX, E * rb, s);
-      EG_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_csr_long_gaps, dim3(1024), dim3(kNT), 0, s, index, (const GapPiece*)gaps,
-                         (const uint32_t*)ngaps, max_gaps);
+    hipLaunchKernelGGL(k_deg_u64, dim3(gv), dim3(kNT), 0, s, (const uint32_t*)out_deg, V, index);
+    EG_HIP(hipGetLastError());
+    prof_begin(s);
+    EG_TRY(exclusive_scan_u64(index, index, uint64_t(V) + 1, g->scan_tmp.as<uint64_t>(), s));
+    prof_end(PK_CSR_INDEX, (uint64_t(V) + 1) * 16, s);
+    hipLaunchKernelGGL(k_bitmap_from_deg, dim3(grid_for(words)), dim3(kNT), 0, s, (const uint32_t*)out_deg, V, words,
+                       g->csr_bitmap.as<uint64_t>());
+    EG_HIP(hipGetLastError());
+  } else {
+    // --- compressed CSC index ---
+    {
+      const R* a = g->csc_list.as<R>();
+      KeyQ<true> key{qmap, g->vbits};
+      const uint64_t nch = ceil_div(E, kChunk);
+      uint32_t* tc = g->tile_cnt.as<uint32_t>();
+      CIdx* cidx = g->csc_cidx.as<CIdx>();
+      if (E) {
+        hipLaunchKernelGGL((k_run_count<R, KeyQ<true>>), dim3(chunk_grid(E)), dim3(kNT), 0, s, a, E, key, tc, nch);
+        EG_HIP(hipGetLastError());
+        EG_TRY(exclusive_scan_u32(tc, tc, nch + 1, stmp32, s));
+        prof_begin(s);
+        hipLaunchKernelGGL((k_cindex_write<R>), dim3(chunk_grid(E)), dim3(kNT), 0, s, a, E, key, (const uint32_t*)tc,
+                           meta, cidx);
+        prof_end(PK_CINDEX, E * rb, s);
+        EG_HIP(hipGetLastError());
+      } else {
+        EG_TRY(fill<uint32_t>(tc, 2, 0u, s));
+      }
+      hipLaunchKernelGGL((k_cindex_sentinel<R>), dim3(Q), dim3(kNT), 0, s, a, E, key, (const uint32_t*)tc, V, meta,
+                         cidx);
       EG_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_csr_empty_rows, dim3(64, unsigned(Q)), dim3(kNT), 0, s, index, V,
-                       (const eg_graph::Meta*)meta);
-    EG_HIP(hipGetLastError());
+
+    // --- dense CSR index + bitmap: runs write their gaps, long gaps go to a
+    // list written by a second launch, empty sub-partitions by a third ---
+    {
+      uint64_t* index = g->csr_index.as<uint64_t>();
+      uint64_t* bitmap = g->csr_bitmap.as<uint64_t>();
+      // a gap of L > kShortGap entries takes ceil(L / kGapPiece) <= L / 65 + 1
+      // pieces and the gaps of a row add up to at most V + 1 entries, so the
+      // list cannot fill up (the fallback in put_gap is never taken)
+      const uint64_t max_pieces = 2 * idx_n / (kShortGap + 1) + 1;
+      if (max_pieces >= (uint64_t(1) << 32)) return fail(EG_ERR_ARG, "CSR index of %llu entries is too large",
+                                                         (unsigned long long)idx_n);
+      const uint32_t max_gaps = uint32_t(max_pieces);
+      EG_TRY(g->idx_tmp.resize(uint64_t(max_gaps) * sizeof(GapPiece) + 64));
+      GapPiece* gaps = g->idx_tmp.as<GapPiece>();
+      uint32_t* ngaps = reinterpret_cast<uint32_t*>(gaps + max_gaps);
+      EG_TRY(fill<uint64_t>(bitmap, uint64_t(Q) * words, uint64_t(0), s));
+      EG_TRY(fill<uint32_t>(ngaps, 1, 0u, s));
+      if (E) {
+        prof_begin(s);
+        hipLaunchKernelGGL((k_csr_gaps<R>), dim3(chunk_grid(E)), dim3(kNT), 0, s, g->csr_list.as<R>(), E,
+                           KeyQ<false>{qmap, g->vbits}, V, (const eg_graph::Meta*)meta, index,
+                           reinterpret_cast<unsigned long long*>(bitmap), gaps, ngaps, max_gaps);
+        prof_end(PK_CSR_INDEX, E * rb, s);
+        EG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_csr_long_gaps, dim3(1024), dim3(kNT), 0, s, index, (const GapPiece*)gaps,
+                           (const uint32_t*)ngaps, max_gaps);
+        EG_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL(k_csr_empty_rows, dim3(64, unsigned(Q)), dim3(kNT), 0, s, index, V,
+                         (const eg_graph::Meta*)meta);
+      EG_HIP(hipGetLastError());
+    }
   }
 
   // --- small tables back to the host ---
