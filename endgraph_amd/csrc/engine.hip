@@ -324,8 +324,10 @@ int ensure_engine(eg_graph* g, hipStream_t s) {
 // Both hold the CSC rows' terms; a different order moves an FP64 row sum by
 // a few ulps and a compensated FP32 row sum not at all at the FP32 rounding.
 struct KeyDstField {
+  static constexpr int kField = 1;
   template <class R>
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
+  __device__ __forceinline__ uint64_t of_field(uint32_t k) const { return k; }
 };
 
 // record v = (v, E - deg[v]): ascending key = descending degree
@@ -770,19 +772,16 @@ __global__ void k_copy_u32_v(const uint32_t* __restrict__ a, uint32_t* __restric
     b[i] = a[i];
 }
 
-__global__ __launch_bounds__(kNT) void k_view_soa(const Rec8* __restrict__ lst, uint64_t E, uint32_t* __restrict__ src,
-                                                   uint64_t* __restrict__ rowflag, uint32_t* __restrict__ cnt) {
+// Row starts of the sorted list from its destination array: bit e of the
+// row bits is set where the destination changes; cnt = heads per word.
+__global__ __launch_bounds__(kNT) void k_view_flags(const uint32_t* __restrict__ dst, uint64_t E,
+                                                     uint64_t* __restrict__ rowflag, uint32_t* __restrict__ cnt) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t nwords = ceil_div(E, 64);
   const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
   for (uint64_t wd = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave; wd < nwords; wd += nw) {
     const uint64_t e = wd * 64 + lane;
-    bool head = false;
-    if (e < E) {
-      const Rec8 r = lst[e];
-      src[e] = r.src;
-      head = e == 0 || lst[e - 1].dst != r.dst;
-    }
+    const bool head = e < E && (e == 0 || dst[e - 1] != dst[e]);
     const uint64_t m = __ballot(head);
     if (lane == 0) {
       rowflag[wd] = m;
@@ -791,13 +790,13 @@ __global__ __launch_bounds__(kNT) void k_view_soa(const Rec8* __restrict__ lst, 
   }
 }
 
-__global__ void k_view_row_dst(const Rec8* __restrict__ lst, uint64_t E, const uint64_t* __restrict__ rowflag,
+__global__ void k_view_row_dst(const uint32_t* __restrict__ dst, uint64_t E, const uint64_t* __restrict__ rowflag,
                                const uint32_t* __restrict__ wordrank, const uint32_t* __restrict__ perm,
                                uint32_t* __restrict__ row_dst) {
   for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += uint64_t(gridDim.x) * blockDim.x) {
     const uint64_t f = rowflag[e >> 6];
     if ((f >> (e & 63)) & 1ull)
-      row_dst[wordrank[e >> 6] + uint32_t(__popcll(f & ((1ull << (e & 63)) - 1)))] = perm[lst[e].dst];
+      row_dst[wordrank[e >> 6] + uint32_t(__popcll(f & ((1ull << (e & 63)) - 1)))] = perm[dst[e]];
   }
 }
 
@@ -814,23 +813,28 @@ int sorted_rows(eg_graph* g, hipStream_t s) {
   EG_TRY(fill<uint32_t>(noff + V, 1, 0u, s));
   EG_TRY(exclusive_scan_u32(noff, noff, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
   EG_TRY(g->t1.resize(E * 8));
-  EG_TRY(g->t2.resize(E * 8));
-  EG_TRY(g->t3.resize(E * 8));
+  EG_TRY(g->t2.resize(E * 4));
+  EG_TRY(g->t3.resize(rx_soa_bytes<Rec8>(E)));
   Rec8* rec = g->t1.as<Rec8>();
-  Rec8* sorted = g->t2.as<Rec8>();
+  uint32_t* sdst = g->t2.as<uint32_t>();  // destinations of the sorted rows
   hipLaunchKernelGGL((k_expand<RowCopyOp>), dim3(unsigned(ceil_div(E, kExpSlots))), dim3(kNT), 0, s,
                      (const uint32_t*)g->pv_inv.as<uint32_t>(), (const uint32_t*)noff, V,
                      (const uint32_t*)g->e_out_ptr.as<uint32_t>(), RowCopyOp{g->e_out_dst.as<uint32_t>(), rec});
   EG_HIP(hipGetLastError());
   EG_TRY(g->sortw.resize_segs(1));
   EG_TRY(fill<uint32_t>(g->sortw.err.as<uint32_t>(), 1, 0u, s));
-  EG_TRY(rx_sort((const Rec8*)rec, sorted, g->t3.as<Rec8>(), std::vector<uint64_t>{0, E}, KeyDstField{}, g->vbits,
-                 g->sortw, s));
+  // sorted by destination straight into field arrays: the sources are the
+  // view's, the destinations give the row starts
+  RxSoa<Rec8> out;
+  out.s = g->pv_src.as<uint32_t>();
+  out.d = sdst;
+  EG_TRY(rx_sort_to_fields((const Rec8*)rec, out, rx_soa_in<Rec8>(g->t3.ptr, E), std::vector<uint64_t>{0, E},
+                           KeyDstField{}, g->vbits, g->sortw, s));
   EG_TRY(g->pv_rowflag.resize((nwords + 1) * 8));
   EG_TRY(g->pv_wordrank.resize((nwords + 2) * 4));
   uint32_t* wr = g->pv_wordrank.as<uint32_t>();
-  hipLaunchKernelGGL(k_view_soa, dim3(grid_for(nwords, kNT / kWave, 65536)), dim3(kNT), 0, s, (const Rec8*)sorted, E,
-                     g->pv_src.as<uint32_t>(), g->pv_rowflag.as<uint64_t>(), wr);
+  hipLaunchKernelGGL(k_view_flags, dim3(grid_for(nwords, kNT / kWave, 65536)), dim3(kNT), 0, s, (const uint32_t*)sdst,
+                     E, g->pv_rowflag.as<uint64_t>(), wr);
   EG_HIP(hipGetLastError());
   EG_TRY(fill<uint32_t>(wr + nwords, 1, 0u, s));
   EG_TRY(exclusive_scan_u32(wr, wr, nwords + 1, g->scan_tmp.as<uint32_t>(), s));
@@ -838,7 +842,7 @@ int sorted_rows(eg_graph* g, hipStream_t s) {
   EG_HIP(hipMemcpyAsync(&R, wr + nwords, 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
   EG_TRY(g->pv_row_dst.resize((uint64_t(R) + 1) * 4));
-  hipLaunchKernelGGL(k_view_row_dst, dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, (const Rec8*)sorted, E,
+  hipLaunchKernelGGL(k_view_row_dst, dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, (const uint32_t*)sdst, E,
                      (const uint64_t*)g->pv_rowflag.as<uint64_t>(), (const uint32_t*)wr,
                      (const uint32_t*)g->pv_perm.as<uint32_t>(), g->pv_row_dst.as<uint32_t>());
   EG_HIP(hipGetLastError());
