@@ -22,6 +22,10 @@
 // digit's run is written contiguously (bigger tiles => longer runs => fewer
 // partially written lines).  Variants are chosen per record size by
 // select_sort_variant (see eg_autotune_sort).
+//
+// A sort's input, intermediate lists and result are each either records or
+// field arrays (src[], dst[], w[]; RxStorage): where a list is only an
+// intermediate, its later upsweeps read the key's array alone.
 #pragma once
 
 #include <type_traits>
