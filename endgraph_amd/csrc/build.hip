@@ -1194,9 +1194,12 @@ int eg_autotune_sort(eg_graph* g, const void* d_edges, uint64_t E0, int symmetri
     int scheme, nt, bits;
   };
   std::vector<Cand> cands;
+  // 9-bit digits change nothing unless they save a pass on the vertex keys
+  const int vb = std::max(1, counting_bits(g->V - 1));
+  const bool nine = rx_plan(vb, 9).passes < rx_plan(vb, 8).passes;
   for (int sc : {kThreeLaunch, kThreeLaunchXcd, kOnePass}) {
     cands.push_back({sc, 512, 8});
-    cands.push_back({sc, 512, 9});
+    if (nine) cands.push_back({sc, 512, 9});
     if (sc != kOnePass) cands.push_back({sc, 256, 8});
     if (g->rec == 8) cands.push_back({sc, 1024, 8});
   }

@@ -49,6 +49,12 @@ struct RxPlan {
   int bits[kRxMaxPasses] = {0};
 };
 
+// Per-wave digit counters of a tile (after the scan, digit starts in the
+// tile) stay below 2^16, a tile holding at most 16384 records: 9-bit digits
+// keep them as 16-bit words so that two 512-thread tiles share a CU's LDS.
+template <int BM>
+using RxWc = std::conditional_t<(BM > 8), uint16_t, uint32_t>;
+
 inline RxPlan rx_plan(int key_bits, int max_bits) {
   RxPlan p;
   if (key_bits <= 0) key_bits = 1;
@@ -264,7 +270,7 @@ constexpr unsigned kLgkmZero = 0xC07F;
 template <int NT, int BM, bool SIN, class R, class KeyFn>
 __device__ __forceinline__ void rx_rank(const RxIn<R>& in, int64_t sh, const RxTile& x, KeyFn key, int shift, int bits,
                                         R (&item)[kRxIPT], uint32_t (&dig)[kRxIPT], uint32_t (&rank)[kRxIPT],
-                                        uint32_t (*wcnt)[1 << BM], uint32_t* tot, uint64_t* peer_tab) {
+                                        RxWc<BM> (*wcnt)[1 << BM], uint32_t* tot, uint64_t* peer_tab) {
   constexpr int W = NT / kWave;
   const int nb = 1 << bits;
   const uint32_t mask = uint32_t(nb - 1);
@@ -333,7 +339,7 @@ __device__ __forceinline__ void rx_rank(const RxIn<R>& in, int64_t sh, const RxT
 template <int NT, int BM, bool SOUT, class R, class KeyFn>
 __device__ __forceinline__ void rx_scatter(const RxOut<R>& out, const RxTile& x, KeyFn key, int shift, int bits,
                                            const R (&item)[kRxIPT], const uint32_t (&dig)[kRxIPT],
-                                           const uint32_t (&rank)[kRxIPT], uint32_t (*wcnt)[1 << BM],
+                                           const uint32_t (&rank)[kRxIPT], RxWc<BM> (*wcnt)[1 << BM],
                                            uint32_t* goff, const uint32_t* dstart, R* stage) {
   const uint32_t mask = uint32_t((1 << bits) - 1);
   const int lane = threadIdx.x & (kWave - 1);
@@ -511,7 +517,8 @@ __global__ __launch_bounds__(NT) void k_rx_down(RxIn<R> in, RxOut<R> out,
                                                 const uint32_t* __restrict__ offs, int xcd_order) {
   constexpr int TILE = NT * kRxIPT;
   constexpr int W = NT / kWave;
-  __shared__ uint32_t wcnt[W][1 << BM];
+  static_assert(TILE < 65536, "16-bit digit counters");
+  __shared__ RxWc<BM> wcnt[W][1 << BM];
   __shared__ uint32_t goff[1 << BM];
   __shared__ uint32_t dstart[1 << BM];
   __shared__ uint32_t wt[W];
@@ -575,7 +582,8 @@ __global__ __launch_bounds__(NT) void k_rx_onepass(RxIn<R> in, RxOut<R> out,
                                                    uint32_t* __restrict__ err) {
   constexpr int TILE = NT * kRxIPT;
   constexpr int W = NT / kWave;
-  __shared__ uint32_t wcnt[W][1 << BM];
+  static_assert(TILE < 65536, "16-bit digit counters");
+  __shared__ RxWc<BM> wcnt[W][1 << BM];
   __shared__ uint32_t goff[1 << BM];
   __shared__ uint32_t dstart[1 << BM];
   __shared__ uint32_t wt[W];
@@ -676,7 +684,7 @@ __global__ __attribute__((unused)) void k_rx_zero(uint32_t* __restrict__ p, uint
 struct SortVariant {
   int scheme = kThreeLaunch;
   int nt = 256;
-  int bits = 8;  // widest digit: 8, or 9 with 512 threads
+  int bits = 8;  // widest digit: 8, or 9 with 512 threads (used where it saves a pass)
 };
 
 // process-wide choice per record size (eg_set_sort_variant / autotuner)
@@ -944,7 +952,10 @@ int rx_sort_with(const SortVariant& v, RxIn<R> in, RxOut<R> out, RxOut<R> tmp, R
   if constexpr (sizeof(R) <= 8) {
     if (v.nt >= 1024) return rx_sort_nt<1024, 8, ST>(in, out, tmp, tmp2, seg, key, key_bits, v.scheme, w, s);
   }
-  if (v.bits >= 9) return rx_sort_nt<512, 9, ST>(in, out, tmp, tmp2, seg, key, key_bits, v.scheme, w, s);
+  // 9-bit digits only where they save a pass (25-27-bit keys: 3 passes
+  // instead of 4); otherwise the 8-bit kernel, whose scatter writes longer runs
+  if (v.bits >= 9 && rx_plan(key_bits, 9).passes < rx_plan(key_bits, 8).passes)
+    return rx_sort_nt<512, 9, ST>(in, out, tmp, tmp2, seg, key, key_bits, v.scheme, w, s);
   return rx_sort_nt<512, 8, ST>(in, out, tmp, tmp2, seg, key, key_bits, v.scheme, w, s);
 }
 
@@ -998,7 +1009,7 @@ int rx_sort_pair_st(const R* in, RxOut<R> out_a, RxOut<R> out_b, RxOut<R> tmp, R
     if (v.nt >= 1024)
       return rx_sort_pair_nt<1024, 8, ST>(in, out_a, out_b, tmp, tmp2, seg, key_a, key_b, key_bits, v.scheme, w, s);
   }
-  if (v.bits >= 9)
+  if (v.bits >= 9 && rx_plan(key_bits, 9).passes < rx_plan(key_bits, 8).passes)
     return rx_sort_pair_nt<512, 9, ST>(in, out_a, out_b, tmp, tmp2, seg, key_a, key_b, key_bits, v.scheme, w, s);
   return rx_sort_pair_nt<512, 8, ST>(in, out_a, out_b, tmp, tmp2, seg, key_a, key_b, key_bits, v.scheme, w, s);
 }

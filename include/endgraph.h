@@ -73,7 +73,8 @@ int eg_profile_read(const char* kernel, double* total_ms, uint64_t* launches, ui
  * pass per digit with decoupled look-back, 2: as 0 with each XCD taking one
  * contiguous range of tiles), a tile of threads x 16 records (threads 256, 512 or 1024;
  * 1024 only for 8-byte records) and digits of up to 8 or 9 bits (9 only with
- * 512 threads).  Process-wide per record size; chosen by eg_autotune_sort or
+ * 512 threads, and taken by a sort only where it saves a pass, as for 25-27
+ * bit keys).  Process-wide per record size; chosen by eg_autotune_sort or
  * set here.  Results never depend on the variant. */
 int eg_set_sort_variant(int rec_bytes, int scheme, int threads, int digit_bits);
 int eg_get_sort_variant(int rec_bytes, int* scheme, int* threads, int* digit_bits);

@@ -37,17 +37,19 @@ def test_build_random_skewed(eg, oracle_lib, P, S, sym, weighted):
     assert_build_equal(g, og)
 
 
+@pytest.mark.parametrize("bits", [8, 9])
 @pytest.mark.parametrize("weighted", [False, True])
-def test_build_four_digit_passes(eg, oracle_lib, weighted):
-    """More than 2^24 vertices at P = 3: 25-bit keys take four 8-bit passes,
-    so the partition-local sorts pass their field arrays through both
-    temporaries twice before writing the records."""
-    print(f"build V > 2^24 P=3 weighted={weighted}", flush=True)
+def test_build_four_digit_passes(eg, oracle_lib, weighted, bits):
+    """More than 2^24 vertices at P = 3: 25-bit keys take four passes of at
+    most 8 bits, so the partition-local sorts pass their field arrays through
+    both temporaries twice before writing the records; with 9-bit digits
+    they take three passes (9, 9, 7 bits)."""
+    print(f"build V > 2^24 P=3 weighted={weighted} bits={bits}", flush=True)
     rng = np.random.default_rng(2024)
     V = (1 << 24) + 5003
     edges = H.random_edges(rng, V, 300_007, weighted=weighted)
     keep = eg.get_sort_variant(edges.dtype.itemsize)
-    eg.set_sort_variant(edges.dtype.itemsize, 2, 512, 8)
+    eg.set_sort_variant(edges.dtype.itemsize, 2, 512, bits)
     try:
         g, _ = _gpu_build(eg, edges, V, 3, 1, 1, 1, False)
         og = oracle_lib.OracleGraph(edges, V, P=3, S=1, eta=1, m=1, threads=8)
@@ -122,7 +124,9 @@ def test_build_every_sort_variant(eg, oracle_lib, scheme, threads, bits, rec):
         pytest.skip("12-byte records use at most 512 threads (LDS)")
     print(f"sort variant scheme={scheme} threads={threads} bits={bits} rec={rec}", flush=True)
     rng = np.random.default_rng(77)
-    V = 9 * 4096 + 100
+    # 9-bit digits are taken only where they save a pass: 18-bit keys (2
+    # passes instead of 3)
+    V = 9 * 4096 + 100 if bits == 8 else 200_000 + 17
     edges = H.random_edges(rng, V, 70001, weighted=(rec == 12))
     keep = eg.get_sort_variant(rec)
     eg.set_sort_variant(rec, scheme, threads, bits)
