@@ -461,9 +461,31 @@ __global__ __launch_bounds__(NT) void k_rx_up(RxIn<R> in, const uint64_t* __rest
         v[k] = fa[rx_src_index(in, sh, x.base + (li < x.count ? li : 0u))];
       }
     }
+    uint32_t dg[kRxIPT];
 #pragma unroll
-    for (int k = 0; k < kRxIPT; ++k)
-      if (quads || uint32_t(k) * NT + threadIdx.x < x.count) atomicAdd(&hw[uint32_t(key.of_field(v[k]) >> shift) & mask], 1u);
+    for (int k = 0; k < kRxIPT; ++k) dg[k] = uint32_t(key.of_field(v[k]) >> shift) & mask;
+    if (quads) {
+      // a load's four keys are consecutive records: a run of one digit
+      // inside it (a vertex of high degree) is one atomic of its length
+#pragma unroll
+      for (int q = 0; q < kRxIPT / 4; ++q) {
+        uint32_t run = 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = 4 * q + j;
+          if (j < 3 && dg[k + 1] == dg[k]) {
+            ++run;
+          } else {
+            atomicAdd(&hw[dg[k]], run);
+            run = 1;
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kRxIPT; ++k)
+        if (uint32_t(k) * NT + threadIdx.x < x.count) atomicAdd(&hw[dg[k]], 1u);
+    }
   } else {
   R r[kRxIPT];
   bool pairs = false;
