@@ -305,14 +305,25 @@ __device__ __forceinline__ void rx_rank(const RxIn<R>& in, int64_t sh, const RxT
     // count read; the compiler emitted it only inside the branch of the
     // count write).  The empty asm statements keep the compiler from moving
     // memory accesses across those points.
-    uint64_t peers = 0;
-    if (valid) __hip_atomic_fetch_or(tab + d, uint64_t(1) << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    __builtin_amdgcn_s_waitcnt(kLgkmZero);
-    __asm__ volatile("" ::: "memory");
-    if (valid) peers = __hip_atomic_load(tab + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    __builtin_amdgcn_s_waitcnt(kLgkmZero);
-    __asm__ volatile("" ::: "memory");
-    if (valid) __hip_atomic_store(tab + d, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    // The lanes sharing the first lane's digit (in a later pass, often a
+    // run of one vertex of high degree, whose records the earlier passes
+    // brought together) take their peers from one ballot and stay off the
+    // table, whose same-entry ORs would serialise; a wave with no other
+    // digit skips the table.
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const bool lead = valid && d == d0;
+    const uint64_t m0 = __ballot(lead);
+    const bool use_tab = valid && !lead;
+    uint64_t peers = lead ? m0 : 0ull;
+    if (__ballot(use_tab)) {
+      if (use_tab) __hip_atomic_fetch_or(tab + d, uint64_t(1) << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      __builtin_amdgcn_s_waitcnt(kLgkmZero);
+      __asm__ volatile("" ::: "memory");
+      if (use_tab) peers = __hip_atomic_load(tab + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      __builtin_amdgcn_s_waitcnt(kLgkmZero);
+      __asm__ volatile("" ::: "memory");
+      if (use_tab) __hip_atomic_store(tab + d, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
     const uint32_t prior = wcnt[w][d];
     __builtin_amdgcn_s_waitcnt(kLgkmZero);
     __asm__ volatile("" ::: "memory");

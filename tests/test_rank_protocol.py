@@ -2,8 +2,10 @@
 (endgraph_amd/csrc/radix.hpp, rx_rank): every lane ORs its bit into its
 digit's entry of a per-wave LDS table of lane masks, reads the entry back,
 clears it, then reads the wave's running count of the digit, and the highest
-peer writes the count back.  The table and the counts belong to one wave, so
-other waves never touch them; what can go wrong is the order in which one
+peer writes the count back.  Lanes whose digit is the first lane's take
+their peers from a ballot instead and stay off the table (a wave with no
+other digit skips the table); lead_ok=False models the table alone.  The table and the counts belong to one wave,
+so other waves never touch them; what can go wrong is the order in which one
 wave's LDS accesses complete.
 
 Two memory semantics are modelled:
@@ -58,25 +60,35 @@ def windows(seq, n_records):
     return [w for w in out if w]
 
 
-def run(dig, valid, seq, order_of):
+def run(dig, valid, seq, order_of, lead_ok=True):
     """Executes the stream; order_of(window) gives the completion order of
     a window's accesses.  Returns the ranks per (record, lane)."""
     tab, count = {}, {}
     peers = np.zeros(dig.shape, np.uint64)
     prior = np.zeros(dig.shape, np.int64)
     rank = np.zeros(dig.shape, np.int64)
+    lead = np.zeros(dig.shape, bool)
+    for k in range(dig.shape[0]):
+        # the ballot of the lanes sharing lane 0's digit (lane 0 is the first
+        # active lane: every lane of the wave runs the loop)
+        lead[k] = valid[k] & (dig[k] == dig[k, 0]) & lead_ok
+        m0 = sum(1 << l for l in range(WAVE) if lead[k, l])
+        for l in range(WAVE):
+            if lead[k, l]:
+                peers[k, l] = m0
     for win in windows(seq, dig.shape[0]):
         for k, op in order_of(win):
             ds = dig[k]
             lanes = [l for l in range(WAVE) if valid[k, l]]
+            tabbed = [l for l in lanes if not lead[k, l]]
             if op == "or":
-                for l in lanes:
+                for l in tabbed:
                     tab[ds[l]] = tab.get(ds[l], 0) | (1 << l)
             elif op == "read":
-                for l in lanes:
+                for l in tabbed:
                     peers[k, l] = tab.get(ds[l], 0)
             elif op == "clear":
-                for l in lanes:
+                for l in tabbed:
                     tab[ds[l]] = 0
             elif op == "rcount":
                 for l in range(WAVE):
@@ -107,7 +119,7 @@ def respects_data(order):
     return True
 
 
-def all_orders(dig, valid, seq, limit=2000):
+def all_orders(dig, valid, seq, limit=2000, lead_ok=True):
     """Ranks under every completion order the waits-only semantics allow
     (the product of the windows' permutations, capped)."""
     wins = windows(seq, dig.shape[0])
@@ -117,7 +129,7 @@ def all_orders(dig, valid, seq, limit=2000):
         if n >= limit:
             break
         table = {tuple(w): list(c) for w, c in zip(wins, choice)}
-        results.append(run(dig, valid, seq, lambda win: table[tuple(win)]))
+        results.append(run(dig, valid, seq, lambda win: table[tuple(win)], lead_ok))
     return results
 
 
@@ -134,22 +146,30 @@ def cases():
     valid = np.ones(dig.shape, bool)
     valid[2, 40:] = False
     out.append((dig, valid))
+    # a run of lane 0's digit over most of the wave, other digits around it
+    dig = rng.integers(1, 4, (3, WAVE))
+    dig[:, :50] = 0
+    dig[1, 10] = 2
+    valid = np.ones(dig.shape, bool)
+    out.append((dig, valid))
     return out
 
 
-@pytest.mark.parametrize("case", range(4))
-def test_shipped_sequence_in_order(case):
+@pytest.mark.parametrize("lead_ok", [True, False])
+@pytest.mark.parametrize("case", range(5))
+def test_shipped_sequence_in_order(case, lead_ok):
     dig, valid = cases()[case]
-    got = run(dig, valid, SHIPPED, lambda w: w)
+    got = run(dig, valid, SHIPPED, lambda w: w, lead_ok)
     exp = expected_ranks(dig, valid)
     assert np.array_equal(got[valid], exp[valid])
 
 
-@pytest.mark.parametrize("case", range(4))
-def test_shipped_sequence_under_every_waits_only_order(case):
+@pytest.mark.parametrize("lead_ok", [True, False])
+@pytest.mark.parametrize("case", range(5))
+def test_shipped_sequence_under_every_waits_only_order(case, lead_ok):
     dig, valid = cases()[case]
     exp = expected_ranks(dig, valid)
-    for got in all_orders(dig, valid, SHIPPED):
+    for got in all_orders(dig, valid, SHIPPED, lead_ok=lead_ok):
         assert np.array_equal(got[valid], exp[valid])
 
 
@@ -164,7 +184,9 @@ def test_each_wait_is_needed_without_in_order_lds(which, breaks):
     """Dropping any one wait lets some waits-only completion order give wrong
     ranks; the third is the subtle one: the clear of record k and the OR of
     record k+1 on the same digit then share a window."""
-    dig = np.zeros((3, WAVE), np.int64)  # one digit: every record collides
+    # every lane but lane 0 shares one digit: they all collide in the table
+    dig = np.ones((3, WAVE), np.int64)
+    dig[:, 0] = 0
     valid = np.ones(dig.shape, bool)
     exp = expected_ranks(dig, valid)
     seq = drop_wait(SHIPPED, which)
