@@ -404,10 +404,14 @@ def twitter_block(eg, args, cores):
         g.build(edges.ptr, E)
     eg.synchronize()
     tb = (time.perf_counter() - t0) / steps
+    bb = build_algorithmic_bytes(g, E)
     out = {"workload": desc, "vertices": V, "edges": E, "build_ms": round(tb * 1e3, 2),
            "build_medges_per_s": round(E / tb / 1e6, 1), "build_steps": steps,
-           "sort_variant": dict(zip(("scheme", "threads", "digit_bits"), eg.get_sort_variant(8)))}
-    log(f"[bench] twitter build: {tb * 1e3:.1f} ms, {E / tb / 1e6:.0f} Medges/s")
+           "sort_variant": dict(zip(("scheme", "threads", "digit_bits"), eg.get_sort_variant(8))),
+           "build_level": {"algorithmic_bytes_per_build": int(bb), "achieved_gbs": round(bb / tb / 1e9, 1),
+                           "frac": round(bb / tb / 1e9 / HBM_PEAK_GBS, 4)}}
+    log(f"[bench] twitter build: {tb * 1e3:.1f} ms, {E / tb / 1e6:.0f} Medges/s, "
+        f"{out['build_level']['frac']:.3f} of peak as a whole")
     prep_ms = g.prepare_engine()
     tv_ms = g.prepare_traversal_view()
     if not args.no_autotune:
