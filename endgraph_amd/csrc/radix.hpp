@@ -4,8 +4,9 @@
 // a stable LSD sort whose result depends only on stability, so any stable sort
 // by the same key gives the identical order.  A sort here covers nseg
 // independent contiguous segments (one per rank slice, or one) and takes
-// ceil(key_bits/8) balanced digit passes.  Two pass schemes, each with the
-// tile size as a template parameter (NT threads x 16 records):
+// ceil(key_bits/8) balanced digit passes, or ceil(key_bits/9) with 9-bit
+// digits where that is fewer.  Two pass schemes, each with the tile size as a
+// template parameter (NT threads x 16 records):
 //
 //   three-launch   upsweep: per-tile digit histogram (LDS atomics) into a
 //                  (segment, digit, tile)-major table; device scan of the
@@ -17,11 +18,13 @@
 //
 // Ranking inside a tile is stable: wave w owns a contiguous run of the tile,
 // handled in rounds of 64 records; the lanes holding the same digit find each
-// other through a per-wave LDS table of lane masks, and per-wave running
-// counters in LDS order the rounds.  The tile is then staged in LDS in digit order, so each
-// digit's run is written contiguously (bigger tiles => longer runs => fewer
-// partially written lines).  Variants are chosen per record size by
-// select_sort_variant (see eg_autotune_sort).
+// other through a per-wave LDS table of lane masks (those sharing the round's
+// first digit through one ballot), and per-wave running counters in LDS order
+// the rounds.  The tile is then staged in LDS in digit order, so each digit's
+// run is written contiguously (bigger tiles => longer runs => fewer partially
+// written lines).  The variant (scheme, threads, digit width) is process-wide
+// per record size: sort_variant(), set by eg_set_sort_variant or
+// eg_autotune_sort.
 //
 // A sort's input, intermediate lists and result are each either records or
 // field arrays (src[], dst[], w[]; RxStorage): where a list is only an
