@@ -613,15 +613,15 @@ __global__ void k_cindex_from_deg(const uint32_t* __restrict__ in_deg, const uin
   }
 }
 // bit u of the CSR bitmap: u has out-edges; one thread per word
-__global__ void k_bitmap_from_deg(const uint32_t* __restrict__ deg, uint32_t V, uint64_t words,
-                                  uint64_t* __restrict__ bitmap) {
-  for (uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words; w += uint64_t(gridDim.x) * blockDim.x) {
-    uint64_t b = 0;
-    for (int i = 0; i < 64; ++i) {
-      const uint64_t u = w * 64 + i;
-      if (u < V && deg[u] > 0) b |= 1ull << i;
-    }
-    bitmap[w] = b;
+__global__ __launch_bounds__(kNT) void k_bitmap_from_deg(const uint32_t* __restrict__ deg, uint32_t V,
+                                                          uint64_t words, uint64_t* __restrict__ bitmap) {
+  // one wave per word: lane i reads the degree of vertex 64w + i (coalesced)
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t nw = uint64_t(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t w = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave; w < words; w += nw) {
+    const uint64_t u = w * 64 + lane;
+    const uint64_t b = __ballot(u < V && deg[u] > 0);
+    if (lane == 0) bitmap[w] = b;
   }
 }
 
@@ -998,8 +998,8 @@ int build_finish(eg_graph* g, hipStream_t s) {
     prof_begin(s);
     EG_TRY(exclusive_scan_u64(index, index, uint64_t(V) + 1, g->scan_tmp.as<uint64_t>(), s));
     prof_end(PK_CSR_INDEX, (uint64_t(V) + 1) * 16, s);
-    hipLaunchKernelGGL(k_bitmap_from_deg, dim3(grid_for(words)), dim3(kNT), 0, s, (const uint32_t*)out_deg, V, words,
-                       g->csr_bitmap.as<uint64_t>());
+    hipLaunchKernelGGL(k_bitmap_from_deg, dim3(grid_for(words, kNT / kWave, 65536)), dim3(kNT), 0, s,
+                       (const uint32_t*)out_deg, V, words, g->csr_bitmap.as<uint64_t>());
     EG_HIP(hipGetLastError());
   } else {
     // --- compressed CSC index ---
