@@ -117,24 +117,33 @@ __global__ __launch_bounds__(kNT) void k_prep_rows(const R* __restrict__ lst, ui
   }
 }
 
-// Merged CSR: edge at position li of sub-partition q's list (source u) goes to
-// out_ptr[u] + (edges of u in sub-partitions < q) + (li - index_q[u]).
+// Merged CSR (every source's out-edges from sub-partition 0, then 1, ...)
+// in two steps.  k_prep_base: per (sub-partition q, source u) the write base
+// out_ptr[u] + (u's edges in sub-partitions before q) - index[q][u], modulo
+// 2^32 (a position is below 2^32), so that edge i of sub-partition q (local
+// index li) goes to base[q][u] + li.  k_prep_csr: one read of that base per
+// edge.
+__global__ void k_prep_base(const uint64_t* __restrict__ index, uint32_t V, int Q,
+                            const uint32_t* __restrict__ out_ptr, uint32_t* __restrict__ base) {
+  for (uint64_t u = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; u < V; u += uint64_t(gridDim.x) * blockDim.x) {
+    uint32_t at = out_ptr[u];
+    for (int q = 0; q < Q; ++q) {
+      const uint64_t* ix = index + uint64_t(q) * (V + 1);
+      const uint64_t lo = ix[u], hi = ix[u + 1];
+      base[uint64_t(q) * V + u] = at - uint32_t(lo);
+      at += uint32_t(hi - lo);
+    }
+  }
+}
+
 template <class R>
-__global__ void k_prep_csr(const R* __restrict__ lst, uint64_t E, const uint64_t* __restrict__ index, uint32_t V, int Q,
-                           const uint64_t* __restrict__ q_off, const uint32_t* __restrict__ out_ptr,
-                           uint32_t* __restrict__ dst, float* __restrict__ w) {
+__global__ void k_prep_csr(const R* __restrict__ lst, uint64_t E, uint32_t V, int Q, const uint64_t* __restrict__ q_off,
+                           const uint32_t* __restrict__ base, uint32_t* __restrict__ dst, float* __restrict__ w) {
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += uint64_t(gridDim.x) * blockDim.x) {
     int q = 0;
     while (q + 1 < Q && q_off[q + 1] <= i) ++q;
     const R r = lst[i];
-    const uint32_t u = r.src;
-    uint64_t before = 0;
-    for (int qq = 0; qq < q; ++qq) {
-      const uint64_t* ix = index + uint64_t(qq) * (V + 1);
-      before += ix[u + 1] - ix[u];
-    }
-    const uint64_t li = i - q_off[q];
-    const uint64_t pos = uint64_t(out_ptr[u]) + before + (li - index[uint64_t(q) * (V + 1) + u]);
+    const uint32_t pos = base[uint64_t(q) * V + r.src] + uint32_t(i - q_off[q]);
     dst[pos] = r.dst;
     if constexpr (sizeof(R) == 12) w[pos] = reinterpret_cast<const Rec12&>(r).w;
   }
@@ -243,9 +252,15 @@ int prepare_impl(eg_graph* g, hipStream_t s) {
     EG_TRY(fill<uint32_t>(out_ptr + V, 1, 0u, s));
     EG_TRY(exclusive_scan_u32(out_ptr, out_ptr, uint64_t(V) + 1, g->scan_tmp.as<uint32_t>(), s));
     if (E) {
-      hipLaunchKernelGGL((k_prep_csr<R>), dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, csr, E,
-                         (const uint64_t*)g->csr_index.as<uint64_t>(), V, Q, g->dmeta()->q_off_csr,
-                         (const uint32_t*)out_ptr, g->e_out_dst.as<uint32_t>(), g->e_out_w.as<float>());
+      // the write bases in the build's sort temporary (free after the build)
+      EG_TRY(g->t3.resize(std::max<uint64_t>(g->t3.bytes, uint64_t(Q) * V * 4)));
+      uint32_t* base = g->t3.as<uint32_t>();
+      hipLaunchKernelGGL(k_prep_base, dim3(grid_for(V, kNT, 65536)), dim3(kNT), 0, s,
+                         (const uint64_t*)g->csr_index.as<uint64_t>(), V, Q, (const uint32_t*)out_ptr, base);
+      EG_HIP(hipGetLastError());
+      hipLaunchKernelGGL((k_prep_csr<R>), dim3(grid_for(E, kNT, 65536)), dim3(kNT), 0, s, csr, E, V, Q,
+                         g->dmeta()->q_off_csr, (const uint32_t*)base, g->e_out_dst.as<uint32_t>(),
+                         g->e_out_w.as<float>());
       EG_HIP(hipGetLastError());
     }
   }
