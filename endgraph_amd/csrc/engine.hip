@@ -62,16 +62,16 @@ int fill(T* p, uint64_t n, T v, hipStream_t s) {
 
 // ------------------------------------------------------------- prepare --
 // SoA source (and weight) of every CSC edge, and the row-start bit per edge
-// (a row = one compressed-index entry: a destination within a sub-partition;
-// without qmap, one destination).
+// (a row = one compressed-index entry: a destination within a sub-partition).
+// Here a row starts where the destination changes; the starts of the
+// sub-partitions, where it may not, are added by k_q_heads.
 // A wave takes kPrepU consecutive words per step, their loads issued before
 // any is used (one word per step left a wave with a single load in flight:
 // 1.9 TB/s on the twitter-shaped graph's 11.7 GB list).
 constexpr int kPrepU = 4;
 template <class R>
 __global__ __launch_bounds__(kNT) void k_prep_csc(const R* __restrict__ lst, uint64_t E, uint32_t* __restrict__ src,
-                                                  float* __restrict__ w, uint64_t* __restrict__ rowflag,
-                                                  const uint16_t* __restrict__ qmap) {
+                                                  float* __restrict__ w, uint64_t* __restrict__ rowflag) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t nwords = ceil_div(E, 64);
   const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
@@ -92,12 +92,20 @@ __global__ __launch_bounds__(kNT) void k_prep_csc(const R* __restrict__ lst, uin
       if (e < E) {
         src[e] = r[k].src;
         if constexpr (sizeof(R) == 12) w[e] = reinterpret_cast<const Rec12&>(r[k]).w;
-        head = e == 0 || p[k].dst != r[k].dst ||
-               (qmap && qmap[p[k].src >> kPageShift] != qmap[r[k].src >> kPageShift]);
+        head = e == 0 || p[k].dst != r[k].dst;
       }
       const uint64_t m = __ballot(head);
       if (lane == 0 && w0 + k < nwords) rowflag[w0 + k] = m;
     }
+  }
+}
+
+// The first edge of every non-empty sub-partition starts a row (the CSC is
+// sorted by sub-partition, then destination).
+__global__ void k_q_heads(uint64_t* __restrict__ rowflag, const uint64_t* __restrict__ q_off, int Q, uint64_t E) {
+  for (int q = int(blockIdx.x * blockDim.x + threadIdx.x); q < Q; q += int(gridDim.x * blockDim.x)) {
+    const uint64_t e = q_off[q];
+    if (e < E) atomicOr(reinterpret_cast<unsigned long long*>(rowflag + (e >> 6)), 1ull << (e & 63));
   }
 }
 
@@ -238,8 +246,12 @@ int prepare_impl(eg_graph* g, hipStream_t s) {
   if (E) {
     hipLaunchKernelGGL((k_prep_csc<R>), dim3(grid_for(ceil_div(nwords, kPrepU), kNT / kWave, 65536)), dim3(kNT), 0, s,
                        csc, E,
-                       g->e_csc_src.as<uint32_t>(), g->e_csc_w.as<float>(), rowflag, g->qmap.as<uint16_t>());
+                       g->e_csc_src.as<uint32_t>(), g->e_csc_w.as<float>(), rowflag);
     EG_HIP(hipGetLastError());
+    if (Q > 1) {
+      hipLaunchKernelGGL(k_q_heads, dim3(1), dim3(kNT), 0, s, rowflag, (const uint64_t*)g->dmeta()->q_off_csc, Q, E);
+      EG_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_popc_words, dim3(grid_for(nwords, kNT, 65536)), dim3(kNT), 0, s, rowflag, nwords, wordrank);
     EG_HIP(hipGetLastError());
   }
