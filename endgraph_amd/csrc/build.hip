@@ -304,18 +304,49 @@ struct KeyDst {
   __device__ __forceinline__ uint64_t operator()(const R& r) const { return r.dst; }
   __device__ __forceinline__ uint64_t of_field(uint32_t k) const { return k; }
 };
-// (q(owner) << vbits) | other endpoint.  kPull: owner = src, key = dst.
+// (q(owner) << vbits) | other endpoint of a record of the CSC (kPull: owner
+// = src, key = dst) or CSR list.  The list is sorted by sub-partition, so q
+// is the sub-partition of the record's position, [q_off[q], q_off[q+1]):
+// taken from the offsets (a search per chunk, scalar loads) instead of a
+// qmap lookup of the owner, a dependent load per record.
 template <bool kPull>
 struct KeyQ {
-  const uint16_t* qmap;
+  const uint64_t* q_off;  // Q + 1 sub-partition offsets of the list
+  int Q;
   int vbits;
+  // sub-partition of position i: the last q with q_off[q] <= i (empty
+  // sub-partitions before a non-empty one share its offset)
+  __device__ __forceinline__ uint32_t q_at(uint64_t i) const {
+    int lo = 0, hi = Q;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (q_off[mid] <= i)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    return uint32_t(lo);
+  }
+  // ... from the sub-partition q of an earlier position on
+  __device__ __forceinline__ uint32_t q_from(uint32_t q, uint64_t i) const {
+    while (int(q) + 1 < Q && q_off[q + 1] <= i) ++q;
+    return q;
+  }
   template <class R>
-  __device__ __forceinline__ uint64_t operator()(const R& r) const {
-    const uint32_t own = kPull ? r.src : r.dst;
-    const uint32_t k = kPull ? r.dst : r.src;
-    return (uint64_t(qmap[own >> kPageShift]) << vbits) | k;
+  __device__ __forceinline__ uint64_t of(const R& r, uint32_t q) const {
+    return (uint64_t(q) << vbits) | (kPull ? r.dst : r.src);
   }
 };
+
+// Key of record i of list a: the functor's own, or KeyQ's from the position.
+template <class R, class KeyFn>
+__device__ __forceinline__ uint64_t key_at(const R* a, uint64_t i, KeyFn key) {
+  return key(a[i]);
+}
+template <class R, bool kPull>
+__device__ __forceinline__ uint64_t key_at(const R* a, uint64_t i, KeyQ<kPull> key) {
+  return key.of(a[i], key.q_at(i));
+}
 
 // ------------------------------------------------------------- indexes --
 // Run-boundary kernels over a sorted list: wave g owns the 512 consecutive
@@ -348,6 +379,34 @@ __device__ __forceinline__ void chunk_keys(const R* __restrict__ a, uint64_t n, 
   for (int j = 0; j < kCR; ++j) c.k[j] = key(r[j]);
   c.prev = c0 > 0 ? key(rp) : kNoKey;
   c.next = nx < n ? key(rn) : kNoKey;
+}
+
+// The same for KeyQ: the sub-partition of the chunk's previous record by a
+// search, then of each record from there on.
+template <class R, bool kPull>
+__device__ __forceinline__ void chunk_keys(const R* __restrict__ a, uint64_t n, uint64_t c0, KeyQ<kPull> key,
+                                           ChunkKeys& c) {
+  const int lane = threadIdx.x & (kWave - 1);
+  R r[kCR];
+#pragma unroll
+  for (int j = 0; j < kCR; ++j) {
+    const uint64_t i = c0 + uint64_t(j) * kWave + lane;
+    r[j] = a[i < n ? i : n - 1];
+  }
+  const R rp = a[c0 > 0 ? c0 - 1 : 0];
+  const uint64_t nx = c0 + kChunk;
+  const R rn = a[nx < n ? nx : n - 1];
+  const uint32_t q0 = key.q_at(c0 > 0 ? c0 - 1 : 0);
+  // the next sub-partition's start; a chunk usually ends before it
+  const uint64_t nb = int(q0) + 1 < key.Q ? key.q_off[q0 + 1] : ~0ull;
+#pragma unroll
+  for (int j = 0; j < kCR; ++j) {
+    const uint64_t i = c0 + uint64_t(j) * kWave + lane;
+    const uint64_t ic = i < n ? i : n - 1;
+    c.k[j] = key.of(r[j], ic < nb ? q0 : key.q_from(q0, ic));
+  }
+  c.prev = c0 > 0 ? key.of(rp, q0) : kNoKey;
+  c.next = nx < n ? key.of(rn, nx < nb ? q0 : key.q_from(q0, nx)) : kNoKey;
 }
 
 // head: first record of a run (key differs from the previous record)
@@ -429,7 +488,7 @@ __device__ uint64_t runs_before(const R* a, uint64_t n, uint64_t x, KeyFn key, c
   const uint64_t ch = x / kChunk;
   const uint64_t b = ch * kChunk;
   uint32_t c = 0;
-  for (uint64_t i = b + threadIdx.x; i < x; i += kNT) c += (i == 0 || key(a[i]) != key(a[i - 1])) ? 1u : 0u;
+  for (uint64_t i = b + threadIdx.x; i < x; i += kNT) c += (i == 0 || key_at(a, i, key) != key_at(a, i - 1, key)) ? 1u : 0u;
   c = wave_reduce(c, OpSum{});
   if ((threadIdx.x & (kWave - 1)) == 0) wt[threadIdx.x / kWave] = c;
   __syncthreads();
@@ -1005,7 +1064,7 @@ int build_finish(eg_graph* g, hipStream_t s) {
     // --- compressed CSC index ---
     {
       const R* a = g->csc_list.as<R>();
-      KeyQ<true> key{qmap, g->vbits};
+      const KeyQ<true> key{meta->q_off_csc, Q, g->vbits};
       const uint64_t nch = ceil_div(E, kChunk);
       uint32_t* tc = g->tile_cnt.as<uint32_t>();
       CIdx* cidx = g->csc_cidx.as<CIdx>();
@@ -1046,7 +1105,7 @@ int build_finish(eg_graph* g, hipStream_t s) {
       if (E) {
         prof_begin(s);
         hipLaunchKernelGGL((k_csr_gaps<R>), dim3(chunk_grid(E)), dim3(kNT), 0, s, g->csr_list.as<R>(), E,
-                           KeyQ<false>{qmap, g->vbits}, V, (const eg_graph::Meta*)meta, index,
+                           KeyQ<false>{meta->q_off_csr, Q, g->vbits}, V, (const eg_graph::Meta*)meta, index,
                            reinterpret_cast<unsigned long long*>(bitmap), gaps, ngaps, max_gaps);
         prof_end(PK_CSR_INDEX, E * rb, s);
         EG_HIP(hipGetLastError());
