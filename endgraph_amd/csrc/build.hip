@@ -409,21 +409,33 @@ __device__ __forceinline__ void chunk_keys(const R* __restrict__ a, uint64_t n, 
   c.next = nx < n ? key.of(rn, nx < nb ? q0 : key.q_from(q0, nx)) : kNoKey;
 }
 
+// Neighbour lanes through DPP wave shifts, with no LDS round trip: lane l
+// gets lane l-1's value (wave_shr:1; lane 0 gets `fill`), or lane l+1's
+// (wave_shl:1; lane 63 gets `fill`).  With bound_ctrl off, a lane whose
+// source is outside the wave keeps the `old` operand, and so does a lane
+// whose source is disabled in EXEC: call these with all 64 lanes active, at
+// the top of a function, never under a condition (a short-circuit `&&`
+// included) that can switch a neighbour off.
+template <int kCtrl>
+__device__ __forceinline__ uint64_t dpp_shift_u64(uint64_t v, uint64_t fill) {
+  const int lo = __builtin_amdgcn_update_dpp(int(uint32_t(fill)), int(uint32_t(v)), kCtrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(int(uint32_t(fill >> 32)), int(uint32_t(v >> 32)), kCtrl, 0xf, 0xf, false);
+  return (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo);
+}
+__device__ __forceinline__ uint64_t lane_prev(uint64_t v, uint64_t fill) { return dpp_shift_u64<0x138>(v, fill); }
+__device__ __forceinline__ uint64_t lane_next(uint64_t v, uint64_t fill) { return dpp_shift_u64<0x130>(v, fill); }
+
 // head: first record of a run (key differs from the previous record)
 __device__ __forceinline__ bool chunk_head(const ChunkKeys& c, int j, uint64_t i, uint64_t n) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t up = shfl_up_t(c.k[j], 1);
   const uint64_t last_prev = j > 0 ? shfl_t(c.k[j > 0 ? j - 1 : 0], kWave - 1) : c.prev;
-  const uint64_t pk = lane > 0 ? up : last_prev;
+  const uint64_t pk = lane_prev(c.k[j], last_prev);  // every lane, before the branch
   return i < n && pk != c.k[j];
 }
 
 // tail: last record of a run (key differs from the next record)
 __device__ __forceinline__ bool chunk_tail(const ChunkKeys& c, int j, uint64_t i, uint64_t n) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t dn = __shfl_down(c.k[j], 1, kWave);
   const uint64_t first_next = j < kCR - 1 ? shfl_t(c.k[j < kCR - 1 ? j + 1 : j], 0) : c.next;
-  const uint64_t nk = lane < kWave - 1 ? dn : first_next;
+  const uint64_t nk = lane_next(c.k[j], first_next);  // every lane, before the branch
   return i < n && (i == n - 1 || nk != c.k[j]);
 }
 
@@ -465,6 +477,9 @@ __global__ __launch_bounds__(kNT) void k_cindex_write(const R* __restrict__ a, u
     ChunkKeys c;
     chunk_keys(a, n, c0, key, c);
     uint64_t run = off[chunk];
+    // the chunk's first sub-partition start, loaded once (wave-uniform)
+    const uint32_t qc = uint32_t(shfl_t(c.k[0], 0) >> key.vbits);
+    const uint64_t qb0 = meta->q_off_csc[qc < uint32_t(key.Q) ? qc : 0];
 #pragma unroll
     for (int j = 0; j < kCR; ++j) {
       const uint64_t i = c0 + uint64_t(j) * kWave + lane;
@@ -472,8 +487,10 @@ __global__ __launch_bounds__(kNT) void k_cindex_write(const R* __restrict__ a, u
       const uint64_t H = __ballot(h);
       if (h) {
         const uint32_t q = uint32_t(c.k[j] >> key.vbits);
+        uint64_t qbase = qb0;
+        if (q != qc) qbase = meta->q_off_csc[q];
         CIdx e;
-        e.index = i - meta->q_off_csc[q];
+        e.index = i - qbase;
         e.vertex = uint32_t(c.k[j] & vmask);
         out[run + __popcll(H & lt) + q] = e;
       }
@@ -585,12 +602,10 @@ __global__ __launch_bounds__(kNT) void k_csr_gaps(const R* __restrict__ a, uint6
       const uint64_t k = c.k[j];
       const uint32_t q = uint32_t(k >> key.vbits);
       const uint32_t u = uint32_t(k & vmask);
-      const uint64_t up = shfl_up_t(k, 1);
       const uint64_t last_prev = j > 0 ? shfl_t(c.k[j > 0 ? j - 1 : 0], kWave - 1) : c.prev;
-      const uint64_t pk = lane > 0 ? up : last_prev;
-      const uint64_t dn = __shfl_down(k, 1, kWave);
+      const uint64_t pk = lane_prev(k, last_prev);
       const uint64_t first_next = j < kCR - 1 ? shfl_t(c.k[j < kCR - 1 ? j + 1 : j], 0) : c.next;
-      const uint64_t nk = lane < kWave - 1 ? dn : first_next;
+      const uint64_t nk = lane_next(k, first_next);
       const bool valid = i < n;
       if (valid && pk != k) {  // run head
         const uint64_t qbase = meta->q_off_csr[q];

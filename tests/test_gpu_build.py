@@ -90,6 +90,27 @@ def test_build_edge_cases(eg, oracle_lib):
         assert_build_equal(g, og)
 
 
+@pytest.mark.parametrize("E", [40 * 512, 40 * 512 - 64, 40 * 512 + 2, 512])
+@pytest.mark.parametrize("P", [1, 2, 8])
+def test_build_runs_ending_at_a_chunk_end(eg, oracle_lib, E, P):
+    """The lists sorted by source and by destination end in a run of several
+    records of the top vertex, and the list ends on the last lane of a
+    512-record wave chunk (E a multiple of 512), on the last lane of an
+    earlier row, or two records into a chunk: each record of the last run
+    counts once in the degrees, and the run ends only at the list's end."""
+    print(f"build runs at a chunk end E={E} P={P}", flush=True)
+    rng = np.random.default_rng(E + P)
+    V = 5000
+    edges = H.random_edges(rng, V - 1, E, skew=False)  # the top vertex only where set below
+    k = min(E // 4, 37)
+    edges["src"][-k:] = V - 1
+    edges["dst"][:k] = V - 1
+    g, _ = _gpu_build(eg, edges, V, P, 1, 1, 1, False)
+    og = oracle_lib.OracleGraph(edges, V, P=P, S=1, eta=1, m=1, threads=4)
+    assert g.out_degree()[V - 1] == k and g.in_degree()[V - 1] == k
+    assert_build_equal(g, og)
+
+
 def test_build_rejects_out_of_range_ids(eg):
     e = np.zeros(10, H.REC8)
     e["src"] = np.arange(10)
